@@ -1,0 +1,317 @@
+"""cuts_amd -- MI355X trie-based subgraph matcher, Python binding.
+
+A thin ctypes layer over ``libcuts_amd.so`` (C ABI in ``include/cuts.h``).
+The matcher itself runs in hand-written HIP kernels; this module only moves
+arguments across the boundary.  It raises ``RuntimeError`` when the native
+library has not been built -- there is no Python or CPU fallback.
+
+Mirrors of the reference's host functions (``inc/host_funcs.h:21-25``):
+``search(query_file, data_file, write_to_disk)`` and
+``search_dfs_bfs_strategy(query_file, data_file, write_to_disk, trunks)``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcuts_amd.so")
+CLI_PATH = os.path.join(_HERE, "cuts")
+
+FLAG_NO_SHORTCUT = 1
+FLAG_NO_FUSE = 2
+FLAG_WRITE = 4
+FLAG_NO_XCD = 8
+FLAG_PROFILE = 16
+MAX_QUERY = 16
+KERNEL_CLASSES = ["level0", "plan", "scan", "tile_index", "expand_write", "expand_count", "expand_fused",
+                  "closed_form"]
+
+_u32p = C.POINTER(C.c_uint32)
+
+
+class _GraphView(C.Structure):
+    _fields_ = [("V", C.c_uint32), ("arcs", C.c_uint64), ("symmetric", C.c_int),
+                ("out_off", _u32p), ("out_adj", _u32p), ("in_off", _u32p), ("in_adj", _u32p),
+                ("sig", _u32p)]
+
+
+class _Opts(C.Structure):
+    _fields_ = [("trunks", C.c_uint32), ("flags", C.c_uint32), ("budget_bytes", C.c_uint64),
+                ("root_begin", C.c_uint32), ("root_end", C.c_uint32), ("root_stride", C.c_uint32),
+                ("blocks_per_cu", C.c_uint32), ("emb_rows", _u32p), ("emb_cap", C.c_uint64)]
+
+
+class _Result(C.Structure):
+    _fields_ = [("count", C.c_uint64), ("level_sizes", C.c_uint64 * 16), ("device_ms", C.c_double),
+                ("emb_written", C.c_uint64), ("chunks", C.c_uint32), ("reserved", C.c_uint32),
+                ("kernel_ms", C.c_double * 8), ("kernel_launches", C.c_uint64 * 8), ("kernel_bytes", C.c_double * 8)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libcuts_amd.so (once).  Fails loudly if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C cuts_amd")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        pp = C.POINTER(C.c_void_p)
+        sig = {
+            "cuts_last_error": (C.c_char_p, []),
+            "cuts_version": (C.c_char_p, []),
+            "cuts_graph_load": (C.c_int, [C.c_char_p, pp]),
+            "cuts_graph_from_arcs": (C.c_int, [C.c_uint32, _u32p, _u32p, C.c_uint64, C.c_int, pp]),
+            "cuts_graph_save": (C.c_int, [vp, C.c_char_p]),
+            "cuts_graph_view_of": (C.c_int, [vp, C.POINTER(_GraphView)]),
+            "cuts_graph_free": (None, [vp]),
+            "cuts_gen_chung_lu": (C.c_int, [C.c_uint32, C.c_uint64, C.c_double, C.c_uint32, C.c_uint64, pp]),
+            "cuts_gen_rmat": (C.c_int, [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
+                                        C.c_uint64, pp]),
+            "cuts_find_pattern": (C.c_int, [vp, C.c_uint32, _u32p, C.c_uint32, C.c_uint64, _u32p]),
+            "cuts_walk_query": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint64, _u32p, _u32p, _u32p]),
+            "cuts_plan_create": (C.c_int, [vp, _u32p, pp]),
+            "cuts_plan_info": (C.c_int, [vp, _u32p, _u32p, _u32p, _u32p, _u32p, _u32p]),
+            "cuts_plan_free": (None, [vp]),
+            "cuts_context_create": (C.c_int, [C.c_int, pp]),
+            "cuts_context_destroy": (None, [vp]),
+            "cuts_device_graph_upload": (C.c_int, [vp, vp, pp]),
+            "cuts_device_graph_free": (None, [vp]),
+            "cuts_match_options_default": (None, [C.POINTER(_Opts)]),
+            "cuts_match": (C.c_int, [vp, vp, vp, C.POINTER(_Opts), C.POINTER(_Result)]),
+            "cuts_search": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.POINTER(C.c_uint64)]),
+            "cuts_search_dfs_bfs_strategy": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_uint32,
+                                                       C.POINTER(C.c_uint64)]),
+            "cuts_write_matches": (C.c_int, [C.c_char_p, vp, _u32p, C.c_uint64]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class CutsError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"cuts error {code}: {msg}")
+        self.code = code
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise CutsError(rc, lib().cuts_last_error().decode())
+
+
+def _u32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint32))
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_u32p)
+
+
+class Graph:
+    """Host CSR graph (``Graph`` of src/graph.cpp:18-61)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.cuts_graph_free(self._h)
+            self._h = None
+
+    @classmethod
+    def load(cls, path: str) -> "Graph":
+        h = C.c_void_p()
+        _check(lib().cuts_graph_load(path.encode(), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_arcs(cls, V: int, src, dst, symmetrize: bool = False) -> "Graph":
+        s, d = _u32(src), _u32(dst)
+        h = C.c_void_p()
+        _check(lib().cuts_graph_from_arcs(V, _ptr(s), _ptr(d), len(s), int(symmetrize), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_edges(cls, V: int, edges: Sequence[Sequence[int]]) -> "Graph":
+        """Undirected graph: every edge becomes both arcs."""
+        e = np.asarray(edges, dtype=np.uint32).reshape(-1, 2)
+        return cls.from_arcs(V, e[:, 0], e[:, 1], symmetrize=True)
+
+    @classmethod
+    def chung_lu(cls, n, m, gamma, max_deg=0, seed=0) -> "Graph":
+        h = C.c_void_p()
+        _check(lib().cuts_gen_chung_lu(n, m, gamma, max_deg, seed, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def rmat(cls, scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=0) -> "Graph":
+        h = C.c_void_p()
+        _check(lib().cuts_gen_rmat(scale, edge_factor, a, b, c, seed, C.byref(h)))
+        return cls(h.value)
+
+    def save(self, path: str):
+        _check(lib().cuts_graph_save(self._h, path.encode()))
+
+    def view(self):
+        v = _GraphView()
+        _check(lib().cuts_graph_view_of(self._h, C.byref(v)))
+        n, a = v.V, v.arcs
+
+        def arr(p, cnt):
+            if cnt == 0:
+                return np.zeros(0, dtype=np.uint32)
+            return np.ctypeslib.as_array(p, shape=(cnt,)).copy()
+        return dict(V=n, arcs=a, symmetric=bool(v.symmetric), out_off=arr(v.out_off, n + 1),
+                    out_adj=arr(v.out_adj, a), in_off=arr(v.in_off, n + 1), in_adj=arr(v.in_adj, a),
+                    sig=arr(v.sig, 2 * n))
+
+    @property
+    def V(self) -> int:
+        v = _GraphView()
+        _check(lib().cuts_graph_view_of(self._h, C.byref(v)))
+        return v.V
+
+    @property
+    def arcs(self) -> int:
+        v = _GraphView()
+        _check(lib().cuts_graph_view_of(self._h, C.byref(v)))
+        return v.arcs
+
+    def find_pattern(self, k: int, edges, seed: int = 0) -> np.ndarray:
+        e = _u32(edges).reshape(-1)
+        w = np.zeros(k, dtype=np.uint32)
+        _check(lib().cuts_find_pattern(self._h, k, _ptr(e), len(e) // 2, seed, _ptr(w)))
+        return w
+
+    def walk_query(self, k: int, n_edges: int, seed: int):
+        cap = k * (k - 1) // 2
+        e = np.zeros(2 * cap, dtype=np.uint32)
+        n = C.c_uint32()
+        w = np.zeros(k, dtype=np.uint32)
+        _check(lib().cuts_walk_query(self._h, k, n_edges, seed, _ptr(e), C.byref(n), _ptr(w)))
+        return e[: 2 * n.value].reshape(-1, 2).copy(), w
+
+
+class Plan:
+    """Query + matching order (``Graph::sort_search_order``, src/graph.cpp:62-134)."""
+
+    def __init__(self, query: Graph, order: Optional[Sequence[int]] = None):
+        h = C.c_void_p()
+        o = None if order is None else _u32(order)
+        _check(lib().cuts_plan_create(query._h, None if o is None else _ptr(o), C.byref(h)))
+        self._h = h
+        self.query = query
+        k = C.c_uint32()
+        _check(lib().cuts_plan_info(self._h, C.byref(k), None, None, None, None, None))
+        self.k = k.value
+        order_a = np.zeros(self.k, np.uint32)
+        po = np.zeros(self.k + 1, np.uint32)
+        co = np.zeros(self.k + 1, np.uint32)
+        _check(lib().cuts_plan_info(self._h, None, _ptr(order_a), _ptr(po), None, _ptr(co), None))
+        par = np.zeros(max(int(po[-1]), 1), np.uint32)
+        chi = np.zeros(max(int(co[-1]), 1), np.uint32)
+        _check(lib().cuts_plan_info(self._h, None, None, None, _ptr(par), None, _ptr(chi)))
+        self.order = order_a.tolist()
+        self.parents_offset = po.tolist()
+        self.children_offset = co.tolist()
+        self.parents = par[: po[-1]].tolist()
+        self.children = chi[: co[-1]].tolist()
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.cuts_plan_free(self._h)
+            self._h = None
+
+
+@dataclass
+class MatchResult:
+    count: int
+    level_sizes: List[int]
+    device_ms: float
+    chunks: int
+    rows: Optional[np.ndarray] = None  # (n, k) data vertices per matching-order position
+    kernels: Optional[dict] = None     # FLAG_PROFILE: class -> (ms, launches, bytes)
+
+
+class DeviceGraph:
+    def __init__(self, ctx: "Context", handle):
+        self.ctx = ctx
+        self._h = C.c_void_p(handle)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.cuts_device_graph_free(self._h)
+            self._h = None
+
+
+class Context:
+    """One GPU: stream, events and the trie-table arena."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(lib().cuts_context_create(device, C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.cuts_context_destroy(self._h)
+            self._h = None
+
+    def upload(self, g: Graph) -> DeviceGraph:
+        h = C.c_void_p()
+        _check(lib().cuts_device_graph_upload(self._h, g._h, C.byref(h)))
+        return DeviceGraph(self, h.value)
+
+    def match(self, dg: DeviceGraph, plan: Plan, trunks: int = 0, flags: int = 0, budget: int = 0,
+              roots: Optional[tuple] = None, write: bool = False, emb_cap: int = 0,
+              blocks_per_cu: int = 0) -> MatchResult:
+        o = _Opts()
+        lib().cuts_match_options_default(C.byref(o))
+        o.trunks, o.flags, o.budget_bytes, o.blocks_per_cu = trunks, flags, budget, blocks_per_cu
+        if roots is not None:
+            o.root_begin, o.root_end, o.root_stride = roots
+        rows = None
+        if write:
+            o.flags |= FLAG_WRITE
+            rows = np.zeros((max(emb_cap, 1), plan.k), dtype=np.uint32)
+            o.emb_rows = _ptr(rows)
+            o.emb_cap = emb_cap
+        r = _Result()
+        _check(lib().cuts_match(self._h, dg._h, plan._h, C.byref(o), C.byref(r)))
+        res = MatchResult(int(r.count), [int(x) for x in r.level_sizes[: plan.k]], float(r.device_ms),
+                          int(r.chunks))
+        if write:
+            res.rows = rows[: r.emb_written].copy()
+        if o.flags & FLAG_PROFILE:
+            res.kernels = {KERNEL_CLASSES[i]: (float(r.kernel_ms[i]), int(r.kernel_launches[i]),
+                                               float(r.kernel_bytes[i])) for i in range(8)}
+        return res
+
+
+def search(query_file: str, data_file: str, write_to_disk: bool = False) -> int:
+    c = C.c_uint64()
+    _check(lib().cuts_search(query_file.encode(), data_file.encode(), int(write_to_disk), C.byref(c)))
+    return c.value
+
+
+def search_dfs_bfs_strategy(query_file: str, data_file: str, write_to_disk: bool, trunks: int) -> int:
+    c = C.c_uint64()
+    _check(lib().cuts_search_dfs_bfs_strategy(query_file.encode(), data_file.encode(), int(write_to_disk),
+                                              trunks, C.byref(c)))
+    return c.value
+
+
+def write_matches(path: str, plan: Plan, rows: np.ndarray):
+    r = _u32(rows)
+    _check(lib().cuts_write_matches(path.encode(), plan._h, _ptr(r), r.shape[0] if r.ndim == 2 else 0))

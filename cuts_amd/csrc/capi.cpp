@@ -1,0 +1,348 @@
+// extern "C" boundary (include/cuts.h).  Every entry point converts
+// exceptions into status codes; nothing here calls exit().
+#include "../../include/cuts.h"
+
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <memory>
+#include <sstream>
+
+#include "device.hpp"
+
+struct cuts_graph {
+  cuts::Graph g;
+};
+struct cuts_plan {
+  cuts::Plan p;
+};
+struct cuts_context {
+  cuts::Context c;
+};
+struct cuts_device_graph {
+  cuts::DeviceGraph dg;
+};
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+template <typename F>
+int guard(F&& f) {
+  try {
+    return f();
+  } catch (const cuts::Error& e) {
+    return fail(e.code, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(CUTS_E_NOMEM, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(CUTS_E_ARG, e.what());
+  }
+}
+
+int check(const std::string& err, int code) {
+  if (!err.empty()) throw cuts::Error(code, err);
+  return CUTS_OK;
+}
+
+// src/common.cpp:19-26: text after the last '/', or "" when there is none.
+std::string base_name(const std::string& s) {
+  size_t i = s.rfind('/');
+  return i == std::string::npos ? std::string() : s.substr(i + 1);
+}
+
+// Default ostream formatting of a float, as `cout << time_milli_sec`.
+std::string ostream_float(float f) {
+  std::ostringstream os;
+  os << f;
+  return os.str();
+}
+
+void write_matches(const std::string& path, const cuts::Plan& p, const uint32_t* rows, uint64_t n) {
+  std::ofstream out(path, std::ios::binary);
+  if (!out) throw cuts::Error(CUTS_E_IO, "cannot open " + path);
+  std::string buf;
+  buf.reserve(1 << 22);
+  char tmp[32];
+  for (uint64_t r = 0; r < n; ++r) {
+    for (uint32_t i = 0; i < p.k; ++i) {
+      int len = snprintf(tmp, sizeof tmp, "(%u,%u)", p.order[i], rows[r * p.k + i]);
+      buf.append(tmp, len);
+    }
+    buf += '\n';
+    if (buf.size() > (1u << 22)) { out << buf; buf.clear(); }
+  }
+  out << buf;
+  if (!out.good()) throw cuts::Error(CUTS_E_IO, "write failed for " + path);
+}
+
+void plan_for(const cuts::Graph& q, const uint32_t* order, cuts::Plan& p) {
+  if (q.V == 0) throw cuts::Error(CUTS_E_QUERY, "empty query graph");
+  if (q.V > cuts::kMaxQuery) throw cuts::Error(CUTS_E_QUERY, "query has more than 16 vertices");
+  if (!cuts::weakly_connected(q)) throw cuts::Error(CUTS_E_QUERY, "query graph is not weakly connected");
+  std::vector<uint32_t> ord;
+  if (order) ord.assign(order, order + q.V);
+  else check(cuts::heuristic_order(q, ord), CUTS_E_QUERY);
+  check(cuts::make_plan(q, ord, p), CUTS_E_QUERY);
+}
+
+cuts::MatchOpts to_opts(const cuts_match_options* o) {
+  cuts::MatchOpts m;
+  if (!o) return m;
+  m.trunks = o->trunks;
+  m.flags = o->flags;
+  m.budget = o->budget_bytes;
+  m.root_begin = o->root_begin;
+  m.root_end = o->root_end;
+  m.root_stride = o->root_stride;
+  m.blocks_per_cu = o->blocks_per_cu;
+  m.emb_rows = o->emb_rows;
+  m.emb_cap = o->emb_cap;
+  return m;
+}
+
+// search / search_dfs_bfs_strategy (src/host_funcs.cu:39-171).
+int search_impl(const char* query_file, const char* data_file, int write_to_disk, bool trunked, uint32_t trunks,
+                uint64_t* count) {
+  std::cout << "start loading graph file from disk to memory..." << std::endl;
+  cuts::Graph q, d;
+  check(cuts::load_graph(query_file, q), CUTS_E_IO);
+  check(cuts::load_graph(data_file, d), CUTS_E_IO);
+  std::cout << "graph loading complete..." << std::endl;
+  cuts::Plan p;
+  plan_for(q, nullptr, p);
+  const std::string qb = base_name(query_file), db = base_name(data_file);
+  auto ctx = std::make_unique<cuts::Context>();
+  cuts::context_init(*ctx, 0);
+  std::cout << "start copying graph to gpu..." << std::endl;
+  auto dg = std::make_unique<cuts::DeviceGraph>();
+  cuts::upload_graph(*ctx, d, *dg);
+  std::cout << (trunked ? "finish copying graph to gpu..." : "end copying graph to gpu...") << std::endl;
+  cuts::MatchOpts o;
+  o.trunks = trunked ? trunks : 0;
+  cuts::MatchResult r;
+  cuts::run_match(*ctx, *dg, p, o, r);
+  std::cout << db << "," << qb << "," << ostream_float((float)r.device_ms) << "ms," << r.count << std::endl;
+  if (write_to_disk) {
+    std::cout << "start writting matching results to disk,ans.txt" << std::endl;
+    const uint64_t words = r.count * (uint64_t)p.k;
+    if (words > (1ull << 34)) throw cuts::Error(CUTS_E_NOMEM, "too many embeddings to write");
+    std::vector<uint32_t> rows(std::max<uint64_t>(words, 1));
+    cuts::MatchOpts ow = o;
+    ow.flags |= CUTS_FLAG_WRITE;
+    ow.emb_rows = rows.data();
+    ow.emb_cap = r.count;
+    cuts::MatchResult rw;
+    cuts::run_match(*ctx, *dg, p, ow, rw);
+    write_matches("ans.txt", p, rows.data(), rw.emb_written);
+    std::cout << "finish writting matching results to disk,ans.txt" << std::endl;
+  }
+  if (count) *count = r.count;
+  return CUTS_OK;
+}
+}  // namespace
+
+extern "C" {
+
+const char* cuts_last_error(void) { return g_err.c_str(); }
+const char* cuts_version(void) { return "cuts_amd 0.1 (gfx950)"; }
+
+int cuts_graph_load(const char* path, cuts_graph** out) {
+  return guard([&]() -> int {
+    if (!path || !out) return fail(CUTS_E_ARG, "null argument");
+    auto g = std::make_unique<cuts_graph>();
+    check(cuts::load_graph(path, g->g), CUTS_E_IO);
+    *out = g.release();
+    return CUTS_OK;
+  });
+}
+
+int cuts_graph_from_arcs(uint32_t V, const uint32_t* src, const uint32_t* dst, uint64_t m, int symmetrize,
+                         cuts_graph** out) {
+  return guard([&]() -> int {
+    if (!out || (m && (!src || !dst))) return fail(CUTS_E_ARG, "null argument");
+    std::vector<uint64_t> arcs(m);
+    for (uint64_t i = 0; i < m; ++i) arcs[i] = ((uint64_t)src[i] << 32) | dst[i];
+    auto g = std::make_unique<cuts_graph>();
+    check(cuts::build_graph(V, std::move(arcs), symmetrize != 0, g->g), CUTS_E_ARG);
+    *out = g.release();
+    return CUTS_OK;
+  });
+}
+
+int cuts_graph_save(const cuts_graph* g, const char* path) {
+  return guard([&]() -> int {
+    if (!g || !path) return fail(CUTS_E_ARG, "null argument");
+    return check(cuts::save_graph(path, g->g), CUTS_E_IO);
+  });
+}
+
+int cuts_graph_view_of(const cuts_graph* g, cuts_graph_view* v) {
+  if (!g || !v) return fail(CUTS_E_ARG, "null argument");
+  v->V = g->g.V;
+  v->arcs = g->g.arcs();
+  v->symmetric = g->g.symmetric;
+  v->out_off = g->g.out_off.data();
+  v->out_adj = g->g.out_adj.data();
+  v->in_off = g->g.in_off.data();
+  v->in_adj = g->g.in_adj.data();
+  v->sig = g->g.sig.data();
+  return CUTS_OK;
+}
+
+void cuts_graph_free(cuts_graph* g) { delete g; }
+
+int cuts_gen_chung_lu(uint32_t n, uint64_t m, double gamma, uint32_t max_deg, uint64_t seed, cuts_graph** out) {
+  return guard([&]() -> int {
+    if (!out) return fail(CUTS_E_ARG, "null argument");
+    std::vector<uint64_t> e;
+    check(cuts::gen_chung_lu(n, m, gamma, max_deg, seed, e), CUTS_E_ARG);
+    auto g = std::make_unique<cuts_graph>();
+    check(cuts::build_graph(n, std::move(e), true, g->g), CUTS_E_ARG);
+    *out = g.release();
+    return CUTS_OK;
+  });
+}
+
+int cuts_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c, uint64_t seed,
+                  cuts_graph** out) {
+  return guard([&]() -> int {
+    if (!out) return fail(CUTS_E_ARG, "null argument");
+    std::vector<uint64_t> e;
+    check(cuts::gen_rmat(scale, edge_factor, a, b, c, seed, e), CUTS_E_ARG);
+    auto g = std::make_unique<cuts_graph>();
+    check(cuts::build_graph(1u << scale, std::move(e), true, g->g), CUTS_E_ARG);
+    *out = g.release();
+    return CUTS_OK;
+  });
+}
+
+int cuts_find_pattern(const cuts_graph* d, uint32_t k, const uint32_t* edges, uint32_t ne, uint64_t seed,
+                      uint32_t* witness) {
+  return guard([&]() -> int {
+    if (!d || !edges || !witness) return fail(CUTS_E_ARG, "null argument");
+    std::vector<uint32_t> pe(edges, edges + 2 * (size_t)ne), w;
+    check(cuts::find_pattern(d->g, k, pe, seed, 100000, w), CUTS_E_QUERY);
+    std::copy(w.begin(), w.end(), witness);
+    return CUTS_OK;
+  });
+}
+
+int cuts_walk_query(const cuts_graph* d, uint32_t k, uint32_t n_edges, uint64_t seed, uint32_t* edges,
+                    uint32_t* n_out, uint32_t* witness) {
+  return guard([&]() -> int {
+    if (!d || !edges || !n_out || !witness) return fail(CUTS_E_ARG, "null argument");
+    std::vector<uint32_t> qe, w;
+    check(cuts::walk_query(d->g, k, n_edges, seed, qe, w), CUTS_E_QUERY);
+    std::copy(qe.begin(), qe.end(), edges);
+    *n_out = (uint32_t)(qe.size() / 2);
+    std::copy(w.begin(), w.end(), witness);
+    return CUTS_OK;
+  });
+}
+
+int cuts_plan_create(const cuts_graph* query, const uint32_t* order, cuts_plan** out) {
+  return guard([&]() -> int {
+    if (!query || !out) return fail(CUTS_E_ARG, "null argument");
+    auto p = std::make_unique<cuts_plan>();
+    plan_for(query->g, order, p->p);
+    *out = p.release();
+    return CUTS_OK;
+  });
+}
+
+int cuts_plan_info(const cuts_plan* p, uint32_t* k, uint32_t* order, uint32_t* par_off, uint32_t* par,
+                   uint32_t* chi_off, uint32_t* chi) {
+  if (!p) return fail(CUTS_E_ARG, "null argument");
+  const cuts::Plan& P = p->p;
+  if (k) *k = P.k;
+  if (order) std::copy(P.order.begin(), P.order.end(), order);
+  if (par_off) std::copy(P.par_off.begin(), P.par_off.end(), par_off);
+  if (par) std::copy(P.par.begin(), P.par.end(), par);
+  if (chi_off) std::copy(P.chi_off.begin(), P.chi_off.end(), chi_off);
+  if (chi) std::copy(P.chi.begin(), P.chi.end(), chi);
+  return CUTS_OK;
+}
+
+void cuts_plan_free(cuts_plan* p) { delete p; }
+
+int cuts_context_create(int device, cuts_context** out) {
+  return guard([&]() -> int {
+    if (!out) return fail(CUTS_E_ARG, "null argument");
+    auto c = std::make_unique<cuts_context>();
+    cuts::context_init(c->c, device);
+    *out = c.release();
+    return CUTS_OK;
+  });
+}
+
+void cuts_context_destroy(cuts_context* ctx) { delete ctx; }
+
+int cuts_device_graph_upload(cuts_context* ctx, const cuts_graph* g, cuts_device_graph** out) {
+  return guard([&]() -> int {
+    if (!ctx || !g || !out) return fail(CUTS_E_ARG, "null argument");
+    auto d = std::make_unique<cuts_device_graph>();
+    cuts::upload_graph(ctx->c, g->g, d->dg);
+    *out = d.release();
+    return CUTS_OK;
+  });
+}
+
+void cuts_device_graph_free(cuts_device_graph* dg) { delete dg; }
+
+void cuts_match_options_default(cuts_match_options* o) {
+  if (o) std::memset(o, 0, sizeof(*o));
+}
+
+int cuts_match(cuts_context* ctx, const cuts_device_graph* dg, const cuts_plan* plan, const cuts_match_options* opt,
+               cuts_match_result* res) {
+  return guard([&]() -> int {
+    if (!ctx || !dg || !plan || !res) return fail(CUTS_E_ARG, "null argument");
+    if (opt && (opt->flags & CUTS_FLAG_WRITE) && opt->emb_cap && !opt->emb_rows)
+      return fail(CUTS_E_ARG, "CUTS_FLAG_WRITE needs emb_rows");
+    cuts::MatchResult r;
+    cuts::run_match(ctx->c, const_cast<cuts::DeviceGraph&>(dg->dg), plan->p, to_opts(opt), r);
+    std::memset(res, 0, sizeof(*res));
+    res->count = r.count;
+    for (uint32_t i = 0; i < cuts::kMaxQuery; ++i) res->level_sizes[i] = r.level[i];
+    res->device_ms = r.device_ms;
+    res->emb_written = r.emb_written;
+    res->chunks = r.chunks;
+    for (int i = 0; i < 8; ++i) {
+      res->kernel_ms[i] = r.kernel_ms[i];
+      res->kernel_launches[i] = r.kernel_launches[i];
+      res->kernel_bytes[i] = r.kernel_bytes[i];
+    }
+    return CUTS_OK;
+  });
+}
+
+int cuts_search(const char* query_file, const char* data_file, int write_to_disk, uint64_t* count) {
+  return guard([&]() -> int {
+    if (!query_file || !data_file) return fail(CUTS_E_ARG, "null argument");
+    return search_impl(query_file, data_file, write_to_disk, false, 0, count);
+  });
+}
+
+int cuts_search_dfs_bfs_strategy(const char* query_file, const char* data_file, int write_to_disk, uint32_t trunks,
+                                 uint64_t* count) {
+  return guard([&]() -> int {
+    if (!query_file || !data_file) return fail(CUTS_E_ARG, "null argument");
+    return search_impl(query_file, data_file, write_to_disk, true, trunks, count);
+  });
+}
+
+int cuts_write_matches(const char* path, const cuts_plan* plan, const uint32_t* rows, uint64_t n) {
+  return guard([&]() -> int {
+    if (!path || !plan || (n && !rows)) return fail(CUTS_E_ARG, "null argument");
+    write_matches(path, plan->p, rows, n);
+    return CUTS_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,105 @@
+// Internal interface between the C ABI layer (capi.cpp) and the HIP code
+// (match.hip, batch.hip).  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "host.hpp"
+
+namespace cuts {
+
+// Error carrying a cuts.h status code.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what);
+#define HIPCHECK(x) ::cuts::hip_check((x), #x)
+
+// Plan in the form the kernels read: per position the list of constraints
+// (earlier position | direction<<7) and the signature thresholds, plus the
+// closed-form last-level description.  Passed by value as a kernel argument.
+struct KPlan {
+  uint32_t k;
+  uint32_t sig_in[kMaxQuery], sig_out[kMaxQuery];
+  uint32_t nj[kMaxQuery];
+  uint8_t jc[kMaxQuery][2 * kMaxQuery];
+  // last position counted as |N(row[sc_pos])| minus prefix members
+  uint32_t sc_ok;      // 1 if the closed form applies
+  uint32_t sc_pos;     // position whose list is counted
+  uint32_t sc_adjmask; // positions certainly inside that list
+  uint32_t sc_thr;     // degree threshold of the last position
+};
+
+KPlan make_kplan(const Plan& p, bool symmetric_data);
+
+struct DeviceGraph {
+  int device = 0;
+  uint32_t V = 0;
+  uint64_t A = 0;
+  bool symmetric = false;
+  uint32_t max_deg = 0;
+  uint32_t *off_out = nullptr, *adj_out = nullptr, *off_in = nullptr, *adj_in = nullptr;
+  uint32_t* sig = nullptr;                 // 2V: in, out
+  std::map<uint32_t, uint32_t*> cnt_tables; // degree threshold -> #neighbours meeting it
+  ~DeviceGraph();
+};
+
+// Per-level scratch of the trie: rows plus the arrays that plan the
+// expansion of those rows.
+struct LevelBuf {
+  uint32_t* rows = nullptr;  // cap rows of (level+1) words
+  uint64_t cap = 0;
+  uint32_t* minlen = nullptr;   // cap
+  uint8_t* minsel = nullptr;    // cap
+  uint64_t* work_off = nullptr; // cap+1
+  uint64_t* bsum = nullptr;     // scan partials
+  uint64_t bsum_cap = 0;
+  uint32_t* tile_p = nullptr;   // tiles+1
+  uint64_t tile_cap = 0;
+};
+
+struct Context {
+  int device = 0;
+  int cus = 256;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  uint64_t default_budget = 0;
+  LevelBuf lv[kMaxQuery];
+  unsigned long long* counters = nullptr; // [0]=embeddings, [1]=fused level size, [2]=write count, [3]=bytes
+  hipEvent_t kev[2] = {nullptr, nullptr};
+  uint64_t* host_pinned = nullptr;        // small pinned staging
+  ~Context();
+};
+
+struct MatchOpts {
+  uint32_t trunks = 0;
+  uint32_t flags = 0;
+  uint64_t budget = 0;
+  uint32_t root_begin = 0, root_end = 0, root_stride = 0;
+  uint32_t blocks_per_cu = 0;
+  uint32_t* emb_rows = nullptr;
+  uint64_t emb_cap = 0;
+};
+struct MatchResult {
+  uint64_t count = 0;
+  uint64_t level[kMaxQuery] = {0};
+  double device_ms = 0;
+  uint64_t emb_written = 0;
+  uint32_t chunks = 0;
+  double kernel_ms[8] = {0};
+  uint64_t kernel_launches[8] = {0};
+  double kernel_bytes[8] = {0};
+};
+
+void context_init(Context& c, int device);
+void upload_graph(Context& c, const Graph& g, DeviceGraph& dg);
+void run_match(Context& c, DeviceGraph& dg, const Plan& p, const MatchOpts& o, MatchResult& r);
+
+}  // namespace cuts

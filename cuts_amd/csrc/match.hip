@@ -1,0 +1,871 @@
+// Trie-based partial-match expansion on gfx950 (MI355X).
+//
+// Semantics follow the reference's kernels: level 0 is every data vertex
+// whose signature dominates that of the first query vertex
+// (src/device_funcs.cu:267-279); level l keeps, for each prefix, the
+// candidates v that lie in out(pre[j]) for every parent position j and in
+// in(pre[j]) for every child position j, are not already used by the prefix,
+// and whose signature dominates that of the query vertex at position l
+// (src/device_funcs.cu:280-407); the last level is counted, not stored
+// (src/device_funcs.cu:363-391).
+//
+// Layout and schedule are MI355X-first rather than a translation:
+//   * a trie level is a dense row table (row = the data vertices of all
+//     positions so far), so a prefix is one contiguous read instead of the
+//     reference's parent-pointer chase (src/device_funcs.cu:39-55);
+//   * expansion work is flattened: per prefix the shortest constraint list
+//     gives an exact upper bound L_p on its children; an exclusive scan of
+//     L_p maps work items to (prefix, list index), so a hub's list is spread
+//     over many wavefronts and every wave64 lane checks one candidate;
+//   * the per-tile slice of the scan is staged in LDS and searched there;
+//   * survivors are compacted with ballot/mbcnt inside the wave, an LDS
+//     prefix over the block's waves, and one global atomic per block;
+//   * the scan total bounds the next level exactly, so a chunk of the work
+//     space is sized to the HBM budget before it runs: trunking is automatic
+//     and no table can overflow (the reference has no overflow check);
+//   * when the last position has a single constraint on a symmetric graph,
+//     its candidates are counted in closed form (|N(u)| meeting the degree
+//     threshold, minus prefix members of N(u)), and that count is fused into
+//     the expansion of the level before, so the largest level is never
+//     materialised.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+
+#include "device.hpp"
+
+namespace cuts {
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw Error(-4, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace {
+
+constexpr int BLOCK = 256;  // 4 wave64s
+constexpr int IPT = 4;      // work items per thread per tile
+constexpr int TILE = BLOCK * IPT;
+constexpr int LDS_OFF = 2048;      // staged work offsets per tile
+constexpr int SCAN_IPT = 8;
+constexpr int SCAN_TILE = BLOCK * SCAN_IPT;
+constexpr uint64_t MAX_CHUNK = 1ull << 31;  // work items per launch sequence
+
+struct GView {
+  const uint32_t *off_out, *adj_out, *off_in, *adj_in, *sig;
+  uint32_t V;
+};
+
+__device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// lower-bound membership test of x in adj[lo, hi)
+__device__ __forceinline__ bool member_safe(const uint32_t* __restrict__ adj, uint32_t lo, uint32_t hi,
+                                            uint32_t x) {
+  const uint32_t end = hi;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (adj[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo < end && adj[lo] == x;
+}
+
+// Demand bytes of one lower-bound search over `len` words: the probes plus
+// the final compare.
+__device__ __forceinline__ uint32_t bs_bytes(uint32_t len) { return 4u * ((32u - __clz(len)) + 1u); }
+
+// Wave-then-block sum of a 64-bit value; one atomic per wave.
+__device__ __forceinline__ void add_wave(unsigned long long* dst, uint64_t v) {
+  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+  // 64-bit sum through two 32-bit reductions with carry handled in 64-bit
+  uint64_t s = v;
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t l2 = __shfl_xor(lo, o), h2 = __shfl_xor(hi, o);
+    uint64_t other = ((uint64_t)h2 << 32) | l2;
+    s += other;
+    lo = (uint32_t)s;
+    hi = (uint32_t)(s >> 32);
+  }
+  if ((threadIdx.x & 63) == 0 && s) atomicAdd(dst, (unsigned long long)s);
+}
+
+// ---------------------------------------------------------------- level 0
+__global__ __launch_bounds__(BLOCK) void k_level0(const uint32_t* __restrict__ sig, uint32_t tin, uint32_t tout,
+                                                 uint32_t rb, uint32_t re, uint32_t rs, uint32_t n,
+                                                 uint32_t* __restrict__ out, uint32_t* __restrict__ cnt) {
+  __shared__ uint32_t s_w[BLOCK / 64], s_pre[BLOCK / 64], s_base;
+  const uint32_t tid = threadIdx.x, wave = tid >> 6;
+  for (uint64_t i0 = (uint64_t)blockIdx.x * BLOCK; i0 < n; i0 += (uint64_t)gridDim.x * BLOCK) {
+    const uint64_t i = i0 + tid;
+    bool ok = false;
+    uint32_t u = 0;
+    if (i < n) {
+      u = rb + (uint32_t)i * rs;
+      ok = u < re && sig[2 * (uint64_t)u] >= tin && sig[2 * (uint64_t)u + 1] >= tout;
+    }
+    const uint64_t m = __ballot(ok);
+    if ((tid & 63) == 0) s_w[wave] = __popcll(m);
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t t = 0;
+      for (int w = 0; w < BLOCK / 64; ++w) { s_pre[w] = t; t += s_w[w]; }
+      s_base = t ? atomicAdd(cnt, t) : 0;
+    }
+    __syncthreads();
+    if (ok) out[s_base + s_pre[wave] + lane_rank(m)] = u;
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------ planning
+// Per prefix: the constraint whose list is shortest (upper bound on the
+// prefix's children).  Ties keep the lowest constraint index.
+template <int L>
+__global__ __launch_bounds__(BLOCK) void k_plan(GView g, KPlan P, const uint32_t* __restrict__ rows, uint64_t n,
+                                               uint32_t* __restrict__ minlen, uint8_t* __restrict__ minsel) {
+  const uint32_t nj = P.nj[L];
+  for (uint64_t p = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (uint64_t)gridDim.x * BLOCK) {
+    uint32_t row[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) row[l] = rows[p * L + l];
+    uint32_t best = 0xFFFFFFFFu, sel = 0;
+    for (uint32_t c = 0; c < nj; ++c) {
+      const uint32_t jc = P.jc[L][c];
+      const uint32_t pos = jc & 31;
+      uint32_t u = 0;
+#pragma unroll
+      for (int l = 0; l < L; ++l)
+        if (l == (int)pos) u = row[l];
+      const uint32_t* off = (jc >> 7) ? g.off_in : g.off_out;
+      const uint32_t len = off[u + 1] - off[u];
+      if (len < best) { best = len; sel = c; }
+    }
+    minlen[p] = best;
+    minsel[p] = (uint8_t)sel;
+  }
+}
+
+// -------------------------------------------------- exclusive scan (u32->u64)
+__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v) {
+  const uint32_t lane = threadIdx.x & 63;
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t lo = __shfl_up((uint32_t)v, o), hi = __shfl_up((uint32_t)(v >> 32), o);
+    uint64_t t = ((uint64_t)hi << 32) | lo;
+    if (lane >= (uint32_t)o) v += t;
+  }
+  return v;
+}
+
+// block-wide exclusive scan of one value per thread; returns the exclusive
+// prefix, *total gets the block sum
+__device__ __forceinline__ uint64_t block_excl_scan(uint64_t v, uint64_t* s_tmp, uint64_t* total) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  uint64_t inc = wave_incl_scan(v);
+  if (lane == 63) s_tmp[wave] = inc;
+  __syncthreads();
+  uint64_t wpre = 0, tot = 0;
+  for (int w = 0; w < BLOCK / 64; ++w) {
+    if (w < (int)wave) wpre += s_tmp[w];
+    tot += s_tmp[w];
+  }
+  __syncthreads();
+  *total = tot;
+  return wpre + inc - v;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scan_reduce(const uint32_t* __restrict__ in, uint64_t n,
+                                                      uint64_t* __restrict__ bsum) {
+  __shared__ uint64_t s_tmp[BLOCK / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+  uint64_t s = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_IPT; ++i) {
+    uint64_t idx = base + (uint64_t)i * BLOCK + threadIdx.x;
+    if (idx < n) s += in[idx];
+  }
+  uint64_t tot;
+  block_excl_scan(s, s_tmp, &tot);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// single block: exclusive scan of bsum[0..nb) in place; bsum[nb] = total
+__global__ __launch_bounds__(BLOCK) void k_scan_partials(uint64_t* __restrict__ bsum, uint64_t nb) {
+  __shared__ uint64_t s_tmp[BLOCK / 64];
+  uint64_t carry = 0;
+  for (uint64_t b0 = 0; b0 < nb; b0 += BLOCK) {
+    uint64_t idx = b0 + threadIdx.x;
+    uint64_t v = idx < nb ? bsum[idx] : 0;
+    uint64_t tot;
+    uint64_t ex = block_excl_scan(v, s_tmp, &tot);
+    if (idx < nb) bsum[idx] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) bsum[nb] = carry;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scan_down(const uint32_t* __restrict__ in, uint64_t n,
+                                                    const uint64_t* __restrict__ bsum, uint64_t nb,
+                                                    uint64_t* __restrict__ out) {
+  __shared__ uint64_t s_tmp[BLOCK / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+  // each thread owns SCAN_IPT consecutive items
+  uint32_t v[SCAN_IPT];
+  uint64_t s = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_IPT; ++i) {
+    uint64_t idx = base + (uint64_t)threadIdx.x * SCAN_IPT + i;
+    v[i] = idx < n ? in[idx] : 0;
+    s += v[i];
+  }
+  uint64_t tot;
+  uint64_t ex = block_excl_scan(s, s_tmp, &tot) + bsum[blockIdx.x];
+#pragma unroll
+  for (int i = 0; i < SCAN_IPT; ++i) {
+    uint64_t idx = base + (uint64_t)threadIdx.x * SCAN_IPT + i;
+    if (idx < n) out[idx] = ex;
+    ex += v[i];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = bsum[nb];
+}
+
+// tile_p[t] = prefix owning work item w_begin + t*TILE (t < ntiles),
+// tile_p[ntiles] = prefix owning item w_end-1
+__global__ __launch_bounds__(BLOCK) void k_tile_index(const uint64_t* __restrict__ off, uint64_t n, uint64_t w_begin,
+                                                     uint64_t w_end, uint64_t ntiles, uint32_t* __restrict__ tile_p) {
+  for (uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; t <= ntiles; t += (uint64_t)gridDim.x * BLOCK) {
+    const uint64_t w = t < ntiles ? w_begin + t * TILE : w_end - 1;
+    uint64_t lo = 0, hi = n + 1;  // upper_bound over off[0..n]
+    while (lo < hi) {
+      uint64_t mid = (lo + hi) >> 1;
+      if (off[mid] <= w) lo = mid + 1;
+      else hi = mid;
+    }
+    tile_p[t] = (uint32_t)(lo - 1);
+  }
+}
+
+// ------------------------------------------------ closed-form last level
+// Number of candidates of the last position given all earlier positions
+// r[0..N): |{x in N(u) : deg(x) >= thr}| - |{i : r[i] in N(u), deg(r[i]) >= thr}|
+// with u = r[sc_pos] (symmetric graphs only).
+template <int N>
+__device__ __forceinline__ uint64_t closed_form(const GView& g, const KPlan& P, const uint32_t (&r)[N],
+                                                const uint32_t* __restrict__ tbl, uint64_t& bytes) {
+  uint32_t u = 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+    if (i == (int)P.sc_pos) u = r[i];
+  const uint32_t lo = g.off_out[u], hi = g.off_out[u + 1];
+  int64_t c = tbl ? (int64_t)tbl[u] : (int64_t)(hi - lo);
+  const bool need_deg = P.sc_thr > 1;
+  bytes += 8 + (tbl ? 4 : 0);
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const uint32_t x = r[i];
+    bool in = (P.sc_adjmask >> i) & 1;
+    if (!in) {
+      in = member_safe(g.adj_out, lo, hi, x);
+      bytes += bs_bytes(hi - lo);
+    }
+    if (in && need_deg) bytes += 4;
+    if (in && (!need_deg || g.sig[2 * (uint64_t)x] >= P.sc_thr)) --c;
+  }
+  return (uint64_t)c;
+}
+
+template <int L>
+__global__ __launch_bounds__(BLOCK) void k_closed_form(GView g, KPlan P, const uint32_t* __restrict__ rows, uint64_t n,
+                                                      const uint32_t* __restrict__ tbl,
+                                                      unsigned long long* __restrict__ counters, int stats) {
+  uint64_t acc = 0, bytes = 0;
+  for (uint64_t p = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (uint64_t)gridDim.x * BLOCK) {
+    uint32_t r[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) r[l] = rows[p * L + l];
+    bytes += 4 * L;
+    acc += closed_form<L>(g, P, r, tbl, bytes);
+  }
+  add_wave(&counters[0], acc);
+  if (stats) add_wave(&counters[3], bytes);
+}
+
+// Degree-threshold table: tbl[u] = |{x in N(u) : deg(x) >= thr}|, one wave per vertex.
+__global__ __launch_bounds__(BLOCK) void k_cnt_table(GView g, uint32_t thr, uint32_t* __restrict__ tbl) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t nw = (uint64_t)gridDim.x * (BLOCK / 64);
+  for (uint64_t u = (uint64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6); u < g.V; u += nw) {
+    const uint32_t lo = g.off_out[u], hi = g.off_out[u + 1];
+    uint32_t c = 0;
+    for (uint32_t i = lo + lane; i < hi; i += 64) c += g.sig[2 * (uint64_t)g.adj_out[i]] >= thr;
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if (lane == 0) tbl[u] = c;
+  }
+}
+
+// ------------------------------------------------------------ expansion
+enum { M_WRITE = 0, M_COUNT = 1, M_FUSED = 2 };
+
+struct ExpandArgs {
+  const uint32_t* rows;      // n rows of L words
+  const uint64_t* work_off;  // n+1
+  const uint8_t* minsel;
+  const uint32_t* tile_p;    // ntiles+1
+  uint64_t w_begin, w_end, ntiles;
+  uint32_t* out_rows;        // M_WRITE: rows of L+1 words
+  uint32_t* out_count;
+  unsigned long long* counters;
+  const uint32_t* tbl;       // M_FUSED closed-form table (null: plain degree)
+  uint32_t xcd;              // 1: XCD-contiguous tile mapping
+  uint32_t stats;            // 1: accumulate demand bytes into counters[3]
+};
+
+template <int L, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a) {
+  __shared__ uint64_t s_off[LDS_OFF + 1];
+  __shared__ uint32_t s_w[BLOCK / 64], s_pre[BLOCK / 64], s_base;
+  const uint32_t tid = threadIdx.x, wave = tid >> 6;
+  const uint32_t nj = P.nj[L];
+  const uint32_t tin = P.sig_in[L], tout = P.sig_out[L];
+  uint64_t acc = 0, valid = 0, bytes = 0;
+  // XCD-aware mapping: blocks are dispatched round-robin over 8 XCDs; give
+  // each XCD a contiguous run of tiles so neighbouring prefixes share an L2.
+  const uint32_t G = gridDim.x;
+  uint32_t vb = blockIdx.x;
+  if (a.xcd && (G & 7) == 0) vb = (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
+  for (uint64_t tile = vb; tile < a.ntiles; tile += G) {
+    const uint64_t w0 = a.w_begin + tile * TILE;
+    const uint64_t w1 = min(w0 + (uint64_t)TILE, a.w_end);
+    const uint32_t plo = a.tile_p[tile], phi = a.tile_p[tile + 1];
+    const uint32_t span = phi - plo + 1;  // offsets plo .. phi+1
+    const bool lds = span <= LDS_OFF;
+    if (lds)
+      for (uint32_t i = tid; i <= span; i += BLOCK) s_off[i] = a.work_off[plo + i];
+    __syncthreads();
+    for (int it = 0; it < IPT; ++it) {
+      const uint64_t w = w0 + (uint64_t)it * BLOCK + tid;
+      bool ok = false;
+      uint32_t v = 0;
+      uint32_t row[L];
+#pragma unroll
+      for (int l = 0; l < L; ++l) row[l] = 0;
+      if (w < w1) {
+        uint32_t lo = 0, hi = span + 1;
+        if (lds) {
+          while (lo < hi) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (s_off[mid] <= w) lo = mid + 1;
+            else hi = mid;
+          }
+        } else {
+          while (lo < hi) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (a.work_off[plo + mid] <= w) lo = mid + 1;
+            else hi = mid;
+          }
+        }
+        const uint32_t pi = lo - 1;
+        const uint64_t p = (uint64_t)plo + pi;
+        const uint64_t pbase = lds ? s_off[pi] : a.work_off[p];
+        const uint32_t idx = (uint32_t)(w - pbase);
+#pragma unroll
+        for (int l = 0; l < L; ++l) row[l] = a.rows[p * L + l];
+        const uint32_t sel = a.minsel[p];
+        const uint32_t jc = P.jc[L][sel];
+        uint32_t u = 0;
+#pragma unroll
+        for (int l = 0; l < L; ++l)
+          if (l == (int)(jc & 31)) u = row[l];
+        const uint32_t* off = (jc >> 7) ? g.off_in : g.off_out;
+        const uint32_t* adj = (jc >> 7) ? g.adj_in : g.adj_out;
+        v = adj[off[u] + idx];
+        // demand bytes: the candidate word and its signature; once per
+        // prefix its row, selector, scan entry and constraint offsets
+        bytes += 12;
+        if (idx == 0) bytes += 4 * L + 9 + 8 * nj;
+        ok = g.sig[2 * (uint64_t)v] >= tin && g.sig[2 * (uint64_t)v + 1] >= tout;
+#pragma unroll
+        for (int l = 0; l < L; ++l) ok = ok && v != row[l];
+        for (uint32_t c = 0; c < nj && ok; ++c) {
+          if (c == sel) continue;
+          const uint32_t jc2 = P.jc[L][c];
+          uint32_t u2 = 0;
+#pragma unroll
+          for (int l = 0; l < L; ++l)
+            if (l == (int)(jc2 & 31)) u2 = row[l];
+          const uint32_t* off2 = (jc2 >> 7) ? g.off_in : g.off_out;
+          const uint32_t* adj2 = (jc2 >> 7) ? g.adj_in : g.adj_out;
+          const uint32_t lo2 = off2[u2], hi2 = off2[u2 + 1];
+          ok = member_safe(adj2, lo2, hi2, v);
+          bytes += bs_bytes(hi2 - lo2);
+        }
+      }
+      if (MODE == M_WRITE) {
+        const uint64_t m = __ballot(ok);
+        if ((tid & 63) == 0) s_w[wave] = __popcll(m);
+        __syncthreads();
+        if (tid == 0) {
+          uint32_t t = 0;
+          for (int ww = 0; ww < BLOCK / 64; ++ww) { s_pre[ww] = t; t += s_w[ww]; }
+          s_base = t ? atomicAdd(a.out_count, t) : 0;
+        }
+        __syncthreads();
+        if (ok) {
+          bytes += 4 * (L + 1);
+          uint32_t* dst = a.out_rows + (uint64_t)(s_base + s_pre[wave] + lane_rank(m)) * (L + 1);
+#pragma unroll
+          for (int l = 0; l < L; ++l) dst[l] = row[l];
+          dst[L] = v;
+        }
+      } else if (MODE == M_COUNT) {
+        acc += ok;
+      } else {
+        if (ok) {
+          uint32_t r[L + 1];
+#pragma unroll
+          for (int l = 0; l < L; ++l) r[l] = row[l];
+          r[L] = v;
+          ++valid;
+          acc += closed_form<L + 1>(g, P, r, a.tbl, bytes);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (MODE != M_WRITE) {
+    add_wave(&a.counters[0], acc);
+    if (MODE == M_FUSED) add_wave(&a.counters[1], valid);
+  }
+  if (a.stats) add_wave(&a.counters[3], bytes);
+}
+
+// ---------------------------------------------------------- dispatch
+#define CUTS_DISPATCH_L(L, CALL) \
+  switch (L) {                   \
+    case 1: CALL(1); break;      \
+    case 2: CALL(2); break;      \
+    case 3: CALL(3); break;      \
+    case 4: CALL(4); break;      \
+    case 5: CALL(5); break;      \
+    case 6: CALL(6); break;      \
+    case 7: CALL(7); break;      \
+    case 8: CALL(8); break;      \
+    case 9: CALL(9); break;      \
+    case 10: CALL(10); break;    \
+    case 11: CALL(11); break;    \
+    case 12: CALL(12); break;    \
+    case 13: CALL(13); break;    \
+    case 14: CALL(14); break;    \
+    case 15: CALL(15); break;    \
+    default: throw Error(-3, "query position out of range"); \
+  }
+
+uint32_t grid_for(uint64_t work, uint64_t per_block, uint32_t cap) {
+  uint64_t b = (work + per_block - 1) / per_block;
+  if (b < 1) b = 1;
+  return (uint32_t)std::min<uint64_t>(b, cap);
+}
+
+template <typename T>
+void dev_alloc(T*& p, uint64_t n) {
+  if (p) HIPCHECK(hipFree(p));
+  p = nullptr;
+  hipError_t e = hipMalloc(&p, std::max<uint64_t>(n, 1) * sizeof(T));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw Error(-5, "device allocation of " + std::to_string(n * sizeof(T)) + " bytes failed");
+  }
+}
+
+// Makes sure level buffer `lv` can hold `rows` rows of `words` words plus
+// the planning arrays for them.
+void ensure_rows(LevelBuf& b, uint64_t rows, uint32_t words) {
+  if (b.cap >= rows && b.rows) return;
+  uint64_t cap = std::max<uint64_t>(rows, 1024);
+  if (b.cap) cap = std::max<uint64_t>(cap, b.cap + b.cap / 2);
+  dev_alloc(b.rows, cap * words);
+  dev_alloc(b.minlen, cap);
+  dev_alloc(b.minsel, cap);
+  dev_alloc(b.work_off, cap + 1);
+  b.cap = cap;
+}
+void ensure_rows_exact(LevelBuf& b, uint64_t rows, uint32_t words) {
+  // like ensure_rows but never grows beyond `rows` (budget-bound levels)
+  if (b.cap >= rows && b.rows) return;
+  dev_alloc(b.rows, std::max<uint64_t>(rows, 1) * words);
+  dev_alloc(b.minlen, std::max<uint64_t>(rows, 1));
+  dev_alloc(b.minsel, std::max<uint64_t>(rows, 1));
+  dev_alloc(b.work_off, std::max<uint64_t>(rows, 1) + 1);
+  b.cap = rows;
+}
+void ensure_bsum(LevelBuf& b, uint64_t nb) {
+  if (b.bsum_cap >= nb + 1 && b.bsum) return;
+  dev_alloc(b.bsum, nb + 1);
+  b.bsum_cap = nb + 1;
+}
+void ensure_tiles(LevelBuf& b, uint64_t nt) {
+  if (b.tile_cap >= nt + 1 && b.tile_p) return;
+  dev_alloc(b.tile_p, nt + 1);
+  b.tile_cap = nt + 1;
+}
+
+struct Matcher {
+  Context& c;
+  DeviceGraph& dg;
+  const Plan& plan;
+  const MatchOpts& o;
+  MatchResult& r;
+  KPlan kp;
+  GView g;
+  uint32_t k;
+  bool closed_last = false, fuse = false, write = false;
+  const uint32_t* tbl = nullptr;
+  uint64_t budget = 0;
+  uint64_t level_cap[kMaxQuery] = {0};
+  uint32_t grid_cap = 0;
+
+  Matcher(Context& c_, DeviceGraph& dg_, const Plan& p_, const MatchOpts& o_, MatchResult& r_)
+      : c(c_), dg(dg_), plan(p_), o(o_), r(r_) {}
+
+  bool prof = false;
+  void pbegin(bool dev_bytes) {
+    if (!prof) return;
+    if (dev_bytes) HIPCHECK(hipMemsetAsync(c.counters + 3, 0, 8, c.stream));
+    HIPCHECK(hipEventRecord(c.kev[0], c.stream));
+  }
+  void pend(int cls, double host_bytes, bool dev_bytes) {
+    if (!prof) return;
+    HIPCHECK(hipEventRecord(c.kev[1], c.stream));
+    HIPCHECK(hipEventSynchronize(c.kev[1]));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, c.kev[0], c.kev[1]));
+    r.kernel_ms[cls] += ms;
+    r.kernel_launches[cls]++;
+    r.kernel_bytes[cls] += host_bytes + (dev_bytes ? (double)read_u64(c.counters + 3) : 0.0);
+  }
+
+  uint64_t read_u64(const void* dev) {
+    HIPCHECK(hipMemcpyAsync(c.host_pinned, dev, 8, hipMemcpyDeviceToHost, c.stream));
+    HIPCHECK(hipStreamSynchronize(c.stream));
+    return c.host_pinned[0];
+  }
+  uint32_t read_u32(const void* dev) {
+    HIPCHECK(hipMemcpyAsync(c.host_pinned, dev, 4, hipMemcpyDeviceToHost, c.stream));
+    HIPCHECK(hipStreamSynchronize(c.stream));
+    return (uint32_t)(c.host_pinned[0] & 0xFFFFFFFFu);
+  }
+
+  // exclusive scan of b.minlen[0..n) into b.work_off[0..n], returns total
+  uint64_t scan(LevelBuf& b, uint64_t n) {
+    const uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    ensure_bsum(b, nb);
+    pbegin(false);
+    hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)nb), dim3(BLOCK), 0, c.stream, b.minlen, n, b.bsum);
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(BLOCK), 0, c.stream, b.bsum, nb);
+    hipLaunchKernelGGL(k_scan_down, dim3((uint32_t)nb), dim3(BLOCK), 0, c.stream, b.minlen, n, b.bsum, nb,
+                       b.work_off);
+    pend(2, 16.0 * n, false);
+    return read_u64(b.work_off + n);
+  }
+
+  // rows: level L-1 table (n rows of L words) living in lv[L-1].rows (or an
+  // offset into it); produces position L.
+  void level(uint32_t L, const uint32_t* rows, uint64_t n) {
+    if (n == 0) return;
+    LevelBuf& in = c.lv[L - 1];
+    if (L == k - 1 && closed_last) {
+      const uint32_t grid = grid_for(n, BLOCK, grid_cap);
+      pbegin(true);
+#define CF(LL) hipLaunchKernelGGL((k_closed_form<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, tbl, c.counters, (int)prof)
+      CUTS_DISPATCH_L(L, CF)
+#undef CF
+      HIPCHECK(hipGetLastError());
+      pend(7, 0, true);
+      return;
+    }
+    // plan + scan over the input rows; arrays live with the input level
+    {
+      const uint32_t grid = grid_for(n, BLOCK, grid_cap);
+      pbegin(false);
+#define PL(LL) hipLaunchKernelGGL((k_plan<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.minlen, in.minsel)
+      CUTS_DISPATCH_L(L, PL)
+#undef PL
+      HIPCHECK(hipGetLastError());
+      pend(1, (double)n * (4.0 * L + 8.0 * kp.nj[L] + 5.0), false);
+    }
+    const uint64_t W = scan(in, n);
+    if (W == 0) return;
+    int mode = M_WRITE;
+    if (L == k - 1 && !write) mode = M_COUNT;
+    else if (L == k - 2 && fuse) mode = M_FUSED;
+    uint64_t chunk = MAX_CHUNK;
+    LevelBuf& out = c.lv[L];
+    if (mode == M_WRITE) {
+      const uint64_t cap = level_cap[L];
+      chunk = std::min<uint64_t>(chunk, std::max<uint64_t>(cap, 1));
+      ensure_rows_exact(out, std::min<uint64_t>(chunk, W), L + 1);
+      chunk = std::min<uint64_t>(chunk, out.cap);
+    }
+    for (uint64_t w0 = 0; w0 < W; w0 += chunk) {
+      const uint64_t w1 = std::min(W, w0 + chunk);
+      const uint64_t nt = (w1 - w0 + TILE - 1) / TILE;
+      ensure_tiles(in, nt);
+      pbegin(false);
+      hipLaunchKernelGGL(k_tile_index, dim3(grid_for(nt + 1, BLOCK, 65535)), dim3(BLOCK), 0, c.stream, in.work_off,
+                         n, w0, w1, nt, in.tile_p);
+      pend(3, (double)(nt + 1) * (8.0 * (64 - __builtin_clzll(n + 2)) + 4.0), false);
+      ExpandArgs a{};
+      a.rows = rows;
+      a.work_off = in.work_off;
+      a.minsel = in.minsel;
+      a.tile_p = in.tile_p;
+      a.w_begin = w0;
+      a.w_end = w1;
+      a.ntiles = nt;
+      a.counters = c.counters;
+      a.tbl = tbl;
+      a.xcd = (o.flags & 8u) ? 0 : 1;
+      a.stats = prof ? 1 : 0;
+      uint32_t grid = grid_for(nt, 1, grid_cap);
+      if (grid > 8) grid &= ~7u;
+      r.chunks++;
+      if (mode == M_WRITE) {
+        a.out_rows = out.rows;
+        a.out_count = (uint32_t*)(c.counters + 2);
+        HIPCHECK(hipMemsetAsync(c.counters + 2, 0, 8, c.stream));
+        pbegin(true);
+#define EW(LL) hipLaunchKernelGGL((k_expand<LL, M_WRITE>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, a)
+        CUTS_DISPATCH_L(L, EW)
+#undef EW
+        HIPCHECK(hipGetLastError());
+        pend(4, 0, true);
+        const uint64_t m = read_u32(c.counters + 2);
+        r.level[L] += m;
+        if (L == k - 1) {
+          // embeddings: copy rows to the host sink
+          r.count += m;
+          const uint64_t room = o.emb_cap > r.emb_written ? o.emb_cap - r.emb_written : 0;
+          const uint64_t take = std::min(room, m);
+          if (take && o.emb_rows) {
+            HIPCHECK(hipMemcpyAsync(o.emb_rows + r.emb_written * k, out.rows, take * k * 4, hipMemcpyDeviceToHost,
+                                    c.stream));
+            HIPCHECK(hipStreamSynchronize(c.stream));
+            r.emb_written += take;
+          }
+        } else {
+          level(L + 1, out.rows, m);
+        }
+      } else if (mode == M_COUNT) {
+        pbegin(true);
+#define EC(LL) hipLaunchKernelGGL((k_expand<LL, M_COUNT>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, a)
+        CUTS_DISPATCH_L(L, EC)
+#undef EC
+        HIPCHECK(hipGetLastError());
+        pend(5, 0, true);
+      } else {
+        pbegin(true);
+#define EF(LL) hipLaunchKernelGGL((k_expand<LL, M_FUSED>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, a)
+        CUTS_DISPATCH_L(L, EF)
+#undef EF
+        HIPCHECK(hipGetLastError());
+        pend(6, 0, true);
+      }
+    }
+  }
+
+  void run() {
+    k = plan.k;
+    if (k == 0 || k > kMaxQuery) throw Error(-3, "query must have between 1 and 16 vertices");
+    kp = make_kplan(plan, dg.symmetric);
+    g = GView{dg.off_out, dg.adj_out, dg.off_in, dg.adj_in, dg.sig, dg.V};
+    write = (o.flags & 4u) != 0;
+    prof = (o.flags & 16u) != 0;
+    closed_last = !write && !(o.flags & 1u) && kp.sc_ok && k >= 2;
+    fuse = closed_last && !(o.flags & 2u) && k >= 3;
+    grid_cap = (uint32_t)c.cus * (o.blocks_per_cu ? o.blocks_per_cu : 8);
+    budget = o.budget ? o.budget : c.default_budget;
+    // split the budget evenly over the materialised levels 1..k-2 (or k-1
+    // when writing embeddings); each row of level l costs (l+1)*4 bytes plus
+    // 13 bytes of planning arrays.
+    uint32_t nlev = 0;
+    for (uint32_t l = 1; l < k; ++l)
+      if (l < k - 1 || write) ++nlev;
+    for (uint32_t l = 1; l < k; ++l) {
+      const uint64_t per = nlev ? budget / nlev : budget;
+      level_cap[l] = std::max<uint64_t>(per / ((l + 1) * 4 + 13), 1024);
+    }
+    if (closed_last && kp.sc_thr > 1) {
+      auto it = dg.cnt_tables.find(kp.sc_thr);
+      if (it == dg.cnt_tables.end()) {
+        uint32_t* t = nullptr;
+        dev_alloc(t, dg.V + 1);
+        hipLaunchKernelGGL(k_cnt_table, dim3(grid_for(dg.V, BLOCK / 64, 65535)), dim3(BLOCK), 0, c.stream, g,
+                           kp.sc_thr, t);
+        HIPCHECK(hipGetLastError());
+        it = dg.cnt_tables.emplace(kp.sc_thr, t).first;
+      }
+      tbl = it->second;
+    }
+    // level 0
+    const uint32_t rb = o.root_stride ? o.root_begin : 0;
+    const uint32_t re = o.root_stride ? std::min(o.root_end, dg.V) : dg.V;
+    const uint32_t rs = o.root_stride ? o.root_stride : 1;
+    const uint32_t n0c = re > rb ? (re - rb + rs - 1) / rs : 0;
+    ensure_rows(c.lv[0], std::max<uint32_t>(n0c, 1), 1);
+    HIPCHECK(hipMemsetAsync(c.counters, 0, 4 * 8, c.stream));
+    HIPCHECK(hipEventRecord(c.ev0, c.stream));
+    pbegin(false);
+    hipLaunchKernelGGL(k_level0, dim3(grid_for(n0c, BLOCK, 65535)), dim3(BLOCK), 0, c.stream, dg.sig,
+                       kp.sig_in[0], kp.sig_out[0], rb, re, rs, n0c, c.lv[0].rows, (uint32_t*)(c.counters + 2));
+    HIPCHECK(hipGetLastError());
+    pend(0, 8.0 * n0c, false);
+    const uint64_t n0 = read_u32(c.counters + 2);
+    r.level[0] = n0;
+    if (prof) r.kernel_bytes[0] += 4.0 * n0;
+    if (k == 1) {
+      r.count = n0;
+    } else if (n0) {
+      uint32_t trunks = o.trunks ? o.trunks : 1;
+      if (trunks > n0) trunks = (uint32_t)n0;
+      const uint64_t ts = (n0 + trunks - 1) / trunks;
+      for (uint64_t s = 0; s < n0; s += ts) level(1, c.lv[0].rows + s, std::min(ts, n0 - s));
+      if (!write) {
+        HIPCHECK(hipMemcpyAsync(c.host_pinned, c.counters, 16, hipMemcpyDeviceToHost, c.stream));
+        HIPCHECK(hipStreamSynchronize(c.stream));
+        r.count = c.host_pinned[0];
+        r.level[k - 1] = r.count;
+        if (fuse) r.level[k - 2] = c.host_pinned[1];
+      }
+    }
+    HIPCHECK(hipEventRecord(c.ev1, c.stream));
+    HIPCHECK(hipEventSynchronize(c.ev1));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    r.device_ms = ms;
+  }
+};
+
+}  // namespace
+
+KPlan make_kplan(const Plan& p, bool symmetric) {
+  KPlan kp;
+  std::memset(&kp, 0, sizeof(kp));
+  kp.k = p.k;
+  for (uint32_t i = 0; i < p.k; ++i) {
+    kp.sig_in[i] = p.sig_in[i];
+    kp.sig_out[i] = p.sig_out[i];
+    uint32_t n = 0;
+    if (symmetric) {
+      const uint32_t m = p.pmask[i] | p.cmask[i];
+      for (uint32_t j = 0; j < i; ++j)
+        if (m >> j & 1) kp.jc[i][n++] = (uint8_t)j;
+    } else {
+      for (uint32_t j = 0; j < i; ++j)
+        if (p.pmask[i] >> j & 1) kp.jc[i][n++] = (uint8_t)j;
+      for (uint32_t j = 0; j < i; ++j)
+        if (p.cmask[i] >> j & 1) kp.jc[i][n++] = (uint8_t)(j | 0x80);
+    }
+    kp.nj[i] = n;
+  }
+  const uint32_t last = p.k - 1;
+  if (symmetric && p.k >= 2 && kp.nj[last] == 1) {
+    kp.sc_ok = 1;
+    kp.sc_pos = kp.jc[last][0] & 31;
+    kp.sc_thr = std::max(p.sig_in[last], p.sig_out[last]);
+    uint32_t adj = 0;
+    for (uint32_t i = 0; i < last; ++i) {
+      if (i == kp.sc_pos) continue;
+      const uint32_t a = std::max(i, kp.sc_pos), b = std::min(i, kp.sc_pos);
+      if (((p.pmask[a] | p.cmask[a]) >> b) & 1) adj |= 1u << i;
+    }
+    kp.sc_adjmask = adj;
+  }
+  return kp;
+}
+
+DeviceGraph::~DeviceGraph() {
+  int prev = 0;
+  (void)hipGetDevice(&prev);
+  (void)hipSetDevice(device);
+  for (uint32_t* p : {off_out, adj_out, sig}) (void)hipFree(p);
+  if (!symmetric) {
+    (void)hipFree(off_in);
+    (void)hipFree(adj_in);
+  }
+  for (auto& kv : cnt_tables) (void)hipFree(kv.second);
+  (void)hipSetDevice(prev);
+}
+
+Context::~Context() {
+  (void)hipSetDevice(device);
+  for (auto& b : lv) {
+    for (void* p : {(void*)b.rows, (void*)b.minlen, (void*)b.minsel, (void*)b.work_off, (void*)b.bsum,
+                    (void*)b.tile_p})
+      if (p) (void)hipFree(p);
+  }
+  if (counters) (void)hipFree(counters);
+  if (host_pinned) (void)hipHostFree(host_pinned);
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  for (hipEvent_t e : kev)
+    if (e) (void)hipEventDestroy(e);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+void context_init(Context& c, int device) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw Error(-6, "no HIP device available");
+  if (device < 0 || device >= n) throw Error(-1, "device index out of range");
+  c.device = device;
+  HIPCHECK(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIPCHECK(hipGetDeviceProperties(&prop, device));
+  c.cus = prop.multiProcessorCount;
+  HIPCHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  HIPCHECK(hipEventCreate(&c.ev0));
+  HIPCHECK(hipEventCreate(&c.ev1));
+  HIPCHECK(hipEventCreate(&c.kev[0]));
+  HIPCHECK(hipEventCreate(&c.kev[1]));
+  HIPCHECK(hipMalloc(&c.counters, 8 * sizeof(unsigned long long)));
+  HIPCHECK(hipHostMalloc(&c.host_pinned, 64, hipHostMallocDefault));
+  size_t fr = 0, tot = 0;
+  HIPCHECK(hipMemGetInfo(&fr, &tot));
+  // trie tables may take 40% of the free HBM by default
+  c.default_budget = (uint64_t)(fr * 0.4);
+}
+
+void upload_graph(Context& c, const Graph& g, DeviceGraph& dg) {
+  HIPCHECK(hipSetDevice(c.device));
+  dg.device = c.device;
+  dg.V = g.V;
+  dg.A = g.arcs();
+  dg.symmetric = g.symmetric;
+  uint32_t md = 0;
+  for (uint32_t v = 0; v < g.V; ++v) md = std::max(md, g.out_off[v + 1] - g.out_off[v]);
+  dg.max_deg = md;
+  auto up = [&](uint32_t*& d, const std::vector<uint32_t>& h) {
+    dev_alloc(d, h.size());
+    HIPCHECK(hipMemcpy(d, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+  };
+  up(dg.off_out, g.out_off);
+  up(dg.adj_out, g.out_adj);
+  up(dg.sig, g.sig);
+  if (g.symmetric) {
+    dg.off_in = dg.off_out;
+    dg.adj_in = dg.adj_out;
+  } else {
+    up(dg.off_in, g.in_off);
+    up(dg.adj_in, g.in_adj);
+  }
+}
+
+void run_match(Context& c, DeviceGraph& dg, const Plan& p, const MatchOpts& o, MatchResult& r) {
+  HIPCHECK(hipSetDevice(c.device));
+  r = MatchResult();
+  Matcher m(c, dg, p, o, r);
+  m.run();
+}
+
+}  // namespace cuts

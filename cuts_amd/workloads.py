@@ -1,0 +1,99 @@
+"""The synthetic workloads the matcher is measured on.
+
+Data graphs are rebuilt deterministically from seeds by the native
+generators; query shapes are fixed, and each is located once in its data
+graph by a seeded random-walk search (the witness proves count >= 1).
+
+Query vertex numbering follows the walk: shape vertices are numbered in the
+order the walk reaches them.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from functools import lru_cache
+from typing import List, Tuple
+
+import numpy as np
+
+from . import Graph
+
+Edge = Tuple[int, int]
+
+
+@dataclass(frozen=True)
+class Shape:
+    name: str
+    k: int
+    edges: Tuple[Edge, ...]
+
+
+# triangle 0-1-2 plus a 2-edge tail 2-3-4 (5 vertices, 5 edges)
+TRI_TAIL2 = Shape("triangle+2-edge-tail", 5, ((0, 1), (1, 2), (2, 0), (2, 3), (3, 4)))
+# 4-cycle 0-1-2-3 with pendants 4 (on 0) and 5 (on 2)
+C4_2PEND = Shape("4-cycle+2-pendants", 6, ((0, 1), (1, 2), (2, 3), (3, 0), (0, 4), (2, 5)))
+# K4 minus edge 0-3 (diamond) with a 2-edge tail 3-4-5
+K4E_TAIL2 = Shape("K4-e+2-tail", 6, ((0, 1), (0, 2), (1, 2), (1, 3), (2, 3), (3, 4), (4, 5)))
+# 5-cycle 0-1-2-3-4 with chord 0-2, pendants 5 (on 3) and 6 (on 4)
+C5CHORD_2TAIL = Shape("5-cycle+chord+2-tails", 7,
+                      ((0, 1), (1, 2), (2, 3), (3, 4), (4, 0), (0, 2), (3, 5), (4, 6)))
+
+
+def shape_arcs(s: Shape) -> List[Edge]:
+    out = []
+    for a, b in s.edges:
+        out += [(a, b), (b, a)]
+    return out
+
+
+def query_graph(s: Shape) -> Graph:
+    return Graph.from_edges(s.k, list(s.edges))
+
+
+@lru_cache(maxsize=None)
+def config1_graph() -> Graph:
+    """Chung-Lu, n=4,096, m=32,768, exponent 2.3, seed 0."""
+    return Graph.chung_lu(4096, 32768, 2.3, 0, 0)
+
+
+@lru_cache(maxsize=None)
+def config2_graph() -> Graph:
+    """Chung-Lu, n=36,692, m=183,831, exponent 2.1, degree capped at 1,400, seed 0."""
+    return Graph.chung_lu(36692, 183831, 2.1, 1400, 0)
+
+
+@lru_cache(maxsize=None)
+def config3_graph() -> Graph:
+    """R-MAT scale 20, edge factor 16, (a,b,c)=(0.57,0.19,0.19), seed 0,
+    deduplicated, loops removed, symmetrised."""
+    return Graph.rmat(20, 16, 0.57, 0.19, 0.19, 0)
+
+
+@lru_cache(maxsize=None)
+def rmat18_graph(seed: int) -> Graph:
+    return Graph.rmat(18, 16, 0.57, 0.19, 0.19, seed)
+
+
+CONFIGS = {
+    1: ("chung-lu-4k", config1_graph, [TRI_TAIL2]),
+    2: ("chung-lu-36.7k", config2_graph, [C4_2PEND, K4E_TAIL2]),
+    3: ("rmat-20", config3_graph, [C5CHORD_2TAIL]),
+}
+
+
+def witness(cfg: int, shape: Shape, seed: int = 0) -> np.ndarray:
+    g = CONFIGS[cfg][1]()
+    return g.find_pattern(shape.k, list(shape.edges), seed)
+
+
+def csr_fingerprint(g: Graph) -> str:
+    """FNV-1a-style 64-bit fold over (a deterministic sample of at most 64Ki)
+    8-byte words of out_off || out_adj, plus the byte length."""
+    v = g.view()
+    h = 0xCBF29CE484222325
+    data = np.concatenate([v["out_off"], v["out_adj"]]).astype("<u4").tobytes()
+    # vectorised FNV is awkward; fold 8-byte words instead of bytes
+    words = np.frombuffer(data + b"\0" * (-len(data) % 8), dtype="<u8")
+    for w in words[:: max(1, len(words) // 65536)]:  # bounded sample for large graphs
+        h ^= int(w)
+        h = (h * 0x100000001B3) & 0xFFFFFFFFFFFFFFFF
+    return f"{h:016x}:{len(data)}"

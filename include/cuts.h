@@ -1,0 +1,156 @@
+/*
+ * cuts_amd -- C ABI of the MI355X trie-based subgraph matcher.
+ *
+ * The reference (cuTS) exposes its path through three de-facto boundaries:
+ * the `cuts <data_graph> <query_graph> [trunks]` process (main.cu:24-46),
+ * the host functions `search` / `search_dfs_bfs_strategy`
+ * (inc/host_funcs.h:21-25) and the kernels `initialize_searching` /
+ * `search_kernel` (inc/device_funcs.h:54-58).  This header replaces the
+ * second and third: plain pointers and sizes, status codes instead of
+ * exit(-1), no C++ or torch types.  Every entry point returns CUTS_OK (0) or
+ * a negative status; cuts_last_error() gives the message of the last
+ * failure on the calling thread.
+ */
+#ifndef CUTS_AMD_H
+#define CUTS_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  CUTS_OK = 0,
+  CUTS_E_ARG = -1,     /* invalid argument */
+  CUTS_E_IO = -2,      /* file could not be read / parsed / written */
+  CUTS_E_QUERY = -3,   /* query unsupported: k > 16, not weakly connected */
+  CUTS_E_HIP = -4,     /* HIP runtime error */
+  CUTS_E_NOMEM = -5,   /* device or host allocation failed */
+  CUTS_E_NODEV = -6    /* no usable GPU */
+};
+
+typedef struct cuts_graph cuts_graph;               /* host CSR graph */
+typedef struct cuts_plan cuts_plan;                 /* query + matching order */
+typedef struct cuts_context cuts_context;           /* one GPU: stream + trie arena */
+typedef struct cuts_device_graph cuts_device_graph; /* data graph resident in HBM */
+
+const char *cuts_last_error(void);
+const char *cuts_version(void);
+
+/* ---- host graphs ------------------------------------------------------ */
+/* Edge-list reader; replaces file_reader + Graph::Graph (src/util.cpp:19-52,
+ * src/graph.cpp:18-61). Same format: first line V, then "s<TAB>t" arcs. */
+int cuts_graph_load(const char *path, cuts_graph **out);
+/* Builds a graph from m arcs src[i]->dst[i]; symmetrize adds dst->src. */
+int cuts_graph_from_arcs(uint32_t V, const uint32_t *src, const uint32_t *dst, uint64_t m,
+                         int symmetrize, cuts_graph **out);
+int cuts_graph_save(const cuts_graph *g, const char *path);
+typedef struct {
+  uint32_t V;
+  uint64_t arcs;
+  int symmetric;
+  const uint32_t *out_off, *out_adj, *in_off, *in_adj, *sig; /* sig[2v]=in, sig[2v+1]=out */
+} cuts_graph_view;
+int cuts_graph_view_of(const cuts_graph *g, cuts_graph_view *view);
+void cuts_graph_free(cuts_graph *g);
+
+/* ---- synthetic inputs (no reference counterpart) ---------------------- */
+int cuts_gen_chung_lu(uint32_t n, uint64_t m, double gamma, uint32_t max_deg, uint64_t seed,
+                      cuts_graph **out);
+int cuts_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c, uint64_t seed,
+                  cuts_graph **out);
+/* One occurrence of an undirected pattern (k vertices, ne edges as pairs). */
+int cuts_find_pattern(const cuts_graph *d, uint32_t k, const uint32_t *edges, uint32_t ne,
+                      uint64_t seed, uint32_t *witness);
+/* Random-walk query with k vertices and up to n_edges edges; writes
+ * *n_out edge pairs into edges (capacity k*(k-1)/2 pairs) and the walk's
+ * data vertices into witness[k]. */
+int cuts_walk_query(const cuts_graph *d, uint32_t k, uint32_t n_edges, uint64_t seed, uint32_t *edges,
+                    uint32_t *n_out, uint32_t *witness);
+
+/* ---- query plans: Graph::sort_search_order (src/graph.cpp:62-134) ----- */
+/* order == NULL computes the score order of src/score.cpp:18-99. */
+int cuts_plan_create(const cuts_graph *query, const uint32_t *order, cuts_plan **out);
+/* Copies k, order[k], P/C CSR (par_off[k+1], par[], chi_off[k+1], chi[]);
+ * any output pointer may be NULL. */
+int cuts_plan_info(const cuts_plan *p, uint32_t *k, uint32_t *order, uint32_t *par_off, uint32_t *par,
+                   uint32_t *chi_off, uint32_t *chi);
+void cuts_plan_free(cuts_plan *p);
+
+/* ---- device ----------------------------------------------------------- */
+int cuts_context_create(int device, cuts_context **out);
+void cuts_context_destroy(cuts_context *ctx);
+/* Copies the CSR + signatures to HBM (src/gpu_memory_allocation.cu:26-36). */
+int cuts_device_graph_upload(cuts_context *ctx, const cuts_graph *g, cuts_device_graph **out);
+void cuts_device_graph_free(cuts_device_graph *dg);
+
+enum {
+  CUTS_FLAG_NO_SHORTCUT = 1u << 0, /* last level: always scan candidate lists */
+  CUTS_FLAG_NO_FUSE = 1u << 1,     /* never fuse the last two levels */
+  CUTS_FLAG_WRITE = 1u << 2,       /* materialise embeddings into emb_rows */
+  CUTS_FLAG_NO_XCD = 1u << 3,      /* plain blockIdx -> tile mapping */
+  CUTS_FLAG_PROFILE = 1u << 4      /* time every kernel class with HIP events */
+};
+
+/* Kernel classes reported by CUTS_FLAG_PROFILE. */
+enum {
+  CUTS_K_LEVEL0 = 0,   /* level-0 signature filter */
+  CUTS_K_PLAN = 1,     /* shortest-list selection per prefix */
+  CUTS_K_SCAN = 2,     /* exclusive scan of the list lengths */
+  CUTS_K_TILE = 3,     /* tile -> prefix index */
+  CUTS_K_EXPAND = 4,   /* expansion writing the next trie level */
+  CUTS_K_COUNT = 5,    /* expansion counting the last level */
+  CUTS_K_FUSED = 6,    /* expansion + closed-form last level */
+  CUTS_K_CLOSED = 7,   /* closed-form last level over stored rows */
+  CUTS_K_CLASSES = 8
+};
+
+typedef struct {
+  uint32_t trunks;       /* level-0 chunks, as the reference's 3rd CLI arg; 0 = automatic */
+  uint32_t flags;        /* CUTS_FLAG_* */
+  uint64_t budget_bytes; /* trie-table budget in HBM; 0 = default */
+  /* level-0 restriction: only data vertices root_begin + i*root_stride
+   * (< root_end) may match position 0; all zero = every vertex. */
+  uint32_t root_begin, root_end, root_stride;
+  uint32_t blocks_per_cu; /* launch shape for the expansion kernels; 0 = default */
+  /* CUTS_FLAG_WRITE: host buffer receiving up to emb_cap rows of k words,
+   * row = data vertices in matching-order positions (ans.txt layout). */
+  uint32_t *emb_rows;
+  uint64_t emb_cap;
+} cuts_match_options;
+
+typedef struct {
+  uint64_t count;           /* exact number of embeddings */
+  uint64_t level_sizes[16]; /* |T_l| for every order position l < k */
+  double device_ms;         /* first matcher kernel .. final count readback */
+  uint64_t emb_written;     /* rows copied to emb_rows */
+  uint32_t chunks;          /* expansion chunks run (automatic trunking) */
+  uint32_t reserved;
+  /* CUTS_FLAG_PROFILE: per kernel class, summed HIP-event time, launches and
+   * algorithmic (demand) bytes as defined next to each kernel */
+  double kernel_ms[8];
+  uint64_t kernel_launches[8];
+  double kernel_bytes[8];
+} cuts_match_result;
+
+void cuts_match_options_default(cuts_match_options *o);
+/* One query against a resident data graph; replaces the level loop of
+ * search / search_dfs_bfs_strategy (src/host_funcs.cu:39-171). */
+int cuts_match(cuts_context *ctx, const cuts_device_graph *dg, const cuts_plan *plan,
+               const cuts_match_options *opt, cuts_match_result *res);
+
+/* ---- reference-compatible entry points ------------------------------- */
+/* search(query, data, write_to_disk) and search_dfs_bfs_strategy(...,
+ * trunks) of inc/host_funcs.h:21-25: load both files, print the progress
+ * lines and the "<data>,<query>,<ms>ms,<count>" report line on stdout,
+ * write ans.txt in the current directory when write_to_disk. */
+int cuts_search(const char *query_file, const char *data_file, int write_to_disk, uint64_t *count);
+int cuts_search_dfs_bfs_strategy(const char *query_file, const char *data_file, int write_to_disk,
+                                 uint32_t trunks, uint64_t *count);
+/* ans.txt writer of src/util.cpp:53-74 for n rows of k words (positions). */
+int cuts_write_matches(const char *path, const cuts_plan *plan, const uint32_t *rows, uint64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

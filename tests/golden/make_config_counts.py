@@ -1,0 +1,54 @@
+"""Writes tests/golden/config_counts.json with the oracle's counts for the
+benchmark workloads: the full config-1 count, and for the larger configs the
+count restricted to a sample of position-0 images (root_begin + i*stride),
+small enough for the CPU oracle.  Each entry records the data-graph
+fingerprint so a generator change cannot pass silently.
+
+Run from the repository root after building cuts_amd and the oracle.
+"""
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+import cuts_amd  # noqa: E402
+import oracle  # noqa: E402
+from cuts_amd import workloads as W  # noqa: E402
+
+SAMPLES = {
+    "config2a": (W.config2_graph, W.C4_2PEND, (2000, 36692, 211)),
+    "config2b": (W.config2_graph, W.K4E_TAIL2, (2000, 36692, 211)),
+    "config3": (W.config3_graph, W.C5CHORD_2TAIL, (1 << 19, 1 << 20, 8191)),
+}
+
+
+def main(keys):
+    path = os.path.join(HERE, "config_counts.json")
+    out = json.load(open(path)) if os.path.exists(path) else {}
+    for key in keys:
+        t0 = time.time()
+        if key == "config1":
+            g = W.config1_graph()
+            og = oracle.OGraph.from_view(g.view())
+            qa = oracle.qmatrix(5, W.shape_arcs(W.TRI_TAIL2))
+            c = oracle.count(og, qa)
+            out[key] = dict(fingerprint=W.csr_fingerprint(g), shape=W.TRI_TAIL2.name, count=c)
+        else:
+            gf, shape, roots = SAMPLES[key]
+            g = gf()
+            og = oracle.OGraph.from_view(g.view())
+            qa = oracle.qmatrix(shape.k, W.shape_arcs(shape))
+            root_q = cuts_amd.Plan(W.query_graph(shape)).order[0]
+            c = oracle.count_ex(og, qa, root_q=root_q, roots=roots)
+            out[key] = dict(fingerprint=W.csr_fingerprint(g), shape=shape.name, root_q=root_q, roots=list(roots),
+                            count=c)
+        print(key, out[key], f"{time.time() - t0:.1f}s", flush=True)
+        with open(path, "w") as f:
+            json.dump(out, f, indent=1)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:] or ["config1", "config2a", "config2b", "config3"])

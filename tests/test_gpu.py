@@ -1,0 +1,219 @@
+"""GPU parity: the HIP matcher (through the C ABI) against the oracle.
+
+Exact, bit-for-bit equality of uint64 counts and of embedding sets:
+networkx-pinned cases, random graphs vs the backtracking oracle under every
+execution mode (closed-form last level on/off, fused last two levels on/off,
+XCD tile mapping on/off, manual trunks, tiny HBM budgets that force many
+chunks), per-level table sizes vs the CPU trie baseline, the full config-1
+count, root-sampled config-2/3 counts, and the reference-compatible CLI.
+"""
+import json
+import os
+import random
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import cuts_amd
+import oracle
+from cuts_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def load(name):
+    with open(os.path.join(GOLD, name)) as f:
+        return json.load(f)
+
+
+MODES = [
+    dict(),
+    dict(flags=cuts_amd.FLAG_NO_FUSE),
+    dict(flags=cuts_amd.FLAG_NO_SHORTCUT),
+    dict(flags=cuts_amd.FLAG_NO_XCD),
+    dict(trunks=3),
+    dict(budget=4096),  # a few hundred rows per level: many chunks
+]
+
+
+def arcs_graph(V, arcs):
+    a = np.asarray(arcs, dtype=np.uint32).reshape(-1, 2)
+    return cuts_amd.Graph.from_arcs(V, a[:, 0], a[:, 1])
+
+
+NX = load("nx_counts.json")["cases"]
+
+
+@pytest.mark.parametrize("case", NX, ids=[c["name"] for c in NX])
+def test_networkx_cases(ctx, case):
+    g = arcs_graph(case["V"], case["arcs"])
+    dg = ctx.upload(g)
+    p = cuts_amd.Plan(arcs_graph(case["k"], case["qarcs"]))
+    for m in MODES:
+        assert ctx.match(dg, p, **m).count == case["count"], m
+
+
+def random_case(seed):
+    rng = random.Random(seed)
+    n = rng.randint(60, 400)
+    g = cuts_amd.Graph.chung_lu(n, rng.randint(2 * n, 6 * n), rng.uniform(2.1, 2.8), 0, seed)
+    k = rng.randint(2, 8)
+    e, _ = g.walk_query(k, rng.randint(k - 1, 2 * k), seed + 1)
+    return g, k, e.tolist()
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_random_undirected_vs_oracle(ctx, seed):
+    g, k, edges = random_case(seed)
+    og = oracle.OGraph.from_view(g.view())
+    qa = oracle.qmatrix(k, oracle.undirected(edges))
+    want = oracle.count(og, qa)
+    dg = ctx.upload(g)
+    p = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges))
+    for m in MODES:
+        r = ctx.match(dg, p, **m)
+        assert r.count == want, (m, r)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_directed_vs_oracle(ctx, seed):
+    rng = random.Random(1000 + seed)
+    n = rng.randint(30, 120)
+    arcs = sorted({(rng.randrange(n), rng.randrange(n)) for _ in range(rng.randint(3 * n, 8 * n))})
+    k = rng.randint(2, 6)
+    q = set()
+    for v in range(1, k):
+        u = rng.randrange(v)
+        q.add((u, v) if rng.random() < 0.5 else (v, u))
+        if rng.random() < 0.3:
+            q.add((v, u))
+    q = sorted(q)
+    og = oracle.OGraph.from_arcs(n, arcs)
+    want = oracle.count(og, oracle.qmatrix(k, q))
+    dg = ctx.upload(arcs_graph(n, arcs))
+    p = cuts_amd.Plan(arcs_graph(k, q))
+    for m in MODES:
+        assert ctx.match(dg, p, **m).count == want, m
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_embedding_sets_vs_oracle(ctx, seed):
+    g, k, edges = random_case(200 + seed)
+    og = oracle.OGraph.from_view(g.view())
+    qa = oracle.qmatrix(k, oracle.undirected(edges))
+    want = oracle.enumerate_embeddings(og, qa, cap=1 << 21)
+    if len(want) > 200000:
+        pytest.skip("too many embeddings for a set comparison")
+    p = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges))
+    dg = ctx.upload(g)
+    for budget in (0, 2048):
+        r = ctx.match(dg, p, write=True, emb_cap=len(want) + 16, budget=budget)
+        assert r.count == len(want)
+        got = {tuple(row) for row in r.rows.tolist()}
+        # oracle rows are indexed by query vertex; GPU rows by order position
+        exp = {tuple(row[u] for u in p.order) for row in want.tolist()}
+        assert got == exp and len(got) == len(want)
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_level_sizes_vs_cpu_trie(ctx, seed):
+    g, k, edges = random_case(400 + seed)
+    og = oracle.OGraph.from_view(g.view())
+    qa = oracle.qmatrix(k, oracle.undirected(edges))
+    p = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges))
+    cpu = oracle.trie_cpu(og, qa, p.order, nthreads=4)
+    dg = ctx.upload(g)
+    for m in (dict(), dict(flags=cuts_amd.FLAG_NO_FUSE | cuts_amd.FLAG_NO_SHORTCUT), dict(budget=8192)):
+        r = ctx.match(dg, p, **m)
+        assert r.count == cpu.count
+        assert r.level_sizes == [cpu.level[i] for i in range(k)], m
+
+
+def test_root_restriction_partitions(ctx):
+    g, k, edges = random_case(77)
+    dg = ctx.upload(g)
+    p = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges))
+    total = ctx.match(dg, p).count
+    V = g.V
+    assert sum(ctx.match(dg, p, roots=(r, V, 4)).count for r in range(4)) == total
+
+
+CFG = load("config_counts.json")
+
+
+def test_config1_full_count(ctx):
+    c = CFG["config1"]
+    g = W.config1_graph()
+    assert W.csr_fingerprint(g) == c["fingerprint"]
+    dg = ctx.upload(g)
+    p = cuts_amd.Plan(W.query_graph(W.TRI_TAIL2))
+    for m in (dict(), dict(flags=cuts_amd.FLAG_NO_FUSE), dict(trunks=4), dict(budget=1 << 24)):
+        assert ctx.match(dg, p, **m).count == c["count"], m
+
+
+@pytest.mark.parametrize("key", ["config2a", "config2b", "config3"])
+def test_config_sampled_counts(ctx, key):
+    c = CFG[key]
+    g = {"config2a": W.config2_graph, "config2b": W.config2_graph, "config3": W.config3_graph}[key]()
+    assert W.csr_fingerprint(g) == c["fingerprint"]
+    shape = {"config2a": W.C4_2PEND, "config2b": W.K4E_TAIL2, "config3": W.C5CHORD_2TAIL}[key]
+    p = cuts_amd.Plan(W.query_graph(shape))
+    assert p.order[0] == c["root_q"]
+    dg = ctx.upload(g)
+    r = ctx.match(dg, p, roots=tuple(c["roots"]))
+    assert r.count == c["count"]
+
+
+def test_cli_report_line_and_trunks(tmp_path):
+    g = W.config1_graph()
+    data = tmp_path / "cl4k.g"
+    query = tmp_path / "tri2.g"
+    g.save(str(data))
+    W.query_graph(W.TRI_TAIL2).save(str(query))
+    want = CFG["config1"]["count"]
+    for extra in ([], ["4"], ["0"]):
+        r = subprocess.run([cuts_amd.CLI_PATH, str(data), str(query)] + extra, capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        lines = r.stdout.strip().splitlines()
+        assert lines[0] == "start loading graph file from disk to memory..."
+        assert lines[-2] == ("end copying graph to gpu..." if not extra else "finish copying graph to gpu...")
+        m = re.fullmatch(r"cl4k\.g,tri2\.g,([0-9.e+-]+)ms,(\d+)", lines[-1])
+        assert m, lines[-1]
+        assert int(m.group(2)) == want
+
+
+def test_cli_write_ans_txt(tmp_path):
+    g, k, edges = random_case(31)
+    og = oracle.OGraph.from_view(g.view())
+    qa = oracle.qmatrix(k, oracle.undirected(edges))
+    want = oracle.enumerate_embeddings(og, qa, cap=1 << 21)
+    if len(want) > 300000:
+        pytest.skip("too many embeddings")
+    g.save(str(tmp_path / "d.g"))
+    q = cuts_amd.Graph.from_edges(k, edges)
+    q.save(str(tmp_path / "q.g"))
+    p = cuts_amd.Plan(q)
+    r = subprocess.run([cuts_amd.CLI_PATH, "d.g", "q.g", "--write"], capture_output=True, text=True,
+                       cwd=str(tmp_path), timeout=120)
+    assert r.returncode == 0, r.stdout
+    # basename of a path without '/' is "" (src/common.cpp:19-26)
+    assert re.fullmatch(r",,[0-9.e+-]+ms,%d" % len(want), r.stdout.strip().splitlines()[-3])
+    lines = (tmp_path / "ans.txt").read_text().splitlines()
+    exp = sorted("".join(f"({u},{row[u]})" for u in p.order) for row in want.tolist())
+    assert sorted(lines) == exp
+
+
+def test_search_entry_points(tmp_path):
+    g, k, edges = random_case(5)
+    og = oracle.OGraph.from_view(g.view())
+    want = oracle.count(og, oracle.qmatrix(k, oracle.undirected(edges)))
+    g.save(str(tmp_path / "d.g"))
+    cuts_amd.Graph.from_edges(k, edges).save(str(tmp_path / "q.g"))
+    assert cuts_amd.search(str(tmp_path / "q.g"), str(tmp_path / "d.g")) == want
+    assert cuts_amd.search_dfs_bfs_strategy(str(tmp_path / "q.g"), str(tmp_path / "d.g"), False, 3) == want

@@ -1,0 +1,172 @@
+"""Host-side parity (no GPU): the C ABI library loads and exports what
+include/cuts.h declares; the edge-list reader, the CSR/signature build, the
+matching order with its P/C constraints and the ans.txt writer agree with
+what the reference's own host code produces (tests/golden/ref_*.json);
+generators are deterministic and hit their specified sizes."""
+import json
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import cuts_amd
+import oracle
+from cuts_amd import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def load(name):
+    with open(os.path.join(GOLD, name)) as f:
+        return json.load(f)
+
+
+def test_abi_exports_every_declared_symbol():
+    with open(os.path.join(ROOT, "include", "cuts.h")) as f:
+        text = f.read()
+    names = set(re.findall(r"\b(cuts_[a-z_0-9]+)\s*\(", text))
+    assert len(names) >= 20
+    L = cuts_amd.lib()
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+
+
+@pytest.mark.parametrize("f", load("ref_csr.json")["files"], ids=lambda f: f["name"])
+def test_reader_matches_reference_csr(f, tmp_path):
+    p = tmp_path / "g.txt"
+    p.write_bytes(f["text"].encode())
+    v = cuts_amd.Graph.load(str(p)).view()
+    assert v["V"] == f["V"] and v["arcs"] == f["arcs"]
+    for key in ("out_off", "out_adj", "in_off", "in_adj", "sig"):
+        assert v[key].tolist() == f[key], key
+
+
+@pytest.mark.parametrize("text", ["", "x\n0\t1\n", "3\n0\t7\n", "3\n-1\t2\n", "3\n1\t\n", "3\nz\t1\n"])
+def test_reader_rejects_what_the_reference_cannot_parse(text, tmp_path):
+    p = tmp_path / "bad.txt"
+    p.write_text(text)
+    with pytest.raises(cuts_amd.CutsError):
+        cuts_amd.Graph.load(str(p))
+
+
+def test_reader_missing_file():
+    with pytest.raises(cuts_amd.CutsError) as e:
+        cuts_amd.Graph.load("/nonexistent/file.g")
+    assert e.value.code == -2
+
+
+PLANS = load("ref_plans.json")["plans"]
+
+
+@pytest.mark.parametrize("plan", PLANS, ids=[p["name"] for p in PLANS])
+def test_plan_matches_reference(plan):
+    k = plan["k"]
+    a = np.asarray(plan["qarcs"], dtype=np.uint32).reshape(-1, 2)
+    q = cuts_amd.Graph.from_arcs(k, a[:, 0], a[:, 1])
+    p = cuts_amd.Plan(q)
+    assert p.order == plan["order"]
+    assert p.parents_offset == plan["parents_offset"]
+    assert p.parents == plan["parents"]
+    assert p.children_offset == plan["children_offset"]
+    assert p.children == plan["children"]
+
+
+def test_plan_rejects_disconnected_and_large():
+    q = cuts_amd.Graph.from_edges(4, [(0, 1), (2, 3)])
+    with pytest.raises(cuts_amd.CutsError):
+        cuts_amd.Plan(q)
+    q = cuts_amd.Graph.from_edges(17, [(i, i + 1) for i in range(16)])
+    with pytest.raises(cuts_amd.CutsError):
+        cuts_amd.Plan(q)
+
+
+def test_plan_with_explicit_order_validates():
+    q = cuts_amd.Graph.from_edges(4, [(0, 1), (1, 2), (2, 3)])
+    cuts_amd.Plan(q, [1, 2, 0, 3])
+    with pytest.raises(cuts_amd.CutsError):
+        cuts_amd.Plan(q, [0, 2, 1, 3])  # position 1 has no earlier neighbour
+
+
+@pytest.mark.parametrize("case", load("ref_ans.json")["cases"], ids=lambda c: str(c["k"]))
+def test_ans_writer_matches_reference(case, tmp_path):
+    k = case["k"]
+    # a query whose heuristic order is the fixture's order: a path through
+    # the order, relabelled; simpler: build the plan with that explicit order
+    order = case["order"]
+    edges = [(order[i], order[i + 1]) for i in range(k - 1)]
+    q = cuts_amd.Graph.from_edges(k, edges)
+    p = cuts_amd.Plan(q, order)
+    out = tmp_path / "ans.txt"
+    cuts_amd.write_matches(str(out), p, np.asarray(case["rows"], dtype=np.uint32))
+    assert out.read_text() == case["text"]
+
+
+def test_cli_usage_line():
+    r = subprocess.run([cuts_amd.CLI_PATH], capture_output=True, text=True)
+    assert r.returncode == 255
+    assert r.stdout.strip() == "args data_graph path,query_graph_path,first depth trunks(optional)"
+
+
+def test_generators_deterministic_and_sized():
+    g1 = W.config1_graph()
+    assert g1.V == 4096 and g1.arcs == 2 * 32768
+    again = cuts_amd.Graph.chung_lu(4096, 32768, 2.3, 0, 0)
+    assert W.csr_fingerprint(again) == W.csr_fingerprint(g1)
+    other = cuts_amd.Graph.chung_lu(4096, 32768, 2.3, 0, 1)
+    assert W.csr_fingerprint(other) != W.csr_fingerprint(g1)
+    g2 = W.config2_graph()
+    v = g2.view()
+    assert v["V"] == 36692 and v["arcs"] == 2 * 183831 and v["symmetric"]
+    assert np.diff(v["out_off"]).max() <= 1400
+    # no self-loops, sorted, duplicate-free
+    off, adj = v["out_off"], v["out_adj"]
+    for u in range(0, v["V"], 97):
+        lst = adj[off[u]:off[u + 1]]
+        assert np.all(np.diff(lst.astype(np.int64)) > 0) and u not in lst
+
+
+def test_rmat_small_properties():
+    g = cuts_amd.Graph.rmat(10, 8, 0.57, 0.19, 0.19, 3)
+    v = g.view()
+    assert v["V"] == 1024 and v["symmetric"]
+    assert v["arcs"] <= 2 * 8 * 1024 and v["arcs"] % 2 == 0
+    assert W.csr_fingerprint(g) == W.csr_fingerprint(cuts_amd.Graph.rmat(10, 8, 0.57, 0.19, 0.19, 3))
+
+
+@pytest.mark.parametrize("shape", [W.TRI_TAIL2, W.C4_2PEND, W.K4E_TAIL2], ids=lambda s: s.name)
+def test_pattern_witness_is_an_embedding(shape):
+    g = W.config1_graph()
+    w = g.find_pattern(shape.k, list(shape.edges), seed=5)
+    assert len(set(w.tolist())) == shape.k
+    v = g.view()
+    for a, b in shape.edges:
+        lst = v["out_adj"][v["out_off"][w[a]]:v["out_off"][w[a] + 1]]
+        assert w[b] in lst
+
+
+def test_walk_query_connected_and_present():
+    g = W.config1_graph()
+    v = g.view()
+    for seed in range(20):
+        k = 4 + seed % 9
+        e, w = g.walk_query(k, int(1.8 * k), seed)
+        assert len(e) >= k - 1 and len(set(w.tolist())) == k
+        q = cuts_amd.Graph.from_edges(k, e.tolist())
+        cuts_amd.Plan(q)  # connected: a plan exists
+        for a, b in e.tolist():
+            assert w[b] in v["out_adj"][v["out_off"][w[a]]:v["out_off"][w[a] + 1]]
+
+
+def test_product_order_equals_oracle_restatement():
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        k = int(rng.integers(2, 12))
+        edges = [(int(rng.integers(0, v)), v) for v in range(1, k)]
+        edges += [tuple(map(int, rng.choice(k, 2, replace=False))) for _ in range(int(rng.integers(0, k)))]
+        q = cuts_amd.Graph.from_edges(k, edges)
+        qa = oracle.qmatrix(k, oracle.undirected(edges))
+        assert cuts_amd.Plan(q).order == oracle.heuristic_order(qa)
