@@ -1,0 +1,158 @@
+"""Benchmark: exact embeddings/s and p50 time per query on one MI355X.
+
+Workload: the mid-size power-law Chung-Lu data graph (n=36,692,
+m=183,831, exponent 2.1, degree cap 1,400, seed 0) resident in HBM, and its
+two 6-vertex queries (a 4-cycle with two pendants; K4 minus an edge with a
+2-edge tail).  One step = matching both queries exactly (uint64 counts) with
+the heuristic score order.  value = total embeddings / total device time of
+the timed steps.
+
+The dominant kernel's roofline comes from HIP events recorded around every
+launch on the library's own stream during the timed steps
+(CUTS_FLAG_PROFILE), with the kernel's algorithmic demand bytes counted on
+the device.  The CPU baseline times the OpenMP port of the same trie
+algorithm (oracle/, test infrastructure) on a bounded sample of level 0.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(g, shapes, budget_s=20.0):
+    """OpenMP trie baseline (same algorithm, same order) on a level-0 sample:
+    every 64th level-0 vertex, stopped between row groups after
+    budget_s/len(shapes) seconds per query; the rate counts only embeddings
+    of sub-tries that were finished."""
+    import oracle
+    import cuts_amd
+    from cuts_amd import workloads as W
+    og = oracle.OGraph.from_view(g.view())
+    # the GPU box grants 16 host cores per GPU whatever os.cpu_count() says
+    threads = min(16, os.cpu_count() or 1)
+    total_emb, total_s, samples = 0, 0.0, []
+    for shape in shapes:
+        qa = oracle.qmatrix(shape.k, W.shape_arcs(shape))
+        order = cuts_amd.Plan(W.query_graph(shape)).order
+        stride = 64
+        r = oracle.trie_cpu(og, qa, order, sample_stride=stride, max_rows=1 << 22, nthreads=threads,
+                            max_seconds=budget_s / len(shapes))
+        total_emb += r.count
+        total_s += r.seconds
+        samples.append(f"{shape.name}: every {stride}th level-0 vertex, {'all' if r.complete else 'time-capped'}; "
+                       f"{r.count} embeddings in {r.seconds:.2f}s")
+    return dict(value=total_emb / total_s if total_s else 0.0, unit="embeddings/s", cores=threads, kind="port",
+                sample="; ".join(samples))
+
+
+def traffic_for(kernel):
+    """HBM bytes per dispatch from the committed rocprofv3 --pmc summary."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    e = t.get(kernel)
+    return None if e is None else e.get("bytes_per_dispatch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: only one GPU is used by this build (asked for --gpus {args.gpus}); "
+              "the matcher shards by level-0 chunks, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import cuts_amd
+    from cuts_amd import workloads as W
+
+    g = W.config2_graph()
+    shapes = [W.C4_2PEND, W.K4E_TAIL2]
+    ctx = cuts_amd.Context(0)
+    dg = ctx.upload(g)
+    plans = [cuts_amd.Plan(W.query_graph(s)) for s in shapes]
+
+    for i in range(args.warmup):
+        for p in plans:
+            ctx.match(dg, p)
+        print(f"bench: warmup step {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+
+    per_query_ms, counts = [], []
+    kern = {}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for p in plans:
+            r = ctx.match(dg, p, flags=cuts_amd.FLAG_PROFILE)
+            per_query_ms.append(r.device_ms)
+            counts.append(r.count)
+            for name, (ms, n, b) in r.kernels.items():
+                k = kern.setdefault(name, [0.0, 0, 0.0])
+                k[0] += ms
+                k[1] += n
+                k[2] += b
+        print(f"bench: timed step done, last query {per_query_ms[-1]:.2f} ms", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    total_ms = sum(per_query_ms)
+    total_emb = sum(counts)
+    dom = max(kern.items(), key=lambda kv: kv[1][0])
+    dname, (dms, dn, dbytes) = dom
+    avg_launch_s = dms / dn / 1e3
+    bytes_per_launch = dbytes / dn
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    line = {
+        "metric": "exact embeddings/sec + p50 time/query; learned vs heuristic order, same GPU",
+        "value": total_emb / (total_ms / 1e3),
+        "unit": "embeddings/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total_ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (seeded Chung-Lu generator; queries located by seeded random walk)",
+        "config": {
+            "workload": "chung-lu n=36692 m=183831 gamma=2.1 maxdeg=1400 seed=0; "
+                        "queries 4-cycle+2-pendants and K4-e+2-tail; heuristic order; one step = both queries",
+            "counts": counts[: len(plans)],
+            "p50_ms_per_query": statistics.median(per_query_ms),
+            "p99_ms_per_query": sorted(per_query_ms)[min(len(per_query_ms) - 1, int(0.99 * len(per_query_ms)))],
+            "wall_s": wall,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dname,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic_for(dname),
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "avg_launch_ms": avg_launch_s * 1e3,
+            "launches": dn,
+        },
+        "kernels_ms": {k: v[0] for k, v in kern.items() if v[1]},
+    }
+    if not args.no_cpu:
+        print("bench: CPU baseline (about 20 s)", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(g, shapes)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

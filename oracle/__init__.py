@@ -31,7 +31,7 @@ class _OGraph(C.Structure):
 
 class TrieResult(C.Structure):
     _fields_ = [("count", C.c_uint64), ("level", C.c_uint64 * 32), ("bytes", C.c_double),
-                ("seconds", C.c_double), ("threads", C.c_int)]
+                ("seconds", C.c_double), ("threads", C.c_int), ("complete", C.c_int)]
 
 
 _lib = None
@@ -53,6 +53,9 @@ def lib():
         L.trie_cpu_run.restype = C.c_int
         L.trie_cpu_run.argtypes = [C.POINTER(_OGraph), C.c_uint32, C.POINTER(C.c_uint8), _u32p, C.c_uint32,
                                    C.c_uint64, C.c_int, C.POINTER(TrieResult)]
+        L.trie_cpu_run_timed.restype = C.c_int
+        L.trie_cpu_run_timed.argtypes = [C.POINTER(_OGraph), C.c_uint32, C.POINTER(C.c_uint8), _u32p, C.c_uint32,
+                                         C.c_uint64, C.c_int, C.c_double, C.POINTER(TrieResult)]
         _lib = L
     return _lib
 
@@ -133,12 +136,12 @@ def heuristic_order(qa: np.ndarray) -> Optional[list]:
 
 
 def trie_cpu(g: OGraph, qa: np.ndarray, order: Sequence[int], sample_stride: int = 1, max_rows: int = 0,
-             nthreads: int = 0) -> TrieResult:
+             nthreads: int = 0, max_seconds: float = 0.0) -> TrieResult:
     qa, p = _qptr(qa)
     o = np.ascontiguousarray(order, dtype=np.uint32)
     r = TrieResult()
-    rc = lib().trie_cpu_run(C.byref(g.c), qa.shape[0], p, o.ctypes.data_as(_u32p), sample_stride, max_rows,
-                            nthreads, C.byref(r))
+    rc = lib().trie_cpu_run_timed(C.byref(g.c), qa.shape[0], p, o.ctypes.data_as(_u32p), sample_stride,
+                                  max_rows, nthreads, max_seconds, C.byref(r))
     if rc:
         raise ValueError("trie_cpu_run failed")
     return r

@@ -301,6 +301,8 @@ typedef struct {
   int nthreads;
   uint64_t level[MAXK];
   double bytes;
+  double t0, max_seconds;
+  int stopped;
 } trie_ctx;
 
 /* Expands the n rows of R (each `len` words = positions 0..len-1) into
@@ -313,6 +315,10 @@ static void trie_expand(trie_ctx *c, const uint32_t *R, uint64_t n, uint32_t len
   /* candidate upper bounds for grouping */
   uint64_t start = 0;
   while (start < n) {
+    if (c->max_seconds > 0 && omp_get_wtime() - c->t0 > c->max_seconds) {
+      c->stopped = 1;
+      return;
+    }
     uint64_t end = start, acc = 0;
     while (end < n) {
       const uint32_t *f = R + end * len;
@@ -398,6 +404,12 @@ static void trie_expand(trie_ctx *c, const uint32_t *R, uint64_t n, uint32_t len
 
 int trie_cpu_run(const oracle_graph *d, uint32_t k, const uint8_t *qa, const uint32_t *order,
                  uint32_t sample_stride, uint64_t max_rows, int nthreads, trie_cpu_result *res) {
+  return trie_cpu_run_timed(d, k, qa, order, sample_stride, max_rows, nthreads, 0.0, res);
+}
+
+int trie_cpu_run_timed(const oracle_graph *d, uint32_t k, const uint8_t *qa, const uint32_t *order,
+                       uint32_t sample_stride, uint64_t max_rows, int nthreads, double max_seconds,
+                       trie_cpu_result *res) {
   if (k == 0 || k > MAXK) return -1;
   trie_ctx *c = (trie_ctx *)calloc(1, sizeof(trie_ctx));
   c->d = d;
@@ -408,6 +420,8 @@ int trie_cpu_run(const oracle_graph *d, uint32_t k, const uint8_t *qa, const uin
   c->nthreads = nthreads > 0 ? nthreads : omp_get_max_threads();
   if (sample_stride == 0) sample_stride = 1;
   double t0 = omp_get_wtime();
+  c->t0 = t0;
+  c->max_seconds = max_seconds;
   uint32_t *l0 = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)d->V + 1));
   uint64_t n0 = 0, seen = 0;
   uint32_t dummy[MAXK] = {0};
@@ -429,6 +443,7 @@ int trie_cpu_run(const oracle_graph *d, uint32_t k, const uint8_t *qa, const uin
   res->bytes = c->bytes;
   res->seconds = t1 - t0;
   res->threads = c->nthreads;
+  res->complete = !c->stopped;
   free(c);
   return 0;
 }

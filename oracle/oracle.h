@@ -63,6 +63,7 @@ typedef struct {
   double bytes;            /* merge-model algorithmic bytes */
   double seconds;          /* wall time of the matching phase */
   int threads;
+  int complete;            /* 0 if max_seconds stopped the run early */
 } trie_cpu_result;
 
 /* Level-by-level trie expansion with the given order (order[i] = query
@@ -71,6 +72,13 @@ typedef struct {
  * table (chunked depth-first over prefix groups). Returns 0 or -1. */
 int trie_cpu_run(const oracle_graph *d, uint32_t k, const uint8_t *qa, const uint32_t *order,
                  uint32_t sample_stride, uint64_t max_rows, int nthreads, trie_cpu_result *res);
+
+/* As trie_cpu_run, but stops between row groups once max_seconds have
+ * passed (0 = no limit); count/level then cover only the sub-tries that were
+ * completed, and res->complete is 0. */
+int trie_cpu_run_timed(const oracle_graph *d, uint32_t k, const uint8_t *qa, const uint32_t *order,
+                       uint32_t sample_stride, uint64_t max_rows, int nthreads, double max_seconds,
+                       trie_cpu_result *res);
 
 #ifdef __cplusplus
 }

@@ -59,12 +59,20 @@ def test_networkx_cases(ctx, case):
 
 
 def random_case(seed):
+    """Small power-law graph and a random-walk query; sized so the oracle
+    finishes in well under a second."""
     rng = random.Random(seed)
-    n = rng.randint(60, 400)
-    g = cuts_amd.Graph.chung_lu(n, rng.randint(2 * n, 6 * n), rng.uniform(2.1, 2.8), 0, seed)
-    k = rng.randint(2, 8)
+    n = rng.randint(40, 160)
+    g = cuts_amd.Graph.chung_lu(n, rng.randint(2 * n, 4 * n), rng.uniform(2.3, 2.9), 0, seed)
+    k = rng.randint(2, 7)
     e, _ = g.walk_query(k, rng.randint(k - 1, 2 * k), seed + 1)
     return g, k, e.tolist()
+
+
+def modes_for(count):
+    """Every execution mode; the tiny-budget mode (1024-row chunks, one
+    host round trip each) only where the tables stay small."""
+    return [m for m in MODES if "budget" not in m or count <= 20_000_000]
 
 
 @pytest.mark.parametrize("seed", range(24))
@@ -75,7 +83,7 @@ def test_random_undirected_vs_oracle(ctx, seed):
     want = oracle.count(og, qa)
     dg = ctx.upload(g)
     p = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges))
-    for m in MODES:
+    for m in modes_for(want):
         r = ctx.match(dg, p, **m)
         assert r.count == want, (m, r)
 
@@ -97,7 +105,7 @@ def test_random_directed_vs_oracle(ctx, seed):
     want = oracle.count(og, oracle.qmatrix(k, q))
     dg = ctx.upload(arcs_graph(n, arcs))
     p = cuts_amd.Plan(arcs_graph(k, q))
-    for m in MODES:
+    for m in modes_for(want):
         assert ctx.match(dg, p, **m).count == want, m
 
 
@@ -106,9 +114,9 @@ def test_embedding_sets_vs_oracle(ctx, seed):
     g, k, edges = random_case(200 + seed)
     og = oracle.OGraph.from_view(g.view())
     qa = oracle.qmatrix(k, oracle.undirected(edges))
-    want = oracle.enumerate_embeddings(og, qa, cap=1 << 21)
-    if len(want) > 200000:
+    if oracle.count(og, qa) > 200000:
         pytest.skip("too many embeddings for a set comparison")
+    want = oracle.enumerate_embeddings(og, qa, cap=1 << 21)
     p = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges))
     dg = ctx.upload(g)
     for budget in (0, 2048):
@@ -128,7 +136,7 @@ def test_level_sizes_vs_cpu_trie(ctx, seed):
     p = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges))
     cpu = oracle.trie_cpu(og, qa, p.order, nthreads=4)
     dg = ctx.upload(g)
-    for m in (dict(), dict(flags=cuts_amd.FLAG_NO_FUSE | cuts_amd.FLAG_NO_SHORTCUT), dict(budget=8192)):
+    for m in modes_for(cpu.count)[:3] + [dict(flags=cuts_amd.FLAG_NO_FUSE | cuts_amd.FLAG_NO_SHORTCUT)]:
         r = ctx.match(dg, p, **m)
         assert r.count == cpu.count
         assert r.level_sizes == [cpu.level[i] for i in range(k)], m
@@ -192,9 +200,9 @@ def test_cli_write_ans_txt(tmp_path):
     g, k, edges = random_case(31)
     og = oracle.OGraph.from_view(g.view())
     qa = oracle.qmatrix(k, oracle.undirected(edges))
-    want = oracle.enumerate_embeddings(og, qa, cap=1 << 21)
-    if len(want) > 300000:
+    if oracle.count(og, qa) > 300000:
         pytest.skip("too many embeddings")
+    want = oracle.enumerate_embeddings(og, qa, cap=1 << 21)
     g.save(str(tmp_path / "d.g"))
     q = cuts_amd.Graph.from_edges(k, edges)
     q.save(str(tmp_path / "q.g"))
