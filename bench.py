@@ -71,6 +71,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="also run N dispatches of a 1 GiB plain copy (byte-counter yardstick)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: only one GPU is used by this build (asked for --gpus {args.gpus}); "
@@ -148,6 +150,8 @@ def main():
         },
         "kernels_ms": {k: v[0] for k, v in kern.items() if v[1]},
     }
+    if args.copy_probe:
+        line["copy_probe_GBs"] = ctx.copy_probe(1 << 30, args.copy_probe)
     if not args.no_cpu:
         print("bench: CPU baseline (about 20 s)", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(g, shapes)

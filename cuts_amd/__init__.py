@@ -43,13 +43,15 @@ class _GraphView(C.Structure):
 class _Opts(C.Structure):
     _fields_ = [("trunks", C.c_uint32), ("flags", C.c_uint32), ("budget_bytes", C.c_uint64),
                 ("root_begin", C.c_uint32), ("root_end", C.c_uint32), ("root_stride", C.c_uint32),
-                ("blocks_per_cu", C.c_uint32), ("emb_rows", _u32p), ("emb_cap", C.c_uint64)]
+                ("blocks_per_cu", C.c_uint32), ("emb_rows", _u32p), ("emb_cap", C.c_uint64),
+                ("time_limit_ms", C.c_double)]
 
 
 class _Result(C.Structure):
     _fields_ = [("count", C.c_uint64), ("level_sizes", C.c_uint64 * 16), ("device_ms", C.c_double),
-                ("emb_written", C.c_uint64), ("chunks", C.c_uint32), ("reserved", C.c_uint32),
-                ("kernel_ms", C.c_double * 8), ("kernel_launches", C.c_uint64 * 8), ("kernel_bytes", C.c_double * 8)]
+                ("emb_written", C.c_uint64), ("chunks", C.c_uint32), ("complete", C.c_uint32),
+                ("kernel_ms", C.c_double * 8), ("kernel_launches", C.c_uint64 * 8), ("kernel_bytes", C.c_double * 8),
+                ("roots_done", C.c_uint64), ("last_root", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 _lib = None
@@ -90,6 +92,7 @@ def lib():
             "cuts_search_dfs_bfs_strategy": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_uint32,
                                                        C.POINTER(C.c_uint64)]),
             "cuts_write_matches": (C.c_int, [C.c_char_p, vp, _u32p, C.c_uint64]),
+            "cuts_copy_probe": (C.c_int, [vp, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -242,6 +245,9 @@ class MatchResult:
     chunks: int
     rows: Optional[np.ndarray] = None  # (n, k) data vertices per matching-order position
     kernels: Optional[dict] = None     # FLAG_PROFILE: class -> (ms, launches, bytes)
+    complete: bool = True              # False if time_limit_ms stopped it between trunks
+    roots_done: int = 0                # level-0 vertices whose subtrees were counted
+    last_root: int = 0                 # the last of them (ascending ids), when incomplete
 
 
 class DeviceGraph:
@@ -268,6 +274,12 @@ class Context:
             _lib.cuts_context_destroy(self._h)
             self._h = None
 
+    def copy_probe(self, nbytes: int, iters: int = 10) -> float:
+        """Read+write GB/s of a plain copy of nbytes (HBM yardstick)."""
+        gbps = C.c_double()
+        _check(lib().cuts_copy_probe(self._h, nbytes, iters, C.byref(gbps)))
+        return gbps.value
+
     def upload(self, g: Graph) -> DeviceGraph:
         h = C.c_void_p()
         _check(lib().cuts_device_graph_upload(self._h, g._h, C.byref(h)))
@@ -275,10 +287,11 @@ class Context:
 
     def match(self, dg: DeviceGraph, plan: Plan, trunks: int = 0, flags: int = 0, budget: int = 0,
               roots: Optional[tuple] = None, write: bool = False, emb_cap: int = 0,
-              blocks_per_cu: int = 0) -> MatchResult:
+              blocks_per_cu: int = 0, time_limit_ms: float = 0.0) -> MatchResult:
         o = _Opts()
         lib().cuts_match_options_default(C.byref(o))
         o.trunks, o.flags, o.budget_bytes, o.blocks_per_cu = trunks, flags, budget, blocks_per_cu
+        o.time_limit_ms = time_limit_ms
         if roots is not None:
             o.root_begin, o.root_end, o.root_stride = roots
         rows = None
@@ -290,7 +303,8 @@ class Context:
         r = _Result()
         _check(lib().cuts_match(self._h, dg._h, plan._h, C.byref(o), C.byref(r)))
         res = MatchResult(int(r.count), [int(x) for x in r.level_sizes[: plan.k]], float(r.device_ms),
-                          int(r.chunks))
+                          int(r.chunks), complete=bool(r.complete), roots_done=int(r.roots_done),
+                          last_root=int(r.last_root))
         if write:
             res.rows = rows[: r.emb_written].copy()
         if o.flags & FLAG_PROFILE:

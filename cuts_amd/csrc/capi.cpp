@@ -103,6 +103,7 @@ cuts::MatchOpts to_opts(const cuts_match_options* o) {
   m.blocks_per_cu = o->blocks_per_cu;
   m.emb_rows = o->emb_rows;
   m.emb_cap = o->emb_cap;
+  m.time_limit_ms = o->time_limit_ms;
   return m;
 }
 
@@ -313,11 +314,22 @@ int cuts_match(cuts_context* ctx, const cuts_device_graph* dg, const cuts_plan* 
     res->device_ms = r.device_ms;
     res->emb_written = r.emb_written;
     res->chunks = r.chunks;
+    res->complete = r.complete;
+    res->roots_done = r.roots_done;
+    res->last_root = r.last_root;
     for (int i = 0; i < 8; ++i) {
       res->kernel_ms[i] = r.kernel_ms[i];
       res->kernel_launches[i] = r.kernel_launches[i];
       res->kernel_bytes[i] = r.kernel_bytes[i];
     }
+    return CUTS_OK;
+  });
+}
+
+int cuts_copy_probe(cuts_context* ctx, uint64_t bytes, uint32_t iters, double* gbps) {
+  return guard([&]() -> int {
+    if (!ctx || !gbps || !iters) return fail(CUTS_E_ARG, "invalid argument");
+    *gbps = cuts::copy_probe(ctx->c, bytes, iters);
     return CUTS_OK;
   });
 }

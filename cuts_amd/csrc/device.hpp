@@ -86,6 +86,7 @@ struct MatchOpts {
   uint32_t blocks_per_cu = 0;
   uint32_t* emb_rows = nullptr;
   uint64_t emb_cap = 0;
+  double time_limit_ms = 0;
 };
 struct MatchResult {
   uint64_t count = 0;
@@ -93,6 +94,9 @@ struct MatchResult {
   double device_ms = 0;
   uint64_t emb_written = 0;
   uint32_t chunks = 0;
+  uint32_t complete = 1;
+  uint64_t roots_done = 0;
+  uint32_t last_root = 0;
   double kernel_ms[8] = {0};
   uint64_t kernel_launches[8] = {0};
   double kernel_bytes[8] = {0};
@@ -101,5 +105,8 @@ struct MatchResult {
 void context_init(Context& c, int device);
 void upload_graph(Context& c, const Graph& g, DeviceGraph& dg);
 void run_match(Context& c, DeviceGraph& dg, const Plan& p, const MatchOpts& o, MatchResult& r);
+// Streams `bytes` through a plain copy kernel `iters` times; returns GB/s
+// (read + write).
+double copy_probe(Context& c, uint64_t bytes, uint32_t iters);
 
 }  // namespace cuts

@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 
@@ -93,31 +94,23 @@ __device__ __forceinline__ void add_wave(unsigned long long* dst, uint64_t v) {
 }
 
 // ---------------------------------------------------------------- level 0
-__global__ __launch_bounds__(BLOCK) void k_level0(const uint32_t* __restrict__ sig, uint32_t tin, uint32_t tout,
-                                                 uint32_t rb, uint32_t re, uint32_t rs, uint32_t n,
-                                                 uint32_t* __restrict__ out, uint32_t* __restrict__ cnt) {
-  __shared__ uint32_t s_w[BLOCK / 64], s_pre[BLOCK / 64], s_base;
-  const uint32_t tid = threadIdx.x, wave = tid >> 6;
-  for (uint64_t i0 = (uint64_t)blockIdx.x * BLOCK; i0 < n; i0 += (uint64_t)gridDim.x * BLOCK) {
-    const uint64_t i = i0 + tid;
-    bool ok = false;
-    uint32_t u = 0;
-    if (i < n) {
-      u = rb + (uint32_t)i * rs;
-      ok = u < re && sig[2 * (uint64_t)u] >= tin && sig[2 * (uint64_t)u + 1] >= tout;
-    }
-    const uint64_t m = __ballot(ok);
-    if ((tid & 63) == 0) s_w[wave] = __popcll(m);
-    __syncthreads();
-    if (tid == 0) {
-      uint32_t t = 0;
-      for (int w = 0; w < BLOCK / 64; ++w) { s_pre[w] = t; t += s_w[w]; }
-      s_base = t ? atomicAdd(cnt, t) : 0;
-    }
-    __syncthreads();
-    if (ok) out[s_base + s_pre[wave] + lane_rank(m)] = u;
-    __syncthreads();
+// Level 0 in ascending vertex order: a 0/1 flag per candidate vertex, an
+// exclusive scan of the flags, and a scatter.  A deterministic level 0 makes
+// a run stopped between trunks correspond to an exact root range.
+__global__ __launch_bounds__(BLOCK) void k_level0_flags(const uint32_t* __restrict__ sig, uint32_t tin,
+                                                       uint32_t tout, uint32_t rb, uint32_t re, uint32_t rs,
+                                                       uint32_t n, uint32_t* __restrict__ flags) {
+  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK) {
+    const uint32_t u = rb + (uint32_t)i * rs;
+    flags[i] = u < re && sig[2 * (uint64_t)u] >= tin && sig[2 * (uint64_t)u + 1] >= tout;
   }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_level0_scatter(const uint32_t* __restrict__ flags,
+                                                         const uint64_t* __restrict__ pos, uint32_t rb, uint32_t rs,
+                                                         uint32_t n, uint32_t* __restrict__ out) {
+  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK)
+    if (flags[i]) out[pos[i]] = rb + (uint32_t)i * rs;
 }
 
 // ------------------------------------------------------------ planning
@@ -251,6 +244,10 @@ __global__ __launch_bounds__(BLOCK) void k_tile_index(const uint64_t* __restrict
 // Number of candidates of the last position given all earlier positions
 // r[0..N): |{x in N(u) : deg(x) >= thr}| - |{i : r[i] in N(u), deg(r[i]) >= thr}|
 // with u = r[sc_pos] (symmetric graphs only).
+// Membership is symmetric (x in N(u) <=> u in N(x)).  When u is the newest
+// position -- a different vertex for every work item of a prefix -- the
+// search runs in the lists of the older positions instead, which all items
+// of the prefix share and which therefore stay in cache.
 template <int N>
 __device__ __forceinline__ uint64_t closed_form(const GView& g, const KPlan& P, const uint32_t (&r)[N],
                                                 const uint32_t* __restrict__ tbl, uint64_t& bytes) {
@@ -261,14 +258,21 @@ __device__ __forceinline__ uint64_t closed_form(const GView& g, const KPlan& P, 
   const uint32_t lo = g.off_out[u], hi = g.off_out[u + 1];
   int64_t c = tbl ? (int64_t)tbl[u] : (int64_t)(hi - lo);
   const bool need_deg = P.sc_thr > 1;
+  const bool newest = P.sc_pos == (uint32_t)(N - 1);
   bytes += 8 + (tbl ? 4 : 0);
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     const uint32_t x = r[i];
     bool in = (P.sc_adjmask >> i) & 1;
     if (!in) {
-      in = member_safe(g.adj_out, lo, hi, x);
-      bytes += bs_bytes(hi - lo);
+      if (newest && i != N - 1) {
+        const uint32_t xl = g.off_out[x], xh = g.off_out[x + 1];
+        in = member_safe(g.adj_out, xl, xh, u);
+        bytes += 8 + bs_bytes(xh - xl);
+      } else {
+        in = member_safe(g.adj_out, lo, hi, x);
+        bytes += bs_bytes(hi - lo);
+      }
     }
     if (in && need_deg) bytes += 4;
     if (in && (!need_deg || g.sig[2 * (uint64_t)x] >= P.sc_thr)) --c;
@@ -530,6 +534,12 @@ struct Matcher {
       : c(c_), dg(dg_), plan(p_), o(o_), r(r_) {}
 
   bool prof = false;
+  bool has_deadline = false, aborted = false;
+  std::chrono::steady_clock::time_point deadline;
+  bool past_deadline() {
+    if (has_deadline && !aborted && std::chrono::steady_clock::now() >= deadline) aborted = true;
+    return aborted;
+  }
   void pbegin(bool dev_bytes) {
     if (!prof) return;
     if (dev_bytes) HIPCHECK(hipMemsetAsync(c.counters + 3, 0, 8, c.stream));
@@ -609,6 +619,7 @@ struct Matcher {
       chunk = std::min<uint64_t>(chunk, out.cap);
     }
     for (uint64_t w0 = 0; w0 < W; w0 += chunk) {
+      if (past_deadline()) return;
       const uint64_t w1 = std::min(W, w0 + chunk);
       const uint64_t nt = (w1 - w0 + TILE - 1) / TILE;
       ensure_tiles(in, nt);
@@ -656,6 +667,7 @@ struct Matcher {
           }
         } else {
           level(L + 1, out.rows, m);
+          if (aborted) return;
         }
       } else if (mode == M_COUNT) {
         pbegin(true);
@@ -713,24 +725,62 @@ struct Matcher {
     const uint32_t re = o.root_stride ? std::min(o.root_end, dg.V) : dg.V;
     const uint32_t rs = o.root_stride ? o.root_stride : 1;
     const uint32_t n0c = re > rb ? (re - rb + rs - 1) / rs : 0;
-    ensure_rows(c.lv[0], std::max<uint32_t>(n0c, 1), 1);
+    LevelBuf& l0 = c.lv[0];
+    ensure_rows(l0, std::max<uint32_t>(n0c, 1), 1);
     HIPCHECK(hipMemsetAsync(c.counters, 0, 4 * 8, c.stream));
     HIPCHECK(hipEventRecord(c.ev0, c.stream));
-    pbegin(false);
-    hipLaunchKernelGGL(k_level0, dim3(grid_for(n0c, BLOCK, 65535)), dim3(BLOCK), 0, c.stream, dg.sig,
-                       kp.sig_in[0], kp.sig_out[0], rb, re, rs, n0c, c.lv[0].rows, (uint32_t*)(c.counters + 2));
-    HIPCHECK(hipGetLastError());
-    pend(0, 8.0 * n0c, false);
-    const uint64_t n0 = read_u32(c.counters + 2);
+    const auto t_start = std::chrono::steady_clock::now();
+    uint64_t n0 = 0;
+    if (n0c) {
+      pbegin(false);
+      hipLaunchKernelGGL(k_level0_flags, dim3(grid_for(n0c, BLOCK, 65535)), dim3(BLOCK), 0, c.stream, dg.sig,
+                         kp.sig_in[0], kp.sig_out[0], rb, re, rs, n0c, l0.minlen);
+      HIPCHECK(hipGetLastError());
+      pend(0, 8.0 * n0c + 4.0 * n0c, false);
+      n0 = scan(l0, n0c);
+      pbegin(false);
+      hipLaunchKernelGGL(k_level0_scatter, dim3(grid_for(n0c, BLOCK, 65535)), dim3(BLOCK), 0, c.stream, l0.minlen,
+                         l0.work_off, rb, rs, n0c, l0.rows);
+      HIPCHECK(hipGetLastError());
+      pend(0, 12.0 * n0c + 4.0 * n0, false);
+    }
     r.level[0] = n0;
-    if (prof) r.kernel_bytes[0] += 4.0 * n0;
+    r.complete = 1;
     if (k == 1) {
       r.count = n0;
+      r.roots_done = n0;
     } else if (n0) {
       uint32_t trunks = o.trunks ? o.trunks : 1;
       if (trunks > n0) trunks = (uint32_t)n0;
       const uint64_t ts = (n0 + trunks - 1) / trunks;
-      for (uint64_t s = 0; s < n0; s += ts) level(1, c.lv[0].rows + s, std::min(ts, n0 - s));
+      if (o.time_limit_ms > 0 && !write) {
+        has_deadline = true;
+        deadline = t_start + std::chrono::microseconds((int64_t)(o.time_limit_ms * 1000.0));
+      }
+      for (uint64_t s = 0; s < n0; s += ts) {
+        // snapshot so that a trunk interrupted by the deadline can be undone
+        uint64_t snap[2] = {0, 0};
+        uint64_t snap_level[kMaxQuery];
+        std::memcpy(snap_level, r.level, sizeof(snap_level));
+        if (has_deadline && !write) {
+          if (past_deadline()) break;
+          HIPCHECK(hipMemcpyAsync(c.host_pinned, c.counters, 16, hipMemcpyDeviceToHost, c.stream));
+          HIPCHECK(hipStreamSynchronize(c.stream));
+          snap[0] = c.host_pinned[0];
+          snap[1] = c.host_pinned[1];
+        }
+        level(1, l0.rows + s, std::min(ts, n0 - s));
+        if (aborted) {
+          c.host_pinned[0] = snap[0];
+          c.host_pinned[1] = snap[1];
+          HIPCHECK(hipMemcpyAsync(c.counters, c.host_pinned, 16, hipMemcpyHostToDevice, c.stream));
+          HIPCHECK(hipStreamSynchronize(c.stream));
+          std::memcpy(r.level, snap_level, sizeof(snap_level));
+          break;
+        }
+        r.roots_done = std::min(n0, s + ts);
+      }
+      if (aborted || r.roots_done < n0) r.complete = 0;
       if (!write) {
         HIPCHECK(hipMemcpyAsync(c.host_pinned, c.counters, 16, hipMemcpyDeviceToHost, c.stream));
         HIPCHECK(hipStreamSynchronize(c.stream));
@@ -738,6 +788,12 @@ struct Matcher {
         r.level[k - 1] = r.count;
         if (fuse) r.level[k - 2] = c.host_pinned[1];
       }
+    }
+    if (r.roots_done) {
+      // the last processed root, for callers that restate the sample
+      HIPCHECK(hipMemcpyAsync(c.host_pinned + 2, l0.rows + r.roots_done - 1, 4, hipMemcpyDeviceToHost, c.stream));
+      HIPCHECK(hipStreamSynchronize(c.stream));
+      r.last_root = (uint32_t)(c.host_pinned[2] & 0xFFFFFFFFu);
     }
     HIPCHECK(hipEventRecord(c.ev1, c.stream));
     HIPCHECK(hipEventSynchronize(c.ev1));
@@ -747,7 +803,35 @@ struct Matcher {
   }
 };
 
+// Plain streaming copy of n 16-byte words: the calibration kernel for HBM
+// byte counters (it moves exactly 16n bytes each way).
+__global__ __launch_bounds__(BLOCK) void k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                     uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLOCK)
+    dst[i] = src[i];
+}
+
 }  // namespace
+
+double copy_probe(Context& c, uint64_t bytes, uint32_t iters) {
+  HIPCHECK(hipSetDevice(c.device));
+  const uint64_t n = std::max<uint64_t>(bytes / 16, 1);
+  uint4 *a = nullptr, *b = nullptr;
+  dev_alloc(a, n);
+  dev_alloc(b, n);
+  HIPCHECK(hipMemsetAsync(a, 1, n * 16, c.stream));
+  const uint32_t grid = (uint32_t)std::min<uint64_t>((n + BLOCK - 1) / BLOCK, (uint64_t)c.cus * 16);
+  hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(BLOCK), 0, c.stream, a, b, n);  // warm
+  HIPCHECK(hipEventRecord(c.kev[0], c.stream));
+  for (uint32_t i = 0; i < iters; ++i) hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(BLOCK), 0, c.stream, a, b, n);
+  HIPCHECK(hipEventRecord(c.kev[1], c.stream));
+  HIPCHECK(hipEventSynchronize(c.kev[1]));
+  float ms = 0;
+  HIPCHECK(hipEventElapsedTime(&ms, c.kev[0], c.kev[1]));
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return (double)iters * 2.0 * (double)n * 16.0 / (ms / 1e3) / 1e9;
+}
 
 KPlan make_kplan(const Plan& p, bool symmetric) {
   KPlan kp;

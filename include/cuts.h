@@ -117,6 +117,11 @@ typedef struct {
    * row = data vertices in matching-order positions (ans.txt layout). */
   uint32_t *emb_rows;
   uint64_t emb_cap;
+  /* > 0 (counting runs only): stop once this much wall time has passed,
+   * checked between expansion chunks; a level-0 trunk cut short is rolled
+   * back, so the result covers exactly the first roots_done level-0
+   * vertices (level 0 is in ascending vertex order). */
+  double time_limit_ms;
 } cuts_match_options;
 
 typedef struct {
@@ -125,12 +130,15 @@ typedef struct {
   double device_ms;         /* first matcher kernel .. final count readback */
   uint64_t emb_written;     /* rows copied to emb_rows */
   uint32_t chunks;          /* expansion chunks run (automatic trunking) */
-  uint32_t reserved;
+  uint32_t complete;        /* 0 if time_limit_ms stopped the run early */
   /* CUTS_FLAG_PROFILE: per kernel class, summed HIP-event time, launches and
    * algorithmic (demand) bytes as defined next to each kernel */
   double kernel_ms[8];
   uint64_t kernel_launches[8];
   double kernel_bytes[8];
+  uint64_t roots_done;      /* level-0 vertices whose subtrees are counted */
+  uint32_t last_root;       /* data vertex of the last of them */
+  uint32_t reserved;
 } cuts_match_result;
 
 void cuts_match_options_default(cuts_match_options *o);
@@ -138,6 +146,11 @@ void cuts_match_options_default(cuts_match_options *o);
  * search / search_dfs_bfs_strategy (src/host_funcs.cu:39-171). */
 int cuts_match(cuts_context *ctx, const cuts_device_graph *dg, const cuts_plan *plan,
                const cuts_match_options *opt, cuts_match_result *res);
+
+/* Calibration (no reference counterpart): copies `bytes` bytes through a
+ * plain streaming kernel `iters` times and reports read+write GB/s.  Its
+ * dispatches are the known-size yardstick for HBM byte counters. */
+int cuts_copy_probe(cuts_context *ctx, uint64_t bytes, uint32_t iters, double *gbps);
 
 /* ---- reference-compatible entry points ------------------------------- */
 /* search(query, data, write_to_disk) and search_dfs_bfs_strategy(...,

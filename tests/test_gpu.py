@@ -151,6 +151,24 @@ def test_root_restriction_partitions(ctx):
     assert sum(ctx.match(dg, p, roots=(r, V, 4)).count for r in range(4)) == total
 
 
+@pytest.mark.parametrize("seed", [3, 9, 17])
+def test_time_limit_counts_exact_root_prefix(ctx, seed):
+    g, k, edges = random_case(seed)
+    dg = ctx.upload(g)
+    p = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges))
+    full = ctx.match(dg, p)
+    n0 = full.level_sizes[0]
+    part = ctx.match(dg, p, trunks=n0, time_limit_ms=0.05)
+    assert part.roots_done <= n0
+    if part.roots_done == 0:
+        assert part.count == 0 and not part.complete
+        return
+    again = ctx.match(dg, p, roots=(0, part.last_root + 1, 1))
+    assert again.level_sizes[0] == part.roots_done
+    assert part.count == again.count
+    assert part.complete == (part.roots_done == n0)
+
+
 CFG = load("config_counts.json")
 
 
