@@ -44,14 +44,25 @@ class _Opts(C.Structure):
     _fields_ = [("trunks", C.c_uint32), ("flags", C.c_uint32), ("budget_bytes", C.c_uint64),
                 ("root_begin", C.c_uint32), ("root_end", C.c_uint32), ("root_stride", C.c_uint32),
                 ("blocks_per_cu", C.c_uint32), ("emb_rows", _u32p), ("emb_cap", C.c_uint64),
-                ("time_limit_ms", C.c_double)]
+                ("time_limit_ms", C.c_double), ("max_work", C.c_uint64)]
 
 
 class _Result(C.Structure):
     _fields_ = [("count", C.c_uint64), ("level_sizes", C.c_uint64 * 16), ("device_ms", C.c_double),
                 ("emb_written", C.c_uint64), ("chunks", C.c_uint32), ("complete", C.c_uint32),
                 ("kernel_ms", C.c_double * 8), ("kernel_launches", C.c_uint64 * 8), ("kernel_bytes", C.c_double * 8),
-                ("roots_done", C.c_uint64), ("last_root", C.c_uint32), ("reserved", C.c_uint32)]
+                ("roots_done", C.c_uint64), ("last_root", C.c_uint32), ("work_capped", C.c_uint32),
+                ("work", C.c_uint64)]
+
+
+class _BatchOpts(C.Structure):
+    _fields_ = [("flags", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("vchunk", C.c_uint32),
+                ("no_graph", C.c_uint32), ("queue_cap", C.c_uint64), ("arena_bytes", C.c_uint64),
+                ("wait_limit_s", C.c_double)]
+
+
+class _BatchResult(C.Structure):
+    _fields_ = [("count", C.c_uint64), ("level_sizes", C.c_uint64 * 16), ("latency_ms", C.c_double)]
 
 
 _lib = None
@@ -93,6 +104,11 @@ def lib():
                                                        C.POINTER(C.c_uint64)]),
             "cuts_write_matches": (C.c_int, [C.c_char_p, vp, _u32p, C.c_uint64]),
             "cuts_copy_probe": (C.c_int, [vp, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
+            "cuts_batch_options_default": (None, [C.POINTER(_BatchOpts)]),
+            "cuts_batch_create": (C.c_int, [vp, vp, C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(_BatchOpts), pp]),
+            "cuts_batch_run": (C.c_int, [vp, C.POINTER(_BatchResult), C.POINTER(C.c_double),
+                                         C.POINTER(C.c_uint64)]),
+            "cuts_batch_free": (None, [vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -247,7 +263,9 @@ class MatchResult:
     kernels: Optional[dict] = None     # FLAG_PROFILE: class -> (ms, launches, bytes)
     complete: bool = True              # False if time_limit_ms stopped it between trunks
     roots_done: int = 0                # level-0 vertices whose subtrees were counted
-    last_root: int = 0                 # the last of them (ascending ids), when incomplete
+    last_root: int = 0                 # the last of them (ascending ids)
+    work_capped: bool = False          # max_work stopped the run; count invalid
+    work: int = 0                      # candidates examined (sum of scan totals)
 
 
 class DeviceGraph:
@@ -287,11 +305,12 @@ class Context:
 
     def match(self, dg: DeviceGraph, plan: Plan, trunks: int = 0, flags: int = 0, budget: int = 0,
               roots: Optional[tuple] = None, write: bool = False, emb_cap: int = 0,
-              blocks_per_cu: int = 0, time_limit_ms: float = 0.0) -> MatchResult:
+              blocks_per_cu: int = 0, time_limit_ms: float = 0.0, max_work: int = 0) -> MatchResult:
         o = _Opts()
         lib().cuts_match_options_default(C.byref(o))
         o.trunks, o.flags, o.budget_bytes, o.blocks_per_cu = trunks, flags, budget, blocks_per_cu
         o.time_limit_ms = time_limit_ms
+        o.max_work = max_work
         if roots is not None:
             o.root_begin, o.root_end, o.root_stride = roots
         rows = None
@@ -304,13 +323,55 @@ class Context:
         _check(lib().cuts_match(self._h, dg._h, plan._h, C.byref(o), C.byref(r)))
         res = MatchResult(int(r.count), [int(x) for x in r.level_sizes[: plan.k]], float(r.device_ms),
                           int(r.chunks), complete=bool(r.complete), roots_done=int(r.roots_done),
-                          last_root=int(r.last_root))
+                          last_root=int(r.last_root), work_capped=bool(r.work_capped), work=int(r.work))
         if write:
             res.rows = rows[: r.emb_written].copy()
         if o.flags & FLAG_PROFILE:
             res.kernels = {KERNEL_CLASSES[i]: (float(r.kernel_ms[i]), int(r.kernel_launches[i]),
                                                float(r.kernel_bytes[i])) for i in range(8)}
         return res
+
+
+@dataclass
+class BatchResult:
+    counts: List[int]
+    level_sizes: List[List[int]]
+    latency_ms: List[float]
+    device_ms: float
+    arena_bytes: int
+
+
+class Batch:
+    """Many queries against one resident data graph in one persistent,
+    hipGraph-captured launch (device work queue)."""
+
+    def __init__(self, ctx: Context, dg: DeviceGraph, plans: Sequence[Plan], flags: int = 0,
+                 blocks_per_cu: int = 0, vchunk: int = 0, use_graph: bool = True, queue_cap: int = 0,
+                 arena_bytes: int = 0, wait_limit_s: float = 0.0):
+        self.ctx, self.dg, self.plans = ctx, dg, list(plans)
+        o = _BatchOpts()
+        lib().cuts_batch_options_default(C.byref(o))
+        o.flags, o.blocks_per_cu, o.vchunk, o.no_graph = flags, blocks_per_cu, vchunk, 0 if use_graph else 1
+        o.queue_cap, o.arena_bytes, o.wait_limit_s = queue_cap, arena_bytes, wait_limit_s
+        arr = (C.c_void_p * len(self.plans))(*[p._h.value for p in self.plans])
+        h = C.c_void_p()
+        _check(lib().cuts_batch_create(ctx._h, dg._h, arr, len(self.plans), C.byref(o), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.cuts_batch_free(self._h)
+            self._h = None
+
+    def run(self) -> BatchResult:
+        n = len(self.plans)
+        res = (_BatchResult * n)()
+        ms = C.c_double()
+        ab = C.c_uint64()
+        _check(lib().cuts_batch_run(self._h, res, C.byref(ms), C.byref(ab)))
+        return BatchResult([int(r.count) for r in res],
+                           [[int(x) for x in r.level_sizes[: p.k]] for r, p in zip(res, self.plans)],
+                           [float(r.latency_ms) for r in res], float(ms.value), int(ab.value))
 
 
 def search(query_file: str, data_file: str, write_to_disk: bool = False) -> int:

@@ -9,7 +9,8 @@
 #include <memory>
 #include <sstream>
 
-#include "device.hpp"
+#include "batch.hpp"
+#include "mpnn.hpp"
 
 struct cuts_graph {
   cuts::Graph g;
@@ -22,6 +23,12 @@ struct cuts_context {
 };
 struct cuts_device_graph {
   cuts::DeviceGraph dg;
+};
+struct cuts_batch {
+  cuts::Batch b;
+};
+struct cuts_mpnn {
+  cuts::Mpnn m;
 };
 
 namespace {
@@ -104,6 +111,7 @@ cuts::MatchOpts to_opts(const cuts_match_options* o) {
   m.emb_rows = o->emb_rows;
   m.emb_cap = o->emb_cap;
   m.time_limit_ms = o->time_limit_ms;
+  m.max_work = o->max_work;
   return m;
 }
 
@@ -317,6 +325,8 @@ int cuts_match(cuts_context* ctx, const cuts_device_graph* dg, const cuts_plan* 
     res->complete = r.complete;
     res->roots_done = r.roots_done;
     res->last_root = r.last_root;
+    res->work_capped = r.work_capped;
+    res->work = r.work;
     for (int i = 0; i < 8; ++i) {
       res->kernel_ms[i] = r.kernel_ms[i];
       res->kernel_launches[i] = r.kernel_launches[i];
@@ -325,6 +335,114 @@ int cuts_match(cuts_context* ctx, const cuts_device_graph* dg, const cuts_plan* 
     return CUTS_OK;
   });
 }
+
+void cuts_batch_options_default(cuts_batch_options* o) {
+  if (o) std::memset(o, 0, sizeof(*o));
+}
+
+int cuts_batch_create(cuts_context* ctx, const cuts_device_graph* dg, const cuts_plan* const* plans, uint32_t n,
+                      const cuts_batch_options* opt, cuts_batch** out) {
+  return guard([&]() -> int {
+    if (!ctx || !dg || !plans || !out || n == 0) return fail(CUTS_E_ARG, "invalid argument");
+    std::vector<const cuts::Plan*> ps(n);
+    for (uint32_t i = 0; i < n; ++i) {
+      if (!plans[i]) return fail(CUTS_E_ARG, "null plan");
+      ps[i] = &plans[i]->p;
+    }
+    cuts::BatchOpts o;
+    if (opt) {
+      o.flags = opt->flags;
+      o.blocks_per_cu = opt->blocks_per_cu;
+      o.vchunk = opt->vchunk;
+      o.use_graph = opt->no_graph ? 0 : 1;
+      o.queue_cap = opt->queue_cap;
+      o.arena_bytes = opt->arena_bytes;
+      o.wait_limit_s = opt->wait_limit_s;
+    }
+    auto b = std::make_unique<cuts_batch>();
+    cuts::batch_create(ctx->c, const_cast<cuts::DeviceGraph&>(dg->dg), ps, o, b->b);
+    *out = b.release();
+    return CUTS_OK;
+  });
+}
+
+int cuts_batch_run(cuts_batch* b, cuts_batch_query_result* results, double* device_ms, uint64_t* arena_bytes) {
+  return guard([&]() -> int {
+    if (!b || !results) return fail(CUTS_E_ARG, "null argument");
+    std::vector<cuts::BatchQueryResult> r;
+    double ms = 0;
+    uint64_t words = 0;
+    cuts::batch_run(b->b, r, ms, words);
+    for (size_t i = 0; i < r.size(); ++i) {
+      results[i].count = r[i].count;
+      for (uint32_t l = 0; l < 16; ++l) results[i].level_sizes[l] = r[i].level[l];
+      results[i].latency_ms = r[i].latency_ms;
+    }
+    if (device_ms) *device_ms = ms;
+    if (arena_bytes) *arena_bytes = words * 4;
+    return CUTS_OK;
+  });
+}
+
+void cuts_batch_free(cuts_batch* b) { delete b; }
+
+uint32_t cuts_mpnn_num_params(void) { return (uint32_t)cuts::kMpnnParams; }
+
+int cuts_mpnn_create(cuts_context* ctx, const float* params, cuts_mpnn** out) {
+  return guard([&]() -> int {
+    if (!ctx || !params || !out) return fail(CUTS_E_ARG, "null argument");
+    auto m = std::make_unique<cuts_mpnn>();
+    cuts::mpnn_init(ctx->c, m->m, params);
+    *out = m.release();
+    return CUTS_OK;
+  });
+}
+
+int cuts_mpnn_get_params(cuts_mpnn* m, float* params) {
+  return guard([&]() -> int {
+    if (!m || !params) return fail(CUTS_E_ARG, "null argument");
+    cuts::mpnn_get(m->m, params);
+    return CUTS_OK;
+  });
+}
+
+int cuts_mpnn_set_params(cuts_mpnn* m, const float* params) {
+  return guard([&]() -> int {
+    if (!m || !params) return fail(CUTS_E_ARG, "null argument");
+    cuts::mpnn_set(m->m, params);
+    return CUTS_OK;
+  });
+}
+
+int cuts_mpnn_forward(cuts_mpnn* m, uint32_t n, const float* x, const uint16_t* adj, const uint32_t* meta, float* y) {
+  return guard([&]() -> int {
+    if (!m || (n && (!x || !adj || !meta || !y))) return fail(CUTS_E_ARG, "null argument");
+    cuts::mpnn_forward(m->m, n, x, adj, meta, y);
+    return CUTS_OK;
+  });
+}
+
+int cuts_mpnn_gradient(cuts_mpnn* m, uint32_t n, const float* x, const uint16_t* adj, const uint32_t* meta,
+                       const float* target, float* grad, double* loss) {
+  return guard([&]() -> int {
+    if (!m || !n || !x || !adj || !meta || !target) return fail(CUTS_E_ARG, "invalid argument");
+    const double l = cuts::mpnn_gradient(m->m, n, x, adj, meta, target, grad);
+    if (loss) *loss = l;
+    return CUTS_OK;
+  });
+}
+
+int cuts_mpnn_train_step(cuts_mpnn* m, uint32_t n, const float* x, const uint16_t* adj, const uint32_t* meta,
+                         const float* target, float lr, double* loss) {
+  return guard([&]() -> int {
+    if (!m || !n || !x || !adj || !meta || !target) return fail(CUTS_E_ARG, "invalid argument");
+    const double l = cuts::mpnn_train_step(m->m, n, x, adj, meta, target, lr);
+    if (loss) *loss = l;
+    return CUTS_OK;
+  });
+}
+
+void cuts_mpnn_free(cuts_mpnn* m) { delete m; }
 
 int cuts_copy_probe(cuts_context* ctx, uint64_t bytes, uint32_t iters, double* gbps) {
   return guard([&]() -> int {

@@ -87,6 +87,7 @@ struct MatchOpts {
   uint32_t* emb_rows = nullptr;
   uint64_t emb_cap = 0;
   double time_limit_ms = 0;
+  uint64_t max_work = 0;
 };
 struct MatchResult {
   uint64_t count = 0;
@@ -97,6 +98,8 @@ struct MatchResult {
   uint32_t complete = 1;
   uint64_t roots_done = 0;
   uint32_t last_root = 0;
+  uint32_t work_capped = 0;
+  uint64_t work = 0;
   double kernel_ms[8] = {0};
   uint64_t kernel_launches[8] = {0};
   double kernel_bytes[8] = {0};

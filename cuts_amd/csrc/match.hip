@@ -35,7 +35,7 @@
 #include <cstdio>
 #include <cstring>
 
-#include "device.hpp"
+#include "batch.hpp"
 
 namespace cuts {
 
@@ -540,6 +540,15 @@ struct Matcher {
     if (has_deadline && !aborted && std::chrono::steady_clock::now() >= deadline) aborted = true;
     return aborted;
   }
+  // deterministic work cap: the candidates examined so far
+  bool add_work(uint64_t w) {
+    r.work += w;
+    if (o.max_work && r.work > o.max_work) {
+      aborted = true;
+      r.work_capped = 1;
+    }
+    return aborted;
+  }
   void pbegin(bool dev_bytes) {
     if (!prof) return;
     if (dev_bytes) HIPCHECK(hipMemsetAsync(c.counters + 3, 0, 8, c.stream));
@@ -586,6 +595,7 @@ struct Matcher {
     if (n == 0) return;
     LevelBuf& in = c.lv[L - 1];
     if (L == k - 1 && closed_last) {
+      if (add_work(n)) return;
       const uint32_t grid = grid_for(n, BLOCK, grid_cap);
       pbegin(true);
 #define CF(LL) hipLaunchKernelGGL((k_closed_form<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, tbl, c.counters, (int)prof)
@@ -607,6 +617,7 @@ struct Matcher {
     }
     const uint64_t W = scan(in, n);
     if (W == 0) return;
+    if (add_work(W)) return;
     int mode = M_WRITE;
     if (L == k - 1 && !write) mode = M_COUNT;
     else if (L == k - 2 && fuse) mode = M_FUSED;
@@ -708,18 +719,7 @@ struct Matcher {
       const uint64_t per = nlev ? budget / nlev : budget;
       level_cap[l] = std::max<uint64_t>(per / ((l + 1) * 4 + 13), 1024);
     }
-    if (closed_last && kp.sc_thr > 1) {
-      auto it = dg.cnt_tables.find(kp.sc_thr);
-      if (it == dg.cnt_tables.end()) {
-        uint32_t* t = nullptr;
-        dev_alloc(t, dg.V + 1);
-        hipLaunchKernelGGL(k_cnt_table, dim3(grid_for(dg.V, BLOCK / 64, 65535)), dim3(BLOCK), 0, c.stream, g,
-                           kp.sc_thr, t);
-        HIPCHECK(hipGetLastError());
-        it = dg.cnt_tables.emplace(kp.sc_thr, t).first;
-      }
-      tbl = it->second;
-    }
+    if (closed_last && kp.sc_thr > 1) tbl = cnt_table(c, dg, kp.sc_thr);
     // level 0
     const uint32_t rb = o.root_stride ? o.root_begin : 0;
     const uint32_t re = o.root_stride ? std::min(o.root_end, dg.V) : dg.V;
@@ -788,6 +788,7 @@ struct Matcher {
         r.level[k - 1] = r.count;
         if (fuse) r.level[k - 2] = c.host_pinned[1];
       }
+      if (r.work_capped) r.count = 0;
     }
     if (r.roots_done) {
       // the last processed root, for callers that restate the sample
@@ -812,6 +813,20 @@ __global__ __launch_bounds__(BLOCK) void k_copy_probe(const uint4* __restrict__ 
 }
 
 }  // namespace
+
+const uint32_t* cnt_table(Context& c, DeviceGraph& dg, uint32_t thr) {
+  auto it = dg.cnt_tables.find(thr);
+  if (it == dg.cnt_tables.end()) {
+    uint32_t* t = nullptr;
+    dev_alloc(t, dg.V + 1);
+    const GView g{dg.off_out, dg.adj_out, dg.off_in, dg.adj_in, dg.sig, dg.V};
+    hipLaunchKernelGGL(k_cnt_table, dim3(grid_for(dg.V, BLOCK / 64, 65535)), dim3(BLOCK), 0, c.stream, g, thr, t);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(c.stream));
+    it = dg.cnt_tables.emplace(thr, t).first;
+  }
+  return it->second;
+}
 
 double copy_probe(Context& c, uint64_t bytes, uint32_t iters) {
   HIPCHECK(hipSetDevice(c.device));

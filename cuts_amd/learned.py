@@ -1,0 +1,349 @@
+"""Learned matching order.
+
+The reference orders a query with a greedy score (src/graph.cpp:62-93,
+src/score.cpp:18-99) and never measures the trie it builds.  Here the
+solver's own per-level table sizes (``MatchResult.level_sizes``: |T_l| for
+the prefix set of the first l+1 order positions) train a message-passing
+network (HIP kernels behind ``cuts_mpnn_*``) that predicts
+``log(1 + |T(S)|) / 10`` for any connected vertex set S of a query.  An
+order is then chosen by dynamic programming over connected prefix sets,
+minimising the predicted number of trie rows the matcher touches:
+
+    cost(order) = sum_{l <= k-2} |T(S_l)|  +  [last vertex has > 1 earlier
+                  neighbour] * |T(V_Q)|
+
+(with a single earlier neighbour the last level is counted in closed form
+and never enumerated).  The score order stays the fallback and the judge:
+the learned order is used only when the network predicts it to be at least
+``margin`` cheaper than the score order, and every learned run's count is
+compared with the score order's.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import Context, Graph, Plan, _check, _u32p, lib
+
+NODES, FEATS, HIDDEN = 16, 24, 64
+SCALE = 10.0  # targets are log(1 + size) / SCALE
+
+
+def _sig():
+    L = lib()
+    if not getattr(L, "_mpnn_sig", False):
+        vp = C.c_void_p
+        f32p = C.POINTER(C.c_float)
+        u16p = C.POINTER(C.c_uint16)
+        L.cuts_mpnn_num_params.restype = C.c_uint32
+        L.cuts_mpnn_num_params.argtypes = []
+        L.cuts_mpnn_create.argtypes = [vp, f32p, C.POINTER(C.c_void_p)]
+        L.cuts_mpnn_get_params.argtypes = [vp, f32p]
+        L.cuts_mpnn_set_params.argtypes = [vp, f32p]
+        L.cuts_mpnn_forward.argtypes = [vp, C.c_uint32, f32p, u16p, _u32p, f32p]
+        L.cuts_mpnn_gradient.argtypes = [vp, C.c_uint32, f32p, u16p, _u32p, f32p, f32p, C.POINTER(C.c_double)]
+        L.cuts_mpnn_train_step.argtypes = [vp, C.c_uint32, f32p, u16p, _u32p, f32p, C.c_float,
+                                           C.POINTER(C.c_double)]
+        L.cuts_mpnn_free.argtypes = [vp]
+        L.cuts_mpnn_free.restype = None
+        L._mpnn_sig = True
+    return L
+
+
+def num_params() -> int:
+    return int(_sig().cuts_mpnn_num_params())
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-style initialisation in the flat layout of cuts_amd/csrc/mpnn.hip."""
+    rng = np.random.default_rng(seed)
+    H, F = HIDDEN, FEATS
+    parts = []
+
+    def mat(fan_in, rows, cols, scale=1.0):
+        return (rng.standard_normal((rows, cols)) * math.sqrt(2.0 / fan_in) * scale).astype(np.float32).ravel()
+
+    parts += [mat(2 * F, F, H), mat(2 * F, F, H), np.zeros(H, np.float32)]
+    for _ in range(2):
+        parts += [mat(2 * H, H, H), mat(2 * H, H, H), np.zeros(H, np.float32)]
+    parts += [mat(2 * H, 2 * H, H), np.zeros(H, np.float32), mat(H, H, 1).ravel() * 0.5, np.zeros(1, np.float32)]
+    p = np.concatenate(parts).astype(np.float32)
+    assert p.size == num_params(), (p.size, num_params())
+    return p
+
+
+# ------------------------------------------------------------- features
+def graph_features(g: Graph) -> np.ndarray:
+    """16 data-graph features: fraction of vertices per log2-degree bin
+    [2^b, 2^(b+1)) for b = 0..13 (last bin open), log2(V)/24 and
+    log2(mean degree)/8."""
+    v = g.view()
+    deg = np.diff(v["out_off"].astype(np.int64))
+    V = max(int(v["V"]), 1)
+    f = np.zeros(16, np.float32)
+    nz = deg[deg > 0]
+    b = np.minimum(np.floor(np.log2(nz)).astype(np.int64), 13)
+    f[:14] = np.bincount(b, minlength=14)[:14] / V
+    f[14] = math.log2(V) / 24.0
+    f[15] = math.log2(max(deg.sum() / V, 1.0)) / 8.0
+    return f
+
+
+def adjacency(k: int, edges: Sequence[Tuple[int, int]]) -> List[int]:
+    adj = [0] * k
+    for a, b in edges:
+        if a != b:
+            adj[a] |= 1 << b
+            adj[b] |= 1 << a
+    return adj
+
+
+def node_features(k: int, adj: List[int], smask: int) -> np.ndarray:
+    x = np.zeros((k, 8), np.float32)
+    for v in range(k):
+        ins = (smask >> v) & 1
+        dq = bin(adj[v]).count("1")
+        ds = bin(adj[v] & smask).count("1")
+        tri = 0
+        nb = adj[v] & smask
+        m = nb
+        while m:
+            a = (m & -m).bit_length() - 1
+            m &= m - 1
+            tri += bin(adj[a] & m).count("1")
+        x[v] = (1.0, ins, dq / 4.0, ds / 4.0, ins * ds / 4.0, tri / 4.0, 1.0 if dq == 1 else 0.0, k / 12.0)
+    return x
+
+
+def encode(samples: Sequence[Tuple[int, List[int], int]], gfeat: np.ndarray):
+    """samples: (k, adj masks, S mask).  Returns x[n,16,24], adj[n,16], meta[n]."""
+    n = len(samples)
+    x = np.zeros((n, NODES, FEATS), np.float32)
+    A = np.zeros((n, NODES), np.uint16)
+    meta = np.zeros(n, np.uint32)
+    for i, (k, adj, sm) in enumerate(samples):
+        x[i, :k, :8] = node_features(k, adj, sm)
+        x[i, :k, 8:] = gfeat
+        A[i, :k] = adj
+        meta[i] = k | (sm << 16)
+    return x, A, meta
+
+
+def target(size: int) -> float:
+    return math.log1p(size) / SCALE
+
+
+# -------------------------------------------------------------- network
+class Mpnn:
+    """Table-size predictor; forward/backward/Adam in HIP (cuts_mpnn_*)."""
+
+    def __init__(self, ctx: Context, params: Optional[np.ndarray] = None, seed: int = 0):
+        L = _sig()
+        p = np.ascontiguousarray(init_params(seed) if params is None else params, dtype=np.float32)
+        h = C.c_void_p()
+        _check(L.cuts_mpnn_create(ctx._h, p.ctypes.data_as(C.POINTER(C.c_float)), C.byref(h)))
+        self._h = h
+        self.ctx = ctx
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _sig().cuts_mpnn_free(self._h)
+            self._h = None
+
+    @staticmethod
+    def _ptrs(x, A, meta):
+        return (x.ctypes.data_as(C.POINTER(C.c_float)), A.ctypes.data_as(C.POINTER(C.c_uint16)),
+                meta.ctypes.data_as(_u32p))
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(num_params(), np.float32)
+        _check(_sig().cuts_mpnn_get_params(self._h, p.ctypes.data_as(C.POINTER(C.c_float))))
+        return p
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(_sig().cuts_mpnn_set_params(self._h, p.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def forward(self, x, A, meta) -> np.ndarray:
+        y = np.zeros(len(meta), np.float32)
+        if len(meta) == 0:
+            return y
+        _check(_sig().cuts_mpnn_forward(self._h, len(meta), *self._ptrs(x, A, meta),
+                                        y.ctypes.data_as(C.POINTER(C.c_float))))
+        return y
+
+    def gradient(self, x, A, meta, t):
+        g = np.zeros(num_params(), np.float32)
+        loss = C.c_double()
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        _check(_sig().cuts_mpnn_gradient(self._h, len(meta), *self._ptrs(x, A, meta),
+                                         t.ctypes.data_as(C.POINTER(C.c_float)),
+                                         g.ctypes.data_as(C.POINTER(C.c_float)), C.byref(loss)))
+        return loss.value, g
+
+    def train_step(self, x, A, meta, t, lr: float) -> float:
+        loss = C.c_double()
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        _check(_sig().cuts_mpnn_train_step(self._h, len(meta), *self._ptrs(x, A, meta),
+                                           t.ctypes.data_as(C.POINTER(C.c_float)), lr, C.byref(loss)))
+        return loss.value
+
+    def fit(self, x, A, meta, t, epochs: int = 30, batch: int = 4096, lr: float = 2e-3, seed: int = 0,
+            log=None) -> List[float]:
+        rng = np.random.default_rng(seed)
+        n = len(meta)
+        hist = []
+        for ep in range(epochs):
+            perm = rng.permutation(n)
+            tot, cnt = 0.0, 0
+            for b0 in range(0, n, batch):
+                idx = perm[b0:b0 + batch]
+                l = self.train_step(np.ascontiguousarray(x[idx]), np.ascontiguousarray(A[idx]),
+                                    np.ascontiguousarray(meta[idx]), np.ascontiguousarray(t[idx]), lr)
+                tot += l * len(idx)
+                cnt += len(idx)
+            hist.append(tot / cnt)
+            if log:
+                log(f"epoch {ep + 1}/{epochs}: train mse {hist[-1]:.5f}")
+        return hist
+
+
+# --------------------------------------------------------- order search
+def connected_subsets(k: int, adj: List[int]) -> List[int]:
+    """All non-empty vertex sets that induce a connected subgraph."""
+    out = []
+    for sm in range(1, 1 << k):
+        start = (sm & -sm).bit_length() - 1
+        seen = 1 << start
+        frontier = seen
+        while frontier:
+            v = (frontier & -frontier).bit_length() - 1
+            frontier &= frontier - 1
+            new = adj[v] & sm & ~seen
+            seen |= new
+            frontier |= new
+        if seen == sm:
+            out.append(sm)
+    return out
+
+
+def order_cost(order: Sequence[int], adj: List[int], f: Dict[int, float]) -> float:
+    """Predicted rows touched: sum of |T| over prefixes of length <= k-1, plus
+    |T(V_Q)| when the last vertex has more than one earlier neighbour."""
+    k = len(order)
+    sm, cost = 0, 0.0
+    for l, v in enumerate(order):
+        sm |= 1 << v
+        if l <= k - 2:
+            cost += f[sm]
+    last = order[-1]
+    if bin(adj[last] & (sm & ~(1 << last))).count("1") > 1:
+        cost += f[sm]
+    return cost
+
+
+def best_order(k: int, adj: List[int], f: Dict[int, float]) -> Tuple[List[int], float]:
+    """Exact minimum of order_cost over connected orders (every prefix
+    connected), by dynamic programming over connected subsets."""
+    full = (1 << k) - 1
+    best: Dict[int, Tuple[float, int]] = {}
+    for sm in sorted(f, key=lambda m: bin(m).count("1")):
+        n = bin(sm).count("1")
+        if n == 1:
+            best[sm] = (f[sm], -1)
+            continue
+        if sm == full:
+            continue
+        cand = None
+        m = sm
+        while m:
+            v = (m & -m).bit_length() - 1
+            m &= m - 1
+            rest = sm & ~(1 << v)
+            if rest in best and adj[v] & rest:
+                c = best[rest][0] + f[sm]
+                if cand is None or c < cand[0]:
+                    cand = (c, v)
+        if cand is not None:
+            best[sm] = cand
+    # last vertex: closed form when it has exactly one earlier neighbour
+    top = None
+    for v in range(k):
+        rest = full & ~(1 << v)
+        if rest in best and adj[v] & rest:
+            c = best[rest][0] + (f[full] if bin(adj[v] & rest).count("1") > 1 else 0.0)
+            if top is None or c < top[0]:
+                top = (c, v)
+    order = [top[1]]
+    sm = full & ~(1 << top[1])
+    while sm:
+        v = best[sm][1]
+        if v < 0:
+            order.append((sm & -sm).bit_length() - 1)
+            break
+        order.append(v)
+        sm &= ~(1 << v)
+    order.reverse()
+    return order, top[0]
+
+
+@dataclass
+class OrderChoice:
+    order: List[int]
+    learned: bool          # False: fell back to the score order
+    predicted_cost: float
+    predicted_heuristic_cost: float
+
+
+def choose_orders(net: Mpnn, queries: Sequence[Tuple[int, List[Tuple[int, int]]]], gfeat: np.ndarray,
+                  heuristic: Sequence[List[int]], margin: float = 0.25) -> List[OrderChoice]:
+    """One forward launch over every connected subset of every query, then
+    the DP per query.  Falls back to the score order unless the network
+    predicts at least `margin` (relative) fewer rows."""
+    samples, spans = [], []
+    adjs = []
+    for k, edges in queries:
+        adj = adjacency(k, edges)
+        adjs.append(adj)
+        subs = connected_subsets(k, adj)
+        spans.append((len(samples), subs))
+        samples += [(k, adj, sm) for sm in subs]
+    x, A, meta = encode(samples, gfeat)
+    y = net.forward(x, A, meta)
+    sizes = np.expm1(np.clip(y.astype(np.float64), 0, None) * SCALE)
+    out = []
+    for (k, edges), adj, (s0, subs), h in zip(queries, adjs, spans, heuristic):
+        f = {sm: float(sizes[s0 + i]) for i, sm in enumerate(subs)}
+        order, c = best_order(k, adj, f)
+        ch = order_cost(h, adj, f)
+        if c < (1.0 - margin) * ch:
+            out.append(OrderChoice(order, True, c, ch))
+        else:
+            out.append(OrderChoice(list(h), False, ch, ch))
+    return out
+
+
+# ------------------------------------------------------------- training
+def table_samples(k: int, edges, order: Sequence[int], level_sizes: Sequence[int]):
+    """(k, adj, S mask, |T(S)|) for every prefix of a completed run."""
+    adj = adjacency(k, edges)
+    out, sm = [], 0
+    for l, v in enumerate(order):
+        sm |= 1 << v
+        out.append((k, adj, sm, int(level_sizes[l])))
+    return out
+
+
+def random_connected_order(k: int, adj: List[int], rng) -> List[int]:
+    start = int(rng.integers(k))
+    order, sm = [start], 1 << start
+    while len(order) < k:
+        front = [v for v in range(k) if not (sm >> v & 1) and adj[v] & sm]
+        v = int(front[int(rng.integers(len(front)))])
+        order.append(v)
+        sm |= 1 << v
+    return order

@@ -97,3 +97,53 @@ def csr_fingerprint(g: Graph) -> str:
         h ^= int(w)
         h = (h * 0x100000001B3) & 0xFFFFFFFFFFFFFFFF
     return f"{h:016x}:{len(data)}"
+
+
+# ---------------------------------------------------------------- batches
+def batch_candidates(g: Graph, seed: int = 0):
+    """Endless seeded stream of random-walk queries: k uniform in [4, 12],
+    edge/vertex ratio uniform in [1.0, 2.5] (edges clamped to
+    [k-1, k(k-1)/2]; a walk's induced subgraph may hold fewer)."""
+    rng = np.random.default_rng(seed)
+    i = 0
+    while True:
+        k = int(rng.integers(4, 13))
+        ratio = float(rng.uniform(1.0, 2.5))
+        ne = max(k - 1, min(int(round(ratio * k)), k * (k - 1) // 2))
+        edges, _ = g.walk_query(k, ne, seed * 1_000_003 + i)
+        yield i, k, ne, [tuple(map(int, e)) for e in edges.tolist()]
+        i += 1
+
+
+# Admission rule for fixed query sets: the heuristic-order run must finish
+# within MAX_WORK candidates and materialise at most MAX_ROWS trie rows
+# (levels 1..k-3; the last two are fused).  Sparse 12-vertex queries on a
+# power-law graph have counts beyond 2^64, so some rule is unavoidable; this
+# one is deterministic (work and level sizes do not depend on timing).
+MAX_WORK = 1 << 31
+MAX_ROWS = 1 << 21
+
+
+def materialised_rows(level_sizes) -> int:
+    k = len(level_sizes)
+    return int(sum(level_sizes[1:max(1, k - 2)]))
+
+
+def admit(ctx, dg, g: Graph, n: int, seed: int = 0, max_work: int = MAX_WORK, max_rows: int = MAX_ROWS,
+          log=None):
+    """First n candidates of batch_candidates(g, seed) that pass the rule."""
+    from . import Plan
+    out, tried = [], 0
+    for i, k, ne, edges in batch_candidates(g, seed):
+        tried += 1
+        p = Plan(Graph.from_edges(k, edges))
+        r = ctx.match(dg, p, max_work=max_work)
+        ok = not r.work_capped and materialised_rows(r.level_sizes) <= max_rows
+        if ok:
+            out.append(dict(index=i, k=k, edges=edges, count=r.count, level_sizes=r.level_sizes, work=r.work,
+                            order=p.order))
+            if log:
+                log(f"admitted {len(out)}/{n} (candidate {i}, k={k}, count={r.count})")
+            if len(out) == n:
+                break
+    return out, tried

@@ -122,6 +122,11 @@ typedef struct {
    * back, so the result covers exactly the first roots_done level-0
    * vertices (level 0 is in ascending vertex order). */
   double time_limit_ms;
+  /* > 0: give up once the expansion work (sum of the per-level scan totals,
+   * i.e. candidates examined) exceeds this; deterministic, used to admit
+   * queries into fixed workloads.  The result then has work_capped = 1 and
+   * no valid count. */
+  uint64_t max_work;
 } cuts_match_options;
 
 typedef struct {
@@ -138,7 +143,8 @@ typedef struct {
   double kernel_bytes[8];
   uint64_t roots_done;      /* level-0 vertices whose subtrees are counted */
   uint32_t last_root;       /* data vertex of the last of them */
-  uint32_t reserved;
+  uint32_t work_capped;     /* 1 if max_work stopped the run (count invalid) */
+  uint64_t work;            /* candidates examined (sum of scan totals) */
 } cuts_match_result;
 
 void cuts_match_options_default(cuts_match_options *o);
@@ -146,6 +152,59 @@ void cuts_match_options_default(cuts_match_options *o);
  * search / search_dfs_bfs_strategy (src/host_funcs.cu:39-171). */
 int cuts_match(cuts_context *ctx, const cuts_device_graph *dg, const cuts_plan *plan,
                const cuts_match_options *opt, cuts_match_result *res);
+
+/* ---- batched matching (no reference counterpart) --------------------- */
+/* Many queries against one resident data graph in one persistent launch
+ * (device work queue of (query, level, row chunk) items, captured once in a
+ * hipGraph and replayed).  The reference runs one query per process with a
+ * host sync per level (src/host_funcs.cu:141-157). */
+typedef struct cuts_batch cuts_batch;
+typedef struct {
+  uint32_t flags;         /* CUTS_FLAG_NO_SHORTCUT / CUTS_FLAG_NO_FUSE */
+  uint32_t blocks_per_cu; /* persistent grid = CUs x this, 1..4; 0 = 2 */
+  uint32_t vchunk;        /* level-0 vertices per initial item; 0 = 4096 */
+  uint32_t no_graph;      /* 1: plain launches instead of a hipGraph */
+  uint64_t queue_cap;     /* work-queue slots; 0 = 4M */
+  uint64_t arena_bytes;   /* trie-row arena; 0 = 8 GiB */
+  double wait_limit_s;    /* bound on any wait inside the kernel; 0 = 300 */
+} cuts_batch_options;
+typedef struct {
+  uint64_t count;
+  uint64_t level_sizes[16];
+  double latency_ms;      /* launch start .. this query's last item done */
+} cuts_batch_query_result;
+void cuts_batch_options_default(cuts_batch_options *o);
+/* The context, the device graph and the plans must outlive the batch. */
+int cuts_batch_create(cuts_context *ctx, const cuts_device_graph *dg, const cuts_plan *const *plans, uint32_t n,
+                      const cuts_batch_options *opt, cuts_batch **out);
+/* results: n entries.  Fails with CUTS_E_NOMEM if the queue or arena filled
+ * up (counts are then invalid). */
+int cuts_batch_run(cuts_batch *b, cuts_batch_query_result *results, double *device_ms, uint64_t *arena_bytes);
+void cuts_batch_free(cuts_batch *b);
+
+/* ---- table-size predictor (no reference counterpart) ------------------ */
+/* A 3-layer message-passing network (hidden 64, BF16 weights, FP32
+ * accumulation) mapping (query graph, prefix set S, data-graph features) to
+ * y ~ log(1 + |T(S)|) / 10, where |T(S)| is the trie table size of a prefix
+ * with vertex set S.  Forward, backward and Adam run in HIP kernels.
+ * A sample is k <= 16 vertices: x[16][24] input features (rows >= k
+ * ignored), adj[16] neighbour bit masks, meta = k | (S bit mask << 16). */
+enum { CUTS_MPNN_NODES = 16, CUTS_MPNN_FEATS = 24, CUTS_MPNN_HIDDEN = 64 };
+typedef struct cuts_mpnn cuts_mpnn;
+uint32_t cuts_mpnn_num_params(void);
+/* params: cuts_mpnn_num_params() floats (FP32 master copy). */
+int cuts_mpnn_create(cuts_context *ctx, const float *params, cuts_mpnn **out);
+int cuts_mpnn_get_params(cuts_mpnn *m, float *params);
+int cuts_mpnn_set_params(cuts_mpnn *m, const float *params);
+int cuts_mpnn_forward(cuts_mpnn *m, uint32_t n, const float *x, const uint16_t *adj, const uint32_t *meta, float *y);
+/* Mean squared error over the n samples and its gradient w.r.t. the BF16
+ * weights (grad may be NULL). */
+int cuts_mpnn_gradient(cuts_mpnn *m, uint32_t n, const float *x, const uint16_t *adj, const uint32_t *meta,
+                       const float *target, float *grad, double *loss);
+/* gradient + one Adam step (beta 0.9/0.999, eps 1e-8). */
+int cuts_mpnn_train_step(cuts_mpnn *m, uint32_t n, const float *x, const uint16_t *adj, const uint32_t *meta,
+                         const float *target, float lr, double *loss);
+void cuts_mpnn_free(cuts_mpnn *m);
 
 /* Calibration (no reference counterpart): copies `bytes` bytes through a
  * plain streaming kernel `iters` times and reports read+write GB/s.  Its

@@ -1,0 +1,161 @@
+"""Learned matching order: the FP64 reference network is pinned by finite
+differences; the HIP forward/backward kernels are checked against it; the
+order DP is checked against brute force; training reduces the loss; and a
+network trained on the solver's own logged tables chooses orders that touch
+fewer trie rows than the score order while giving identical counts."""
+import itertools
+import math
+import os
+import random
+
+import numpy as np
+import pytest
+
+import cuts_amd
+from cuts_amd import learned as LN
+
+torch = pytest.importorskip("torch")
+from oracle import mpnn_ref  # noqa: E402
+
+
+def random_samples(n, seed):
+    rng = np.random.default_rng(seed)
+    samples = []
+    for _ in range(n):
+        k = int(rng.integers(2, 13))
+        edges = [(int(rng.integers(0, v)), v) for v in range(1, k)]
+        edges += [tuple(map(int, rng.choice(k, 2, replace=False))) for _ in range(int(rng.integers(0, k)))]
+        adj = LN.adjacency(k, edges)
+        sm = int(rng.integers(1, 1 << k))
+        samples.append((k, adj, sm))
+    gfeat = rng.random(16).astype(np.float32) * 0.3
+    x, A, meta = LN.encode(samples, gfeat)
+    t = rng.random(n).astype(np.float32) * 3
+    return x, A, meta, t
+
+
+def test_reference_gradient_matches_finite_differences():
+    x, A, meta, t = random_samples(3, 1)
+    p = LN.init_params(3).astype(np.float64)
+    p = mpnn_ref.bf16_round(p)  # exactly representable start point
+    _, g, _ = mpnn_ref.loss_and_grad(p, x, A, meta, t)
+    rng = np.random.default_rng(0)
+    idx = list(rng.choice(len(p), 25, replace=False)) + [len(p) - 1, len(p) - 2]
+    tp = torch.tensor(p)
+    for i in idx:
+        h = 1e-6
+        a, b = tp.clone(), tp.clone()
+        a[i] += h
+        b[i] -= h
+        la = float(((mpnn_ref.forward(a, x, A, meta) - torch.tensor(t, dtype=torch.float64)) ** 2).mean())
+        lb = float(((mpnn_ref.forward(b, x, A, meta) - torch.tensor(t, dtype=torch.float64)) ** 2).mean())
+        fd = (la - lb) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(g[i]), (i, fd, g[i])
+
+
+def brute_best(k, adj, f):
+    best = None
+    for perm in itertools.permutations(range(k)):
+        ok, sm = True, 1 << perm[0]
+        for v in perm[1:]:
+            if not adj[v] & sm:
+                ok = False
+                break
+            sm |= 1 << v
+        if ok:
+            c = LN.order_cost(list(perm), adj, f)
+            best = c if best is None else min(best, c)
+    return best
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_order_dp_is_exact(seed):
+    rng = random.Random(seed)
+    k = rng.randint(2, 7)
+    edges = [(rng.randrange(v), v) for v in range(1, k)] + [tuple(rng.sample(range(k), 2)) for _ in range(rng.randint(0, 4))]
+    adj = LN.adjacency(k, edges)
+    f = {sm: rng.uniform(1, 1000) for sm in LN.connected_subsets(k, adj)}
+    order, cost = LN.best_order(k, adj, f)
+    assert sorted(order) == list(range(k))
+    assert math.isclose(cost, LN.order_cost(order, adj, f))
+    assert math.isclose(cost, brute_best(k, adj, f))
+
+
+def test_connected_subset_counts():
+    for k in range(1, 9):
+        path = LN.adjacency(k, [(i, i + 1) for i in range(k - 1)])
+        assert len(LN.connected_subsets(k, path)) == k * (k + 1) // 2
+        full = LN.adjacency(k, [(i, j) for i in range(k) for j in range(i + 1, k)])
+        assert len(LN.connected_subsets(k, full)) == 2 ** k - 1
+
+
+@pytest.mark.gpu
+def test_mpnn_forward_and_gradient_match_fp64_reference(ctx):
+    x, A, meta, t = random_samples(40, 2)
+    p = LN.init_params(5)
+    net = LN.Mpnn(ctx, p)
+    y = net.forward(x, A, meta)
+    loss, g = net.gradient(x, A, meta, t)
+    rl, rg, ry = mpnn_ref.loss_and_grad(p, x, A, meta, t)
+    # FP32 accumulation against FP64: tolerances relative to the magnitudes
+    assert np.allclose(y, ry, rtol=1e-4, atol=1e-4)
+    assert abs(loss - rl) <= 1e-4 * max(1.0, rl)
+    assert np.max(np.abs(g - rg)) <= 1e-3 * np.max(np.abs(rg)) + 1e-6
+
+
+@pytest.mark.gpu
+def test_mpnn_training_reduces_loss(ctx):
+    x, A, meta, _ = random_samples(2000, 3)
+    # a learnable target: a fixed function of the subset size and degrees
+    t = (np.array([bin(int(m) >> 16).count("1") for m in meta]) * 0.3 + x[:, :, 2].sum(1) * 0.1).astype(np.float32)
+    net = LN.Mpnn(ctx, seed=1)
+    hist = net.fit(x, A, meta, t, epochs=25, batch=500, lr=3e-3)
+    assert hist[-1] < 0.2 * hist[0]
+
+
+@pytest.mark.gpu
+def test_learned_orders_touch_fewer_rows_and_count_the_same(ctx):
+    g = cuts_amd.Graph.chung_lu(4096, 32768, 2.3, 0, 11)
+    dg = ctx.upload(g)
+    gfeat = LN.graph_features(g)
+    rng = np.random.default_rng(0)
+
+    def queries(seed, n):
+        out = []
+        for i, k, ne, edges in cuts_amd.workloads.batch_candidates(g, seed):
+            if not 5 <= k <= 8:
+                continue
+            p = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges))
+            r = ctx.match(dg, p, max_work=1 << 27)
+            if not r.work_capped:
+                out.append((k, edges, p, r))
+            if len(out) == n:
+                return out
+
+    train = queries(1, 200)
+    samples, targets = [], []
+    for k, edges, p, r in train:
+        runs = [(p.order, r)]
+        for _ in range(2):
+            o = LN.random_connected_order(k, LN.adjacency(k, edges), rng)
+            rr = ctx.match(dg, cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges), o), max_work=1 << 27)
+            if not rr.work_capped:
+                runs.append((o, rr))
+        for o, rr in runs:
+            for kk, adj, sm, size in LN.table_samples(k, edges, o, rr.level_sizes):
+                samples.append((kk, adj, sm))
+                targets.append(LN.target(size))
+    x, A, meta = LN.encode(samples, gfeat)
+    net = LN.Mpnn(ctx, seed=2)
+    hist = net.fit(x, A, meta, np.asarray(targets, np.float32), epochs=40, batch=1024, lr=2e-3)
+    assert hist[-1] < hist[0]
+
+    test = queries(2, 60)
+    choices = LN.choose_orders(net, [(k, e) for k, e, _, _ in test], gfeat, [p.order for _, _, p, _ in test])
+    rows_h = rows_l = 0
+    for (k, edges, p, r), ch in zip(test, choices):
+        rl = ctx.match(dg, cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges), ch.order))
+        assert rl.count == r.count
+        rows_h += sum(r.level_sizes[: k - 1])
+        rows_l += sum(rl.level_sizes[: k - 1])
+    assert rows_l < rows_h
