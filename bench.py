@@ -14,6 +14,16 @@ the device.  The CPU baseline times the OpenMP port of the same trie
 algorithm (oracle/, test infrastructure) on a bounded sample of level 0.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Other workloads (one JSON line each; the default above is the headline):
+    --workload chunglu4k   triangle + 2-edge tail on Chung-Lu n=4,096 (full count)
+    --workload rmat20      5-cycle + chord + 2 tails on R-MAT scale 20: the full
+                           count exceeds 2^64, so the run is time-limited and
+                           reports the exact count of a root prefix
+    --workload batch256    256 random-walk queries, one persistent hipGraph
+                           launch, learned orders, autotuned launch shape
+    --workload learned     train the table-size network on R-MAT scale 18,
+                           evaluate learned vs score order on R-MAT scale 20
 """
 import argparse
 import json
@@ -65,6 +75,266 @@ def traffic_for(kernel):
     return None if e is None else e.get("bytes_per_dispatch")
 
 
+def emit(line):
+    print(json.dumps(line), flush=True)
+
+
+def log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+def base_line(metric, value, unit, steps, warmup, ms_per_step, workload, extra):
+    d = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+         "dtype": "u32", "data": "synthetic (seeded generators; queries located by seeded random walk)",
+         "config": {"workload": workload}}
+    d["config"].update(extra)
+    return d
+
+
+def run_chunglu4k(args):
+    import cuts_amd
+    from cuts_amd import workloads as W
+    ctx = cuts_amd.Context(0)
+    dg = ctx.upload(W.config1_graph())
+    p = cuts_amd.Plan(W.query_graph(W.TRI_TAIL2))
+    for _ in range(args.warmup):
+        ctx.match(dg, p)
+    ms, counts = [], []
+    for _ in range(args.steps):
+        r = ctx.match(dg, p)
+        ms.append(r.device_ms)
+        counts.append(r.count)
+    emit(base_line("exact embeddings/sec + p50 time/query", sum(counts) / (sum(ms) / 1e3), "embeddings/s",
+                   args.steps, args.warmup, sum(ms) / args.steps,
+                   "chung-lu n=4096 m=32768 gamma=2.3 seed=0; triangle+2-edge-tail; heuristic order",
+                   {"count": counts[0], "p50_ms_per_query": statistics.median(ms)}))
+
+
+def run_rmat20(args, limit_s=20.0):
+    import cuts_amd
+    from cuts_amd import workloads as W
+    g = W.config3_graph()
+    ctx = cuts_amd.Context(0)
+    dg = ctx.upload(g)
+    p = cuts_amd.Plan(W.query_graph(W.C5CHORD_2TAIL))
+    roots = (1 << 19, 1 << 20, 97)
+    log(f"R-MAT 20 uploaded ({g.arcs} arcs); roots {roots}, {limit_s}s limit, one root per trunk")
+    r = ctx.match(dg, p, roots=roots, trunks=1 << 20, time_limit_ms=limit_s * 1e3, flags=cuts_amd.FLAG_PROFILE)
+    dname, (dms, dn, dbytes) = max(r.kernels.items(), key=lambda kv: kv[1][0])
+    ach = dbytes / dn / (dms / dn / 1e3) / 1e9 if dn else 0.0
+    line = base_line("exact embeddings/sec (time-limited exact root prefix)", r.count / (r.device_ms / 1e3),
+                     "embeddings/s", 1, 0, r.device_ms,
+                     "rmat scale=20 ef=16 a,b,c=.57,.19,.19 seed=0; 5-cycle+chord+2-tails; heuristic order",
+                     {"roots": list(roots), "roots_done": r.roots_done, "last_root": r.last_root,
+                      "count_of_root_prefix": r.count, "complete": r.complete, "level_sizes": r.level_sizes,
+                      "time_limit_s": limit_s})
+    line["roofline"] = {"bound": "hbm", "kernel": dname, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": ach / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": dms / max(dn, 1),
+                        "launches": dn}
+    emit(line)
+
+
+def load_batch_fixture():
+    path = os.path.join(ROOT, "tests", "golden", "batch_queries.json")
+    with open(path) as f:
+        return json.load(f)
+
+
+def run_batch256(args):
+    import numpy as np
+    import cuts_amd
+    from cuts_amd import autotune, learned as LN, workloads as W
+    g = W.config2_graph()
+    ctx = cuts_amd.Context(0)
+    dg = ctx.upload(g)
+    fx = load_batch_fixture()
+    qs = [(q["k"], [tuple(e) for e in q["edges"]]) for q in fx["queries"]]
+    heur = [cuts_amd.Plan(cuts_amd.Graph.from_edges(k, e)) for k, e in qs]
+    # heuristic order, one query at a time
+    for p in heur[:8]:
+        ctx.match(dg, p)
+    single = [ctx.match(dg, p) for p in heur]
+    s_ms = [r.device_ms for r in single]
+    counts = [r.count for r in single]
+    assert counts == [q["count"] for q in fx["queries"]], "single-query counts differ from the fixture"
+    log(f"single heuristic: p50 {statistics.median(s_ms):.3f} ms, total {sum(s_ms):.1f} ms")
+    # learned orders from a network trained on tables logged on other queries of the same graph
+    net, info = train_on_graph(ctx, dg, g, seed=1, n_queries=args.train_queries, log=log)
+    gfeat = LN.graph_features(g)
+    choices = []
+    for i in range(0, len(qs), 32):
+        choices += LN.choose_orders(net, qs[i:i + 32], gfeat, [p.order for p in heur[i:i + 32]])
+    plans = [cuts_amd.Plan(cuts_amd.Graph.from_edges(k, e), ch.order) for (k, e), ch in zip(qs, choices)]
+    fallback = sum(1 for c in choices if not c.learned) / len(choices)
+    key = f"batch256:{W.csr_fingerprint(g)}:learned"
+    tb = autotune.tune_batch(ctx, dg, plans, key, use_cache=False, log=log)
+    best = tb["best"]
+    b = cuts_amd.Batch(ctx, dg, plans, blocks_per_cu=best["blocks_per_cu"], vchunk=best["vchunk"],
+                       arena_bytes=32 << 30)
+    runs = []
+    for _ in range(args.warmup):
+        b.run()
+    for _ in range(args.steps):
+        r = b.run()
+        assert r.counts == counts, "batched learned-order counts differ from single-query score-order counts"
+        runs.append(r)
+    bh = cuts_amd.Batch(ctx, dg, heur, blocks_per_cu=best["blocks_per_cu"], vchunk=best["vchunk"],
+                        arena_bytes=32 << 30)
+    rh = [bh.run() for _ in range(2)][-1]
+    assert rh.counts == counts
+    ms = statistics.median([r.device_ms for r in runs])
+    lat = runs[-1].latency_ms
+    cum = list(np.cumsum(s_ms))
+    amort = ms / len(qs)
+    emit(base_line("exact embeddings/sec + p50 time/query; learned vs heuristic order, same GPU",
+                   sum(counts) / (ms / 1e3), "embeddings/s", args.steps, args.warmup, ms,
+                   "256 random-walk queries (k 4-12, edge/vertex 1.0-2.5) on chung-lu n=36692; one persistent "
+                   "hipGraph launch; learned orders; autotuned launch shape",
+                   {"queries": len(qs), "sum_count": sum(counts),
+                    "batch_learned_ms": ms, "batch_heuristic_ms": rh.device_ms,
+                    "amortised_ms_per_query_batch_learned": amort,
+                    "p50_ms_single_heuristic": statistics.median(s_ms),
+                    "p99_ms_single_heuristic": sorted(s_ms)[int(0.99 * len(s_ms))],
+                    "speedup_p50_amortised_batch_learned_vs_single_heuristic": statistics.median(s_ms) / amort,
+                    "p50_latency_ms_batch_learned_all_submitted_at_t0": statistics.median(lat),
+                    "p99_latency_ms_batch_learned_all_submitted_at_t0": sorted(lat)[int(0.99 * len(lat))],
+                    "p50_latency_ms_single_heuristic_sequential_all_submitted_at_t0": statistics.median(cum),
+                    "speedup_p50_latency": statistics.median(cum) / statistics.median(lat),
+                    "fallback_rate": fallback, "tuned": best, "training": info}))
+
+
+def train_on_graph(ctx, dg, g, seed, n_queries, log, kmin=5, kmax=10, max_work=1 << 27):
+    """Tables logged by the solver itself on seeded random-walk queries:
+    score order plus one random connected order per query."""
+    import numpy as np
+    import cuts_amd
+    from cuts_amd import learned as LN, workloads as W
+    rng = np.random.default_rng(seed)
+    samples, targets, used = [], [], 0
+    t0 = time.perf_counter()
+    for i, k, ne, edges in W.batch_candidates(g, seed):
+        if not kmin <= k <= kmax:
+            continue
+        adj = LN.adjacency(k, edges)
+        q = cuts_amd.Graph.from_edges(k, edges)
+        for order in (None, LN.random_connected_order(k, adj, rng)):
+            p = cuts_amd.Plan(q, order)
+            r = ctx.match(dg, p, max_work=max_work)
+            if r.work_capped:
+                continue
+            for kk, a, sm, size in LN.table_samples(k, edges, p.order, r.level_sizes):
+                samples.append((kk, a, sm))
+                targets.append(LN.target(size))
+        used += 1
+        if used % 500 == 0:
+            log(f"logged tables of {used} training queries ({len(samples)} samples)")
+        if used == n_queries:
+            break
+    t1 = time.perf_counter()
+    x, A, meta = LN.encode(samples, LN.graph_features(g))
+    net = LN.Mpnn(ctx, seed=seed)
+    hist = net.fit(x, A, meta, np.asarray(targets, np.float32), epochs=30, batch=4096, lr=2e-3)
+    t2 = time.perf_counter()
+    log(f"trained on {len(samples)} samples: mse {hist[0]:.4f} -> {hist[-1]:.4f}")
+    return net, {"queries": used, "samples": len(samples), "mse_first": hist[0], "mse_last": hist[-1],
+                 "log_s": t1 - t0, "train_s": t2 - t1}
+
+
+def run_learned(args):
+    """Train on R-MAT scale-18 graphs, evaluate on the R-MAT scale-20 graph."""
+    import numpy as np
+    import cuts_amd
+    from cuts_amd import learned as LN, workloads as W
+    ctx = cuts_amd.Context(0)
+    samples, targets, info = [], [], []
+    rng = np.random.default_rng(0)
+    per_graph = args.train_queries // 4
+    for gs in (1, 2, 3, 4):
+        g18 = W.rmat18_graph(gs)
+        dg18 = ctx.upload(g18)
+        gf = LN.graph_features(g18)
+        used = 0
+        for i, k, ne, edges in W.batch_candidates(g18, 100 + gs):
+            if not 5 <= k <= 10:
+                continue
+            adj = LN.adjacency(k, edges)
+            q = cuts_amd.Graph.from_edges(k, edges)
+            for order in (None, LN.random_connected_order(k, adj, rng)):
+                p = cuts_amd.Plan(q, order)
+                r = ctx.match(dg18, p, max_work=1 << 27)
+                if r.work_capped:
+                    continue
+                for kk, a, sm, size in LN.table_samples(k, edges, p.order, r.level_sizes):
+                    x1, A1, m1 = LN.encode([(kk, a, sm)], gf)
+                    samples.append((x1[0], A1[0], m1[0]))
+                    targets.append(LN.target(size))
+            used += 1
+            if used == per_graph:
+                break
+        log(f"R-MAT 18 seed {gs}: {used} queries logged, {len(samples)} samples so far")
+        del dg18
+    x = np.stack([s[0] for s in samples])
+    A = np.stack([s[1] for s in samples])
+    meta = np.asarray([s[2] for s in samples], np.uint32)
+    net = LN.Mpnn(ctx, seed=0)
+    hist = net.fit(x, A, meta, np.asarray(targets, np.float32), epochs=30, batch=4096, lr=2e-3)
+    log(f"trained: mse {hist[0]:.4f} -> {hist[-1]:.4f} on {len(meta)} samples")
+    if args.save_weights:
+        np.save(args.save_weights, net.params())
+    g = W.config3_graph()
+    dg = ctx.upload(g)
+    gf = LN.graph_features(g)
+    evals, tried = [], 0
+    for i, k, ne, edges in W.batch_candidates(g, 7):
+        if not 6 <= k <= 10:
+            continue
+        tried += 1
+        p = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges))
+        r = ctx.match(dg, p, max_work=1 << 29)
+        if r.work_capped:
+            continue
+        evals.append((k, edges, p, r))
+        if len(evals) % 100 == 0:
+            log(f"admitted {len(evals)} evaluation queries of {tried}")
+        if len(evals) == args.eval_queries or tried >= 20 * args.eval_queries:
+            break
+    choices = []
+    for j in range(0, len(evals), 32):
+        part = evals[j:j + 32]
+        choices += LN.choose_orders(net, [(k, e) for k, e, _, _ in part], gf, [p.order for _, _, p, _ in part])
+    h_ms, l_ms, model_fb, run_fb, mism = [], [], 0, 0, 0
+    for (k, edges, p, r), ch in zip(evals, choices):
+        rh = ctx.match(dg, p)  # timed again, warm
+        h_ms.append(rh.device_ms)
+        if not ch.learned:
+            model_fb += 1
+            l_ms.append(rh.device_ms)
+            continue
+        rl = ctx.match(dg, cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges), ch.order), max_work=4 * r.work + 1024)
+        if rl.work_capped:
+            run_fb += 1
+            l_ms.append(rl.device_ms + rh.device_ms)
+            continue
+        if rl.count != rh.count:
+            mism += 1
+        l_ms.append(rl.device_ms)
+    ratio = [a / b for a, b in zip(h_ms, l_ms) if b > 0]
+    geo = float(np.exp(np.mean(np.log(ratio)))) if ratio else 0.0
+    line = base_line("exact embeddings/sec + p50 time/query; learned vs heuristic order, same GPU",
+                     sum(r.count for _, _, _, r in evals) / (sum(l_ms) / 1e3), "embeddings/s", 1, 0, sum(l_ms),
+                     "train: 4 R-MAT scale-18 graphs (seeds 1-4), 5-10 vertex walk queries; evaluate: 6-10 vertex "
+                     "walk queries on R-MAT scale 20 (seed 0) admitted at 2^29 candidates",
+                     {"train_samples": int(len(meta)), "mse_first": hist[0], "mse_last": hist[-1],
+                      "eval_queries": len(evals), "tried": tried,
+                      "p50_ms_heuristic": statistics.median(h_ms), "p50_ms_learned": statistics.median(l_ms),
+                      "speedup_p50": statistics.median(h_ms) / statistics.median(l_ms),
+                      "speedup_total": sum(h_ms) / sum(l_ms), "speedup_geomean": geo,
+                      "fallback_rate_model": model_fb / len(evals), "fallback_rate_runtime": run_fb / len(evals),
+                      "count_mismatches": mism})
+    emit(line)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,11 +343,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run N dispatches of a 1 GiB plain copy (byte-counter yardstick)")
+    ap.add_argument("--workload", default="chunglu36k",
+                    choices=["chunglu36k", "chunglu4k", "rmat20", "batch256", "learned"])
+    ap.add_argument("--train-queries", type=int, default=2000)
+    ap.add_argument("--eval-queries", type=int, default=500)
+    ap.add_argument("--save-weights", default="")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: only one GPU is used by this build (asked for --gpus {args.gpus}); "
               "the matcher shards by level-0 chunks, see DESIGN.md", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "chunglu36k":
+        {"chunglu4k": run_chunglu4k, "rmat20": run_rmat20, "batch256": run_batch256,
+         "learned": run_learned}[args.workload](args)
+        return
 
     import cuts_amd
     from cuts_amd import workloads as W

@@ -27,6 +27,7 @@ FLAG_NO_FUSE = 2
 FLAG_WRITE = 4
 FLAG_NO_XCD = 8
 FLAG_PROFILE = 16
+FLAG_NO_BITMAP = 32
 MAX_QUERY = 16
 KERNEL_CLASSES = ["level0", "plan", "scan", "tile_index", "expand_write", "expand_count", "expand_fused",
                   "closed_form"]
@@ -104,6 +105,7 @@ def lib():
                                                        C.POINTER(C.c_uint64)]),
             "cuts_write_matches": (C.c_int, [C.c_char_p, vp, _u32p, C.c_uint64]),
             "cuts_copy_probe": (C.c_int, [vp, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
+            "cuts_log_tables": (C.c_int, [C.c_char_p, C.c_char_p, vp, C.POINTER(_Result)]),
             "cuts_batch_options_default": (None, [C.POINTER(_BatchOpts)]),
             "cuts_batch_create": (C.c_int, [vp, vp, C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(_BatchOpts), pp]),
             "cuts_batch_run": (C.c_int, [vp, C.POINTER(_BatchResult), C.POINTER(C.c_double),
@@ -324,6 +326,7 @@ class Context:
         res = MatchResult(int(r.count), [int(x) for x in r.level_sizes[: plan.k]], float(r.device_ms),
                           int(r.chunks), complete=bool(r.complete), roots_done=int(r.roots_done),
                           last_root=int(r.last_root), work_capped=bool(r.work_capped), work=int(r.work))
+        res._raw = r
         if write:
             res.rows = rows[: r.emb_written].copy()
         if o.flags & FLAG_PROFILE:
@@ -385,6 +388,12 @@ def search_dfs_bfs_strategy(query_file: str, data_file: str, write_to_disk: bool
     _check(lib().cuts_search_dfs_bfs_strategy(query_file.encode(), data_file.encode(), int(write_to_disk),
                                               trunks, C.byref(c)))
     return c.value
+
+
+def log_tables(path: str, label: str, plan: Plan, result: MatchResult):
+    """Appends the run's per-level table sizes as one JSON line (the solver's
+    own table log; see cuts_log_tables in include/cuts.h)."""
+    _check(lib().cuts_log_tables(path.encode(), label.encode(), plan._h, C.byref(result._raw)))
 
 
 def write_matches(path: str, plan: Plan, rows: np.ndarray):

@@ -1,0 +1,112 @@
+"""Launch-shape autotuner.
+
+Sweeps the tunable launch parameters of the matcher on a given workload and
+keeps the fastest point; every point's counts must equal every other
+point's (counts do not depend on launch shape), so a tuning run is also a
+consistency check.  Results are cached per (device, graph fingerprint,
+workload) in ``$CUTS_TUNE_CACHE`` (default ``~/.cache/cuts_amd/tune.json``).
+
+Tunables:
+  single query -- blocks per CU of the expansion kernels (grid = CUs x n) and
+                  the XCD-contiguous tile mapping;
+  batch        -- persistent blocks per CU (1..4) and level-0 vertices per
+                  initial work item.
+"""
+from __future__ import annotations
+
+import json
+import os
+import statistics
+import time
+from typing import Dict, List, Optional, Sequence
+
+from . import FLAG_NO_XCD, Batch, Context, DeviceGraph, Plan
+
+CACHE = os.environ.get("CUTS_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "cuts_amd", "tune.json"))
+
+
+def _load() -> Dict:
+    try:
+        with open(CACHE) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(d: Dict):
+    try:
+        os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+        with open(CACHE, "w") as f:
+            json.dump(d, f, indent=1)
+    except OSError:
+        pass
+
+
+class CountMismatch(AssertionError):
+    pass
+
+
+def tune_single(ctx: Context, dg: DeviceGraph, plans: Sequence[Plan], key: str, reps: int = 2,
+                grid: Sequence[int] = (2, 4, 8, 16), xcd: Sequence[bool] = (True, False), log=None,
+                use_cache: bool = True) -> Dict:
+    cache = _load()
+    if use_cache and key in cache:
+        return cache[key]
+    points: List[Dict] = []
+    ref_counts = None
+    for bpc in grid:
+        for x in xcd:
+            flags = 0 if x else FLAG_NO_XCD
+            ms = []
+            counts = []
+            for _ in range(reps):
+                t = 0.0
+                counts = []
+                for p in plans:
+                    r = ctx.match(dg, p, flags=flags, blocks_per_cu=bpc)
+                    t += r.device_ms
+                    counts.append(r.count)
+                ms.append(t)
+            if ref_counts is None:
+                ref_counts = counts
+            elif counts != ref_counts:
+                raise CountMismatch(f"counts differ at blocks_per_cu={bpc} xcd={x}")
+            pt = dict(blocks_per_cu=bpc, xcd=x, ms=min(ms))
+            points.append(pt)
+            if log:
+                log(f"tune single: {pt}")
+    best = min(points, key=lambda p: p["ms"])
+    res = dict(best=best, points=points, counts=ref_counts)
+    cache[key] = res
+    _store(cache)
+    return res
+
+
+def tune_batch(ctx: Context, dg: DeviceGraph, plans: Sequence[Plan], key: str, reps: int = 2,
+               blocks: Sequence[int] = (1, 2, 4), vchunks: Sequence[int] = (1024, 4096, 16384),
+               arena_bytes: int = 32 << 30, log=None, use_cache: bool = True) -> Dict:
+    cache = _load()
+    if use_cache and key in cache:
+        return cache[key]
+    points: List[Dict] = []
+    ref = None
+    for bpc in blocks:
+        for vc in vchunks:
+            b = Batch(ctx, dg, plans, blocks_per_cu=bpc, vchunk=vc, arena_bytes=arena_bytes)
+            runs = [b.run() for _ in range(reps)]
+            for r in runs:
+                if ref is None:
+                    ref = r.counts
+                elif r.counts != ref:
+                    raise CountMismatch(f"batch counts differ at blocks_per_cu={bpc} vchunk={vc}")
+            pt = dict(blocks_per_cu=bpc, vchunk=vc, ms=min(r.device_ms for r in runs),
+                      p50_latency_ms=statistics.median(runs[-1].latency_ms))
+            points.append(pt)
+            if log:
+                log(f"tune batch: {pt}")
+            del b
+    best = min(points, key=lambda p: p["ms"])
+    res = dict(best=best, points=points)
+    cache[key] = res
+    _store(cache)
+    return res

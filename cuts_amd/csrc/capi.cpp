@@ -3,6 +3,7 @@
 #include "../../include/cuts.h"
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iostream>
@@ -115,6 +116,33 @@ cuts::MatchOpts to_opts(const cuts_match_options* o) {
   return m;
 }
 
+std::string json_escape(const std::string& in) {
+  std::string o;
+  for (char ch : in) {
+    if (ch == '"' || ch == '\\') { o += '\\'; o += ch; }
+    else if ((unsigned char)ch < 0x20) { char b[8]; snprintf(b, sizeof b, "\\u%04x", ch); o += b; }
+    else o += ch;
+  }
+  return o;
+}
+
+void log_tables(const std::string& path, const std::string& label, const cuts::Plan& p, uint64_t count,
+                const uint64_t* level, uint64_t work, double ms, int complete) {
+  std::ofstream out(path, std::ios::app);
+  if (!out) throw cuts::Error(CUTS_E_IO, "cannot append to " + path);
+  std::ostringstream os;
+  os << "{\"label\": \"" << json_escape(label) << "\", \"k\": " << p.k << ", \"order\": [";
+  for (uint32_t i = 0; i < p.k; ++i) os << (i ? ", " : "") << p.order[i];
+  os << "], \"level_sizes\": [";
+  for (uint32_t i = 0; i < p.k; ++i) os << (i ? ", " : "") << level[i];
+  char msb[64];
+  snprintf(msb, sizeof msb, "%.6f", ms);
+  os << "], \"count\": " << count << ", \"work\": " << work << ", \"device_ms\": " << msb
+     << ", \"complete\": " << complete << "}\n";
+  out << os.str();
+  if (!out.good()) throw cuts::Error(CUTS_E_IO, "write failed for " + path);
+}
+
 // search / search_dfs_bfs_strategy (src/host_funcs.cu:39-171).
 int search_impl(const char* query_file, const char* data_file, int write_to_disk, bool trunked, uint32_t trunks,
                 uint64_t* count) {
@@ -137,6 +165,8 @@ int search_impl(const char* query_file, const char* data_file, int write_to_disk
   cuts::MatchResult r;
   cuts::run_match(*ctx, *dg, p, o, r);
   std::cout << db << "," << qb << "," << ostream_float((float)r.device_ms) << "ms," << r.count << std::endl;
+  if (const char* lp = std::getenv("CUTS_TABLE_LOG"))
+    log_tables(lp, db + "," + qb, p, r.count, r.level, r.work, r.device_ms, (int)r.complete);
   if (write_to_disk) {
     std::cout << "start writting matching results to disk,ans.txt" << std::endl;
     const uint64_t words = r.count * (uint64_t)p.k;
@@ -443,6 +473,14 @@ int cuts_mpnn_train_step(cuts_mpnn* m, uint32_t n, const float* x, const uint16_
 }
 
 void cuts_mpnn_free(cuts_mpnn* m) { delete m; }
+
+int cuts_log_tables(const char* path, const char* label, const cuts_plan* plan, const cuts_match_result* r) {
+  return guard([&]() -> int {
+    if (!path || !plan || !r) return fail(CUTS_E_ARG, "null argument");
+    log_tables(path, label ? label : "", plan->p, r->count, r->level_sizes, r->work, r->device_ms, (int)r->complete);
+    return CUTS_OK;
+  });
+}
 
 int cuts_copy_probe(cuts_context* ctx, uint64_t bytes, uint32_t iters, double* gbps) {
   return guard([&]() -> int {

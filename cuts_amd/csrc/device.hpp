@@ -48,6 +48,11 @@ struct DeviceGraph {
   uint32_t *off_out = nullptr, *adj_out = nullptr, *off_in = nullptr, *adj_in = nullptr;
   uint32_t* sig = nullptr;                 // 2V: in, out
   std::map<uint32_t, uint32_t*> cnt_tables; // degree threshold -> #neighbours meeting it
+  // adjacency bitmaps of hub vertices (symmetric graphs): hub_id[v] = row of
+  // v in `bits` or -1; a row is `words` uint64 covering all V vertices
+  int32_t* hub_id = nullptr;
+  uint64_t* bits = nullptr;
+  uint32_t words = 0, hubs = 0, hub_min_deg = 0;
   ~DeviceGraph();
 };
 

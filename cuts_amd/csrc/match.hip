@@ -56,7 +56,14 @@ constexpr uint64_t MAX_CHUNK = 1ull << 31;  // work items per launch sequence
 struct GView {
   const uint32_t *off_out, *adj_out, *off_in, *adj_in, *sig;
   uint32_t V;
+  const int32_t* hub_id;  // null when the graph has no hub bitmaps
+  const uint64_t* bits;
+  uint32_t words;
 };
+
+GView view_of(const DeviceGraph& dg) {
+  return GView{dg.off_out, dg.adj_out, dg.off_in, dg.adj_in, dg.sig, dg.V, dg.hub_id, dg.bits, dg.words};
+}
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -77,6 +84,36 @@ __device__ __forceinline__ bool member_safe(const uint32_t* __restrict__ adj, ui
 // Demand bytes of one lower-bound search over `len` words: the probes plus
 // the final compare.
 __device__ __forceinline__ uint32_t bs_bytes(uint32_t len) { return 4u * ((32u - __clz(len)) + 1u); }
+
+// x in out-list(u) of a symmetric graph: one bit test when u is a hub,
+// else a lower-bound search of u's sorted list.
+__device__ __forceinline__ bool adj_has(const GView& g, uint32_t u, uint32_t x, uint64_t& bytes) {
+  if (g.hub_id) {
+    const int32_t h = g.hub_id[u];
+    bytes += 4;
+    if (h >= 0) {
+      bytes += 8;
+      return (g.bits[(uint64_t)h * g.words + (x >> 6)] >> (x & 63)) & 1ull;
+    }
+  }
+  const uint32_t lo = g.off_out[u], hi = g.off_out[u + 1];
+  bytes += 8 + bs_bytes(hi - lo);
+  return member_safe(g.adj_out, lo, hi, x);
+}
+
+// One block per hub: sets the bits of its neighbours.
+__global__ __launch_bounds__(256) void k_hub_bits(const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
+                                                  const uint32_t* __restrict__ hubs, uint32_t nh, uint32_t words,
+                                                  uint64_t* __restrict__ bits) {
+  for (uint32_t h = blockIdx.x; h < nh; h += gridDim.x) {
+    const uint32_t u = hubs[h];
+    uint64_t* row = bits + (uint64_t)h * words;
+    for (uint32_t i = off[u] + threadIdx.x; i < off[u + 1]; i += 256) {
+      const uint32_t x = adj[i];
+      atomicOr((unsigned long long*)&row[x >> 6], 1ull << (x & 63));
+    }
+  }
+}
 
 // Wave-then-block sum of a 64-bit value; one atomic per wave.
 __device__ __forceinline__ void add_wave(unsigned long long* dst, uint64_t v) {
@@ -265,14 +302,8 @@ __device__ __forceinline__ uint64_t closed_form(const GView& g, const KPlan& P, 
     const uint32_t x = r[i];
     bool in = (P.sc_adjmask >> i) & 1;
     if (!in) {
-      if (newest && i != N - 1) {
-        const uint32_t xl = g.off_out[x], xh = g.off_out[x + 1];
-        in = member_safe(g.adj_out, xl, xh, u);
-        bytes += 8 + bs_bytes(xh - xl);
-      } else {
-        in = member_safe(g.adj_out, lo, hi, x);
-        bytes += bs_bytes(hi - lo);
-      }
+      if (newest && i != N - 1) in = adj_has(g, x, u, bytes);
+      else in = adj_has(g, u, x, bytes);
     }
     if (in && need_deg) bytes += 4;
     if (in && (!need_deg || g.sig[2 * (uint64_t)x] >= P.sc_thr)) --c;
@@ -399,11 +430,15 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
 #pragma unroll
           for (int l = 0; l < L; ++l)
             if (l == (int)(jc2 & 31)) u2 = row[l];
-          const uint32_t* off2 = (jc2 >> 7) ? g.off_in : g.off_out;
-          const uint32_t* adj2 = (jc2 >> 7) ? g.adj_in : g.adj_out;
-          const uint32_t lo2 = off2[u2], hi2 = off2[u2 + 1];
-          ok = member_safe(adj2, lo2, hi2, v);
-          bytes += bs_bytes(hi2 - lo2);
+          if (g.hub_id) {
+            ok = adj_has(g, u2, v, bytes);  // symmetric: every constraint is an out-list
+          } else {
+            const uint32_t* off2 = (jc2 >> 7) ? g.off_in : g.off_out;
+            const uint32_t* adj2 = (jc2 >> 7) ? g.adj_in : g.adj_out;
+            const uint32_t lo2 = off2[u2], hi2 = off2[u2 + 1];
+            ok = member_safe(adj2, lo2, hi2, v);
+            bytes += bs_bytes(hi2 - lo2);
+          }
         }
       }
       if (MODE == M_WRITE) {
@@ -702,7 +737,8 @@ struct Matcher {
     k = plan.k;
     if (k == 0 || k > kMaxQuery) throw Error(-3, "query must have between 1 and 16 vertices");
     kp = make_kplan(plan, dg.symmetric);
-    g = GView{dg.off_out, dg.adj_out, dg.off_in, dg.adj_in, dg.sig, dg.V};
+    g = view_of(dg);
+    if (o.flags & 32u) g.hub_id = nullptr;  // CUTS_FLAG_NO_BITMAP
     write = (o.flags & 4u) != 0;
     prof = (o.flags & 16u) != 0;
     closed_last = !write && !(o.flags & 1u) && kp.sc_ok && k >= 2;
@@ -819,7 +855,7 @@ const uint32_t* cnt_table(Context& c, DeviceGraph& dg, uint32_t thr) {
   if (it == dg.cnt_tables.end()) {
     uint32_t* t = nullptr;
     dev_alloc(t, dg.V + 1);
-    const GView g{dg.off_out, dg.adj_out, dg.off_in, dg.adj_in, dg.sig, dg.V};
+    const GView g = view_of(dg);
     hipLaunchKernelGGL(k_cnt_table, dim3(grid_for(dg.V, BLOCK / 64, 65535)), dim3(BLOCK), 0, c.stream, g, thr, t);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipStreamSynchronize(c.stream));
@@ -894,6 +930,8 @@ DeviceGraph::~DeviceGraph() {
     (void)hipFree(adj_in);
   }
   for (auto& kv : cnt_tables) (void)hipFree(kv.second);
+  if (hub_id) (void)hipFree(hub_id);
+  if (bits) (void)hipFree(bits);
   (void)hipSetDevice(prev);
 }
 
@@ -957,6 +995,46 @@ void upload_graph(Context& c, const Graph& g, DeviceGraph& dg) {
   } else {
     up(dg.off_in, g.in_off);
     up(dg.adj_in, g.in_adj);
+  }
+  // Hub bitmaps: the lowest power-of-two degree threshold (>= 32) whose
+  // bitmaps fit the budget (1/32 of free HBM, at most 4 GiB).
+  if (g.symmetric && g.V >= 64) {
+    size_t fr = 0, tot = 0;
+    HIPCHECK(hipMemGetInfo(&fr, &tot));
+    const uint64_t cap = std::min<uint64_t>(fr / 32, 4ull << 30);
+    const uint32_t words = (g.V + 63) / 64;
+    std::vector<uint32_t> deg(g.V);
+    for (uint32_t v = 0; v < g.V; ++v) deg[v] = g.out_off[v + 1] - g.out_off[v];
+    uint32_t t = 32;
+    for (; t <= (1u << 30); t *= 2) {
+      uint64_t nh = 0;
+      for (uint32_t d : deg) nh += d >= t;
+      if (nh * words * 8 <= cap) break;
+    }
+    std::vector<uint32_t> hubs;
+    std::vector<int32_t> hid(g.V, -1);
+    for (uint32_t v = 0; v < g.V; ++v)
+      if (deg[v] >= t) {
+        hid[v] = (int32_t)hubs.size();
+        hubs.push_back(v);
+      }
+    if (!hubs.empty()) {
+      dg.words = words;
+      dg.hubs = (uint32_t)hubs.size();
+      dg.hub_min_deg = t;
+      dev_alloc(dg.hub_id, g.V);
+      HIPCHECK(hipMemcpy(dg.hub_id, hid.data(), g.V * 4, hipMemcpyHostToDevice));
+      dev_alloc(dg.bits, (uint64_t)hubs.size() * words);
+      HIPCHECK(hipMemset(dg.bits, 0, (uint64_t)hubs.size() * words * 8));
+      uint32_t* dh = nullptr;
+      dev_alloc(dh, hubs.size());
+      HIPCHECK(hipMemcpy(dh, hubs.data(), hubs.size() * 4, hipMemcpyHostToDevice));
+      hipLaunchKernelGGL(k_hub_bits, dim3((uint32_t)std::min<size_t>(hubs.size(), 65535)), dim3(256), 0, c.stream,
+                         dg.off_out, dg.adj_out, dh, (uint32_t)hubs.size(), words, dg.bits);
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipStreamSynchronize(c.stream));
+      (void)hipFree(dh);
+    }
   }
 }
 

@@ -89,7 +89,8 @@ enum {
   CUTS_FLAG_NO_FUSE = 1u << 1,     /* never fuse the last two levels */
   CUTS_FLAG_WRITE = 1u << 2,       /* materialise embeddings into emb_rows */
   CUTS_FLAG_NO_XCD = 1u << 3,      /* plain blockIdx -> tile mapping */
-  CUTS_FLAG_PROFILE = 1u << 4      /* time every kernel class with HIP events */
+  CUTS_FLAG_PROFILE = 1u << 4,     /* time every kernel class with HIP events */
+  CUTS_FLAG_NO_BITMAP = 1u << 5    /* lower-bound searches even in hub lists */
 };
 
 /* Kernel classes reported by CUTS_FLAG_PROFILE. */
@@ -152,6 +153,14 @@ void cuts_match_options_default(cuts_match_options *o);
  * search / search_dfs_bfs_strategy (src/host_funcs.cu:39-171). */
 int cuts_match(cuts_context *ctx, const cuts_device_graph *dg, const cuts_plan *plan,
                const cuts_match_options *opt, cuts_match_result *res);
+
+/* Table log (no reference counterpart; the reference keeps the level sizes
+ * in `lengths`, src/host_funcs.cu:95,153, and drops them).  Appends one JSON
+ * line to `path`: {"label", "k", "order", "level_sizes", "count", "work",
+ * "device_ms", "complete"} -- the training data of the table-size network.
+ * The CLI writes it when the environment variable CUTS_TABLE_LOG names a
+ * file. */
+int cuts_log_tables(const char *path, const char *label, const cuts_plan *plan, const cuts_match_result *r);
 
 /* ---- batched matching (no reference counterpart) --------------------- */
 /* Many queries against one resident data graph in one persistent launch

@@ -199,6 +199,24 @@ def make_ans():
     print("ref_ans.json:", len(cases), "cases")
 
 
+def make_gsi():
+    """The reference's own query files in both formats (data): ours_format
+    {1..33}.g and the matching gsi_format/{5,6,7}nodes/{0..10}.g."""
+    cases = []
+    for q in range(1, 34):
+        d = "5nodes" if q <= 11 else ("6nodes" if q <= 22 else "7nodes")
+        i = (q - 1) % 11
+        with open(os.path.join(QDIR, f"{q}.g")) as f:
+            ours = f.read()
+        with open(os.path.join(REF, "data_set/query/gsi_format", d, f"{i}.g")) as f:
+            gsi = f.read()
+        cases.append(dict(ours=ours, gsi=gsi, name=f"{q}.g"))
+    with open(os.path.join(HERE, "ref_gsi.json"), "w") as f:
+        json.dump(dict(source="reference data_set/query ours_format and gsi_format files", cases=cases), f,
+                  separators=(",", ":"))
+    print("ref_gsi.json:", len(cases), "pairs")
+
+
 if __name__ == "__main__":
     if not oracle.ref_available():
         sys.exit("build oracle/_ref first: make -C oracle ref")
@@ -206,3 +224,4 @@ if __name__ == "__main__":
     make_plans()
     make_csr()
     make_ans()
+    make_gsi()

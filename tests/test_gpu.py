@@ -36,6 +36,7 @@ MODES = [
     dict(flags=cuts_amd.FLAG_NO_FUSE),
     dict(flags=cuts_amd.FLAG_NO_SHORTCUT),
     dict(flags=cuts_amd.FLAG_NO_XCD),
+    dict(flags=cuts_amd.FLAG_NO_BITMAP),
     dict(trunks=3),
     dict(budget=4096),  # a few hundred rows per level: many chunks
 ]
@@ -243,3 +244,46 @@ def test_search_entry_points(tmp_path):
     cuts_amd.Graph.from_edges(k, edges).save(str(tmp_path / "q.g"))
     assert cuts_amd.search(str(tmp_path / "q.g"), str(tmp_path / "d.g")) == want
     assert cuts_amd.search_dfs_bfs_strategy(str(tmp_path / "q.g"), str(tmp_path / "d.g"), False, 3) == want
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_sparse_graphs_vs_oracle(ctx, seed):
+    """E/V <= 1: the reference switches to its 8-lane virtual-warp kernel
+    (src/host_funcs.cu:74,145; src/device_funcs.cu:618-733) for these; the
+    flattened kernels need no separate path."""
+    rng = random.Random(500 + seed)
+    n = rng.randint(50, 400)
+    edges = set()
+    for v in range(1, n):
+        if rng.random() < 0.9:
+            edges.add((rng.randrange(v), v))
+    g = cuts_amd.Graph.from_edges(n, sorted(edges))
+    assert g.arcs <= 2 * n  # E/V <= 1 in undirected edges
+    dg = ctx.upload(g)
+    og = oracle.OGraph.from_view(g.view())
+    for q in ([(0, 1), (1, 2)], [(0, 1), (1, 2), (2, 3)], [(0, 1), (0, 2), (0, 3)], [(0, 1), (1, 2), (1, 3), (3, 4)]):
+        k = 1 + max(max(e) for e in q)
+        want = oracle.count(og, oracle.qmatrix(k, oracle.undirected(q)))
+        p = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, q))
+        for m in MODES:
+            assert ctx.match(dg, p, **m).count == want, (q, m)
+
+
+def test_table_log_from_cli_and_library(tmp_path):
+    g, k, edges = random_case(12)
+    g.save(str(tmp_path / "d.g"))
+    q = cuts_amd.Graph.from_edges(k, edges)
+    q.save(str(tmp_path / "q.g"))
+    log = tmp_path / "tables.jsonl"
+    env = dict(os.environ, CUTS_TABLE_LOG=str(log))
+    r = subprocess.run([cuts_amd.CLI_PATH, "d.g", "q.g"], cwd=str(tmp_path), env=env, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout
+    rec = json.loads(log.read_text().splitlines()[-1])
+    ctx = cuts_amd.Context(0)
+    p = cuts_amd.Plan(q)
+    m = ctx.match(ctx.upload(g), p)
+    assert rec["order"] == p.order and rec["level_sizes"] == m.level_sizes and rec["count"] == m.count
+    cuts_amd.log_tables(str(log), "library", p, m)
+    rec2 = json.loads(log.read_text().splitlines()[-1])
+    assert rec2["label"] == "library" and rec2["level_sizes"] == rec["level_sizes"]
