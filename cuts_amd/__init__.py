@@ -28,6 +28,7 @@ FLAG_WRITE = 4
 FLAG_NO_XCD = 8
 FLAG_PROFILE = 16
 FLAG_NO_BITMAP = 32
+FLAG_NO_PREFIX_BASE = 64
 MAX_QUERY = 16
 KERNEL_CLASSES = ["level0", "plan", "scan", "tile_index", "expand_write", "expand_count", "expand_fused",
                   "closed_form"]

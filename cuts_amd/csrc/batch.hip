@@ -36,6 +36,9 @@ constexpr int WROW = 16;     // LDS words per staged row (k <= 16)
 struct GV {
   const uint32_t *off_out, *adj_out, *off_in, *adj_in, *sig;
   uint32_t V;
+  const int32_t* hub_id;  // hub adjacency bitmaps (symmetric graphs) or null
+  const uint64_t* bits;
+  uint32_t words;
 };
 
 __device__ __forceinline__ uint32_t lane_rank64(uint64_t m) {
@@ -50,6 +53,15 @@ __device__ __forceinline__ bool bmember(const uint32_t* __restrict__ adj, uint32
     else hi = mid;
   }
   return lo < end && adj[lo] == x;
+}
+
+// x in out-list(u) of a symmetric graph (bit test for hubs)
+__device__ __forceinline__ bool has_arc(const GV& g, uint32_t u, uint32_t x) {
+  if (g.hub_id) {
+    const int32_t h = g.hub_id[u];
+    if (h >= 0) return (g.bits[(uint64_t)h * g.words + (x >> 6)] >> (x & 63)) & 1ull;
+  }
+  return bmember(g.adj_out, g.off_out[u], g.off_out[u + 1], x);
 }
 
 template <typename T>
@@ -70,10 +82,7 @@ __device__ uint64_t closed_form_rt(const GV& g, const KPlan& P, const uint32_t* 
   for (uint32_t i = 0; i < npos; ++i) {
     const uint32_t x = (i == n) ? v : r[i];
     bool in = (P.sc_adjmask >> i) & 1;
-    if (!in) {
-      if (newest && i != n) in = bmember(g.adj_out, g.off_out[x], g.off_out[x + 1], u);
-      else in = bmember(g.adj_out, lo, hi, x);
-    }
+    if (!in) in = (newest && i != n) ? has_arc(g, x, u) : has_arc(g, u, x);
     if (in && (!need_deg || g.sig[2 * (uint64_t)x] >= P.sc_thr)) --c;
   }
   return (uint64_t)c;
@@ -190,7 +199,7 @@ __device__ void add_block(unsigned long long* dst, uint64_t v) {
 }
 
 __device__ void process(const BatchArgs& a, Shared& s, uint32_t epoch) {
-  const GV g{a.off_out, a.adj_out, a.off_in, a.adj_in, a.sig, a.V};
+  const GV g{a.off_out, a.adj_out, a.off_in, a.adj_in, a.sig, a.V, a.hub_id, a.bits, a.words};
   const Item it = s.item;
   const uint32_t q = it.qL >> 8, L = it.qL & 0xFF;
   const KPlan& P = a.plans[q];
@@ -279,9 +288,13 @@ __device__ void process(const BatchArgs& a, Shared& s, uint32_t epoch) {
         if (c == sel) continue;
         const uint32_t jc2 = P.jc[L][c];
         const uint32_t u2 = row[jc2 & 31];
-        const uint32_t* off2 = (jc2 >> 7) ? g.off_in : g.off_out;
-        const uint32_t* adj2 = (jc2 >> 7) ? g.adj_in : g.adj_out;
-        ok = bmember(adj2, off2[u2], off2[u2 + 1], v);
+        if (g.hub_id) {
+          ok = has_arc(g, u2, v);
+        } else {
+          const uint32_t* off2 = (jc2 >> 7) ? g.off_in : g.off_out;
+          const uint32_t* adj2 = (jc2 >> 7) ? g.adj_in : g.adj_out;
+          ok = bmember(adj2, off2[u2], off2[u2 + 1], v);
+        }
       }
     }
     if (mode == 1) {
@@ -453,6 +466,9 @@ void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& p
   a.adj_in = dg.adj_in;
   a.sig = dg.sig;
   a.V = dg.V;
+  a.hub_id = (o.flags & 32u) ? nullptr : dg.hub_id;
+  a.bits = dg.bits;
+  a.words = dg.words;
   KPlan* dplans = balloc<KPlan>(b, 0, b.nq);
   HIPCHECK(hipMemcpy(dplans, kp.data(), b.nq * sizeof(KPlan), hipMemcpyHostToDevice));
   a.plans = dplans;

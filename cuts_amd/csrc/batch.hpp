@@ -20,6 +20,9 @@ struct BState {
 struct BatchArgs {
   const uint32_t *off_out, *adj_out, *off_in, *adj_in, *sig;
   uint32_t V;
+  const int32_t* hub_id;
+  const uint64_t* bits;
+  uint32_t words;
   const KPlan* plans;
   const uint32_t* const* tbls;  // per query: closed-form table or null
   BState* st;

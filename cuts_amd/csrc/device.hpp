@@ -64,6 +64,7 @@ struct LevelBuf {
   uint32_t* minlen = nullptr;   // cap
   uint8_t* minsel = nullptr;    // cap
   uint64_t* work_off = nullptr; // cap+1
+  uint32_t* cbase = nullptr;    // cap: per-prefix closed-form base (fused levels)
   uint64_t* bsum = nullptr;     // scan partials
   uint64_t bsum_cap = 0;
   uint32_t* tile_p = nullptr;   // tiles+1

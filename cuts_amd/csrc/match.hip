@@ -153,9 +153,13 @@ __global__ __launch_bounds__(BLOCK) void k_level0_scatter(const uint32_t* __rest
 // ------------------------------------------------------------ planning
 // Per prefix: the constraint whose list is shortest (upper bound on the
 // prefix's children).  Ties keep the lowest constraint index.
+// With cbase (fused levels whose closed-form vertex u is an older position)
+// it also stores the part of the closed form that only depends on the
+// prefix: cnt(u) - |{i < L : r_i in N(u), deg(r_i) >= t}|.
 template <int L>
 __global__ __launch_bounds__(BLOCK) void k_plan(GView g, KPlan P, const uint32_t* __restrict__ rows, uint64_t n,
-                                               uint32_t* __restrict__ minlen, uint8_t* __restrict__ minsel) {
+                                               uint32_t* __restrict__ minlen, uint8_t* __restrict__ minsel,
+                                               uint32_t* __restrict__ cbase, const uint32_t* __restrict__ tbl) {
   const uint32_t nj = P.nj[L];
   for (uint64_t p = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (uint64_t)gridDim.x * BLOCK) {
     uint32_t row[L];
@@ -175,6 +179,20 @@ __global__ __launch_bounds__(BLOCK) void k_plan(GView g, KPlan P, const uint32_t
     }
     minlen[p] = best;
     minsel[p] = (uint8_t)sel;
+    if (cbase) {
+      uint32_t u = 0;
+#pragma unroll
+      for (int l = 0; l < L; ++l)
+        if (l == (int)P.sc_pos) u = row[l];
+      int64_t c = tbl ? (int64_t)tbl[u] : (int64_t)(g.off_out[u + 1] - g.off_out[u]);
+      uint64_t dummy = 0;
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        const bool in = ((P.sc_adjmask >> l) & 1) || adj_has(g, u, row[l], dummy);
+        if (in && (P.sc_thr <= 1 || g.sig[2 * (uint64_t)row[l]] >= P.sc_thr)) --c;
+      }
+      cbase[p] = (uint32_t)c;
+    }
   }
 }
 
@@ -353,6 +371,7 @@ struct ExpandArgs {
   uint32_t* out_count;
   unsigned long long* counters;
   const uint32_t* tbl;       // M_FUSED closed-form table (null: plain degree)
+  const uint32_t* cbase;     // M_FUSED with an older closed-form vertex: per-prefix part
   uint32_t xcd;              // 1: XCD-contiguous tile mapping
   uint32_t stats;            // 1: accumulate demand bytes into counters[3]
 };
@@ -383,6 +402,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
       const uint64_t w = w0 + (uint64_t)it * BLOCK + tid;
       bool ok = false;
       uint32_t v = 0;
+      uint64_t fused_contrib = 0;
       uint32_t row[L];
 #pragma unroll
       for (int l = 0; l < L; ++l) row[l] = 0;
@@ -424,7 +444,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
 #pragma unroll
         for (int l = 0; l < L; ++l) ok = ok && v != row[l];
         for (uint32_t c = 0; c < nj && ok; ++c) {
-          if (c == sel) continue;
+          if (c == (uint32_t)sel) continue;
           const uint32_t jc2 = P.jc[L][c];
           uint32_t u2 = 0;
 #pragma unroll
@@ -439,6 +459,18 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
             ok = member_safe(adj2, lo2, hi2, v);
             bytes += bs_bytes(hi2 - lo2);
           }
+        }
+        if (MODE == M_FUSED && ok && a.cbase) {
+          // closed form = per-prefix base - [v in N(u), deg(v) >= t]; v is
+          // certainly in N(u) when it was drawn from u's own list
+          uint32_t us = 0;
+#pragma unroll
+          for (int l = 0; l < L; ++l)
+            if (l == (int)P.sc_pos) us = row[l];
+          bytes += 4;
+          const bool in = (jc & 31) == P.sc_pos || adj_has(g, us, v, bytes);
+          const bool cnt_v = in && (P.sc_thr <= 1 || g.sig[2 * (uint64_t)v] >= P.sc_thr);
+          fused_contrib = (uint64_t)a.cbase[p] - (cnt_v ? 1 : 0);
         }
       }
       if (MODE == M_WRITE) {
@@ -462,12 +494,16 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
         acc += ok;
       } else {
         if (ok) {
-          uint32_t r[L + 1];
-#pragma unroll
-          for (int l = 0; l < L; ++l) r[l] = row[l];
-          r[L] = v;
           ++valid;
-          acc += closed_form<L + 1>(g, P, r, a.tbl, bytes);
+          if (a.cbase) {
+            acc += fused_contrib;
+          } else {
+            uint32_t r[L + 1];
+#pragma unroll
+            for (int l = 0; l < L; ++l) r[l] = row[l];
+            r[L] = v;
+            acc += closed_form<L + 1>(g, P, r, a.tbl, bytes);
+          }
         }
       }
     }
@@ -528,6 +564,7 @@ void ensure_rows(LevelBuf& b, uint64_t rows, uint32_t words) {
   dev_alloc(b.minlen, cap);
   dev_alloc(b.minsel, cap);
   dev_alloc(b.work_off, cap + 1);
+  dev_alloc(b.cbase, cap);
   b.cap = cap;
 }
 void ensure_rows_exact(LevelBuf& b, uint64_t rows, uint32_t words) {
@@ -537,6 +574,7 @@ void ensure_rows_exact(LevelBuf& b, uint64_t rows, uint32_t words) {
   dev_alloc(b.minlen, std::max<uint64_t>(rows, 1));
   dev_alloc(b.minsel, std::max<uint64_t>(rows, 1));
   dev_alloc(b.work_off, std::max<uint64_t>(rows, 1) + 1);
+  dev_alloc(b.cbase, std::max<uint64_t>(rows, 1));
   b.cap = rows;
 }
 void ensure_bsum(LevelBuf& b, uint64_t nb) {
@@ -641,10 +679,11 @@ struct Matcher {
       return;
     }
     // plan + scan over the input rows; arrays live with the input level
+    const bool use_cbase = fuse && L == k - 2 && kp.sc_pos < L && !(o.flags & 64u);
     {
       const uint32_t grid = grid_for(n, BLOCK, grid_cap);
       pbegin(false);
-#define PL(LL) hipLaunchKernelGGL((k_plan<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.minlen, in.minsel)
+#define PL(LL) hipLaunchKernelGGL((k_plan<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.minlen, in.minsel, use_cbase ? in.cbase : nullptr, tbl)
       CUTS_DISPATCH_L(L, PL)
 #undef PL
       HIPCHECK(hipGetLastError());
@@ -683,6 +722,7 @@ struct Matcher {
       a.ntiles = nt;
       a.counters = c.counters;
       a.tbl = tbl;
+      a.cbase = use_cbase ? in.cbase : nullptr;
       a.xcd = (o.flags & 8u) ? 0 : 1;
       a.stats = prof ? 1 : 0;
       uint32_t grid = grid_for(nt, 1, grid_cap);
@@ -939,7 +979,7 @@ Context::~Context() {
   (void)hipSetDevice(device);
   for (auto& b : lv) {
     for (void* p : {(void*)b.rows, (void*)b.minlen, (void*)b.minsel, (void*)b.work_off, (void*)b.bsum,
-                    (void*)b.tile_p})
+                    (void*)b.tile_p, (void*)b.cbase})
       if (p) (void)hipFree(p);
   }
   if (counters) (void)hipFree(counters);

@@ -90,7 +90,8 @@ enum {
   CUTS_FLAG_WRITE = 1u << 2,       /* materialise embeddings into emb_rows */
   CUTS_FLAG_NO_XCD = 1u << 3,      /* plain blockIdx -> tile mapping */
   CUTS_FLAG_PROFILE = 1u << 4,     /* time every kernel class with HIP events */
-  CUTS_FLAG_NO_BITMAP = 1u << 5    /* lower-bound searches even in hub lists */
+  CUTS_FLAG_NO_BITMAP = 1u << 5,   /* lower-bound searches even in hub lists */
+  CUTS_FLAG_NO_PREFIX_BASE = 1u << 6 /* fused closed form fully per work item */
 };
 
 /* Kernel classes reported by CUTS_FLAG_PROFILE. */

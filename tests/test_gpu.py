@@ -37,6 +37,7 @@ MODES = [
     dict(flags=cuts_amd.FLAG_NO_SHORTCUT),
     dict(flags=cuts_amd.FLAG_NO_XCD),
     dict(flags=cuts_amd.FLAG_NO_BITMAP),
+    dict(flags=cuts_amd.FLAG_NO_PREFIX_BASE | cuts_amd.FLAG_NO_BITMAP),
     dict(trunks=3),
     dict(budget=4096),  # a few hundred rows per level: many chunks
 ]

@@ -13,6 +13,7 @@ import pytest
 
 import cuts_amd
 from cuts_amd import learned as LN
+from cuts_amd import workloads as W
 
 torch = pytest.importorskip("torch")
 from oracle import mpnn_ref  # noqa: E402
@@ -122,7 +123,7 @@ def test_learned_orders_touch_fewer_rows_and_count_the_same(ctx):
 
     def queries(seed, n):
         out = []
-        for i, k, ne, edges in cuts_amd.workloads.batch_candidates(g, seed):
+        for i, k, ne, edges in W.batch_candidates(g, seed):
             if not 5 <= k <= 8:
                 continue
             p = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges))
