@@ -373,6 +373,7 @@ struct ExpandArgs {
   const uint32_t* tbl;       // M_FUSED closed-form table (null: plain degree)
   const uint32_t* cbase;     // M_FUSED with an older closed-form vertex: per-prefix part
   uint32_t xcd;              // 1: XCD-contiguous tile mapping
+  unsigned int* tilectr;     // 8 tile counters, zero at launch
   uint32_t stats;            // 1: accumulate demand bytes into counters[3]
 };
 
@@ -384,12 +385,34 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
   const uint32_t nj = P.nj[L];
   const uint32_t tin = P.sig_in[L], tout = P.sig_out[L];
   uint64_t acc = 0, valid = 0, bytes = 0;
-  // XCD-aware mapping: blocks are dispatched round-robin over 8 XCDs; give
-  // each XCD a contiguous run of tiles so neighbouring prefixes share an L2.
-  const uint32_t G = gridDim.x;
-  uint32_t vb = blockIdx.x;
-  if (a.xcd && (G & 7) == 0) vb = (blockIdx.x & 7) * (G >> 3) + (blockIdx.x >> 3);
-  for (uint64_t tile = vb; tile < a.ntiles; tile += G) {
+  // Dynamic, XCD-aware tile fetching: blocks are dispatched round-robin over
+  // the 8 XCDs, so block b works first on XCD (b mod 8)'s contiguous share of
+  // the tiles (neighbouring prefixes share adjacency lists in that XCD's L2)
+  // and, once that share is drained, takes tiles from the other shares.
+  // One atomic per 1,024-item tile; no static imbalance between blocks.
+  __shared__ uint64_t s_tile;
+  const uint32_t nx = a.xcd ? 8u : 1u;
+  const uint32_t myx = a.xcd ? (blockIdx.x & 7u) : 0u;
+  uint32_t tries = 0;
+  while (true) {
+    if (tid == 0) {
+      uint64_t t = ~0ull;
+      while (tries < nx) {
+        const uint32_t xx = (myx + tries) % nx;
+        const uint64_t lo = a.ntiles * xx / nx, hi = a.ntiles * (xx + 1) / nx;
+        const uint64_t i = atomicAdd(&a.tilectr[xx], 1u);
+        if (lo + i < hi) {
+          t = lo + i;
+          break;
+        }
+        ++tries;
+      }
+      s_tile = t;
+    }
+    __syncthreads();
+    const uint64_t tile = s_tile;
+    if (tile == ~0ull) break;
+    {
     const uint64_t w0 = a.w_begin + tile * TILE;
     const uint64_t w1 = min(w0 + (uint64_t)TILE, a.w_end);
     const uint32_t plo = a.tile_p[tile], phi = a.tile_p[tile + 1];
@@ -508,6 +531,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
       }
     }
     __syncthreads();
+    }
   }
   if (MODE != M_WRITE) {
     add_wave(&a.counters[0], acc);
@@ -724,9 +748,10 @@ struct Matcher {
       a.tbl = tbl;
       a.cbase = use_cbase ? in.cbase : nullptr;
       a.xcd = (o.flags & 8u) ? 0 : 1;
+      a.tilectr = (unsigned int*)(c.counters + 4);  // 8 x uint32 in counters[4..7]
+      HIPCHECK(hipMemsetAsync(a.tilectr, 0, 8 * sizeof(unsigned int), c.stream));
       a.stats = prof ? 1 : 0;
       uint32_t grid = grid_for(nt, 1, grid_cap);
-      if (grid > 8) grid &= ~7u;
       r.chunks++;
       if (mode == M_WRITE) {
         a.out_rows = out.rows;
