@@ -101,6 +101,34 @@ struct Shared {
   uint64_t base;
 };
 
+// Data handed from one block to another inside the persistent kernel (items,
+// arena rows) goes through agent-scope atomic loads and stores: the XCDs of
+// MI355X have separate L2 caches that ordinary loads do not keep coherent,
+// so a plain load could return a line cached on the reader's XCD before the
+// writer on another XCD filled it.
+__device__ __forceinline__ void store_item(Item* dst, const Item& it) {
+  unsigned long long* d = reinterpret_cast<unsigned long long*>(dst);
+  const unsigned long long w0 = ((unsigned long long)it.n << 32) | it.qL;
+  __hip_atomic_store(d, w0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(d + 1, (unsigned long long)it.off, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ Item load_item(const Item* src) {
+  unsigned long long* p = const_cast<unsigned long long*>(reinterpret_cast<const unsigned long long*>(src));
+  const unsigned long long w0 = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned long long w1 = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  Item it;
+  it.qL = (uint32_t)w0;
+  it.n = (uint32_t)(w0 >> 32);
+  it.off = w1;
+  return it;
+}
+__device__ __forceinline__ void store_word(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t load_word(const uint32_t* p) {
+  return __hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ bool push_item(const BatchArgs& a, uint32_t epoch, Item it) {
   const uint32_t q = it.qL >> 8;
   atomicAdd(&a.qpend[q], 1ull);
@@ -112,7 +140,7 @@ __device__ bool push_item(const BatchArgs& a, uint32_t epoch, Item it) {
     atomicSub(&a.qpend[q], 1ull);
     return false;
   }
-  a.items[slot] = it;
+  store_item(&a.items[slot], it);
   __threadfence();
   __hip_atomic_store(&a.ready[slot], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   return true;
@@ -137,7 +165,8 @@ __device__ void flush_stage(const BatchArgs& a, Shared& s, uint32_t epoch, uint3
   __syncthreads();
   const uint64_t base = s.base;
   if (base != ~0ull)
-    for (uint32_t i = threadIdx.x; i < n * words; i += BB) a.arena[base + i] = s.stage[(i / words) * WROW + i % words];
+    for (uint32_t i = threadIdx.x; i < n * words; i += BB)
+      store_word(&a.arena[base + i], s.stage[(i / words) * WROW + i % words]);
   __syncthreads();
   if (threadIdx.x == 0) {
     if (base != ~0ull) {
@@ -228,7 +257,7 @@ __device__ void process(const BatchArgs& a, Shared& s, uint32_t epoch) {
 
   const uint32_t n = it.n;
   const uint32_t* src = a.arena + it.off;
-  for (uint32_t i = tid; i < n * L; i += BB) s.rows[(i / L) * WROW + i % L] = src[i];
+  for (uint32_t i = tid; i < n * L; i += BB) s.rows[(i / L) * WROW + i % L] = load_word(&src[i]);
   __syncthreads();
 
   if (L == k - 1 && closed) {
@@ -343,7 +372,7 @@ __global__ __launch_bounds__(BB) void k_batch_init(BatchArgs a, uint32_t nq, uin
     it.qL = q << 8;
     it.off = (uint64_t)c * vchunk;
     it.n = (uint32_t)std::min<uint64_t>(vchunk, a.V - it.off);
-    a.items[i] = it;
+    store_item(&a.items[i], it);
     __threadfence();
     __hip_atomic_store(&a.ready[i], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   }
@@ -372,23 +401,38 @@ __global__ __launch_bounds__(BB) void k_batch(BatchArgs a) {
       const unsigned long long t = atomicAdd(&a.st->head, 1ull);
       if (t < a.qcap) {
         const uint64_t t0 = wall_clock64();
-        while (true) {
+        for (uint32_t spin = 0;; ++spin) {
           if (aload(&a.ready[t]) == epoch) {
             __threadfence();
-            s.item = a.items[t];
+            const Item it = load_item(&a.items[t]);
+            // an item outside the shapes this kernel writes means the queue
+            // is corrupt: stop everything instead of looping over garbage
+            const uint32_t q = it.qL >> 8, L = it.qL & 0xFF;
+            const bool sane = q < a.nq && L < kMaxQuery &&
+                              (L == 0 ? (it.off < a.V && it.n <= a.V - it.off)
+                                      : (it.n >= 1 && it.n <= (uint32_t)RMAX &&
+                                         it.off + (uint64_t)it.n * L <= a.arena_cap));
+            if (!sane) {
+              atomicExch(&a.st->abort, 4u);
+              break;
+            }
+            s.item = it;
             cmd = 0;
             break;
           }
-          if (aload(&a.st->abort)) break;
-          if (aload(&a.st->pending) == 0) {
-            __threadfence();
-            if (aload(&a.st->tail) <= t) break;
+          // the shared counters are consulted on every 8th round only
+          if ((spin & 7) == 7) {
+            if (aload(&a.st->abort)) break;
+            if (aload(&a.st->pending) == 0) {
+              __threadfence();
+              if (aload(&a.st->tail) <= t) break;
+            }
+            if (wall_clock64() - t0 > a.wait_ticks) {
+              atomicExch(&a.st->abort, 3u);
+              break;
+            }
           }
-          if (wall_clock64() - t0 > a.wait_ticks) {
-            atomicExch(&a.st->abort, 3u);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(8);
+          __builtin_amdgcn_s_sleep(32);
         }
       } else {
         atomicExch(&a.st->abort, 1u);
@@ -446,7 +490,7 @@ void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& p
   b.nch = (dg.V + b.vchunk - 1) / b.vchunk;
   if (b.nch == 0) b.nch = 1;
   b.use_graph = o.use_graph != 0;
-  const uint32_t bpc = o.blocks_per_cu ? std::min<uint32_t>(o.blocks_per_cu, 4) : 2;
+  const uint32_t bpc = o.blocks_per_cu ? std::min<uint32_t>(o.blocks_per_cu, 4) : 1;
   b.grid = (uint32_t)c.cus * bpc;
   const uint64_t qcap = o.queue_cap ? o.queue_cap : (1ull << 22);
   if ((uint64_t)b.nq * b.nch > qcap) throw Error(-1, "batch: queue too small for the level-0 items");
@@ -493,6 +537,7 @@ void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& p
   const double lim = o.wait_limit_s > 0 ? o.wait_limit_s : 300.0;
   a.wait_ticks = (uint64_t)(lim * 1000.0 * b.tick_khz);
   a.flags = o.flags & 3u;
+  a.nq = b.nq;
 }
 
 static void batch_launch(Batch& b, hipStream_t s) {
@@ -525,7 +570,10 @@ void batch_run(Batch& b, std::vector<BatchQueryResult>& out, double& device_ms, 
   BState st;
   HIPCHECK(hipMemcpy(&st, b.args.st, sizeof(BState), hipMemcpyDeviceToHost));
   if (st.abort) {
-    const char* why = st.abort == 1 ? "work queue full" : st.abort == 2 ? "row arena full" : "wait limit reached";
+    const char* why = st.abort == 1   ? "work queue full"
+                      : st.abort == 2 ? "row arena full"
+                      : st.abort == 3 ? "wait limit reached"
+                                      : "corrupt work item";
     throw Error(-5, std::string("batch aborted: ") + why + "; run fewer queries per batch or raise the limits");
   }
   std::vector<unsigned long long> cnt(b.nq), lev((size_t)b.nq * kMaxQuery), done(b.nq);

@@ -11,9 +11,22 @@ struct Item {      // 16 bytes
   uint64_t off;    // arena word offset of the rows (first vertex for level 0)
 };
 
+// Queue state.  Every field has a 256-byte line of its own: waiting blocks
+// poll some of them while working blocks update others, and on one shared
+// line those accesses would serialise in a single L2 channel.
 struct BState {
-  unsigned long long head, tail, pending, arena_head;
-  uint32_t abort, epoch;  // abort: 1 queue full, 2 arena full, 3 wait limit
+  unsigned long long head;
+  char pad0[248];
+  unsigned long long tail;
+  char pad1[248];
+  unsigned long long pending;
+  char pad2[248];
+  unsigned long long arena_head;
+  char pad3[248];
+  uint32_t abort;  // 1 queue full, 2 arena full, 3 wait limit, 4 corrupt item
+  char pad4[252];
+  uint32_t epoch;
+  char pad5[252];
   unsigned long long t_start;
 };
 
@@ -34,6 +47,7 @@ struct BatchArgs {
   unsigned long long *counts, *levels, *qpend, *done;
   uint64_t wait_ticks;
   uint32_t flags;                // 1: no closed form, 2: no fusion
+  uint32_t nq;                   // queries in the batch
 };
 
 struct BatchOpts {

@@ -13,6 +13,8 @@ import oracle
 from cuts_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
+# any stall inside the persistent kernel becomes an error within this bound
+WAIT = 60.0
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
@@ -35,7 +37,7 @@ def test_batch_equals_single_and_oracle(ctx, seed):
     plans = [cuts_amd.Plan(cuts_amd.Graph.from_edges(k, e)) for k, e in qs]
     singles = [ctx.match(dg, p) for p in plans]
     for flags in (0, cuts_amd.FLAG_NO_FUSE, cuts_amd.FLAG_NO_SHORTCUT):
-        b = cuts_amd.Batch(ctx, dg, plans, flags=flags, vchunk=64)
+        b = cuts_amd.Batch(ctx, dg, plans, flags=flags, vchunk=64, wait_limit_s=WAIT)
         r = b.run()
         for i, (k, e) in enumerate(qs):
             assert r.counts[i] == singles[i].count, (flags, i)
@@ -50,8 +52,8 @@ def test_batch_graph_replays_agree(ctx):
     qs = small_queries(g, 40, 7)
     plans = [cuts_amd.Plan(cuts_amd.Graph.from_edges(k, e)) for k, e in qs]
     want = [ctx.match(dg, p).count for p in plans]
-    b = cuts_amd.Batch(ctx, dg, plans, use_graph=True)
-    plain = cuts_amd.Batch(ctx, dg, plans, use_graph=False)
+    b = cuts_amd.Batch(ctx, dg, plans, use_graph=True, wait_limit_s=WAIT)
+    plain = cuts_amd.Batch(ctx, dg, plans, use_graph=False, wait_limit_s=WAIT)
     for rep in range(4):
         r = b.run()
         assert r.counts == want, rep
@@ -76,7 +78,7 @@ def test_batch_directed(ctx):
             q.add((u, v) if rng.random() < 0.5 else (v, u))
         qa = np.asarray(sorted(q), dtype=np.uint32)
         plans.append(cuts_amd.Plan(cuts_amd.Graph.from_arcs(k, qa[:, 0], qa[:, 1])))
-    r = cuts_amd.Batch(ctx, dg, plans).run()
+    r = cuts_amd.Batch(ctx, dg, plans, wait_limit_s=WAIT).run()
     assert r.counts == [ctx.match(dg, p).count for p in plans]
 
 
@@ -84,7 +86,7 @@ def test_batch_tiny_queue_fails_cleanly(ctx):
     g = cuts_amd.Graph.chung_lu(300, 1500, 2.4, 0, 3)
     dg = ctx.upload(g)
     plans = [cuts_amd.Plan(cuts_amd.Graph.from_edges(k, e)) for k, e in small_queries(g, 8, 3)]
-    b = cuts_amd.Batch(ctx, dg, plans, queue_cap=48, vchunk=64)  # 40 level-0 items fit, children do not
+    b = cuts_amd.Batch(ctx, dg, plans, queue_cap=48, vchunk=64, wait_limit_s=WAIT)  # 40 level-0 items fit, children do not
     with pytest.raises(cuts_amd.CutsError):
         b.run()
     # the context stays usable
@@ -98,6 +100,6 @@ def test_config4_batch_matches_fixture(ctx):
     dg = ctx.upload(g)
     plans = [cuts_amd.Plan(cuts_amd.Graph.from_edges(q["k"], q["edges"])) for q in fx["queries"]]
     assert [p.order for p in plans] == [q["order"] for q in fx["queries"]]
-    r = cuts_amd.Batch(ctx, dg, plans, arena_bytes=32 << 30).run()
+    r = cuts_amd.Batch(ctx, dg, plans, arena_bytes=32 << 30, wait_limit_s=WAIT).run()
     assert r.counts == [q["count"] for q in fx["queries"]]
     assert r.level_sizes == [q["level_sizes"] for q in fx["queries"]]
