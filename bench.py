@@ -9,8 +9,12 @@ the timed steps.
 
 The dominant kernel's roofline comes from HIP events recorded around every
 launch on the library's own stream during the timed steps
-(CUTS_FLAG_PROFILE), with the kernel's algorithmic demand bytes counted on
-the device.  The CPU baseline times the OpenMP port of the same trie
+(CUTS_FLAG_PROFILE).  Its algorithmic bytes are the merge model's, counted
+exactly on the device for the prefixes that kernel expands: 8 + 4*len per
+distinct backward list of every prefix, 16 per trie node written and read
+back (the sum over all kernels equals the CPU trie walk's figure).  The
+device also counts the kernel's demand bytes (what it actually loads, e.g.
+one word per hub-bitmap test), reported beside it.  The CPU baseline times the OpenMP port of the same trie
 algorithm (oracle/, test infrastructure) on a bounded sample of level 0.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
@@ -20,7 +24,7 @@ Other workloads (one JSON line each; the default above is the headline):
     --workload rmat20      5-cycle + chord + 2 tails on R-MAT scale 20: the full
                            count exceeds 2^64, so the run is time-limited and
                            reports the exact count of a root prefix
-    --workload batch256    256 random-walk queries, one persistent hipGraph
+    --workload batch256    256 random-walk queries, batched level queue in one hipGraph
                            launch, learned orders, autotuned launch shape
     --workload learned     train the table-size network on R-MAT scale 18,
                            evaluate learned vs score order on R-MAT scale 20
@@ -36,6 +40,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+BATCH_ARENA = 48 << 30   # two halves of 24 GiB: the largest trie level of the whole batch
+BATCH_QUEUE = 1 << 26    # work items per level
 
 
 def cpu_baseline(g, shapes, budget_s=20.0):
@@ -122,7 +128,7 @@ def run_rmat20(args, limit_s=20.0):
     log(f"R-MAT 20 uploaded ({g.arcs} arcs); roots {roots}, {limit_s}s limit, one root per trunk")
     r = ctx.match(dg, p, roots=roots, trunks=1 << 20, time_limit_ms=limit_s * 1e3, flags=cuts_amd.FLAG_PROFILE)
     dname, (dms, dn, dbytes) = max(r.kernels.items(), key=lambda kv: kv[1][0])
-    ach = dbytes / dn / (dms / dn / 1e3) / 1e9 if dn else 0.0
+    ach = r.merge_bytes[dname] / dn / (dms / dn / 1e3) / 1e9 if dn else 0.0
     line = base_line("exact embeddings/sec (time-limited exact root prefix)", r.count / (r.device_ms / 1e3),
                      "embeddings/s", 1, 0, r.device_ms,
                      "rmat scale=20 ef=16 a,b,c=.57,.19,.19 seed=0; 5-cycle+chord+2-tails; heuristic order",
@@ -131,7 +137,8 @@ def run_rmat20(args, limit_s=20.0):
                       "time_limit_s": limit_s})
     line["roofline"] = {"bound": "hbm", "kernel": dname, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": ach / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": dms / max(dn, 1),
-                        "launches": dn}
+                        "launches": dn, "bytes_model": "merge model",
+                        "demand_GBs": dbytes / dn / (dms / dn / 1e3) / 1e9 if dn else 0.0}
     emit(line)
 
 
@@ -159,8 +166,11 @@ def run_batch256(args):
     counts = [r.count for r in single]
     assert counts == [q["count"] for q in fx["queries"]], "single-query counts differ from the fixture"
     log(f"single heuristic: p50 {statistics.median(s_ms):.3f} ms, total {sum(s_ms):.1f} ms")
-    # learned orders from a network trained on tables logged on other queries of the same graph
-    net, info = train_on_graph(ctx, dg, g, seed=1, n_queries=args.train_queries, log=log)
+    # learned orders from a network trained on tables logged on other queries
+    # of the same graph (same admission rule, another seed)
+    train, tried = W.admit(ctx, dg, g, min(args.train_queries, 512), seed=1)
+    log(f"admitted {len(train)} training queries (tries per size {tried})")
+    net, info = train_on_queries(ctx, dg, g, [(q["k"], q["edges"], q["work"]) for q in train], seed=1, log=log)
     gfeat = LN.graph_features(g)
     choices = []
     for i in range(0, len(qs), 32):
@@ -168,10 +178,11 @@ def run_batch256(args):
     plans = [cuts_amd.Plan(cuts_amd.Graph.from_edges(k, e), ch.order) for (k, e), ch in zip(qs, choices)]
     fallback = sum(1 for c in choices if not c.learned) / len(choices)
     key = f"batch256:{W.csr_fingerprint(g)}:learned"
-    tb = autotune.tune_batch(ctx, dg, plans, key, use_cache=False, log=log)
+    tb = autotune.tune_batch(ctx, dg, plans, key, use_cache=False, log=log, arena_bytes=BATCH_ARENA,
+                             queue_cap=BATCH_QUEUE)
     best = tb["best"]
-    b = cuts_amd.Batch(ctx, dg, plans, blocks_per_cu=best["blocks_per_cu"], vchunk=best["vchunk"],
-                       arena_bytes=32 << 30)
+    b = cuts_amd.Batch(ctx, dg, plans, blocks_per_cu=best["blocks_per_cu"], item_work=best["item_work"],
+                       arena_bytes=BATCH_ARENA, queue_cap=BATCH_QUEUE)
     runs = []
     for _ in range(args.warmup):
         b.run()
@@ -179,8 +190,8 @@ def run_batch256(args):
         r = b.run()
         assert r.counts == counts, "batched learned-order counts differ from single-query score-order counts"
         runs.append(r)
-    bh = cuts_amd.Batch(ctx, dg, heur, blocks_per_cu=best["blocks_per_cu"], vchunk=best["vchunk"],
-                        arena_bytes=32 << 30)
+    bh = cuts_amd.Batch(ctx, dg, heur, blocks_per_cu=best["blocks_per_cu"], item_work=best["item_work"],
+                        arena_bytes=BATCH_ARENA, queue_cap=BATCH_QUEUE)
     rh = [bh.run() for _ in range(2)][-1]
     assert rh.counts == counts
     ms = statistics.median([r.device_ms for r in runs])
@@ -189,8 +200,8 @@ def run_batch256(args):
     amort = ms / len(qs)
     emit(base_line("exact embeddings/sec + p50 time/query; learned vs heuristic order, same GPU",
                    sum(counts) / (ms / 1e3), "embeddings/s", args.steps, args.warmup, ms,
-                   "256 random-walk queries (k 4-12, edge/vertex 1.0-2.5) on chung-lu n=36692; one persistent "
-                   "hipGraph launch; learned orders; autotuned launch shape",
+                   "256 random-walk queries (k 4-12, edge/vertex 1.0-2.5) on chung-lu n=36692; level-synchronous "
+                   "batch queue replayed as one hipGraph; learned orders; autotuned launch shape",
                    {"queries": len(qs), "sum_count": sum(counts),
                     "batch_learned_ms": ms, "batch_heuristic_ms": rh.device_ms,
                     "amortised_ms_per_query_batch_learned": amort,
@@ -202,6 +213,37 @@ def run_batch256(args):
                     "p50_latency_ms_single_heuristic_sequential_all_submitted_at_t0": statistics.median(cum),
                     "speedup_p50_latency": statistics.median(cum) / statistics.median(lat),
                     "fallback_rate": fallback, "tuned": best, "training": info}))
+
+
+def train_on_queries(ctx, dg, g, queries, seed, log):
+    """Tables logged by the solver itself: for each (k, edges, work) the
+    score order's run plus one random connected order's run (capped at 4x
+    the score order's work)."""
+    import numpy as np
+    import cuts_amd
+    from cuts_amd import learned as LN
+    rng = np.random.default_rng(seed)
+    samples, targets = [], []
+    t0 = time.perf_counter()
+    for k, edges, work in queries:
+        q = cuts_amd.Graph.from_edges(k, edges)
+        adj = LN.adjacency(k, edges)
+        for order, cap in ((None, 0), (LN.random_connected_order(k, adj, rng), 4 * work + 4096)):
+            p = cuts_amd.Plan(q, order)
+            r = ctx.match(dg, p, max_work=cap)
+            if r.work_capped:
+                continue
+            for kk, a, sm, size in LN.table_samples(k, edges, p.order, r.level_sizes):
+                samples.append((kk, a, sm))
+                targets.append(LN.target(size))
+    t1 = time.perf_counter()
+    x, A, meta = LN.encode(samples, LN.graph_features(g))
+    net = LN.Mpnn(ctx, seed=seed)
+    hist = net.fit(x, A, meta, np.asarray(targets, np.float32), epochs=30, batch=4096, lr=2e-3)
+    t2 = time.perf_counter()
+    log(f"trained on {len(samples)} samples of {len(queries)} queries: mse {hist[0]:.4f} -> {hist[-1]:.4f}")
+    return net, {"queries": len(queries), "samples": len(samples), "mse_first": hist[0], "mse_last": hist[-1],
+                 "log_s": t1 - t0, "train_s": t2 - t1}
 
 
 def train_on_graph(ctx, dg, g, seed, n_queries, log, kmin=5, kmax=10, max_work=1 << 27):
@@ -381,18 +423,19 @@ def main():
             per_query_ms.append(r.device_ms)
             counts.append(r.count)
             for name, (ms, n, b) in r.kernels.items():
-                k = kern.setdefault(name, [0.0, 0, 0.0])
+                k = kern.setdefault(name, [0.0, 0, 0.0, 0.0])
                 k[0] += ms
                 k[1] += n
                 k[2] += b
+                k[3] += r.merge_bytes[name]
         print(f"bench: timed step done, last query {per_query_ms[-1]:.2f} ms", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     total_ms = sum(per_query_ms)
     total_emb = sum(counts)
     dom = max(kern.items(), key=lambda kv: kv[1][0])
-    dname, (dms, dn, dbytes) = dom
+    dname, (dms, dn, dbytes, dmerge) = dom
     avg_launch_s = dms / dn / 1e3
-    bytes_per_launch = dbytes / dn
+    bytes_per_launch = dmerge / dn
     achieved = bytes_per_launch / avg_launch_s / 1e9
     line = {
         "metric": "exact embeddings/sec + p50 time/query; learned vs heuristic order, same GPU",
@@ -424,9 +467,13 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic_for(dname),
             "algorithmic_bytes_per_launch": bytes_per_launch,
+            "bytes_model": "merge model (8+4*len per distinct backward list per prefix, 16 per trie node)",
             "avg_launch_ms": avg_launch_s * 1e3,
             "launches": dn,
+            "demand_bytes_per_launch": dbytes / dn,
+            "demand_GBs": dbytes / dn / avg_launch_s / 1e9,
         },
+        "merge_bytes_total": sum(v[3] for v in kern.values()),
         "kernels_ms": {k: v[0] for k, v in kern.items() if v[1]},
     }
     if args.copy_probe:

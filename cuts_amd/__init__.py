@@ -54,13 +54,13 @@ class _Result(C.Structure):
                 ("emb_written", C.c_uint64), ("chunks", C.c_uint32), ("complete", C.c_uint32),
                 ("kernel_ms", C.c_double * 8), ("kernel_launches", C.c_uint64 * 8), ("kernel_bytes", C.c_double * 8),
                 ("roots_done", C.c_uint64), ("last_root", C.c_uint32), ("work_capped", C.c_uint32),
-                ("work", C.c_uint64)]
+                ("work", C.c_uint64), ("merge_bytes", C.c_double * 8)]
 
 
 class _BatchOpts(C.Structure):
     _fields_ = [("flags", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("vchunk", C.c_uint32),
                 ("no_graph", C.c_uint32), ("queue_cap", C.c_uint64), ("arena_bytes", C.c_uint64),
-                ("wait_limit_s", C.c_double)]
+                ("ticket_queue", C.c_uint32), ("item_work", C.c_uint32)]
 
 
 class _BatchResult(C.Structure):
@@ -269,6 +269,7 @@ class MatchResult:
     last_root: int = 0                 # the last of them (ascending ids)
     work_capped: bool = False          # max_work stopped the run; count invalid
     work: int = 0                      # candidates examined (sum of scan totals)
+    merge_bytes: Optional[dict] = None # FLAG_PROFILE: class -> merge-model algorithmic bytes
 
 
 class DeviceGraph:
@@ -333,6 +334,7 @@ class Context:
         if o.flags & FLAG_PROFILE:
             res.kernels = {KERNEL_CLASSES[i]: (float(r.kernel_ms[i]), int(r.kernel_launches[i]),
                                                float(r.kernel_bytes[i])) for i in range(8)}
+            res.merge_bytes = {KERNEL_CLASSES[i]: float(r.merge_bytes[i]) for i in range(8)}
         return res
 
 
@@ -346,17 +348,22 @@ class BatchResult:
 
 
 class Batch:
-    """Many queries against one resident data graph in one persistent,
-    hipGraph-captured launch (device work queue)."""
+    """Many queries against one resident data graph: a level-synchronous
+    device work queue, one persistent-grid launch per trie level, the whole
+    sequence captured once in a hipGraph.  ``queue="ticket"`` selects the
+    asynchronous ticket-queue kernel, which is disabled (creation fails)."""
 
     def __init__(self, ctx: Context, dg: DeviceGraph, plans: Sequence[Plan], flags: int = 0,
                  blocks_per_cu: int = 0, vchunk: int = 0, use_graph: bool = True, queue_cap: int = 0,
-                 arena_bytes: int = 0, wait_limit_s: float = 0.0):
+                 arena_bytes: int = 0, item_work: int = 0, queue: str = "level"):
+        if queue not in ("level", "ticket"):
+            raise ValueError("queue must be 'level' or 'ticket'")
         self.ctx, self.dg, self.plans = ctx, dg, list(plans)
         o = _BatchOpts()
         lib().cuts_batch_options_default(C.byref(o))
         o.flags, o.blocks_per_cu, o.vchunk, o.no_graph = flags, blocks_per_cu, vchunk, 0 if use_graph else 1
-        o.queue_cap, o.arena_bytes, o.wait_limit_s = queue_cap, arena_bytes, wait_limit_s
+        o.queue_cap, o.arena_bytes, o.item_work = queue_cap, arena_bytes, item_work
+        o.ticket_queue = 1 if queue == "ticket" else 0
         arr = (C.c_void_p * len(self.plans))(*[p._h.value for p in self.plans])
         h = C.c_void_p()
         _check(lib().cuts_batch_create(ctx._h, dg._h, arr, len(self.plans), C.byref(o), C.byref(h)))

@@ -9,8 +9,8 @@ workload) in ``$CUTS_TUNE_CACHE`` (default ``~/.cache/cuts_amd/tune.json``).
 Tunables:
   single query -- blocks per CU of the expansion kernels (grid = CUs x n) and
                   the XCD-contiguous tile mapping;
-  batch        -- persistent blocks per CU (1..4) and level-0 vertices per
-                  initial work item.
+  batch        -- blocks per CU of the level launches (1..4) and the candidate
+                  work at which a new work item is started.
 """
 from __future__ import annotations
 
@@ -83,23 +83,23 @@ def tune_single(ctx: Context, dg: DeviceGraph, plans: Sequence[Plan], key: str, 
 
 
 def tune_batch(ctx: Context, dg: DeviceGraph, plans: Sequence[Plan], key: str, reps: int = 2,
-               blocks: Sequence[int] = (1, 2, 4), vchunks: Sequence[int] = (1024, 4096, 16384),
-               arena_bytes: int = 32 << 30, log=None, use_cache: bool = True) -> Dict:
+               blocks: Sequence[int] = (1, 2, 4), item_works: Sequence[int] = (4096, 16384, 65536),
+               arena_bytes: int = 32 << 30, queue_cap: int = 0, log=None, use_cache: bool = True) -> Dict:
     cache = _load()
     if use_cache and key in cache:
         return cache[key]
     points: List[Dict] = []
     ref = None
     for bpc in blocks:
-        for vc in vchunks:
-            b = Batch(ctx, dg, plans, blocks_per_cu=bpc, vchunk=vc, arena_bytes=arena_bytes)
+        for iw in item_works:
+            b = Batch(ctx, dg, plans, blocks_per_cu=bpc, item_work=iw, arena_bytes=arena_bytes, queue_cap=queue_cap)
             runs = [b.run() for _ in range(reps)]
             for r in runs:
                 if ref is None:
                     ref = r.counts
                 elif r.counts != ref:
-                    raise CountMismatch(f"batch counts differ at blocks_per_cu={bpc} vchunk={vc}")
-            pt = dict(blocks_per_cu=bpc, vchunk=vc, ms=min(r.device_ms for r in runs),
+                    raise CountMismatch(f"batch counts differ at blocks_per_cu={bpc} item_work={iw}")
+            pt = dict(blocks_per_cu=bpc, item_work=iw, ms=min(r.device_ms for r in runs),
                       p50_latency_ms=statistics.median(runs[-1].latency_ms))
             points.append(pt)
             if log:

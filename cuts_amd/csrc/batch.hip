@@ -1,22 +1,34 @@
-// Batched matching: many queries against one resident data graph in a single
-// persistent launch.  No reference counterpart (cuTS runs one query per
-// process and syncs the host after every level, src/host_funcs.cu:141-157;
-// its MPI driver distributes level-0 trunks, main_mpi.cu:257-376).
+// Batched matching: many queries against one resident data graph.  No
+// reference counterpart (cuTS runs one query per process and syncs the host
+// after every level, src/host_funcs.cu:141-157; its MPI driver distributes
+// level-0 trunks, main_mpi.cu:257-376).
 //
-// Work items (query, level, row chunk) live in a device ticket queue:
-//   * a block takes the next ticket with one atomicAdd and waits until that
-//     slot is published (ready[slot] == epoch) -- tickets are never reused,
-//     so there is no ABA race and no CAS loop;
+// Level-synchronous device work queue of (query, level, row chunk) items.
+// All items of level L exist before the launch that processes level L:
+//   * launch L runs a persistent grid; a block takes the next item of level
+//     L with one atomicAdd and exits when none is left.  No block ever waits
+//     for another, so every wave ends, whatever the grid size or residency;
 //   * an item's rows are expanded with the same tests as the single-query
 //     kernels (signature, injectivity, constraint membership, closed-form
-//     last level); surviving rows are staged in LDS, flushed to a bump
-//     arena with an exact allocation, and published as new items;
-//   * `pending` counts published-but-unfinished items; a block whose ticket
-//     lies beyond the final tail exits once pending reaches 0, and every
-//     wait is also bounded by a wall-clock limit, so the grid always drains;
-//   * the epoch is bumped on the device by the init kernel, so the whole
-//     launch sequence can be captured once in a hipGraph and replayed
-//     without resetting the ready flags.
+//     last level, fusion of the last two levels); surviving rows are staged
+//     in LDS and flushed to the arena half of level L+1 as new items.  A
+//     flush happens when the stage holds RMAX rows or when the staged rows'
+//     candidate work for level L+1 (their shortest constraint lists) reaches
+//     `item_work`, so hub rows do not pile up in one item;
+//   * the kernel boundary between launches publishes level L+1 to every
+//     XCD (the XCDs' L2 caches are not kept coherent with each other inside
+//     a launch, so no item is consumed in the launch that wrote it);
+//   * all queries advance together: the launch sequence (init + one launch
+//     per level up to the largest query size) is captured once in a hipGraph
+//     and replayed.
+// Loops that contain block barriers keep every branch around their edges
+// wave-uniform: the single-lane work (ticket, arena allocation, item push)
+// is done by all 64 lanes of wave 0 under a wave-uniform branch, with atomic
+// operands masked to lane 0 and results broadcast by readfirstlane.  A
+// `threadIdx.x == 0` region at a loop's back edge lets the compiler split
+// the loop into lane-0 and other-lane latches (nested loops after
+// structurisation); the other lanes then pass the next barrier without lane
+// 0's update and spin on a stale item.
 // Per query: exact uint64 count, per-level table sizes and the wall-clock
 // tick at which its last item finished.
 #include <hip/hip_runtime.h>
@@ -30,7 +42,8 @@ namespace cuts {
 namespace {
 
 constexpr int BB = 256;      // threads per block (4 wave64)
-constexpr int RMAX = 256;    // rows per item
+constexpr int NW = BB / 64;  // waves per block
+constexpr int RMAX = 256;    // rows per item (one per thread in the scan)
 constexpr int WROW = 16;     // LDS words per staged row (k <= 16)
 
 struct GV {
@@ -40,6 +53,14 @@ struct GV {
   const uint64_t* bits;
   uint32_t words;
 };
+
+__device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint64_t uni64(uint64_t x) {
+  return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(x >> 32)) << 32) |
+         __builtin_amdgcn_readfirstlane((uint32_t)x);
+}
+__device__ __forceinline__ uint32_t wave_id() { return uni(threadIdx.x >> 6); }
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63; }
 
 __device__ __forceinline__ uint32_t lane_rank64(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -64,11 +85,6 @@ __device__ __forceinline__ bool has_arc(const GV& g, uint32_t u, uint32_t x) {
   return bmember(g.adj_out, g.off_out[u], g.off_out[u + 1], x);
 }
 
-template <typename T>
-__device__ __forceinline__ T aload(T* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Closed-form count of the last position given positions r[0..n) (LDS row)
 // plus, when has_v, the candidate v at position n.
 __device__ uint64_t closed_form_rt(const GV& g, const KPlan& P, const uint32_t* r, uint32_t n, bool has_v, uint32_t v,
@@ -88,127 +104,37 @@ __device__ uint64_t closed_form_rt(const GV& g, const KPlan& P, const uint32_t* 
   return (uint64_t)c;
 }
 
+// Candidate work of the row row[0..L) + v at its next position L+1: the
+// length of its shortest constraint list (what the consumer will scan), or
+// the closed form's membership tests when L+1 is a closed-form last position.
+__device__ __forceinline__ uint64_t next_work(const GV& g, const KPlan& P, const uint32_t* row, uint32_t L, uint32_t v,
+                                              bool closed) {
+  const uint32_t p = L + 1;
+  if (p == P.k - 1 && closed) return p;
+  uint32_t best = 0xFFFFFFFFu;
+  for (uint32_t c = 0; c < P.nj[p]; ++c) {
+    const uint32_t jc = P.jc[p][c];
+    const uint32_t pos = jc & 31;
+    const uint32_t u = pos == L ? v : row[pos];
+    const uint32_t* off = (jc >> 7) ? g.off_in : g.off_out;
+    best = min(best, off[u + 1] - off[u]);
+  }
+  return best;
+}
+
 struct Shared {
   Item item;
-  int cmd;
-  uint32_t rows[RMAX * WROW];    // input rows of the item
-  uint32_t minlen[RMAX];
-  uint8_t minsel[RMAX];
-  uint32_t off[RMAX + 1];        // exclusive scan of minlen
-  uint32_t stage[RMAX * WROW];   // output rows waiting for a flush
-  uint32_t nstage;
-  uint32_t wcnt[BB / 64], wpre[BB / 64];
+  uint32_t cmd;
+  uint32_t nstage;                // rows waiting in `stage`
+  uint64_t wstage;                // their candidate work at the next level
   uint64_t base;
+  uint32_t wcnt[NW];
+  uint64_t wwork[NW];
+  uint64_t off[RMAX + 1];         // exclusive scan of the rows' shortest list lengths
+  uint8_t minsel[RMAX];
+  uint32_t rows[RMAX * WROW];     // input rows of the item
+  uint32_t stage[RMAX * WROW];    // output rows waiting for a flush
 };
-
-// Data handed from one block to another inside the persistent kernel (items,
-// arena rows) goes through agent-scope atomic loads and stores: the XCDs of
-// MI355X have separate L2 caches that ordinary loads do not keep coherent,
-// so a plain load could return a line cached on the reader's XCD before the
-// writer on another XCD filled it.
-__device__ __forceinline__ void store_item(Item* dst, const Item& it) {
-  unsigned long long* d = reinterpret_cast<unsigned long long*>(dst);
-  const unsigned long long w0 = ((unsigned long long)it.n << 32) | it.qL;
-  __hip_atomic_store(d, w0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(d + 1, (unsigned long long)it.off, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ Item load_item(const Item* src) {
-  unsigned long long* p = const_cast<unsigned long long*>(reinterpret_cast<const unsigned long long*>(src));
-  const unsigned long long w0 = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const unsigned long long w1 = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  Item it;
-  it.qL = (uint32_t)w0;
-  it.n = (uint32_t)(w0 >> 32);
-  it.off = w1;
-  return it;
-}
-__device__ __forceinline__ void store_word(uint32_t* p, uint32_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint32_t load_word(const uint32_t* p) {
-  return __hip_atomic_load(const_cast<uint32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ bool push_item(const BatchArgs& a, uint32_t epoch, Item it) {
-  const uint32_t q = it.qL >> 8;
-  atomicAdd(&a.qpend[q], 1ull);
-  atomicAdd(&a.st->pending, 1ull);
-  const unsigned long long slot = atomicAdd(&a.st->tail, 1ull);
-  if (slot >= a.qcap) {
-    atomicExch(&a.st->abort, 1u);
-    atomicSub(&a.st->pending, 1ull);
-    atomicSub(&a.qpend[q], 1ull);
-    return false;
-  }
-  store_item(&a.items[slot], it);
-  __threadfence();
-  __hip_atomic_store(&a.ready[slot], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-  return true;
-}
-
-// Writes the staged rows (row length `words`) to the arena and publishes
-// them as one item of level `L`.  Called by the whole block.
-__device__ void flush_stage(const BatchArgs& a, Shared& s, uint32_t epoch, uint32_t q, uint32_t words, uint32_t L) {
-  __syncthreads();
-  const uint32_t n = s.nstage;
-  if (n == 0) return;
-  if (threadIdx.x == 0) {
-    const unsigned long long need = (unsigned long long)n * words;
-    const unsigned long long o = atomicAdd(&a.st->arena_head, need);
-    if (o + need > a.arena_cap) {
-      atomicExch(&a.st->abort, 2u);
-      s.base = ~0ull;
-    } else {
-      s.base = o;
-    }
-  }
-  __syncthreads();
-  const uint64_t base = s.base;
-  if (base != ~0ull)
-    for (uint32_t i = threadIdx.x; i < n * words; i += BB)
-      store_word(&a.arena[base + i], s.stage[(i / words) * WROW + i % words]);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (base != ~0ull) {
-      __threadfence();
-      Item it;
-      it.qL = (q << 8) | L;
-      it.n = n;
-      it.off = base;
-      push_item(a, epoch, it);
-    }
-    s.nstage = 0;
-  }
-  __syncthreads();
-}
-
-// Block-wide append of per-thread survivors (row + optional v) to the stage,
-// flushing first when they would not fit.
-__device__ void stage_append(const BatchArgs& a, Shared& s, uint32_t epoch, uint32_t q, bool ok,
-                             const uint32_t* row, uint32_t L, bool has_v, uint32_t v) {
-  const uint32_t tid = threadIdx.x, wave = tid >> 6;
-  const uint64_t m = __ballot(ok);
-  if ((tid & 63) == 0) s.wcnt[wave] = __popcll(m);
-  __syncthreads();
-  if (tid == 0) {
-    uint32_t t = 0;
-    for (int w = 0; w < BB / 64; ++w) { s.wpre[w] = t; t += s.wcnt[w]; }
-    s.wcnt[0] = t;  // reused as the round total after the prefix is taken
-  }
-  __syncthreads();
-  const uint32_t total = s.wcnt[0];
-  const uint32_t words = L + (has_v ? 1 : 0);
-  if (s.nstage + total > RMAX) flush_stage(a, s, epoch, q, words, words);
-  if (ok) {
-    const uint32_t slot = s.nstage + s.wpre[wave] + lane_rank64(m);
-    uint32_t* dst = &s.stage[slot * WROW];
-    for (uint32_t l = 0; l < L; ++l) dst[l] = row[l];
-    if (has_v) dst[L] = v;
-  }
-  __syncthreads();
-  if (tid == 0) s.nstage += total;
-  __syncthreads();
-}
 
 __device__ uint64_t wave_sum(uint64_t v) {
   uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
@@ -222,20 +148,117 @@ __device__ uint64_t wave_sum(uint64_t v) {
   return s;
 }
 
-__device__ void add_block(unsigned long long* dst, uint64_t v) {
-  const uint64_t s = wave_sum(v);
-  if ((threadIdx.x & 63) == 0 && s) atomicAdd(dst, (unsigned long long)s);
+__device__ uint64_t wave_incl_scan(uint64_t x) {
+  const uint32_t lane = threadIdx.x & 63;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t lo = __shfl_up((uint32_t)x, o), hi = __shfl_up((uint32_t)(x >> 32), o);
+    if (lane >= (uint32_t)o) x += ((uint64_t)hi << 32) | lo;
+  }
+  return x;
 }
 
-__device__ void process(const BatchArgs& a, Shared& s, uint32_t epoch) {
+// One atomic per wave (operand masked to lane 0; the wave total is uniform).
+__device__ void add_block(unsigned long long* dst, uint64_t v) {
+  const uint64_t s = uni64(wave_sum(v));
+  if (s) atomicAdd(dst, lane_id() == 0 ? (unsigned long long)s : 0ull);
+}
+
+// Called by all 64 lanes of one wave.
+__device__ void push_item(const BatchArgs& a, uint32_t level, const Item& it) {
+  const bool l0 = lane_id() == 0;
+  const unsigned long long slot = uni64(atomicAdd(&a.st->tail[level].v, l0 ? 1ull : 0ull));
+  if (slot >= a.qcap) {
+    atomicOr(&a.st->abort.v, l0 ? 1ull : 0ull);
+    return;
+  }
+  atomicAdd(&a.qpend[it.qL >> 8], l0 ? 1ull : 0ull);
+  a.items[(uint64_t)(level & 1) * a.qcap + slot] = it;  // same 16 bytes from every lane
+}
+
+// Writes the staged rows (length `words`, i.e. level `words`) to that
+// level's arena half and publishes them as one item.  Whole block.
+__device__ void flush_stage(const BatchArgs& a, Shared& s, uint32_t q, uint32_t words) {
+  __syncthreads();
+  const uint32_t n = uni(s.nstage);
+  if (n == 0) return;
+  const bool l0 = lane_id() == 0;
+  if (wave_id() == 0) {
+    const unsigned long long need = (unsigned long long)n * words;
+    const unsigned long long o = uni64(atomicAdd(&a.st->arena[words].v, l0 ? need : 0ull));
+    unsigned long long base = o;
+    if (o + need > a.half_words) {
+      atomicOr(&a.st->abort.v, l0 ? 2ull : 0ull);
+      base = ~0ull;
+    }
+    s.base = base;
+  }
+  __syncthreads();
+  const uint64_t base = uni64(s.base);
+  if (base != ~0ull) {
+    uint32_t* dst = a.arena + (uint64_t)(words & 1) * a.half_words + base;
+    for (uint32_t i = threadIdx.x; i < n * words; i += BB) dst[i] = s.stage[(i / words) * WROW + i % words];
+    if (wave_id() == 0) {
+      Item it;
+      it.qL = (q << 8) | words;
+      it.n = n;
+      it.off = base;
+      push_item(a, words, it);
+    }
+  }
+  s.nstage = 0;  // every thread read nstage before the barrier above
+  s.wstage = 0;
+  __syncthreads();
+}
+
+// Block-wide append of the survivors row[0..L) + v (one per thread with ok)
+// to the stage, flushing first when they would overfill it.
+__device__ void stage_append(const BatchArgs& a, Shared& s, uint32_t q, bool ok, const uint32_t* row, uint32_t L,
+                             uint32_t v, uint64_t work) {
+  const uint32_t wave = wave_id();
+  const uint64_t m = __ballot(ok);
+  const uint64_t wsum = uni64(wave_sum(ok ? work : 0));
+  s.wcnt[wave] = (uint32_t)__popcll(m);  // same value from every lane of the wave
+  s.wwork[wave] = wsum;
+  __syncthreads();
+  uint32_t total = 0, pre = 0;
+  uint64_t wtot = 0;
+  for (uint32_t w = 0; w < (uint32_t)NW; ++w) {
+    if (w == wave) pre = total;
+    total += uni(s.wcnt[w]);
+    wtot += uni64(s.wwork[w]);
+  }
+  uint32_t nst = uni(s.nstage);
+  uint64_t wst = uni64(s.wstage);
+  if (total) {
+    if (nst + total > RMAX || (nst > 0 && wst + wtot > a.item_work)) {
+      flush_stage(a, s, q, L + 1);
+      nst = 0;
+      wst = 0;
+    }
+    if (ok) {
+      uint32_t* dst = &s.stage[(nst + pre + lane_rank64(m)) * WROW];
+      for (uint32_t l = 0; l < L; ++l) dst[l] = row[l];
+      dst[L] = v;
+    }
+    __syncthreads();
+    s.nstage = nst + total;  // same value from every thread
+    s.wstage = wst + wtot;
+  }
+  __syncthreads();
+}
+
+__device__ void process(const BatchArgs& a, Shared& s, uint32_t L) {
   const GV g{a.off_out, a.adj_out, a.off_in, a.adj_in, a.sig, a.V, a.hub_id, a.bits, a.words};
-  const Item it = s.item;
-  const uint32_t q = it.qL >> 8, L = it.qL & 0xFF;
+  Item it;
+  it.qL = uni(s.item.qL);
+  it.n = uni(s.item.n);
+  it.off = uni64(s.item.off);
+  const uint32_t q = it.qL >> 8;
   const KPlan& P = a.plans[q];
-  const uint32_t k = P.k;
+  const uint32_t k = uni(P.k);
   const uint32_t tid = threadIdx.x;
   const uint32_t* tbl = a.tbls[q];
-  const bool closed = (a.flags & 1u) == 0 && P.sc_ok;
+  const bool closed = (a.flags & 1u) == 0 && uni(P.sc_ok);
   const bool fuse = closed && (a.flags & 2u) == 0 && k >= 3;
   uint64_t cnt = 0, surv = 0;
 
@@ -243,56 +266,65 @@ __device__ void process(const BatchArgs& a, Shared& s, uint32_t epoch) {
     // level-0 generation over vertices [off, off + n)
     for (uint32_t i0 = 0; i0 < it.n; i0 += BB) {
       const uint32_t i = i0 + tid;
-      const uint32_t u = (uint32_t)it.off + i;
-      const bool ok = i < it.n && g.sig[2 * (uint64_t)u] >= P.sig_in[0] && g.sig[2 * (uint64_t)u + 1] >= P.sig_out[0];
+      const bool in = i < it.n;
+      const uint32_t u = (uint32_t)it.off + (in ? i : 0);  // it.n >= 1 whenever this loop runs
+      const bool ok = in && (g.sig[2 * (uint64_t)u] >= P.sig_in[0]) & (g.sig[2 * (uint64_t)u + 1] >= P.sig_out[0]);
       surv += ok;
-      if (k == 1) continue;
-      stage_append(a, s, epoch, q, ok, &u, 1, false, 0);
+      if (k > 1) stage_append(a, s, q, ok, nullptr, 0, u, ok ? next_work(g, P, nullptr, 0, u, closed) : 0);
     }
-    if (k > 1) flush_stage(a, s, epoch, q, 1, 1);
+    if (k > 1) flush_stage(a, s, q, 1);
     add_block(&a.levels[q * kMaxQuery + 0], surv);
     if (k == 1) add_block(&a.counts[q], surv);
     return;
   }
 
   const uint32_t n = it.n;
-  const uint32_t* src = a.arena + it.off;
-  for (uint32_t i = tid; i < n * L; i += BB) s.rows[(i / L) * WROW + i % L] = load_word(&src[i]);
+  const uint32_t* src = a.arena + (uint64_t)(L & 1) * a.half_words + it.off;
+  for (uint32_t i = tid; i < n * L; i += BB) s.rows[(i / L) * WROW + i % L] = src[i];
   __syncthreads();
 
   if (L == k - 1 && closed) {
-    for (uint32_t r = tid; r < n; r += BB) cnt += closed_form_rt(g, P, &s.rows[r * WROW], L, false, 0, tbl);
+    if (tid < n) cnt = closed_form_rt(g, P, &s.rows[tid * WROW], L, false, 0, tbl);
     add_block(&a.counts[q], cnt);
     add_block(&a.levels[q * kMaxQuery + L], cnt);
     return;
   }
 
-  // shortest constraint list per row
-  const uint32_t nj = P.nj[L];
-  for (uint32_t r = tid; r < n; r += BB) {
+  // shortest constraint list per row (one row per thread), then a block scan
+  const uint32_t nj = uni(P.nj[L]);
+  uint64_t len = 0;
+  if (tid < n) {
     uint32_t best = 0xFFFFFFFFu, sel = 0;
     for (uint32_t c = 0; c < nj; ++c) {
       const uint32_t jc = P.jc[L][c];
-      const uint32_t u = s.rows[r * WROW + (jc & 31)];
+      const uint32_t u = s.rows[tid * WROW + (jc & 31)];
       const uint32_t* off = (jc >> 7) ? g.off_in : g.off_out;
-      const uint32_t len = off[u + 1] - off[u];
-      if (len < best) { best = len; sel = c; }
+      const uint32_t l = off[u + 1] - off[u];
+      if (l < best) {
+        best = l;
+        sel = c;
+      }
     }
-    s.minlen[r] = best;
-    s.minsel[r] = (uint8_t)sel;
+    s.minsel[tid] = (uint8_t)sel;
+    len = best;
   }
+  const uint64_t incl = wave_incl_scan(len);
+  if ((tid & 63) == 63) s.wwork[tid >> 6] = incl;
   __syncthreads();
-  if (tid == 0) {
-    uint32_t t = 0;
-    for (uint32_t r = 0; r < n; ++r) { s.off[r] = t; t += s.minlen[r]; }
-    s.off[n] = t;
+  uint64_t before = 0, W = 0;
+  for (uint32_t w = 0; w < (uint32_t)NW; ++w) {
+    const uint64_t x = uni64(s.wwork[w]);
+    if (w < wave_id()) before += x;
+    W += x;
   }
+  if (tid < n) s.off[tid] = before + incl - len;
+  s.off[n] = W;  // same value from every thread; no thread writes index n above
   __syncthreads();
-  const uint32_t W = s.off[n];
+
   const int mode = (L == k - 1) ? 1 : ((L == k - 2 && fuse) ? 2 : 0);
   const uint32_t tin = P.sig_in[L], tout = P.sig_out[L];
-  for (uint32_t w0 = 0; w0 < W; w0 += BB) {
-    const uint32_t w = w0 + tid;
+  for (uint64_t w0 = 0; w0 < W; w0 += BB) {
+    const uint64_t w = w0 + tid;
     bool ok = false;
     uint32_t v = 0, r = 0;
     if (w < W) {
@@ -303,7 +335,7 @@ __device__ void process(const BatchArgs& a, Shared& s, uint32_t epoch) {
         else hi = mid;
       }
       r = lo - 1;
-      const uint32_t idx = w - s.off[r];
+      const uint32_t idx = (uint32_t)(w - s.off[r]);
       const uint32_t* row = &s.rows[r * WROW];
       const uint32_t sel = s.minsel[r];
       const uint32_t jc = P.jc[L][sel];
@@ -326,19 +358,20 @@ __device__ void process(const BatchArgs& a, Shared& s, uint32_t epoch) {
         }
       }
     }
+    const uint32_t* row = &s.rows[r * WROW];
     if (mode == 1) {
       cnt += ok;
     } else if (mode == 2) {
       if (ok) {
         ++surv;
-        cnt += closed_form_rt(g, P, &s.rows[r * WROW], L, true, v, tbl);
+        cnt += closed_form_rt(g, P, row, L, true, v, tbl);
       }
     } else {
       surv += ok;
-      stage_append(a, s, epoch, q, ok, &s.rows[r * WROW], L, true, v);
+      stage_append(a, s, q, ok, row, L, v, ok ? next_work(g, P, row, L, v, closed) : 0);
     }
   }
-  if (mode == 0) flush_stage(a, s, epoch, q, L + 1, L + 1);
+  if (mode == 0) flush_stage(a, s, q, L + 1);
   if (mode == 1) {
     add_block(&a.counts[q], cnt);
     add_block(&a.levels[q * kMaxQuery + L], cnt);
@@ -353,102 +386,82 @@ __device__ void process(const BatchArgs& a, Shared& s, uint32_t epoch) {
 
 }  // namespace
 
-// Resets the queue (device side, so it can be replayed inside a hipGraph):
-// bumps the epoch, clears counters and publishes the level-0 items.
-__global__ __launch_bounds__(BB) void k_batch_init(BatchArgs a, uint32_t nq, uint32_t vchunk, uint32_t nch) {
+// Resets the queue on the device (so the sequence replays inside a hipGraph)
+// and publishes the level-0 items: vchunk vertices of one query each.
+__global__ __launch_bounds__(BB) void k_batch_init(BatchArgs a, uint32_t vchunk, uint32_t nch) {
   const uint64_t tid = (uint64_t)blockIdx.x * BB + threadIdx.x;
   const uint64_t nthreads = (uint64_t)gridDim.x * BB;
-  const uint32_t epoch = a.st->epoch + 1;  // every thread reads the old value first
-  __syncthreads();
-  for (uint64_t i = tid; i < (uint64_t)nq * kMaxQuery; i += nthreads) a.levels[i] = 0;
-  for (uint64_t q = tid; q < nq; q += nthreads) {
+  for (uint64_t i = tid; i < (uint64_t)a.nq * kMaxQuery; i += nthreads) a.levels[i] = 0;
+  for (uint64_t q = tid; q < a.nq; q += nthreads) {
     a.counts[q] = 0;
     a.qpend[q] = nch;
     a.done[q] = 0;
   }
-  for (uint64_t i = tid; i < (uint64_t)nq * nch; i += nthreads) {
+  for (uint64_t i = tid; i < (uint64_t)a.nq * nch; i += nthreads) {
     const uint32_t q = (uint32_t)(i / nch), c = (uint32_t)(i % nch);
     Item it;
     it.qL = q << 8;
     it.off = (uint64_t)c * vchunk;
     it.n = (uint32_t)std::min<uint64_t>(vchunk, a.V - it.off);
-    store_item(&a.items[i], it);
-    __threadfence();
-    __hip_atomic_store(&a.ready[i], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    a.items[i] = it;
   }
   if (tid == 0) {
-    a.st->head = 0;
-    a.st->tail = (unsigned long long)nq * nch;
-    a.st->pending = (unsigned long long)nq * nch;
-    a.st->arena_head = 0;
-    a.st->abort = 0;
-    a.st->t_start = wall_clock64();
+    for (uint32_t l = 0; l <= kMaxQuery; ++l) {
+      a.st->head[l].v = 0;
+      a.st->tail[l].v = 0;
+      a.st->arena[l].v = 0;
+    }
+    a.st->tail[0].v = (unsigned long long)a.nq * nch;
+    a.st->abort.v = 0;
+    a.st->t_start.v = wall_clock64();
   }
 }
 
-// Publishes the new epoch after k_batch_init has finished (same stream).
-__global__ void k_batch_epoch(BatchArgs a) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) a.st->epoch = a.st->epoch + 1;
-}
-
-__global__ __launch_bounds__(BB) void k_batch(BatchArgs a) {
+// Processes every item of level L (published by the earlier launches).
+__global__ __launch_bounds__(BB) void k_batch_level(BatchArgs a, uint32_t L) {
   __shared__ Shared s;
-  const uint32_t epoch = aload(&a.st->epoch);
-  if (threadIdx.x == 0) s.nstage = 0;
+  const bool l0 = lane_id() == 0;
+  const unsigned long long n_items = uni64(std::min<unsigned long long>(a.st->tail[L].v, a.qcap));
+  s.nstage = 0;  // same value from every thread
+  s.wstage = 0;
+  uint32_t prev_q = 0xFFFFFFFFu;  // query of the item finished in the last round
   while (true) {
-    if (threadIdx.x == 0) {
-      int cmd = 1;
-      const unsigned long long t = atomicAdd(&a.st->head, 1ull);
-      if (t < a.qcap) {
-        const uint64_t t0 = wall_clock64();
-        for (uint32_t spin = 0;; ++spin) {
-          if (aload(&a.ready[t]) == epoch) {
-            __threadfence();
-            const Item it = load_item(&a.items[t]);
-            // an item outside the shapes this kernel writes means the queue
-            // is corrupt: stop everything instead of looping over garbage
-            const uint32_t q = it.qL >> 8, L = it.qL & 0xFF;
-            const bool sane = q < a.nq && L < kMaxQuery &&
-                              (L == 0 ? (it.off < a.V && it.n <= a.V - it.off)
-                                      : (it.n >= 1 && it.n <= (uint32_t)RMAX &&
-                                         it.off + (uint64_t)it.n * L <= a.arena_cap));
-            if (!sane) {
-              atomicExch(&a.st->abort, 4u);
-              break;
-            }
+    if (wave_id() == 0) {
+      // retire the previous item, then take the next ticket
+      if (prev_q != 0xFFFFFFFFu) {
+        const unsigned long long left = uni64(atomicSub(&a.qpend[prev_q], l0 ? 1ull : 0ull));
+        if (left == 1ull) a.done[prev_q] = wall_clock64();
+      }
+      uint32_t cmd = 1;
+      if (uni64(__hip_atomic_load(&a.st->abort.v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 0) {
+        const unsigned long long t = uni64(atomicAdd(&a.st->head[L].v, l0 ? 1ull : 0ull));
+        if (t < n_items) {
+          const Item* ip = &a.items[(uint64_t)(L & 1) * a.qcap + t];
+          Item it;
+          it.qL = uni(ip->qL);
+          it.n = uni(ip->n);
+          it.off = uni64(ip->off);
+          // an item outside the shapes the kernels write means corrupt
+          // memory: stop instead of reading out of bounds
+          const bool sane = (it.qL >> 8) < a.nq && (it.qL & 0xFF) == L &&
+                            (L == 0 ? (it.off <= a.V && it.n <= a.V - it.off)
+                                    : (it.n >= 1 && it.n <= (uint32_t)RMAX &&
+                                       it.off + (uint64_t)it.n * L <= a.half_words));
+          if (sane) {
             s.item = it;
             cmd = 0;
-            break;
+          } else {
+            atomicOr(&a.st->abort.v, l0 ? 4ull : 0ull);
           }
-          // the shared counters are consulted on every 8th round only
-          if ((spin & 7) == 7) {
-            if (aload(&a.st->abort)) break;
-            if (aload(&a.st->pending) == 0) {
-              __threadfence();
-              if (aload(&a.st->tail) <= t) break;
-            }
-            if (wall_clock64() - t0 > a.wait_ticks) {
-              atomicExch(&a.st->abort, 3u);
-              break;
-            }
-          }
-          __builtin_amdgcn_s_sleep(32);
         }
-      } else {
-        atomicExch(&a.st->abort, 1u);
       }
       s.cmd = cmd;
     }
     __syncthreads();
-    if (s.cmd) break;
-    process(a, s, epoch);
+    if (uni(s.cmd)) break;
+    prev_q = uni(s.item.qL) >> 8;
+    process(a, s, L);
     __syncthreads();
-    if (threadIdx.x == 0) {
-      const uint32_t q = s.item.qL >> 8;
-      __threadfence();
-      if (atomicSub(&a.qpend[q], 1ull) == 1ull) a.done[q] = wall_clock64();
-      atomicSub(&a.st->pending, 1ull);
-    }
   }
 }
 
@@ -480,6 +493,9 @@ Batch::~Batch() {
 }
 
 void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& plans, const BatchOpts& o, Batch& b) {
+  if (o.ticket_queue)
+    throw Error(-1, "the asynchronous ticket-queue batch kernel is disabled: it stalled on MI355X (no progress, "
+                    "no fault) in its first test and again after a fix, with the cause not established");
   HIPCHECK(hipSetDevice(c.device));
   if (plans.empty()) throw Error(-1, "batch: no queries");
   if (plans.size() >= (1u << 24)) throw Error(-1, "batch: too many queries");
@@ -490,17 +506,20 @@ void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& p
   b.nch = (dg.V + b.vchunk - 1) / b.vchunk;
   if (b.nch == 0) b.nch = 1;
   b.use_graph = o.use_graph != 0;
-  const uint32_t bpc = o.blocks_per_cu ? std::min<uint32_t>(o.blocks_per_cu, 4) : 1;
+  const uint32_t bpc = o.blocks_per_cu ? std::min<uint32_t>(o.blocks_per_cu, 8) : 2;
   b.grid = (uint32_t)c.cus * bpc;
   const uint64_t qcap = o.queue_cap ? o.queue_cap : (1ull << 22);
   if ((uint64_t)b.nq * b.nch > qcap) throw Error(-1, "batch: queue too small for the level-0 items");
-  const uint64_t arena_words = (o.arena_bytes ? o.arena_bytes : (8ull << 30)) / 4;
+  const uint64_t half_words = (o.arena_bytes ? o.arena_bytes : (8ull << 30)) / 8;
+  if (half_words < 16) throw Error(-1, "batch: arena too small");
   std::vector<KPlan> kp(b.nq);
   std::vector<const uint32_t*> tb(b.nq, nullptr);
+  b.kmax = 0;
   for (uint32_t q = 0; q < b.nq; ++q) {
     if (plans[q]->k == 0 || plans[q]->k > kMaxQuery) throw Error(-3, "batch: query size out of range");
     kp[q] = make_kplan(*plans[q], dg.symmetric);
     b.k.push_back(plans[q]->k);
+    b.kmax = std::max(b.kmax, plans[q]->k);
     if (kp[q].sc_ok && kp[q].sc_thr > 1 && !(o.flags & 1u)) tb[q] = cnt_table(c, dg, kp[q].sc_thr);
   }
   BatchArgs& a = b.args;
@@ -521,12 +540,10 @@ void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& p
   a.tbls = dtb;
   a.st = balloc<BState>(b, 2, 1);
   HIPCHECK(hipMemset(a.st, 0, sizeof(BState)));
-  a.items = balloc<Item>(b, 3, qcap);
-  a.ready = balloc<uint32_t>(b, 4, qcap);
-  HIPCHECK(hipMemset(a.ready, 0, qcap * 4));
+  a.items = balloc<Item>(b, 3, 2 * qcap);
   a.qcap = qcap;
-  a.arena = balloc<uint32_t>(b, 5, arena_words);
-  a.arena_cap = arena_words;
+  a.arena = balloc<uint32_t>(b, 5, 2 * half_words);
+  a.half_words = half_words;
   a.counts = balloc<unsigned long long>(b, 6, b.nq);
   a.levels = balloc<unsigned long long>(b, 7, (uint64_t)b.nq * kMaxQuery);
   a.qpend = balloc<unsigned long long>(b, 8, b.nq);
@@ -534,8 +551,7 @@ void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& p
   int khz = 0;
   if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c.device) == hipSuccess && khz > 0)
     b.tick_khz = khz;
-  const double lim = o.wait_limit_s > 0 ? o.wait_limit_s : 300.0;
-  a.wait_ticks = (uint64_t)(lim * 1000.0 * b.tick_khz);
+  a.item_work = o.item_work ? o.item_work : 16384;
   a.flags = o.flags & 3u;
   a.nq = b.nq;
 }
@@ -543,10 +559,8 @@ void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& p
 static void batch_launch(Batch& b, hipStream_t s) {
   const uint64_t nitems = (uint64_t)b.nq * b.nch;
   const uint32_t igrid = (uint32_t)std::min<uint64_t>((nitems + BB - 1) / BB, 4096);
-  hipLaunchKernelGGL(k_batch_init, dim3(std::max<uint32_t>(igrid, 1)), dim3(BB), 0, s, b.args, b.nq, b.vchunk,
-                     b.nch);
-  hipLaunchKernelGGL(k_batch_epoch, dim3(1), dim3(64), 0, s, b.args);
-  hipLaunchKernelGGL(k_batch, dim3(b.grid), dim3(BB), 0, s, b.args);
+  hipLaunchKernelGGL(k_batch_init, dim3(std::max<uint32_t>(igrid, 1)), dim3(BB), 0, s, b.args, b.vchunk, b.nch);
+  for (uint32_t L = 0; L < b.kmax; ++L) hipLaunchKernelGGL(k_batch_level, dim3(b.grid), dim3(BB), 0, s, b.args, L);
 }
 
 void batch_run(Batch& b, std::vector<BatchQueryResult>& out, double& device_ms, uint64_t& arena_words) {
@@ -569,11 +583,8 @@ void batch_run(Batch& b, std::vector<BatchQueryResult>& out, double& device_ms, 
   device_ms = ms;
   BState st;
   HIPCHECK(hipMemcpy(&st, b.args.st, sizeof(BState), hipMemcpyDeviceToHost));
-  if (st.abort) {
-    const char* why = st.abort == 1   ? "work queue full"
-                      : st.abort == 2 ? "row arena full"
-                      : st.abort == 3 ? "wait limit reached"
-                                      : "corrupt work item";
+  if (st.abort.v) {
+    const char* why = (st.abort.v & 4) ? "corrupt work item" : (st.abort.v & 1) ? "work queue full" : "row arena full";
     throw Error(-5, std::string("batch aborted: ") + why + "; run fewer queries per batch or raise the limits");
   }
   std::vector<unsigned long long> cnt(b.nq), lev((size_t)b.nq * kMaxQuery), done(b.nq);
@@ -584,9 +595,10 @@ void batch_run(Batch& b, std::vector<BatchQueryResult>& out, double& device_ms, 
   for (uint32_t q = 0; q < b.nq; ++q) {
     out[q].count = cnt[q];
     for (uint32_t l = 0; l < kMaxQuery; ++l) out[q].level[l] = lev[(size_t)q * kMaxQuery + l];
-    out[q].latency_ms = done[q] > st.t_start ? (double)(done[q] - st.t_start) / b.tick_khz : 0.0;
+    out[q].latency_ms = done[q] > st.t_start.v ? (double)(done[q] - st.t_start.v) / b.tick_khz : 0.0;
   }
-  arena_words = st.arena_head;
+  arena_words = 0;
+  for (uint32_t l = 0; l <= kMaxQuery; ++l) arena_words = std::max<uint64_t>(arena_words, st.arena[l].v);
 }
 
 }  // namespace cuts

@@ -1,5 +1,5 @@
-// Batched persistent matcher: device structures shared by batch.hip and the
-// C ABI.  Not part of the public ABI.
+// Batched matcher: device structures shared by batch.hip and the C ABI.
+// Not part of the public ABI.
 #pragma once
 #include "device.hpp"
 
@@ -11,23 +11,18 @@ struct Item {      // 16 bytes
   uint64_t off;    // arena word offset of the rows (first vertex for level 0)
 };
 
-// Queue state.  Every field has a 256-byte line of its own: waiting blocks
-// poll some of them while working blocks update others, and on one shared
-// line those accesses would serialise in a single L2 channel.
+// One counter per 256-byte line: blocks of a launch hit head[L] while
+// others bump tail[L+1] and arena[L+1].
+struct alignas(256) PadCtr {
+  unsigned long long v;
+};
+
 struct BState {
-  unsigned long long head;
-  char pad0[248];
-  unsigned long long tail;
-  char pad1[248];
-  unsigned long long pending;
-  char pad2[248];
-  unsigned long long arena_head;
-  char pad3[248];
-  uint32_t abort;  // 1 queue full, 2 arena full, 3 wait limit, 4 corrupt item
-  char pad4[252];
-  uint32_t epoch;
-  char pad5[252];
-  unsigned long long t_start;
+  PadCtr head[kMaxQuery + 1];   // items of level L taken
+  PadCtr tail[kMaxQuery + 1];   // items of level L published
+  PadCtr arena[kMaxQuery + 1];  // arena words of level L allocated (in half L & 1)
+  PadCtr abort;                 // 1 item queue full, 2 arena full, 4 corrupt item
+  PadCtr t_start;
 };
 
 struct BatchArgs {
@@ -39,15 +34,14 @@ struct BatchArgs {
   const KPlan* plans;
   const uint32_t* const* tbls;  // per query: closed-form table or null
   BState* st;
-  Item* items;
-  uint32_t* ready;
+  Item* items;                  // 2 halves of qcap items; level L uses half L & 1
   uint64_t qcap;
-  uint32_t* arena;
-  uint64_t arena_cap;            // words
+  uint32_t* arena;              // 2 halves of half_words; level L rows in half L & 1
+  uint64_t half_words;
   unsigned long long *counts, *levels, *qpend, *done;
-  uint64_t wait_ticks;
-  uint32_t flags;                // 1: no closed form, 2: no fusion
-  uint32_t nq;                   // queries in the batch
+  uint64_t item_work;           // candidate work at which a staged item is flushed
+  uint32_t flags;               // 1: no closed form, 2: no fusion
+  uint32_t nq;                  // queries in the batch
 };
 
 struct BatchOpts {
@@ -57,7 +51,8 @@ struct BatchOpts {
   uint32_t use_graph = 1;
   uint64_t queue_cap = 0;
   uint64_t arena_bytes = 0;
-  double wait_limit_s = 0;
+  uint32_t ticket_queue = 0;
+  uint32_t item_work = 0;
 };
 
 struct BatchQueryResult {
@@ -69,7 +64,7 @@ struct BatchQueryResult {
 struct Batch {
   Context* ctx = nullptr;
   DeviceGraph* dg = nullptr;
-  uint32_t nq = 0, nch = 0, vchunk = 0, grid = 0;
+  uint32_t nq = 0, nch = 0, vchunk = 0, grid = 0, kmax = 0;
   BatchArgs args{};
   std::vector<uint32_t> k;   // query sizes
   void* allocs[12] = {nullptr};
@@ -82,7 +77,7 @@ struct Batch {
 
 void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& plans, const BatchOpts& o, Batch& b);
 // Runs the whole batch once; fills per-query results, the device time of the
-// launch sequence and the arena words used.
+// launch sequence and the peak arena words of one level.
 void batch_run(Batch& b, std::vector<BatchQueryResult>& out, double& device_ms, uint64_t& arena_words);
 
 // Degree-threshold count table of the closed form (built once per threshold).

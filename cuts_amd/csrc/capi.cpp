@@ -361,6 +361,7 @@ int cuts_match(cuts_context* ctx, const cuts_device_graph* dg, const cuts_plan* 
       res->kernel_ms[i] = r.kernel_ms[i];
       res->kernel_launches[i] = r.kernel_launches[i];
       res->kernel_bytes[i] = r.kernel_bytes[i];
+      res->merge_bytes[i] = r.merge_bytes[i];
     }
     return CUTS_OK;
   });
@@ -387,7 +388,8 @@ int cuts_batch_create(cuts_context* ctx, const cuts_device_graph* dg, const cuts
       o.use_graph = opt->no_graph ? 0 : 1;
       o.queue_cap = opt->queue_cap;
       o.arena_bytes = opt->arena_bytes;
-      o.wait_limit_s = opt->wait_limit_s;
+      o.ticket_queue = opt->ticket_queue;
+      o.item_work = opt->item_work;
     }
     auto b = std::make_unique<cuts_batch>();
     cuts::batch_create(ctx->c, const_cast<cuts::DeviceGraph&>(dg->dg), ps, o, b->b);

@@ -78,7 +78,9 @@ struct Context {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   uint64_t default_budget = 0;
   LevelBuf lv[kMaxQuery];
-  unsigned long long* counters = nullptr; // [0]=embeddings, [1]=fused level size, [2]=write count, [3]=bytes
+  // [0] embeddings, [1] fused level size, [2] write count, [3] demand bytes,
+  // [4..7] eight uint32 tile counters, [8] merge-model bytes
+  unsigned long long* counters = nullptr;
   hipEvent_t kev[2] = {nullptr, nullptr};
   uint64_t* host_pinned = nullptr;        // small pinned staging
   ~Context();
@@ -109,6 +111,7 @@ struct MatchResult {
   double kernel_ms[8] = {0};
   uint64_t kernel_launches[8] = {0};
   double kernel_bytes[8] = {0};
+  double merge_bytes[8] = {0};
 };
 
 void context_init(Context& c, int device);

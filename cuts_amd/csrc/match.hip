@@ -153,19 +153,25 @@ __global__ __launch_bounds__(BLOCK) void k_level0_scatter(const uint32_t* __rest
 // ------------------------------------------------------------ planning
 // Per prefix: the constraint whose list is shortest (upper bound on the
 // prefix's children).  Ties keep the lowest constraint index.
+// With merge (profiling) it adds the merge-model bytes of expanding the
+// prefix: 8 per backward neighbour (offset pair) + 4 per element of every
+// backward neighbour's list.
 // With cbase (fused levels whose closed-form vertex u is an older position)
 // it also stores the part of the closed form that only depends on the
 // prefix: cnt(u) - |{i < L : r_i in N(u), deg(r_i) >= t}|.
 template <int L>
 __global__ __launch_bounds__(BLOCK) void k_plan(GView g, KPlan P, const uint32_t* __restrict__ rows, uint64_t n,
                                                uint32_t* __restrict__ minlen, uint8_t* __restrict__ minsel,
-                                               uint32_t* __restrict__ cbase, const uint32_t* __restrict__ tbl) {
+                                               uint32_t* __restrict__ cbase, const uint32_t* __restrict__ tbl,
+                                               unsigned long long* __restrict__ merge) {
   const uint32_t nj = P.nj[L];
+  uint64_t mb = 0;
   for (uint64_t p = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (uint64_t)gridDim.x * BLOCK) {
     uint32_t row[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) row[l] = rows[p * L + l];
     uint32_t best = 0xFFFFFFFFu, sel = 0;
+    uint64_t sum = 0;
     for (uint32_t c = 0; c < nj; ++c) {
       const uint32_t jc = P.jc[L][c];
       const uint32_t pos = jc & 31;
@@ -175,8 +181,10 @@ __global__ __launch_bounds__(BLOCK) void k_plan(GView g, KPlan P, const uint32_t
         if (l == (int)pos) u = row[l];
       const uint32_t* off = (jc >> 7) ? g.off_in : g.off_out;
       const uint32_t len = off[u + 1] - off[u];
+      sum += len;
       if (len < best) { best = len; sel = c; }
     }
+    mb += 8ull * nj + 4ull * sum;
     minlen[p] = best;
     minsel[p] = (uint8_t)sel;
     if (cbase) {
@@ -194,6 +202,7 @@ __global__ __launch_bounds__(BLOCK) void k_plan(GView g, KPlan P, const uint32_t
       cbase[p] = (uint32_t)c;
     }
   }
+  if (merge) add_wave(merge, mb);
 }
 
 // -------------------------------------------------- exclusive scan (u32->u64)
@@ -333,16 +342,27 @@ template <int L>
 __global__ __launch_bounds__(BLOCK) void k_closed_form(GView g, KPlan P, const uint32_t* __restrict__ rows, uint64_t n,
                                                       const uint32_t* __restrict__ tbl,
                                                       unsigned long long* __restrict__ counters, int stats) {
-  uint64_t acc = 0, bytes = 0;
+  uint64_t acc = 0, bytes = 0, mb = 0;
   for (uint64_t p = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (uint64_t)gridDim.x * BLOCK) {
     uint32_t r[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) r[l] = rows[p * L + l];
     bytes += 4 * L;
     acc += closed_form<L>(g, P, r, tbl, bytes);
+    if (stats) {
+      // merge model: the one backward list of the last position
+      uint32_t u = 0;
+#pragma unroll
+      for (int l = 0; l < L; ++l)
+        if (l == (int)P.sc_pos) u = r[l];
+      mb += 8 + 4ull * (g.off_out[u + 1] - g.off_out[u]);
+    }
   }
   add_wave(&counters[0], acc);
-  if (stats) add_wave(&counters[3], bytes);
+  if (stats) {
+    add_wave(&counters[3], bytes);
+    add_wave(&counters[8], mb);
+  }
 }
 
 // Degree-threshold table: tbl[u] = |{x in N(u) : deg(x) >= thr}|, one wave per vertex.
@@ -384,7 +404,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
   const uint32_t tid = threadIdx.x, wave = tid >> 6;
   const uint32_t nj = P.nj[L];
   const uint32_t tin = P.sig_in[L], tout = P.sig_out[L];
-  uint64_t acc = 0, valid = 0, bytes = 0;
+  uint64_t acc = 0, valid = 0, bytes = 0, mb = 0;
   // Dynamic, XCD-aware tile fetching: blocks are dispatched round-robin over
   // the 8 XCDs, so block b works first on XCD (b mod 8)'s contiguous share of
   // the tiles (neighbouring prefixes share adjacency lists in that XCD's L2)
@@ -518,6 +538,15 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
       } else {
         if (ok) {
           ++valid;
+          if (a.stats) {
+            // merge model: the survivor's trie node (written and read back)
+            // and the last position's one backward list
+            uint32_t us = v;
+#pragma unroll
+            for (int l = 0; l < L; ++l)
+              if (l == (int)P.sc_pos) us = row[l];
+            mb += 16 + 8 + 4ull * (g.off_out[us + 1] - g.off_out[us]);
+          }
           if (a.cbase) {
             acc += fused_contrib;
           } else {
@@ -537,7 +566,10 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
     add_wave(&a.counters[0], acc);
     if (MODE == M_FUSED) add_wave(&a.counters[1], valid);
   }
-  if (a.stats) add_wave(&a.counters[3], bytes);
+  if (a.stats) {
+    add_wave(&a.counters[3], bytes);
+    if (MODE == M_FUSED) add_wave(&a.counters[8], mb);
+  }
 }
 
 // ---------------------------------------------------------- dispatch
@@ -648,7 +680,10 @@ struct Matcher {
   }
   void pbegin(bool dev_bytes) {
     if (!prof) return;
-    if (dev_bytes) HIPCHECK(hipMemsetAsync(c.counters + 3, 0, 8, c.stream));
+    if (dev_bytes) {
+      HIPCHECK(hipMemsetAsync(c.counters + 3, 0, 8, c.stream));
+      HIPCHECK(hipMemsetAsync(c.counters + 8, 0, 8, c.stream));
+    }
     HIPCHECK(hipEventRecord(c.kev[0], c.stream));
   }
   void pend(int cls, double host_bytes, bool dev_bytes) {
@@ -660,6 +695,7 @@ struct Matcher {
     r.kernel_ms[cls] += ms;
     r.kernel_launches[cls]++;
     r.kernel_bytes[cls] += host_bytes + (dev_bytes ? (double)read_u64(c.counters + 3) : 0.0);
+    if (dev_bytes) r.merge_bytes[cls] += (double)read_u64(c.counters + 8);
   }
 
   uint64_t read_u64(const void* dev) {
@@ -704,21 +740,26 @@ struct Matcher {
     }
     // plan + scan over the input rows; arrays live with the input level
     const bool use_cbase = fuse && L == k - 2 && kp.sc_pos < L && !(o.flags & 64u);
+    int mode = M_WRITE;
+    if (L == k - 1 && !write) mode = M_COUNT;
+    else if (L == k - 2 && fuse) mode = M_FUSED;
+    const int mclass = mode == M_WRITE ? 4 : (mode == M_COUNT ? 5 : 6);
     {
       const uint32_t grid = grid_for(n, BLOCK, grid_cap);
+      unsigned long long* merge = prof ? c.counters + 8 : nullptr;
+      if (merge) HIPCHECK(hipMemsetAsync(merge, 0, 8, c.stream));
       pbegin(false);
-#define PL(LL) hipLaunchKernelGGL((k_plan<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.minlen, in.minsel, use_cbase ? in.cbase : nullptr, tbl)
+#define PL(LL) hipLaunchKernelGGL((k_plan<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.minlen, in.minsel, use_cbase ? in.cbase : nullptr, tbl, merge)
       CUTS_DISPATCH_L(L, PL)
 #undef PL
       HIPCHECK(hipGetLastError());
       pend(1, (double)n * (4.0 * L + 8.0 * kp.nj[L] + 5.0), false);
+      // the prefixes' list reads belong to the expansion that performs them
+      if (merge) r.merge_bytes[mclass] += (double)read_u64(merge);
     }
     const uint64_t W = scan(in, n);
     if (W == 0) return;
     if (add_work(W)) return;
-    int mode = M_WRITE;
-    if (L == k - 1 && !write) mode = M_COUNT;
-    else if (L == k - 2 && fuse) mode = M_FUSED;
     uint64_t chunk = MAX_CHUNK;
     LevelBuf& out = c.lv[L];
     if (mode == M_WRITE) {
@@ -765,6 +806,7 @@ struct Matcher {
         pend(4, 0, true);
         const uint64_t m = read_u32(c.counters + 2);
         r.level[L] += m;
+        if (prof && L < k - 1) r.merge_bytes[4] += 16.0 * (double)m;  // trie nodes written and read back
         if (L == k - 1) {
           // embeddings: copy rows to the host sink
           r.count += m;
@@ -847,6 +889,7 @@ struct Matcher {
     }
     r.level[0] = n0;
     r.complete = 1;
+    if (prof) r.merge_bytes[0] = 8.0 * n0c + (k > 1 ? 16.0 * (double)n0 : 0.0);  // signature scan, level-0 nodes
     if (k == 1) {
       r.count = n0;
       r.roots_done = n0;
@@ -1030,7 +1073,7 @@ void context_init(Context& c, int device) {
   HIPCHECK(hipEventCreate(&c.ev1));
   HIPCHECK(hipEventCreate(&c.kev[0]));
   HIPCHECK(hipEventCreate(&c.kev[1]));
-  HIPCHECK(hipMalloc(&c.counters, 8 * sizeof(unsigned long long)));
+  HIPCHECK(hipMalloc(&c.counters, 16 * sizeof(unsigned long long)));
   HIPCHECK(hipHostMalloc(&c.host_pinned, 64, hipHostMallocDefault));
   size_t fr = 0, tot = 0;
   HIPCHECK(hipMemGetInfo(&fr, &tot));

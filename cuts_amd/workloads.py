@@ -11,7 +11,7 @@ from __future__ import annotations
 
 from dataclasses import dataclass
 from functools import lru_cache
-from typing import List, Tuple
+from typing import Dict, List, Tuple
 
 import numpy as np
 
@@ -119,9 +119,13 @@ def batch_candidates(g: Graph, seed: int = 0):
 # within MAX_WORK candidates and materialise at most MAX_ROWS trie rows
 # (levels 1..k-3; the last two are fused).  Sparse 12-vertex queries on a
 # power-law graph have counts beyond 2^64, so some rule is unavoidable; this
-# one is deterministic (work and level sizes do not depend on timing).
-MAX_WORK = 1 << 31
+# one is deterministic (work and level sizes do not depend on timing).  Sizes
+# stay uniform: each k in [KMIN, KMAX] gets an equal quota, filled by the
+# first candidates of that size that pass (on a dense graph the rule keeps
+# the denser, more selective walks of each size).
+MAX_WORK = 1 << 28
 MAX_ROWS = 1 << 21
+KMIN, KMAX = 4, 12
 
 
 def materialised_rows(level_sizes) -> int:
@@ -129,21 +133,38 @@ def materialised_rows(level_sizes) -> int:
     return int(sum(level_sizes[1:max(1, k - 2)]))
 
 
+def quotas(n: int, kmin: int = KMIN, kmax: int = KMAX) -> Dict[int, int]:
+    """n split over the sizes kmin..kmax as evenly as possible (the
+    remainder goes to the smallest sizes)."""
+    m = kmax - kmin + 1
+    return {k: n // m + (1 if k - kmin < n % m else 0) for k in range(kmin, kmax + 1)}
+
+
 def admit(ctx, dg, g: Graph, n: int, seed: int = 0, max_work: int = MAX_WORK, max_rows: int = MAX_ROWS,
-          log=None):
-    """First n candidates of batch_candidates(g, seed) that pass the rule."""
+          kmin: int = KMIN, kmax: int = KMAX, max_tries_per_k: int = 3000, log=None):
+    """Per size k, the first quotas(n)[k] candidates of size k from
+    batch_candidates(g, seed) that pass the rule, in stream order.  A size
+    whose quota is not met within max_tries_per_k candidates keeps what it
+    found.  Returns (queries, tried per size)."""
     from . import Plan
-    out, tried = [], 0
+    want = quotas(n, kmin, kmax)
+    got = {k: 0 for k in want}
+    tried = {k: 0 for k in want}
+    out = []
     for i, k, ne, edges in batch_candidates(g, seed):
-        tried += 1
+        if all(got[x] >= want[x] or tried[x] >= max_tries_per_k for x in want):
+            break
+        if k not in want or got[k] >= want[k] or tried[k] >= max_tries_per_k:
+            continue
+        tried[k] += 1
         p = Plan(Graph.from_edges(k, edges))
         r = ctx.match(dg, p, max_work=max_work)
-        ok = not r.work_capped and materialised_rows(r.level_sizes) <= max_rows
-        if ok:
-            out.append(dict(index=i, k=k, edges=edges, count=r.count, level_sizes=r.level_sizes, work=r.work,
-                            order=p.order))
-            if log:
-                log(f"admitted {len(out)}/{n} (candidate {i}, k={k}, count={r.count})")
-            if len(out) == n:
-                break
+        if r.work_capped or materialised_rows(r.level_sizes) > max_rows:
+            continue
+        got[k] += 1
+        out.append(dict(index=i, k=k, edges=edges, count=r.count, level_sizes=r.level_sizes, work=r.work,
+                        order=p.order))
+        if log:
+            log(f"admitted {len(out)}/{n} (candidate {i}, k={k}: {got[k]}/{want[k]} after {tried[k]} tries, "
+                f"count={r.count})")
     return out, tried

@@ -147,6 +147,14 @@ typedef struct {
   uint32_t last_root;       /* data vertex of the last of them */
   uint32_t work_capped;     /* 1 if max_work stopped the run (count invalid) */
   uint64_t work;            /* candidates examined (sum of scan totals) */
+  /* CUTS_FLAG_PROFILE: merge-model algorithmic bytes per kernel class -- the
+   * level-0 signature scan (8 per vertex) and level-0 nodes (16 each) in
+   * class 0; for every expansion level, 8 per backward neighbour and 4 per
+   * element of its list for each prefix expanded, plus 16 per trie node of
+   * levels below k-1 (written and read back), in the class that performs the
+   * expansion.  The sum does not depend on kernels or modes: it equals the
+   * CPU trie walk's figure for the same graph, query and order. */
+  double merge_bytes[8];
 } cuts_match_result;
 
 void cuts_match_options_default(cuts_match_options *o);
@@ -164,19 +172,23 @@ int cuts_match(cuts_context *ctx, const cuts_device_graph *dg, const cuts_plan *
 int cuts_log_tables(const char *path, const char *label, const cuts_plan *plan, const cuts_match_result *r);
 
 /* ---- batched matching (no reference counterpart) --------------------- */
-/* Many queries against one resident data graph in one persistent launch
- * (device work queue of (query, level, row chunk) items, captured once in a
- * hipGraph and replayed).  The reference runs one query per process with a
- * host sync per level (src/host_funcs.cu:141-157). */
+/* Many queries against one resident data graph.  A level-synchronous device
+ * work queue of (query, level, row chunk) items: one persistent-grid launch
+ * per trie level processes the items of every query at that level and
+ * publishes the next level's items; the whole sequence is captured once in
+ * a hipGraph and replayed.  No block ever waits for another.  The reference
+ * runs one query per process with a host sync per level
+ * (src/host_funcs.cu:141-157). */
 typedef struct cuts_batch cuts_batch;
 typedef struct {
-  uint32_t flags;         /* CUTS_FLAG_NO_SHORTCUT / CUTS_FLAG_NO_FUSE */
-  uint32_t blocks_per_cu; /* persistent grid = CUs x this, 1..4; 0 = 1 */
+  uint32_t flags;         /* CUTS_FLAG_NO_SHORTCUT / CUTS_FLAG_NO_FUSE / CUTS_FLAG_NO_BITMAP */
+  uint32_t blocks_per_cu; /* grid of each level launch = CUs x this, 1..8; 0 = 2 */
   uint32_t vchunk;        /* level-0 vertices per initial item; 0 = 4096 */
   uint32_t no_graph;      /* 1: plain launches instead of a hipGraph */
-  uint64_t queue_cap;     /* work-queue slots; 0 = 4M */
-  uint64_t arena_bytes;   /* trie-row arena; 0 = 8 GiB */
-  double wait_limit_s;    /* bound on any wait inside the kernel; 0 = 300 */
+  uint64_t queue_cap;     /* work items per level; 0 = 4M */
+  uint64_t arena_bytes;   /* trie-row arena, two halves alternating between levels; 0 = 8 GiB */
+  uint32_t ticket_queue;  /* 1: the asynchronous ticket-queue kernel -- disabled, create fails */
+  uint32_t item_work;     /* candidate work at which a new item is started; 0 = 16384 */
 } cuts_batch_options;
 typedef struct {
   uint64_t count;
@@ -188,7 +200,7 @@ void cuts_batch_options_default(cuts_batch_options *o);
 int cuts_batch_create(cuts_context *ctx, const cuts_device_graph *dg, const cuts_plan *const *plans, uint32_t n,
                       const cuts_batch_options *opt, cuts_batch **out);
 /* results: n entries.  Fails with CUTS_E_NOMEM if the queue or arena filled
- * up (counts are then invalid). */
+ * up (counts are then invalid).  arena_bytes: the largest level's rows. */
 int cuts_batch_run(cuts_batch *b, cuts_batch_query_result *results, double *device_ms, uint64_t *arena_bytes);
 void cuts_batch_free(cuts_batch *b);
 

@@ -303,6 +303,7 @@ typedef struct {
   double bytes;
   double t0, max_seconds;
   int stopped;
+  int symmetric; /* in-lists equal out-lists: one list per backward neighbour */
 } trie_ctx;
 
 /* Expands the n rows of R (each `len` words = positions 0..len-1) into
@@ -348,12 +349,15 @@ static void trie_expand(trie_ctx *c, const uint32_t *R, uint64_t n, uint32_t len
         const uint32_t *a;
         uint32_t lo, hi;
         int skip = source(d, p, len, f, &a, &lo, &hi);
-        bytes += 8.0 * p->nc[len];
+        /* merge model: an offset pair and every element of each distinct
+         * backward list (both arcs of a symmetric pair share one list;
+         * plan_from_order puts a position's two constraints side by side) */
         for (uint32_t cc = 0; cc < p->nc[len]; ++cc) {
+          if (c->symmetric && cc > 0 && p->cj[len][cc - 1] == p->cj[len][cc]) continue;
           const uint32_t *aa;
           uint32_t l2, h2;
           list_of(d, p->cd[len][cc], f[p->cj[len][cc]], &aa, &l2, &h2);
-          bytes += 4.0 * (h2 - l2);
+          bytes += 8.0 + 4.0 * (h2 - l2);
         }
         for (uint32_t x = lo; x < hi; ++x) {
           uint32_t v = a ? a[x] : x;
@@ -417,6 +421,9 @@ int trie_cpu_run_timed(const oracle_graph *d, uint32_t k, const uint8_t *qa, con
   for (uint32_t i = 0; i < k; ++i)
     if (c->p.nc[i] > MAXC) { free(c); return -1; }
   c->max_rows = max_rows ? max_rows : (1ull << 26);
+  c->symmetric = 1;
+  for (uint32_t x = 0; x <= d->V && c->symmetric; ++x) c->symmetric = d->off_in[x] == d->off_out[x];
+  for (uint64_t e = 0; c->symmetric && e < d->off_out[d->V]; ++e) c->symmetric = d->adj_in[e] == d->adj_out[e];
   c->nthreads = nthreads > 0 ? nthreads : omp_get_max_threads();
   if (sample_stride == 0) sample_stride = 1;
   double t0 = omp_get_wtime();

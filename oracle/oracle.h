@@ -60,7 +60,11 @@ int oracle_heuristic_order(uint32_t k, const uint8_t *qa, uint32_t *order);
 typedef struct {
   uint64_t count;          /* |T_{k-1}| */
   uint64_t level[32];      /* |T_l| for l < k (level[k-1] == count) */
-  double bytes;            /* merge-model algorithmic bytes */
+  double bytes;            /* merge-model algorithmic bytes: 8 per data vertex
+                              (signature scan); per prefix expanded, 8 + 4*len
+                              for each distinct backward list (a symmetric
+                              graph's in- and out-list are one list); 16 per
+                              trie node of levels below k-1 (written, read) */
   double seconds;          /* wall time of the matching phase */
   int threads;
   int complete;            /* 0 if max_seconds stopped the run early */

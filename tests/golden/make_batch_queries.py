@@ -1,7 +1,7 @@
 """Writes tests/golden/batch_queries.json: the 256 queries of the batched
-workload (random walks on the mid-size Chung-Lu graph), admitted by
-cuts_amd.workloads.admit, with their exact counts and per-level table sizes
-from single-query GPU runs.  Needs a GPU; run from the repository root."""
+workload (random walks on the mid-size Chung-Lu graph; sizes 4..12 in equal
+quotas), admitted by cuts_amd.workloads.admit, with their exact counts and
+per-level table sizes from single-query GPU runs.  Needs a GPU; run from the repository root."""
 import json
 import os
 import sys
@@ -19,6 +19,7 @@ if __name__ == "__main__":
     dg = ctx.upload(g)
     qs, tried = W.admit(ctx, dg, g, 256, seed=0, log=lambda m: print(m, flush=True))
     with open(out, "w") as f:
-        json.dump(dict(graph="chung-lu n=36692 m=183831 gamma=2.1 maxdeg=1400 seed=0", seed=0, tried=tried,
+        json.dump(dict(graph="chung-lu n=36692 m=183831 gamma=2.1 maxdeg=1400 seed=0", seed=0,
+                       tried={str(k): v for k, v in tried.items()}, quotas={str(k): v for k, v in W.quotas(256).items()},
                        max_work=W.MAX_WORK, max_rows=W.MAX_ROWS, queries=qs), f, separators=(",", ":"))
-    print(f"wrote {out}: {len(qs)} admitted of {tried} candidates", flush=True)
+    print(f"wrote {out}: {len(qs)} admitted; tries per size {tried}", flush=True)

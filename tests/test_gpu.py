@@ -144,6 +144,49 @@ def test_level_sizes_vs_cpu_trie(ctx, seed):
         assert r.level_sizes == [cpu.level[i] for i in range(k)], m
 
 
+def _profiled(m):
+    mm = dict(m)
+    mm["flags"] = mm.get("flags", 0) | cuts_amd.FLAG_PROFILE
+    return mm
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_merge_bytes_equal_cpu_trie_undirected(ctx, seed):
+    """The profiled merge-model bytes depend on (graph, query, order) only:
+    every execution mode reports exactly the CPU trie walk's figure."""
+    g, k, edges = random_case(500 + seed)
+    og = oracle.OGraph.from_view(g.view())
+    p = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges))
+    cpu = oracle.trie_cpu(og, oracle.qmatrix(k, oracle.undirected(edges)), p.order, nthreads=4)
+    dg = ctx.upload(g)
+    for m in modes_for(cpu.count):
+        r = ctx.match(dg, p, **_profiled(m))
+        assert r.count == cpu.count, m
+        assert sum(r.merge_bytes.values()) == cpu.bytes, (m, r.merge_bytes)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_merge_bytes_equal_cpu_trie_directed(ctx, seed):
+    rng = random.Random(2000 + seed)
+    n = rng.randint(30, 120)
+    arcs = sorted({(rng.randrange(n), rng.randrange(n)) for _ in range(rng.randint(3 * n, 8 * n))})
+    k = rng.randint(2, 6)
+    q = set()
+    for v in range(1, k):
+        u = rng.randrange(v)
+        q.add((u, v) if rng.random() < 0.5 else (v, u))
+    q = sorted(q)
+    g = arcs_graph(n, arcs)
+    og = oracle.OGraph.from_view(g.view())
+    p = cuts_amd.Plan(arcs_graph(k, q))
+    cpu = oracle.trie_cpu(og, oracle.qmatrix(k, q), p.order, nthreads=4)
+    dg = ctx.upload(g)
+    for m in (dict(), dict(trunks=3), dict(flags=cuts_amd.FLAG_NO_XCD)):
+        r = ctx.match(dg, p, **_profiled(m))
+        assert r.count == cpu.count, m
+        assert sum(r.merge_bytes.values()) == cpu.bytes, (m, r.merge_bytes)
+
+
 def test_root_restriction_partitions(ctx):
     g, k, edges = random_case(77)
     dg = ctx.upload(g)

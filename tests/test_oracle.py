@@ -174,6 +174,86 @@ def test_trie_cpu_counts_and_levels(case):
             assert r.level[l] == want, (order, l)
 
 
+def brute_merge_bytes(V, arcs, k, qarcs, order):
+    """The merge-model bytes from their definition, over trie levels built
+    by brute force (every injective extension checked against every arc):
+    8 per data vertex, per prefix of T_(l-1) 8 + 4*len for each distinct
+    backward list of position l (a symmetric graph's in- and out-list of a
+    vertex are one list), 16 per node of T_0..T_(k-2)."""
+    A = set(map(tuple, arcs))
+    outd, ind = [0] * V, [0] * V
+    for a, b in A:
+        outd[a] += 1
+        ind[b] += 1
+    sym = all((b, a) in A for a, b in A)
+    Q = set(map(tuple, qarcs))
+    qout, qin = [0] * k, [0] * k
+    for a, b in Q:
+        qout[a] += 1
+        qin[b] += 1
+
+    def fits(q, x):
+        return ind[x] >= qin[q] and outd[x] >= qout[q]
+
+    T = [[(x,) for x in range(V) if fits(order[0], x)]]
+    for l in range(1, k):
+        v = order[l]
+        nxt = []
+        for p in T[-1]:
+            for x in range(V):
+                if x in p or not fits(v, x):
+                    continue
+                if all(((order[j], v) not in Q or (p[j], x) in A) and ((v, order[j]) not in Q or (x, p[j]) in A)
+                       for j in range(l)):
+                    nxt.append(p + (x,))
+        T.append(nxt)
+    B = 8 * V
+    for l in range(1, k):
+        v = order[l]
+        lists = []
+        for j in range(l):
+            dirs = ([0] if (order[j], v) in Q else []) + ([1] if (v, order[j]) in Q else [])
+            if sym and dirs:
+                dirs = [0]
+            lists += [(j, d) for d in dirs]
+        for p in T[l - 1]:
+            for j, d in lists:
+                B += 8 + 4 * (outd[p[j]] if d == 0 else ind[p[j]])
+    B += 16 * sum(len(T[l]) for l in range(k - 1))
+    return B, [len(t) for t in T]
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_trie_cpu_merge_bytes_from_definition(seed):
+    rng = random.Random(seed)
+    V = rng.randint(5, 9)
+    directed = seed % 2 == 1
+    arcs = set()
+    for a in range(V):
+        for b in range(V):
+            if a != b and (directed or a < b) and rng.random() < 0.45:
+                arcs.add((a, b))
+                if not directed:
+                    arcs.add((b, a))
+    k = rng.randint(2, 4)
+    q = set()
+    for v in range(1, k):
+        u = rng.randrange(v)
+        q.add((u, v))
+        if not directed:
+            q.add((v, u))
+        elif rng.random() < 0.4:
+            q.add((v, u))
+    qarcs = sorted(q)
+    g = oracle.OGraph.from_arcs(V, sorted(arcs))
+    qa = oracle.qmatrix(k, qarcs)
+    for order in valid_orders(k, qarcs, rng, 3):
+        want, levels = brute_merge_bytes(V, arcs, k, qarcs, order)
+        r = oracle.trie_cpu(g, qa, order, nthreads=2)
+        assert [r.level[i] for i in range(k)] == levels
+        assert r.bytes == want, (order, r.bytes, want)
+
+
 def test_oracle_root_restriction_partitions_count():
     _, g = sym_random(50, 0.15, 11)
     qa = oracle.qmatrix(4, oracle.undirected([(0, 1), (1, 2), (2, 3), (3, 0)]))
