@@ -157,32 +157,53 @@ def run_batch256(args):
     dg = ctx.upload(g)
     fx = load_batch_fixture()
     qs = [(q["k"], [tuple(e) for e in q["edges"]]) for q in fx["queries"]]
+    roots = [tuple(q["roots"]) for q in fx["queries"]]
     heur = [cuts_amd.Plan(cuts_amd.Graph.from_edges(k, e)) for k, e in qs]
-    # heuristic order, one query at a time
-    for p in heur[:8]:
-        ctx.match(dg, p)
-    single = [ctx.match(dg, p) for p in heur]
+    # heuristic order, one query at a time (each on its root sample)
+    for p, rt in zip(heur[:8], roots):
+        ctx.match(dg, p, roots=rt)
+    single = [ctx.match(dg, p, roots=rt) for p, rt in zip(heur, roots)]
     s_ms = [r.device_ms for r in single]
     counts = [r.count for r in single]
     assert counts == [q["count"] for q in fx["queries"]], "single-query counts differ from the fixture"
     log(f"single heuristic: p50 {statistics.median(s_ms):.3f} ms, total {sum(s_ms):.1f} ms")
     # learned orders from a network trained on tables logged on other queries
     # of the same graph (same admission rule, another seed)
-    train, tried = W.admit(ctx, dg, g, min(args.train_queries, 512), seed=1)
+    train, tried = W.admit(ctx, dg, g, min(args.train_queries, 1024), seed=1)
     log(f"admitted {len(train)} training queries (tries per size {tried})")
-    net, info = train_on_queries(ctx, dg, g, [(q["k"], q["edges"], q["work"]) for q in train], seed=1, log=log)
+    net, info = train_on_queries(ctx, dg, g, [(q["k"], q["edges"], q["work"], tuple(q["roots"])) for q in train],
+                                 seed=1, log=log)
     gfeat = LN.graph_features(g)
     choices = []
     for i in range(0, len(qs), 32):
-        choices += LN.choose_orders(net, qs[i:i + 32], gfeat, [p.order for p in heur[i:i + 32]])
-    plans = [cuts_amd.Plan(cuts_amd.Graph.from_edges(k, e), ch.order) for (k, e), ch in zip(qs, choices)]
-    fallback = sum(1 for c in choices if not c.learned) / len(choices)
+        choices += LN.choose_orders(net, qs[i:i + 32], gfeat, [p.order for p in heur[i:i + 32]], keep_first=True)
+    # the score order judges every proposal: a learned order is kept only if
+    # its own run finishes within the score order's work (one capped run)
+    plans, proposed, kept, work_h, work_l = [], 0, 0, 0, 0
+    for (k, e), ch, hp, hr, rt in zip(qs, choices, heur, single, roots):
+        use = hp
+        if ch.learned:
+            proposed += 1
+            lp = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, e), ch.order)
+            lr = ctx.match(dg, lp, roots=rt, max_work=hr.work)
+            if not lr.work_capped and lr.work < hr.work:
+                assert lr.count == hr.count, "learned order changed a count"
+                use, kept = lp, kept + 1
+                work_l += lr.work
+            else:
+                work_l += hr.work
+        else:
+            work_l += hr.work
+        work_h += hr.work
+        plans.append(use)
+    fallback = 1.0 - kept / len(qs)
+    log(f"learned proposals {proposed}, kept by the judge {kept}; candidates {work_l} vs score order {work_h}")
     key = f"batch256:{W.csr_fingerprint(g)}:learned"
     tb = autotune.tune_batch(ctx, dg, plans, key, use_cache=False, log=log, arena_bytes=BATCH_ARENA,
-                             queue_cap=BATCH_QUEUE)
+                             queue_cap=BATCH_QUEUE, roots=roots)
     best = tb["best"]
     b = cuts_amd.Batch(ctx, dg, plans, blocks_per_cu=best["blocks_per_cu"], item_work=best["item_work"],
-                       arena_bytes=BATCH_ARENA, queue_cap=BATCH_QUEUE)
+                       arena_bytes=BATCH_ARENA, queue_cap=BATCH_QUEUE, roots=roots)
     runs = []
     for _ in range(args.warmup):
         b.run()
@@ -191,7 +212,7 @@ def run_batch256(args):
         assert r.counts == counts, "batched learned-order counts differ from single-query score-order counts"
         runs.append(r)
     bh = cuts_amd.Batch(ctx, dg, heur, blocks_per_cu=best["blocks_per_cu"], item_work=best["item_work"],
-                        arena_bytes=BATCH_ARENA, queue_cap=BATCH_QUEUE)
+                        arena_bytes=BATCH_ARENA, queue_cap=BATCH_QUEUE, roots=roots)
     rh = [bh.run() for _ in range(2)][-1]
     assert rh.counts == counts
     ms = statistics.median([r.device_ms for r in runs])
@@ -200,8 +221,9 @@ def run_batch256(args):
     amort = ms / len(qs)
     emit(base_line("exact embeddings/sec + p50 time/query; learned vs heuristic order, same GPU",
                    sum(counts) / (ms / 1e3), "embeddings/s", args.steps, args.warmup, ms,
-                   "256 random-walk queries (k 4-12, edge/vertex 1.0-2.5) on chung-lu n=36692; level-synchronous "
-                   "batch queue replayed as one hipGraph; learned orders; autotuned launch shape",
+                   "256 random-walk queries (k 4-12 in equal quotas, edge/vertex 1.0-2.5) on chung-lu n=36692, "
+                   "each exact on its admitted root sample of order position 0; level-synchronous batch queue "
+                   "replayed as one hipGraph; learned orders (same first vertex); autotuned launch shape",
                    {"queries": len(qs), "sum_count": sum(counts),
                     "batch_learned_ms": ms, "batch_heuristic_ms": rh.device_ms,
                     "amortised_ms_per_query_batch_learned": amort,
@@ -212,34 +234,40 @@ def run_batch256(args):
                     "p99_latency_ms_batch_learned_all_submitted_at_t0": sorted(lat)[int(0.99 * len(lat))],
                     "p50_latency_ms_single_heuristic_sequential_all_submitted_at_t0": statistics.median(cum),
                     "speedup_p50_latency": statistics.median(cum) / statistics.median(lat),
-                    "fallback_rate": fallback, "tuned": best, "training": info}))
+                    "fallback_rate": fallback, "learned_proposed": proposed, "learned_kept": kept,
+                    "candidates_learned": work_l, "candidates_heuristic": work_h,
+                    "tuned": best, "training": info}))
 
 
 def train_on_queries(ctx, dg, g, queries, seed, log):
-    """Tables logged by the solver itself: for each (k, edges, work) the
-    score order's run plus one random connected order's run (capped at 4x
-    the score order's work)."""
+    """Tables logged by the solver itself: for each (k, edges, work, roots)
+    the score order's run on its root sample plus one random connected
+    order's run with the same first vertex (capped at 4x the score order's
+    work).  Sampled table sizes are scaled by the root stride, so targets
+    estimate full tables."""
     import numpy as np
     import cuts_amd
     from cuts_amd import learned as LN
     rng = np.random.default_rng(seed)
     samples, targets = [], []
     t0 = time.perf_counter()
-    for k, edges, work in queries:
+    for k, edges, work, rt in queries:
         q = cuts_amd.Graph.from_edges(k, edges)
         adj = LN.adjacency(k, edges)
-        for order, cap in ((None, 0), (LN.random_connected_order(k, adj, rng), 4 * work + 4096)):
+        h = cuts_amd.Plan(q)
+        extra = [LN.random_connected_order(k, adj, rng, first=h.order[0]) for _ in range(3)]
+        for order, cap in [(None, 0)] + [(o, 4 * work + 4096) for o in extra]:
             p = cuts_amd.Plan(q, order)
-            r = ctx.match(dg, p, max_work=cap)
+            r = ctx.match(dg, p, max_work=cap, roots=rt)
             if r.work_capped:
                 continue
             for kk, a, sm, size in LN.table_samples(k, edges, p.order, r.level_sizes):
                 samples.append((kk, a, sm))
-                targets.append(LN.target(size))
+                targets.append(LN.target(size * rt[2]))
     t1 = time.perf_counter()
     x, A, meta = LN.encode(samples, LN.graph_features(g))
     net = LN.Mpnn(ctx, seed=seed)
-    hist = net.fit(x, A, meta, np.asarray(targets, np.float32), epochs=30, batch=4096, lr=2e-3)
+    hist = net.fit(x, A, meta, np.asarray(targets, np.float32), epochs=60, batch=2048, lr=2e-3)
     t2 = time.perf_counter()
     log(f"trained on {len(samples)} samples of {len(queries)} queries: mse {hist[0]:.4f} -> {hist[-1]:.4f}")
     return net, {"queries": len(queries), "samples": len(samples), "mse_first": hist[0], "mse_last": hist[-1],

@@ -14,7 +14,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 from dataclasses import dataclass, field
-from typing import List, Optional, Sequence
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -60,7 +60,7 @@ class _Result(C.Structure):
 class _BatchOpts(C.Structure):
     _fields_ = [("flags", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("vchunk", C.c_uint32),
                 ("no_graph", C.c_uint32), ("queue_cap", C.c_uint64), ("arena_bytes", C.c_uint64),
-                ("ticket_queue", C.c_uint32), ("item_work", C.c_uint32)]
+                ("ticket_queue", C.c_uint32), ("item_work", C.c_uint32), ("roots", C.POINTER(C.c_uint32))]
 
 
 class _BatchResult(C.Structure):
@@ -355,7 +355,10 @@ class Batch:
 
     def __init__(self, ctx: Context, dg: DeviceGraph, plans: Sequence[Plan], flags: int = 0,
                  blocks_per_cu: int = 0, vchunk: int = 0, use_graph: bool = True, queue_cap: int = 0,
-                 arena_bytes: int = 0, item_work: int = 0, queue: str = "level"):
+                 arena_bytes: int = 0, item_work: int = 0, queue: str = "level",
+                 roots: Optional[Sequence[Tuple[int, int, int]]] = None):
+        """roots: per query (first, end, stride) of the data vertices that
+        order position 0 may take (as Context.match's roots); None = all."""
         if queue not in ("level", "ticket"):
             raise ValueError("queue must be 'level' or 'ticket'")
         self.ctx, self.dg, self.plans = ctx, dg, list(plans)
@@ -364,6 +367,11 @@ class Batch:
         o.flags, o.blocks_per_cu, o.vchunk, o.no_graph = flags, blocks_per_cu, vchunk, 0 if use_graph else 1
         o.queue_cap, o.arena_bytes, o.item_work = queue_cap, arena_bytes, item_work
         o.ticket_queue = 1 if queue == "ticket" else 0
+        if roots is not None:
+            if len(roots) != len(self.plans):
+                raise ValueError("roots: one (first, end, stride) per query")
+            self._roots = np.ascontiguousarray(np.asarray(roots, dtype=np.uint32).reshape(-1, 3))
+            o.roots = self._roots.ctypes.data_as(C.POINTER(C.c_uint32))
         arr = (C.c_void_p * len(self.plans))(*[p._h.value for p in self.plans])
         h = C.c_void_p()
         _check(lib().cuts_batch_create(ctx._h, dg._h, arr, len(self.plans), C.byref(o), C.byref(h)))

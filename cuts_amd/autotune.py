@@ -84,7 +84,8 @@ def tune_single(ctx: Context, dg: DeviceGraph, plans: Sequence[Plan], key: str, 
 
 def tune_batch(ctx: Context, dg: DeviceGraph, plans: Sequence[Plan], key: str, reps: int = 2,
                blocks: Sequence[int] = (1, 2, 4), item_works: Sequence[int] = (4096, 16384, 65536),
-               arena_bytes: int = 32 << 30, queue_cap: int = 0, log=None, use_cache: bool = True) -> Dict:
+               arena_bytes: int = 32 << 30, queue_cap: int = 0, roots=None, log=None,
+               use_cache: bool = True) -> Dict:
     cache = _load()
     if use_cache and key in cache:
         return cache[key]
@@ -92,7 +93,8 @@ def tune_batch(ctx: Context, dg: DeviceGraph, plans: Sequence[Plan], key: str, r
     ref = None
     for bpc in blocks:
         for iw in item_works:
-            b = Batch(ctx, dg, plans, blocks_per_cu=bpc, item_work=iw, arena_bytes=arena_bytes, queue_cap=queue_cap)
+            b = Batch(ctx, dg, plans, blocks_per_cu=bpc, item_work=iw, arena_bytes=arena_bytes, queue_cap=queue_cap,
+                      roots=roots)
             runs = [b.run() for _ in range(reps)]
             for r in runs:
                 if ref is None:

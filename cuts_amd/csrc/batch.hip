@@ -263,11 +263,12 @@ __device__ void process(const BatchArgs& a, Shared& s, uint32_t L) {
   uint64_t cnt = 0, surv = 0;
 
   if (L == 0) {
-    // level-0 generation over vertices [off, off + n)
+    // level-0 generation over roots off .. off+n-1 of the query's root sequence
+    const uint32_t rfirst = uni(a.roots[3 * q]), rstride = uni(a.roots[3 * q + 1]);
     for (uint32_t i0 = 0; i0 < it.n; i0 += BB) {
       const uint32_t i = i0 + tid;
       const bool in = i < it.n;
-      const uint32_t u = (uint32_t)it.off + (in ? i : 0);  // it.n >= 1 whenever this loop runs
+      const uint32_t u = rfirst + (uint32_t)(it.off + (in ? i : 0)) * rstride;  // it.n >= 1 whenever this loop runs
       const bool ok = in && (g.sig[2 * (uint64_t)u] >= P.sig_in[0]) & (g.sig[2 * (uint64_t)u + 1] >= P.sig_out[0]);
       surv += ok;
       if (k > 1) stage_append(a, s, q, ok, nullptr, 0, u, ok ? next_work(g, P, nullptr, 0, u, closed) : 0);
@@ -387,31 +388,24 @@ __device__ void process(const BatchArgs& a, Shared& s, uint32_t L) {
 }  // namespace
 
 // Resets the queue on the device (so the sequence replays inside a hipGraph)
-// and publishes the level-0 items: vchunk vertices of one query each.
-__global__ __launch_bounds__(BB) void k_batch_init(BatchArgs a, uint32_t vchunk, uint32_t nch) {
+// and publishes the level-0 items (up to vchunk roots of one query each).
+__global__ __launch_bounds__(BB) void k_batch_init(BatchArgs a) {
   const uint64_t tid = (uint64_t)blockIdx.x * BB + threadIdx.x;
   const uint64_t nthreads = (uint64_t)gridDim.x * BB;
   for (uint64_t i = tid; i < (uint64_t)a.nq * kMaxQuery; i += nthreads) a.levels[i] = 0;
   for (uint64_t q = tid; q < a.nq; q += nthreads) {
     a.counts[q] = 0;
-    a.qpend[q] = nch;
+    a.qpend[q] = a.qinit[q];
     a.done[q] = 0;
   }
-  for (uint64_t i = tid; i < (uint64_t)a.nq * nch; i += nthreads) {
-    const uint32_t q = (uint32_t)(i / nch), c = (uint32_t)(i % nch);
-    Item it;
-    it.qL = q << 8;
-    it.off = (uint64_t)c * vchunk;
-    it.n = (uint32_t)std::min<uint64_t>(vchunk, a.V - it.off);
-    a.items[i] = it;
-  }
+  for (uint64_t i = tid; i < a.n_init; i += nthreads) a.items[i] = a.init_items[i];
   if (tid == 0) {
     for (uint32_t l = 0; l <= kMaxQuery; ++l) {
       a.st->head[l].v = 0;
       a.st->tail[l].v = 0;
       a.st->arena[l].v = 0;
     }
-    a.st->tail[0].v = (unsigned long long)a.nq * nch;
+    a.st->tail[0].v = a.n_init;
     a.st->abort.v = 0;
     a.st->t_start.v = wall_clock64();
   }
@@ -444,7 +438,7 @@ __global__ __launch_bounds__(BB) void k_batch_level(BatchArgs a, uint32_t L) {
           // an item outside the shapes the kernels write means corrupt
           // memory: stop instead of reading out of bounds
           const bool sane = (it.qL >> 8) < a.nq && (it.qL & 0xFF) == L &&
-                            (L == 0 ? (it.off <= a.V && it.n <= a.V - it.off)
+                            (L == 0 ? (it.off + it.n <= a.roots[3 * (it.qL >> 8) + 2])
                                     : (it.n >= 1 && it.n <= (uint32_t)RMAX &&
                                        it.off + (uint64_t)it.n * L <= a.half_words));
           if (sane) {
@@ -503,13 +497,36 @@ void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& p
   b.dg = &dg;
   b.nq = (uint32_t)plans.size();
   b.vchunk = o.vchunk ? o.vchunk : 4096;
-  b.nch = (dg.V + b.vchunk - 1) / b.vchunk;
-  if (b.nch == 0) b.nch = 1;
   b.use_graph = o.use_graph != 0;
   const uint32_t bpc = o.blocks_per_cu ? std::min<uint32_t>(o.blocks_per_cu, 8) : 2;
   b.grid = (uint32_t)c.cus * bpc;
   const uint64_t qcap = o.queue_cap ? o.queue_cap : (1ull << 22);
-  if ((uint64_t)b.nq * b.nch > qcap) throw Error(-1, "batch: queue too small for the level-0 items");
+  // position-0 roots per query (first, stride, count) and the level-0 items
+  std::vector<uint32_t> roots(3 * (size_t)b.nq);
+  std::vector<unsigned long long> qinit(b.nq);
+  std::vector<Item> init;
+  for (uint32_t q = 0; q < b.nq; ++q) {
+    uint32_t first = 0, end = dg.V, stride = 1;
+    if (o.roots) {
+      first = o.roots[3 * q];
+      end = std::min(o.roots[3 * q + 1], dg.V);
+      stride = o.roots[3 * q + 2];
+      if (stride == 0) throw Error(-1, "batch: root stride must be positive");
+    }
+    const uint32_t cnt = first < end ? (uint32_t)(((uint64_t)end - first + stride - 1) / stride) : 0;
+    roots[3 * q] = first;
+    roots[3 * q + 1] = stride;
+    roots[3 * q + 2] = cnt;
+    for (uint64_t off = 0; off < cnt; off += b.vchunk) {
+      Item it;
+      it.qL = q << 8;
+      it.n = (uint32_t)std::min<uint64_t>(b.vchunk, cnt - off);
+      it.off = off;
+      init.push_back(it);
+      ++qinit[q];
+    }
+  }
+  if (init.size() > qcap) throw Error(-1, "batch: queue too small for the level-0 items");
   const uint64_t half_words = (o.arena_bytes ? o.arena_bytes : (8ull << 30)) / 8;
   if (half_words < 16) throw Error(-1, "batch: arena too small");
   std::vector<KPlan> kp(b.nq);
@@ -548,6 +565,16 @@ void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& p
   a.levels = balloc<unsigned long long>(b, 7, (uint64_t)b.nq * kMaxQuery);
   a.qpend = balloc<unsigned long long>(b, 8, b.nq);
   a.done = balloc<unsigned long long>(b, 9, b.nq);
+  uint32_t* droots = balloc<uint32_t>(b, 4, roots.size());
+  HIPCHECK(hipMemcpy(droots, roots.data(), roots.size() * 4, hipMemcpyHostToDevice));
+  a.roots = droots;
+  unsigned long long* dqinit = balloc<unsigned long long>(b, 10, b.nq);
+  HIPCHECK(hipMemcpy(dqinit, qinit.data(), b.nq * 8, hipMemcpyHostToDevice));
+  a.qinit = dqinit;
+  Item* dinit = balloc<Item>(b, 11, init.size());
+  if (!init.empty()) HIPCHECK(hipMemcpy(dinit, init.data(), init.size() * sizeof(Item), hipMemcpyHostToDevice));
+  a.init_items = dinit;
+  a.n_init = init.size();
   int khz = 0;
   if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c.device) == hipSuccess && khz > 0)
     b.tick_khz = khz;
@@ -557,9 +584,9 @@ void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& p
 }
 
 static void batch_launch(Batch& b, hipStream_t s) {
-  const uint64_t nitems = (uint64_t)b.nq * b.nch;
+  const uint64_t nitems = std::max<uint64_t>(b.args.n_init, b.nq);
   const uint32_t igrid = (uint32_t)std::min<uint64_t>((nitems + BB - 1) / BB, 4096);
-  hipLaunchKernelGGL(k_batch_init, dim3(std::max<uint32_t>(igrid, 1)), dim3(BB), 0, s, b.args, b.vchunk, b.nch);
+  hipLaunchKernelGGL(k_batch_init, dim3(std::max<uint32_t>(igrid, 1)), dim3(BB), 0, s, b.args);
   for (uint32_t L = 0; L < b.kmax; ++L) hipLaunchKernelGGL(k_batch_level, dim3(b.grid), dim3(BB), 0, s, b.args, L);
 }
 

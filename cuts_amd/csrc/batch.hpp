@@ -7,8 +7,8 @@ namespace cuts {
 
 struct Item {      // 16 bytes
   uint32_t qL;     // query << 8 | level (row length in words; 0 = level-0 generation)
-  uint32_t n;      // rows (or vertices for level 0)
-  uint64_t off;    // arena word offset of the rows (first vertex for level 0)
+  uint32_t n;      // rows (or roots for level 0)
+  uint64_t off;    // arena word offset of the rows (for level 0: index of the first root)
 };
 
 // One counter per 256-byte line: blocks of a launch hit head[L] while
@@ -39,6 +39,10 @@ struct BatchArgs {
   uint32_t* arena;              // 2 halves of half_words; level L rows in half L & 1
   uint64_t half_words;
   unsigned long long *counts, *levels, *qpend, *done;
+  const uint32_t* roots;        // per query: first image, stride, count of position-0 roots
+  const unsigned long long* qinit;  // per query: level-0 items
+  const Item* init_items;       // level-0 items, copied into the queue by every replay
+  uint64_t n_init;
   uint64_t item_work;           // candidate work at which a staged item is flushed
   uint32_t flags;               // 1: no closed form, 2: no fusion
   uint32_t nq;                  // queries in the batch
@@ -53,6 +57,7 @@ struct BatchOpts {
   uint64_t arena_bytes = 0;
   uint32_t ticket_queue = 0;
   uint32_t item_work = 0;
+  const uint32_t* roots = nullptr;  // null or 3 words per query: first, end, stride
 };
 
 struct BatchQueryResult {
@@ -64,7 +69,7 @@ struct BatchQueryResult {
 struct Batch {
   Context* ctx = nullptr;
   DeviceGraph* dg = nullptr;
-  uint32_t nq = 0, nch = 0, vchunk = 0, grid = 0, kmax = 0;
+  uint32_t nq = 0, vchunk = 0, grid = 0, kmax = 0;
   BatchArgs args{};
   std::vector<uint32_t> k;   // query sizes
   void* allocs[12] = {nullptr};

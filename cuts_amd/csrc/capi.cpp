@@ -390,6 +390,7 @@ int cuts_batch_create(cuts_context* ctx, const cuts_device_graph* dg, const cuts
       o.arena_bytes = opt->arena_bytes;
       o.ticket_queue = opt->ticket_queue;
       o.item_work = opt->item_work;
+      o.roots = opt->roots;
     }
     auto b = std::make_unique<cuts_batch>();
     cuts::batch_create(ctx->c, const_cast<cuts::DeviceGraph&>(dg->dg), ps, o, b->b);

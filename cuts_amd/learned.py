@@ -246,12 +246,15 @@ def order_cost(order: Sequence[int], adj: List[int], f: Dict[int, float]) -> flo
     return cost
 
 
-def best_order(k: int, adj: List[int], f: Dict[int, float]) -> Tuple[List[int], float]:
+def best_order(k: int, adj: List[int], f: Dict[int, float], first: Optional[int] = None) -> Tuple[List[int], float]:
     """Exact minimum of order_cost over connected orders (every prefix
-    connected), by dynamic programming over connected subsets."""
+    connected), by dynamic programming over connected subsets; with `first`,
+    over the orders that start with that vertex."""
     full = (1 << k) - 1
     best: Dict[int, Tuple[float, int]] = {}
     for sm in sorted(f, key=lambda m: bin(m).count("1")):
+        if first is not None and not (sm >> first) & 1:
+            continue
         n = bin(sm).count("1")
         if n == 1:
             best[sm] = (f[sm], -1)
@@ -271,8 +274,12 @@ def best_order(k: int, adj: List[int], f: Dict[int, float]) -> Tuple[List[int], 
         if cand is not None:
             best[sm] = cand
     # last vertex: closed form when it has exactly one earlier neighbour
+    if k == 1:
+        return [0], f[full]
     top = None
     for v in range(k):
+        if v == first:
+            continue
         rest = full & ~(1 << v)
         if rest in best and adj[v] & rest:
             c = best[rest][0] + (f[full] if bin(adj[v] & rest).count("1") > 1 else 0.0)
@@ -300,10 +307,14 @@ class OrderChoice:
 
 
 def choose_orders(net: Mpnn, queries: Sequence[Tuple[int, List[Tuple[int, int]]]], gfeat: np.ndarray,
-                  heuristic: Sequence[List[int]], margin: float = 0.25) -> List[OrderChoice]:
+                  heuristic: Sequence[List[int]], margin: float = 0.25,
+                  keep_first: bool = False) -> List[OrderChoice]:
     """One forward launch over every connected subset of every query, then
     the DP per query.  Falls back to the score order unless the network
-    predicts at least `margin` (relative) fewer rows."""
+    predicts at least `margin` (relative) fewer rows.  keep_first restricts
+    the DP to orders that start with the score order's first vertex (needed
+    when position 0 is matched on a sample of the data vertices, so that
+    both orders count the same embeddings)."""
     samples, spans = [], []
     adjs = []
     for k, edges in queries:
@@ -318,7 +329,7 @@ def choose_orders(net: Mpnn, queries: Sequence[Tuple[int, List[Tuple[int, int]]]
     out = []
     for (k, edges), adj, (s0, subs), h in zip(queries, adjs, spans, heuristic):
         f = {sm: float(sizes[s0 + i]) for i, sm in enumerate(subs)}
-        order, c = best_order(k, adj, f)
+        order, c = best_order(k, adj, f, first=h[0] if keep_first else None)
         ch = order_cost(h, adj, f)
         if c < (1.0 - margin) * ch:
             out.append(OrderChoice(order, True, c, ch))
@@ -338,8 +349,8 @@ def table_samples(k: int, edges, order: Sequence[int], level_sizes: Sequence[int
     return out
 
 
-def random_connected_order(k: int, adj: List[int], rng) -> List[int]:
-    start = int(rng.integers(k))
+def random_connected_order(k: int, adj: List[int], rng, first: Optional[int] = None) -> List[int]:
+    start = int(rng.integers(k)) if first is None else first
     order, sm = [start], 1 << start
     while len(order) < k:
         front = [v for v in range(k) if not (sm >> v & 1) and adj[v] & sm]

@@ -115,22 +115,32 @@ def batch_candidates(g: Graph, seed: int = 0):
         i += 1
 
 
-# Admission rule for fixed query sets: the heuristic-order run must finish
-# within MAX_WORK candidates and materialise at most MAX_ROWS trie rows
-# (levels 1..k-3; the last two are fused).  Sparse 12-vertex queries on a
-# power-law graph have counts beyond 2^64, so some rule is unavoidable; this
-# one is deterministic (work and level sizes do not depend on timing).  Sizes
+# Admission rule for fixed query sets.  Walk queries of 5+ vertices on the
+# config-2 graph have counts far beyond what any run finishes (12-vertex ones
+# beyond 2^64), so each query is matched on a sample of the data vertices
+# that order position 0 may take: the residue class v = s-1 (mod s), with s
+# a power of two.  These classes are nested (s' = 2s keeps a subset), so work
+# and table sizes fall monotonically with s, and the rule picks the smallest
+# s whose score-order run finishes within MAX_WORK candidates and
+# materialises at most MAX_ROWS trie rows (levels 1..k-3; the last two are
+# fused), with a non-zero count.  Counts are exact for that sample.  Sizes
 # stay uniform: each k in [KMIN, KMAX] gets an equal quota, filled by the
-# first candidates of that size that pass (on a dense graph the rule keeps
-# the denser, more selective walks of each size).
+# first candidates of that size.  Everything is deterministic (work and table
+# sizes do not depend on timing).
 MAX_WORK = 1 << 28
 MAX_ROWS = 1 << 21
+MAX_STRIDE_LOG = 15
 KMIN, KMAX = 4, 12
 
 
 def materialised_rows(level_sizes) -> int:
     k = len(level_sizes)
     return int(sum(level_sizes[1:max(1, k - 2)]))
+
+
+def root_sample(V: int, stride: int) -> Tuple[int, int, int]:
+    """(first, end, stride) of the residue class v = stride-1 (mod stride)."""
+    return (stride - 1, V, stride)
 
 
 def quotas(n: int, kmin: int = KMIN, kmax: int = KMAX) -> Dict[int, int]:
@@ -143,14 +153,20 @@ def quotas(n: int, kmin: int = KMIN, kmax: int = KMAX) -> Dict[int, int]:
 def admit(ctx, dg, g: Graph, n: int, seed: int = 0, max_work: int = MAX_WORK, max_rows: int = MAX_ROWS,
           kmin: int = KMIN, kmax: int = KMAX, max_tries_per_k: int = 3000, log=None):
     """Per size k, the first quotas(n)[k] candidates of size k from
-    batch_candidates(g, seed) that pass the rule, in stream order.  A size
-    whose quota is not met within max_tries_per_k candidates keeps what it
-    found.  Returns (queries, tried per size)."""
+    batch_candidates(g, seed) that pass the rule at some root sample, in
+    stream order, each with its densest passing sample.  A size whose quota
+    is not met within max_tries_per_k candidates keeps what it found.
+    Returns (queries, tried per size)."""
     from . import Plan
     want = quotas(n, kmin, kmax)
     got = {k: 0 for k in want}
     tried = {k: 0 for k in want}
     out = []
+
+    def run(p, j):
+        r = ctx.match(dg, p, roots=root_sample(g.V, 1 << j), max_work=max_work)
+        return (not r.work_capped and materialised_rows(r.level_sizes) <= max_rows), r
+
     for i, k, ne, edges in batch_candidates(g, seed):
         if all(got[x] >= want[x] or tried[x] >= max_tries_per_k for x in want):
             break
@@ -158,13 +174,23 @@ def admit(ctx, dg, g: Graph, n: int, seed: int = 0, max_work: int = MAX_WORK, ma
             continue
         tried[k] += 1
         p = Plan(Graph.from_edges(k, edges))
-        r = ctx.match(dg, p, max_work=max_work)
-        if r.work_capped or materialised_rows(r.level_sizes) > max_rows:
+        ok, r = run(p, MAX_STRIDE_LOG)
+        if not ok:
+            continue
+        lo, hi, best = 0, MAX_STRIDE_LOG, r
+        while lo < hi:  # smallest passing stride exponent
+            mid = (lo + hi) // 2
+            ok, r = run(p, mid)
+            if ok:
+                hi, best = mid, r
+            else:
+                lo = mid + 1
+        if best.count == 0:
             continue
         got[k] += 1
-        out.append(dict(index=i, k=k, edges=edges, count=r.count, level_sizes=r.level_sizes, work=r.work,
-                        order=p.order))
+        out.append(dict(index=i, k=k, edges=edges, roots=list(root_sample(g.V, 1 << hi)), count=best.count,
+                        level_sizes=best.level_sizes, work=best.work, order=p.order))
         if log:
             log(f"admitted {len(out)}/{n} (candidate {i}, k={k}: {got[k]}/{want[k]} after {tried[k]} tries, "
-                f"count={r.count})")
+                f"stride {1 << hi}, count={best.count})")
     return out, tried

@@ -189,6 +189,9 @@ typedef struct {
   uint64_t arena_bytes;   /* trie-row arena, two halves alternating between levels; 0 = 8 GiB */
   uint32_t ticket_queue;  /* 1: the asynchronous ticket-queue kernel -- disabled, create fails */
   uint32_t item_work;     /* candidate work at which a new item is started; 0 = 16384 */
+  const uint32_t *roots;  /* NULL, or 3 words per query: first, end, stride of the data
+                             vertices order position 0 may take (cuts_match_options'
+                             root_begin/root_end/root_stride, per query) */
 } cuts_batch_options;
 typedef struct {
   uint64_t count;

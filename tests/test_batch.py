@@ -79,6 +79,22 @@ def test_level_batch_mixed_sizes_and_one_vertex_queries(ctx):
     assert r.counts[1] == int(g.view()["out_off"][-1])  # an edge matches every arc
 
 
+def test_level_batch_root_samples_equal_single_and_oracle(ctx):
+    g = cuts_amd.Graph.chung_lu(400, 2000, 2.4, 0, 31)
+    dg = ctx.upload(g)
+    og = oracle.OGraph.from_view(g.view())
+    qs = small_queries(g, 20, 9, 3, 7)
+    plans = [cuts_amd.Plan(cuts_amd.Graph.from_edges(k, e)) for k, e in qs]
+    roots = [W.root_sample(g.V, 1 << (i % 5)) for i in range(len(qs))]
+    singles = [ctx.match(dg, p, roots=rt) for p, rt in zip(plans, roots)]
+    r = cuts_amd.Batch(ctx, dg, plans, roots=roots, vchunk=16).run()
+    assert r.counts == [x.count for x in singles]
+    assert r.level_sizes == [x.level_sizes for x in singles]
+    for (k, e), p, rt, x in list(zip(qs, plans, roots, singles))[:8]:
+        print(f"oracle k={k} roots={rt}", flush=True)
+        assert x.count == oracle.count_ex(og, oracle.qmatrix(k, oracle.undirected(e)), root_q=p.order[0], roots=rt)
+
+
 def test_level_batch_graph_replays_agree(ctx):
     g = cuts_amd.Graph.chung_lu(300, 1200, 2.4, 0, 9)
     dg = ctx.upload(g)
@@ -150,7 +166,8 @@ def test_config4_batch_matches_fixture(ctx):
     dg = ctx.upload(g)
     plans = [cuts_amd.Plan(cuts_amd.Graph.from_edges(q["k"], q["edges"])) for q in fx["queries"]]
     assert [p.order for p in plans] == [q["order"] for q in fx["queries"]]
-    r = cuts_amd.Batch(ctx, dg, plans, arena_bytes=48 << 30, queue_cap=1 << 26).run()
+    r = cuts_amd.Batch(ctx, dg, plans, arena_bytes=48 << 30, queue_cap=1 << 26,
+                       roots=[tuple(q["roots"]) for q in fx["queries"]]).run()
     assert r.counts == [q["count"] for q in fx["queries"]]
     assert r.level_sizes == [q["level_sizes"] for q in fx["queries"]]
 

@@ -54,9 +54,11 @@ def test_reference_gradient_matches_finite_differences():
         assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(g[i]), (i, fd, g[i])
 
 
-def brute_best(k, adj, f):
+def brute_best(k, adj, f, first=None):
     best = None
     for perm in itertools.permutations(range(k)):
+        if first is not None and perm[0] != first:
+            continue
         ok, sm = True, 1 << perm[0]
         for v in perm[1:]:
             if not adj[v] & sm:
@@ -80,6 +82,11 @@ def test_order_dp_is_exact(seed):
     assert sorted(order) == list(range(k))
     assert math.isclose(cost, LN.order_cost(order, adj, f))
     assert math.isclose(cost, brute_best(k, adj, f))
+    first = rng.randrange(k)
+    order, cost = LN.best_order(k, adj, f, first=first)
+    assert order[0] == first and sorted(order) == list(range(k))
+    assert math.isclose(cost, LN.order_cost(order, adj, f))
+    assert math.isclose(cost, brute_best(k, adj, f, first))
 
 
 def test_connected_subset_counts():
