@@ -177,25 +177,13 @@ def run_batch256(args):
     choices = []
     for i in range(0, len(qs), 32):
         choices += LN.choose_orders(net, qs[i:i + 32], gfeat, [p.order for p in heur[i:i + 32]], keep_first=True)
-    # the score order judges every proposal: a learned order is kept only if
-    # its own run finishes within the score order's work (one capped run)
-    plans, proposed, kept, work_h, work_l = [], 0, 0, 0, 0
-    for (k, e), ch, hp, hr, rt in zip(qs, choices, heur, single, roots):
-        use = hp
-        if ch.learned:
-            proposed += 1
-            lp = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, e), ch.order)
-            lr = ctx.match(dg, lp, roots=rt, max_work=hr.work)
-            if not lr.work_capped and lr.work < hr.work:
-                assert lr.count == hr.count, "learned order changed a count"
-                use, kept = lp, kept + 1
-                work_l += lr.work
-            else:
-                work_l += hr.work
-        else:
-            work_l += hr.work
-        work_h += hr.work
-        plans.append(use)
+    # the score order judges every proposal (one run capped at its work)
+    judged = LN.judge(ctx, dg, qs, heur, single, choices, roots=roots)
+    plans = [j.plan for j in judged]
+    proposed = sum(1 for c in choices if c.learned)
+    kept = sum(1 for j in judged if j.learned)
+    work_l = sum(j.work for j in judged)
+    work_h = sum(r.work for r in single)
     fallback = 1.0 - kept / len(qs)
     log(f"learned proposals {proposed}, kept by the judge {kept}; candidates {work_l} vs score order {work_h}")
     key = f"batch256:{W.csr_fingerprint(g)}:learned"

@@ -132,6 +132,7 @@ struct Shared {
   uint64_t wwork[NW];
   uint64_t off[RMAX + 1];         // exclusive scan of the rows' shortest list lengths
   uint8_t minsel[RMAX];
+  uint32_t cbase[RMAX];           // fused level: prefix part of the closed form
   uint32_t rows[RMAX * WROW];     // input rows of the item
   uint32_t stage[RMAX * WROW];    // output rows waiting for a flush
 };
@@ -293,8 +294,24 @@ __device__ void process(const BatchArgs& a, Shared& s, uint32_t L) {
 
   // shortest constraint list per row (one row per thread), then a block scan
   const uint32_t nj = uni(P.nj[L]);
+  const int mode = (L == k - 1) ? 1 : ((L == k - 2 && fuse) ? 2 : 0);
+  // fused level with an older closed-form vertex u: the closed form splits
+  // into a per-prefix base, cnt(u) - |{i < L : r_i in N(u), deg(r_i) >= t}|,
+  // and one term for the candidate itself
+  const uint32_t sc_pos = uni(P.sc_pos), sc_thr = uni(P.sc_thr);
+  const bool use_cbase = mode == 2 && sc_pos < L && (a.flags & 4u) == 0;
   uint64_t len = 0;
   if (tid < n) {
+    if (use_cbase) {
+      const uint32_t* row = &s.rows[tid * WROW];
+      const uint32_t u = row[sc_pos];
+      int64_t c = tbl ? (int64_t)tbl[u] : (int64_t)(g.off_out[u + 1] - g.off_out[u]);
+      for (uint32_t l = 0; l < L; ++l) {
+        const bool in = ((P.sc_adjmask >> l) & 1) || has_arc(g, u, row[l]);
+        if (in && (sc_thr <= 1 || g.sig[2 * (uint64_t)row[l]] >= sc_thr)) --c;
+      }
+      s.cbase[tid] = (uint32_t)c;
+    }
     uint32_t best = 0xFFFFFFFFu, sel = 0;
     for (uint32_t c = 0; c < nj; ++c) {
       const uint32_t jc = P.jc[L][c];
@@ -322,12 +339,12 @@ __device__ void process(const BatchArgs& a, Shared& s, uint32_t L) {
   s.off[n] = W;  // same value from every thread; no thread writes index n above
   __syncthreads();
 
-  const int mode = (L == k - 1) ? 1 : ((L == k - 2 && fuse) ? 2 : 0);
   const uint32_t tin = P.sig_in[L], tout = P.sig_out[L];
   for (uint64_t w0 = 0; w0 < W; w0 += BB) {
     const uint64_t w = w0 + tid;
     bool ok = false;
     uint32_t v = 0, r = 0;
+    uint64_t fc = 0;  // fused: closed-form count of the survivor
     if (w < W) {
       uint32_t lo = 0, hi = n + 1;
       while (lo < hi) {
@@ -358,6 +375,16 @@ __device__ void process(const BatchArgs& a, Shared& s, uint32_t L) {
           ok = bmember(adj2, off2[u2], off2[u2 + 1], v);
         }
       }
+      if (mode == 2 && ok) {
+        if (use_cbase) {
+          // v is certainly in N(u) when it was drawn from u's own list
+          const uint32_t us = row[sc_pos];
+          const bool in = (jc & 31) == sc_pos || has_arc(g, us, v);
+          fc = (uint64_t)s.cbase[r] - ((in && (sc_thr <= 1 || g.sig[2 * (uint64_t)v] >= sc_thr)) ? 1 : 0);
+        } else {
+          fc = closed_form_rt(g, P, row, L, true, v, tbl);
+        }
+      }
     }
     const uint32_t* row = &s.rows[r * WROW];
     if (mode == 1) {
@@ -365,7 +392,7 @@ __device__ void process(const BatchArgs& a, Shared& s, uint32_t L) {
     } else if (mode == 2) {
       if (ok) {
         ++surv;
-        cnt += closed_form_rt(g, P, row, L, true, v, tbl);
+        cnt += fc;
       }
     } else {
       surv += ok;
@@ -579,7 +606,7 @@ void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& p
   if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c.device) == hipSuccess && khz > 0)
     b.tick_khz = khz;
   a.item_work = o.item_work ? o.item_work : 16384;
-  a.flags = o.flags & 3u;
+  a.flags = (o.flags & 3u) | ((o.flags & 64u) ? 4u : 0u);
   a.nq = b.nq;
 }
 

@@ -44,7 +44,7 @@ struct BatchArgs {
   const Item* init_items;       // level-0 items, copied into the queue by every replay
   uint64_t n_init;
   uint64_t item_work;           // candidate work at which a staged item is flushed
-  uint32_t flags;               // 1: no closed form, 2: no fusion
+  uint32_t flags;               // 1: no closed form, 2: no fusion, 4: no per-prefix base
   uint32_t nq;                  // queries in the batch
 };
 

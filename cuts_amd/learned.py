@@ -338,6 +338,35 @@ def choose_orders(net: Mpnn, queries: Sequence[Tuple[int, List[Tuple[int, int]]]
     return out
 
 
+@dataclass
+class Judged:
+    plan: Plan
+    learned: bool      # the proposal was kept
+    work: int          # candidates examined by the kept order
+    count: int
+
+
+def judge(ctx: Context, dg, queries: Sequence[Tuple[int, List[Tuple[int, int]]]], heuristic_plans,
+          heuristic_results, choices: Sequence[OrderChoice], roots=None) -> List[Judged]:
+    """The score order judges every learned proposal: a proposal is kept
+    only if its own run, capped at the score order's work, finishes with
+    fewer candidates examined.  A kept order's count must equal the score
+    order's (order never changes a count); anything else raises."""
+    out = []
+    for i, ((k, edges), hp, hr, ch) in enumerate(zip(queries, heuristic_plans, heuristic_results, choices)):
+        rt = None if roots is None else roots[i]
+        kept = None
+        if ch.learned and list(ch.order) != list(hp.order):
+            lp = Plan(Graph.from_edges(k, edges), ch.order)
+            lr = ctx.match(dg, lp, roots=rt, max_work=hr.work)
+            if not lr.work_capped and lr.work < hr.work:
+                if lr.count != hr.count:
+                    raise AssertionError(f"query {i}: learned order counted {lr.count}, score order {hr.count}")
+                kept = Judged(lp, True, lr.work, lr.count)
+        out.append(kept or Judged(hp, False, hr.work, hr.count))
+    return out
+
+
 # ------------------------------------------------------------- training
 def table_samples(k: int, edges, order: Sequence[int], level_sizes: Sequence[int]):
     """(k, adj, S mask, |T(S)|) for every prefix of a completed run."""
