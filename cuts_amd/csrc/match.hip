@@ -441,6 +441,177 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
     if (lds)
       for (uint32_t i = tid; i <= span; i += BLOCK) s_off[i] = a.work_off[plo + i];
     __syncthreads();
+    if constexpr (MODE != M_WRITE) {
+      // Counting modes: each thread's items are processed in lock-step
+      // stages (search, prefix data, list word, signature, constraints,
+      // closed form), G items at a time, so the loads of different items are
+      // independent and in flight together instead of one dependent chain
+      // after another.  Items past w1 are clamped to the tile's last prefix
+      // and masked; their list word is never read.
+      constexpr int G = (MODE == M_FUSED) ? ((L <= 6) ? 2 : 1) : ((L <= 6) ? 4 : (L <= 10 ? 2 : 1));
+      for (int g0 = 0; g0 < IPT; g0 += G) {
+        uint64_t w[G];
+        uint32_t pi[G];
+        bool in[G];
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+          w[j] = w0 + (uint64_t)(g0 + j) * BLOCK + tid;
+          in[j] = w[j] < w1;
+          pi[j] = 0;
+        }
+        // largest pi with off(pi) <= w: a fixed number of halvings for the
+        // whole block (off(0) <= w0 <= w and off(span) >= w1)
+        for (uint32_t n = span + 1; n > 1;) {
+          const uint32_t half = n >> 1;
+#pragma unroll
+          for (int j = 0; j < G; ++j) {
+            const uint64_t o = lds ? s_off[pi[j] + half] : a.work_off[plo + pi[j] + half];
+            pi[j] = (o <= w[j]) ? pi[j] + half : pi[j];
+          }
+          n -= half;
+        }
+        uint32_t row[G][L], sel[G], jc[G], idx[G], lb[G], v[G], cb[G];
+        bool ok[G];
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+          const uint32_t pj = min(pi[j], span - 1);
+          const uint64_t p = (uint64_t)plo + pj;
+          const uint64_t pbase = lds ? s_off[pj] : a.work_off[p];
+          idx[j] = (uint32_t)(w[j] - pbase);
+#pragma unroll
+          for (int l = 0; l < L; ++l) row[j][l] = a.rows[p * L + l];
+          sel[j] = a.minsel[p];
+          cb[j] = (MODE == M_FUSED && a.cbase) ? a.cbase[p] : 0u;
+          if (a.stats) {
+            bytes += 12;
+            if (in[j] && idx[j] == 0) bytes += 4 * L + 9 + 8 * nj;
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+          jc[j] = P.jc[L][sel[j]];
+          uint32_t u = 0;
+#pragma unroll
+          for (int l = 0; l < L; ++l)
+            if (l == (int)(jc[j] & 31)) u = row[j][l];
+          lb[j] = ((jc[j] >> 7) ? g.off_in : g.off_out)[u];
+        }
+#pragma unroll
+        for (int j = 0; j < G; ++j) v[j] = in[j] ? ((jc[j] >> 7) ? g.adj_in : g.adj_out)[lb[j] + idx[j]] : 0u;
+        uint2 sg[G];
+#pragma unroll
+        for (int j = 0; j < G; ++j) sg[j] = *reinterpret_cast<const uint2*>(g.sig + 2 * (uint64_t)v[j]);
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+          ok[j] = in[j] && sg[j].x >= tin && sg[j].y >= tout;
+#pragma unroll
+          for (int l = 0; l < L; ++l) ok[j] = ok[j] && v[j] != row[j][l];
+        }
+        for (uint32_t c = 0; c < nj; ++c) {
+          const uint32_t jc2 = P.jc[L][c];
+          uint32_t u2[G];
+#pragma unroll
+          for (int j = 0; j < G; ++j) {
+            u2[j] = 0;
+#pragma unroll
+            for (int l = 0; l < L; ++l)
+              if (l == (int)(jc2 & 31)) u2[j] = row[j][l];
+          }
+          if (g.hub_id) {  // symmetric: every constraint is an out-list
+            int32_t h[G];
+            uint64_t bw[G];
+#pragma unroll
+            for (int j = 0; j < G; ++j) h[j] = g.hub_id[u2[j]];
+#pragma unroll
+            for (int j = 0; j < G; ++j)
+              bw[j] = g.bits[h[j] >= 0 ? (uint64_t)h[j] * g.words + (v[j] >> 6) : 0];
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+              if (!ok[j] || c == sel[j]) continue;
+              if (h[j] >= 0) {
+                ok[j] = (bw[j] >> (v[j] & 63)) & 1ull;
+                if (a.stats) bytes += 12;
+              } else {
+                const uint32_t lo2 = g.off_out[u2[j]], hi2 = g.off_out[u2[j] + 1];
+                ok[j] = member_safe(g.adj_out, lo2, hi2, v[j]);
+                if (a.stats) bytes += 12 + bs_bytes(hi2 - lo2);
+              }
+            }
+          } else {
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+              if (!ok[j] || c == sel[j]) continue;
+              const uint32_t* off2 = (jc2 >> 7) ? g.off_in : g.off_out;
+              const uint32_t* adj2 = (jc2 >> 7) ? g.adj_in : g.adj_out;
+              const uint32_t lo2 = off2[u2[j]], hi2 = off2[u2[j] + 1];
+              ok[j] = member_safe(adj2, lo2, hi2, v[j]);
+              if (a.stats) bytes += 8 + bs_bytes(hi2 - lo2);
+            }
+          }
+        }
+        if (MODE == M_COUNT) {
+#pragma unroll
+          for (int j = 0; j < G; ++j) acc += ok[j];
+        } else if (a.cbase) {
+          // closed form = per-prefix base - [v in N(u), deg(v) >= t]; v is
+          // certainly in N(u) when it was drawn from u's own list
+          uint32_t us[G];
+          int32_t h[G];
+          uint64_t bw[G];
+#pragma unroll
+          for (int j = 0; j < G; ++j) {
+            us[j] = 0;
+#pragma unroll
+            for (int l = 0; l < L; ++l)
+              if (l == (int)P.sc_pos) us[j] = row[j][l];
+            h[j] = g.hub_id ? g.hub_id[us[j]] : -1;
+          }
+#pragma unroll
+          for (int j = 0; j < G; ++j) bw[j] = h[j] >= 0 ? g.bits[(uint64_t)h[j] * g.words + (v[j] >> 6)] : 0;
+#pragma unroll
+          for (int j = 0; j < G; ++j) {
+            if (!ok[j]) continue;
+            ++valid;
+            bool inn = (jc[j] & 31) == P.sc_pos;
+            if (!inn) {
+              if (h[j] >= 0) {
+                inn = (bw[j] >> (v[j] & 63)) & 1ull;
+              } else {
+                const uint32_t lo2 = g.off_out[us[j]], hi2 = g.off_out[us[j] + 1];
+                inn = member_safe(g.adj_out, lo2, hi2, v[j]);
+                if (a.stats) bytes += 8 + bs_bytes(hi2 - lo2);
+              }
+            }
+            const bool cnt_v = inn && (P.sc_thr <= 1 || sg[j].x >= P.sc_thr);
+            acc += (uint64_t)cb[j] - (cnt_v ? 1 : 0);
+            if (a.stats) {
+              bytes += 8;
+              // merge model: the survivor's trie node (written and read back)
+              // and the last position's one backward list
+              mb += 16 + 8 + 4ull * (g.off_out[us[j] + 1] - g.off_out[us[j]]);
+            }
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < G; ++j) {
+            if (!ok[j]) continue;
+            ++valid;
+            uint32_t r[L + 1];
+#pragma unroll
+            for (int l = 0; l < L; ++l) r[l] = row[j][l];
+            r[L] = v[j];
+            acc += closed_form<L + 1>(g, P, r, a.tbl, bytes);
+            if (a.stats) {
+              uint32_t us = v[j];
+#pragma unroll
+              for (int l = 0; l < L; ++l)
+                if (l == (int)P.sc_pos) us = row[j][l];
+              mb += 16 + 8 + 4ull * (g.off_out[us + 1] - g.off_out[us]);
+            }
+          }
+        }
+      }
+    } else
     for (int it = 0; it < IPT; ++it) {
       const uint64_t w = w0 + (uint64_t)it * BLOCK + tid;
       bool ok = false;

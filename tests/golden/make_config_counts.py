@@ -21,7 +21,9 @@ from cuts_amd import workloads as W  # noqa: E402
 SAMPLES = {
     "config2a": (W.config2_graph, W.C4_2PEND, (2000, 36692, 211)),
     "config2b": (W.config2_graph, W.K4E_TAIL2, (2000, 36692, 211)),
-    "config3": (W.config3_graph, W.C5CHORD_2TAIL, (1 << 19, 1 << 20, 8191)),
+    # v = 511 (mod 512): 2,048 roots of mostly low degree; the full count
+    # exceeds 2^64, this sample has 3.3e8 embeddings (the oracle needs ~20 s)
+    "config3": (W.config3_graph, W.C5CHORD_2TAIL, (511, 1 << 20, 512)),
 }
 
 
