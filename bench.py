@@ -262,134 +262,84 @@ def train_on_queries(ctx, dg, g, queries, seed, log):
                  "log_s": t1 - t0, "train_s": t2 - t1}
 
 
-def train_on_graph(ctx, dg, g, seed, n_queries, log, kmin=5, kmax=10, max_work=1 << 27):
-    """Tables logged by the solver itself on seeded random-walk queries:
-    score order plus one random connected order per query."""
-    import numpy as np
-    import cuts_amd
-    from cuts_amd import learned as LN, workloads as W
-    rng = np.random.default_rng(seed)
-    samples, targets, used = [], [], 0
-    t0 = time.perf_counter()
-    for i, k, ne, edges in W.batch_candidates(g, seed):
-        if not kmin <= k <= kmax:
-            continue
-        adj = LN.adjacency(k, edges)
-        q = cuts_amd.Graph.from_edges(k, edges)
-        for order in (None, LN.random_connected_order(k, adj, rng)):
-            p = cuts_amd.Plan(q, order)
-            r = ctx.match(dg, p, max_work=max_work)
-            if r.work_capped:
-                continue
-            for kk, a, sm, size in LN.table_samples(k, edges, p.order, r.level_sizes):
-                samples.append((kk, a, sm))
-                targets.append(LN.target(size))
-        used += 1
-        if used % 500 == 0:
-            log(f"logged tables of {used} training queries ({len(samples)} samples)")
-        if used == n_queries:
-            break
-    t1 = time.perf_counter()
-    x, A, meta = LN.encode(samples, LN.graph_features(g))
-    net = LN.Mpnn(ctx, seed=seed)
-    hist = net.fit(x, A, meta, np.asarray(targets, np.float32), epochs=30, batch=4096, lr=2e-3)
-    t2 = time.perf_counter()
-    log(f"trained on {len(samples)} samples: mse {hist[0]:.4f} -> {hist[-1]:.4f}")
-    return net, {"queries": used, "samples": len(samples), "mse_first": hist[0], "mse_last": hist[-1],
-                 "log_s": t1 - t0, "train_s": t2 - t1}
-
-
 def run_learned(args):
-    """Train on R-MAT scale-18 graphs, evaluate on the R-MAT scale-20 graph."""
+    """Train on R-MAT scale-18 graphs, evaluate on the R-MAT scale-20 graph.
+    Queries of both sets are admitted on nested root samples of position 0
+    (cuts_amd.workloads.admit); learned orders keep the score order's first
+    vertex and are judged by it."""
     import numpy as np
     import cuts_amd
     from cuts_amd import learned as LN, workloads as W
     ctx = cuts_amd.Context(0)
-    samples, targets, info = [], [], []
+    samples, targets = [], []
     rng = np.random.default_rng(0)
-    per_graph = args.train_queries // 4
+    per_graph = max(1, args.train_queries // 4)
     for gs in (1, 2, 3, 4):
         g18 = W.rmat18_graph(gs)
         dg18 = ctx.upload(g18)
         gf = LN.graph_features(g18)
-        used = 0
-        for i, k, ne, edges in W.batch_candidates(g18, 100 + gs):
-            if not 5 <= k <= 10:
-                continue
+        qs, tried = W.admit(ctx, dg18, g18, per_graph, seed=100 + gs, kmin=5, kmax=10)
+        for q in qs:
+            k, edges, rt = q["k"], q["edges"], tuple(q["roots"])
             adj = LN.adjacency(k, edges)
-            q = cuts_amd.Graph.from_edges(k, edges)
-            for order in (None, LN.random_connected_order(k, adj, rng)):
-                p = cuts_amd.Plan(q, order)
-                r = ctx.match(dg18, p, max_work=1 << 27)
+            qg = cuts_amd.Graph.from_edges(k, edges)
+            for order, cap in ((None, 0), (LN.random_connected_order(k, adj, rng, first=q["order"][0]),
+                                           4 * q["work"] + 4096)):
+                p = cuts_amd.Plan(qg, order)
+                r = ctx.match(dg18, p, roots=rt, max_work=cap)
                 if r.work_capped:
                     continue
                 for kk, a, sm, size in LN.table_samples(k, edges, p.order, r.level_sizes):
                     x1, A1, m1 = LN.encode([(kk, a, sm)], gf)
                     samples.append((x1[0], A1[0], m1[0]))
-                    targets.append(LN.target(size))
-            used += 1
-            if used == per_graph:
-                break
-        log(f"R-MAT 18 seed {gs}: {used} queries logged, {len(samples)} samples so far")
+                    targets.append(LN.target(size * rt[2]))
+        log(f"R-MAT 18 seed {gs}: {len(qs)} queries admitted (tries {tried}), {len(samples)} samples so far")
         del dg18
-    x = np.stack([s[0] for s in samples])
-    A = np.stack([s[1] for s in samples])
-    meta = np.asarray([s[2] for s in samples], np.uint32)
+    x = np.stack([s_[0] for s_ in samples])
+    A = np.stack([s_[1] for s_ in samples])
+    meta = np.asarray([s_[2] for s_ in samples], np.uint32)
     net = LN.Mpnn(ctx, seed=0)
-    hist = net.fit(x, A, meta, np.asarray(targets, np.float32), epochs=30, batch=4096, lr=2e-3)
+    hist = net.fit(x, A, meta, np.asarray(targets, np.float32), epochs=60, batch=2048, lr=2e-3)
     log(f"trained: mse {hist[0]:.4f} -> {hist[-1]:.4f} on {len(meta)} samples")
     if args.save_weights:
         np.save(args.save_weights, net.params())
     g = W.config3_graph()
     dg = ctx.upload(g)
     gf = LN.graph_features(g)
-    evals, tried = [], 0
-    for i, k, ne, edges in W.batch_candidates(g, 7):
-        if not 6 <= k <= 10:
-            continue
-        tried += 1
-        p = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges))
-        r = ctx.match(dg, p, max_work=1 << 29)
-        if r.work_capped:
-            continue
-        evals.append((k, edges, p, r))
-        if len(evals) % 100 == 0:
-            log(f"admitted {len(evals)} evaluation queries of {tried}")
-        if len(evals) == args.eval_queries or tried >= 20 * args.eval_queries:
-            break
+    evq, tried = W.admit(ctx, dg, g, args.eval_queries, seed=7, kmin=6, kmax=10)
+    log(f"admitted {len(evq)} evaluation queries (tries {tried})")
+    qs = [(q["k"], [tuple(e) for e in q["edges"]]) for q in evq]
+    roots = [tuple(q["roots"]) for q in evq]
+    heur = [cuts_amd.Plan(cuts_amd.Graph.from_edges(k, e)) for k, e in qs]
+    single = [ctx.match(dg, p, roots=rt) for p, rt in zip(heur, roots)]
     choices = []
-    for j in range(0, len(evals), 32):
-        part = evals[j:j + 32]
-        choices += LN.choose_orders(net, [(k, e) for k, e, _, _ in part], gf, [p.order for _, _, p, _ in part])
-    h_ms, l_ms, model_fb, run_fb, mism = [], [], 0, 0, 0
-    for (k, edges, p, r), ch in zip(evals, choices):
-        rh = ctx.match(dg, p)  # timed again, warm
+    for j in range(0, len(qs), 32):
+        choices += LN.choose_orders(net, qs[j:j + 32], gf, [p.order for p in heur[j:j + 32]], keep_first=True)
+    judged = LN.judge(ctx, dg, qs, heur, single, choices, roots=roots)
+    h_ms, l_ms = [], []
+    for p, jd, rt in zip(heur, judged, roots):
+        rh = ctx.match(dg, p, roots=rt)  # timed again, warm
         h_ms.append(rh.device_ms)
-        if not ch.learned:
-            model_fb += 1
-            l_ms.append(rh.device_ms)
-            continue
-        rl = ctx.match(dg, cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges), ch.order), max_work=4 * r.work + 1024)
-        if rl.work_capped:
-            run_fb += 1
-            l_ms.append(rl.device_ms + rh.device_ms)
-            continue
-        if rl.count != rh.count:
-            mism += 1
-        l_ms.append(rl.device_ms)
+        l_ms.append(ctx.match(dg, jd.plan, roots=rt).device_ms if jd.learned else rh.device_ms)
     ratio = [a / b for a, b in zip(h_ms, l_ms) if b > 0]
     geo = float(np.exp(np.mean(np.log(ratio)))) if ratio else 0.0
+    counts = [r.count for r in single]
+    proposed = sum(1 for c in choices if c.learned)
+    kept = sum(1 for jd in judged if jd.learned)
     line = base_line("exact embeddings/sec + p50 time/query; learned vs heuristic order, same GPU",
-                     sum(r.count for _, _, _, r in evals) / (sum(l_ms) / 1e3), "embeddings/s", 1, 0, sum(l_ms),
+                     sum(counts) / (sum(l_ms) / 1e3), "embeddings/s", 1, 0, sum(l_ms),
                      "train: 4 R-MAT scale-18 graphs (seeds 1-4), 5-10 vertex walk queries; evaluate: 6-10 vertex "
-                     "walk queries on R-MAT scale 20 (seed 0) admitted at 2^29 candidates",
+                     "walk queries on R-MAT scale 20 (seed 0); every query exact on its admitted root sample of "
+                     "position 0; learned orders keep the first vertex and are judged by the score order",
                      {"train_samples": int(len(meta)), "mse_first": hist[0], "mse_last": hist[-1],
-                      "eval_queries": len(evals), "tried": tried,
+                      "eval_queries": len(evq), "tried": tried, "sum_count": sum(counts),
                       "p50_ms_heuristic": statistics.median(h_ms), "p50_ms_learned": statistics.median(l_ms),
                       "speedup_p50": statistics.median(h_ms) / statistics.median(l_ms),
                       "speedup_total": sum(h_ms) / sum(l_ms), "speedup_geomean": geo,
-                      "fallback_rate_model": model_fb / len(evals), "fallback_rate_runtime": run_fb / len(evals),
-                      "count_mismatches": mism})
+                      "learned_proposed": proposed, "learned_kept": kept,
+                      "fallback_rate": 1.0 - kept / max(1, len(evq)),
+                      "candidates_learned": sum(jd.work for jd in judged),
+                      "candidates_heuristic": sum(r.work for r in single)})
     emit(line)
 
 

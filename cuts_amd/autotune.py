@@ -9,7 +9,7 @@ workload) in ``$CUTS_TUNE_CACHE`` (default ``~/.cache/cuts_amd/tune.json``).
 Tunables:
   single query -- blocks per CU of the expansion kernels (grid = CUs x n) and
                   the XCD-contiguous tile mapping;
-  batch        -- blocks per CU of the level launches (1..4) and the candidate
+  batch        -- blocks per CU of the level launches (2..8) and the candidate
                   work at which a new work item is started.
 """
 from __future__ import annotations
@@ -83,7 +83,7 @@ def tune_single(ctx: Context, dg: DeviceGraph, plans: Sequence[Plan], key: str, 
 
 
 def tune_batch(ctx: Context, dg: DeviceGraph, plans: Sequence[Plan], key: str, reps: int = 2,
-               blocks: Sequence[int] = (1, 2, 4), item_works: Sequence[int] = (4096, 16384, 65536),
+               blocks: Sequence[int] = (2, 4, 6, 8), item_works: Sequence[int] = (4096, 16384, 65536),
                arena_bytes: int = 32 << 30, queue_cap: int = 0, roots=None, log=None,
                use_cache: bool = True) -> Dict:
     cache = _load()

@@ -44,12 +44,14 @@ namespace {
 constexpr int BB = 256;      // threads per block (4 wave64)
 constexpr int NW = BB / 64;  // waves per block
 constexpr int RMAX = 256;    // rows per item (one per thread in the scan)
-constexpr int WROW = 16;     // LDS words per staged row (k <= 16)
+// LDS words per row (input and staged): a template parameter of at least
+// kmax-1, odd so that rows of different lanes fall in different banks.
 
 struct GV {
   const uint32_t *off_out, *adj_out, *off_in, *adj_in, *sig;
   uint32_t V;
   const int32_t* hub_id;  // hub adjacency bitmaps (symmetric graphs) or null
+  const uint2 *nsig_out, *nsig_in;  // signature of every list entry
   const uint64_t* bits;
   uint32_t words;
 };
@@ -122,6 +124,7 @@ __device__ __forceinline__ uint64_t next_work(const GV& g, const KPlan& P, const
   return best;
 }
 
+template <int WROW>
 struct Shared {
   Item item;
   uint32_t cmd;
@@ -178,7 +181,8 @@ __device__ void push_item(const BatchArgs& a, uint32_t level, const Item& it) {
 
 // Writes the staged rows (length `words`, i.e. level `words`) to that
 // level's arena half and publishes them as one item.  Whole block.
-__device__ void flush_stage(const BatchArgs& a, Shared& s, uint32_t q, uint32_t words) {
+template <int WROW>
+__device__ void flush_stage(const BatchArgs& a, Shared<WROW>& s, uint32_t q, uint32_t words) {
   __syncthreads();
   const uint32_t n = uni(s.nstage);
   if (n == 0) return;
@@ -213,7 +217,8 @@ __device__ void flush_stage(const BatchArgs& a, Shared& s, uint32_t q, uint32_t 
 
 // Block-wide append of the survivors row[0..L) + v (one per thread with ok)
 // to the stage, flushing first when they would overfill it.
-__device__ void stage_append(const BatchArgs& a, Shared& s, uint32_t q, bool ok, const uint32_t* row, uint32_t L,
+template <int WROW>
+__device__ void stage_append(const BatchArgs& a, Shared<WROW>& s, uint32_t q, bool ok, const uint32_t* row, uint32_t L,
                              uint32_t v, uint64_t work) {
   const uint32_t wave = wave_id();
   const uint64_t m = __ballot(ok);
@@ -248,8 +253,9 @@ __device__ void stage_append(const BatchArgs& a, Shared& s, uint32_t q, bool ok,
   __syncthreads();
 }
 
-__device__ void process(const BatchArgs& a, Shared& s, uint32_t L) {
-  const GV g{a.off_out, a.adj_out, a.off_in, a.adj_in, a.sig, a.V, a.hub_id, a.bits, a.words};
+template <int WROW>
+__device__ void process(const BatchArgs& a, Shared<WROW>& s, uint32_t L) {
+  const GV g{a.off_out, a.adj_out, a.off_in, a.adj_in, a.sig, a.V, a.hub_id, a.nsig_out, a.nsig_in, a.bits, a.words};
   Item it;
   it.qL = uni(s.item.qL);
   it.n = uni(s.item.n);
@@ -360,8 +366,10 @@ __device__ void process(const BatchArgs& a, Shared& s, uint32_t L) {
       const uint32_t u = row[jc & 31];
       const uint32_t* off = (jc >> 7) ? g.off_in : g.off_out;
       const uint32_t* adj = (jc >> 7) ? g.adj_in : g.adj_out;
-      v = adj[off[u] + idx];
-      ok = g.sig[2 * (uint64_t)v] >= tin && g.sig[2 * (uint64_t)v + 1] >= tout;
+      const uint32_t li = off[u] + idx;
+      v = adj[li];
+      const uint2 sgv = ((jc >> 7) ? g.nsig_in : g.nsig_out)[li];
+      ok = sgv.x >= tin && sgv.y >= tout;
       for (uint32_t l = 0; l < L && ok; ++l) ok = v != row[l];
       for (uint32_t c = 0; c < nj && ok; ++c) {
         if (c == sel) continue;
@@ -439,8 +447,11 @@ __global__ __launch_bounds__(BB) void k_batch_init(BatchArgs a) {
 }
 
 // Processes every item of level L (published by the earlier launches).
-__global__ __launch_bounds__(BB) void k_batch_level(BatchArgs a, uint32_t L) {
-  __shared__ Shared s;
+// At least 6 waves per SIMD: 80 VGPRs; with row widths 7 and 11 the LDS also
+// fits 6 blocks of 4 waves per CU.
+template <int WROW>
+__global__ __launch_bounds__(BB, 6) void k_batch_level(BatchArgs a, uint32_t L) {
+  __shared__ Shared<WROW> s;
   const bool l0 = lane_id() == 0;
   const unsigned long long n_items = uni64(std::min<unsigned long long>(a.st->tail[L].v, a.qcap));
   s.nstage = 0;  // same value from every thread
@@ -525,7 +536,7 @@ void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& p
   b.nq = (uint32_t)plans.size();
   b.vchunk = o.vchunk ? o.vchunk : 4096;
   b.use_graph = o.use_graph != 0;
-  const uint32_t bpc = o.blocks_per_cu ? std::min<uint32_t>(o.blocks_per_cu, 8) : 2;
+  const uint32_t bpc = o.blocks_per_cu ? std::min<uint32_t>(o.blocks_per_cu, 8) : 6;
   b.grid = (uint32_t)c.cus * bpc;
   const uint64_t qcap = o.queue_cap ? o.queue_cap : (1ull << 22);
   // position-0 roots per query (first, stride, count) and the level-0 items
@@ -574,6 +585,8 @@ void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& p
   a.sig = dg.sig;
   a.V = dg.V;
   a.hub_id = (o.flags & 32u) ? nullptr : dg.hub_id;
+  a.nsig_out = dg.nsig_out;
+  a.nsig_in = dg.nsig_in;
   a.bits = dg.bits;
   a.words = dg.words;
   KPlan* dplans = balloc<KPlan>(b, 0, b.nq);
@@ -614,7 +627,11 @@ static void batch_launch(Batch& b, hipStream_t s) {
   const uint64_t nitems = std::max<uint64_t>(b.args.n_init, b.nq);
   const uint32_t igrid = (uint32_t)std::min<uint64_t>((nitems + BB - 1) / BB, 4096);
   hipLaunchKernelGGL(k_batch_init, dim3(std::max<uint32_t>(igrid, 1)), dim3(BB), 0, s, b.args);
-  for (uint32_t L = 0; L < b.kmax; ++L) hipLaunchKernelGGL(k_batch_level, dim3(b.grid), dim3(BB), 0, s, b.args, L);
+  for (uint32_t L = 0; L < b.kmax; ++L) {
+    if (b.kmax <= 8) hipLaunchKernelGGL(k_batch_level<7>, dim3(b.grid), dim3(BB), 0, s, b.args, L);
+    else if (b.kmax <= 12) hipLaunchKernelGGL(k_batch_level<11>, dim3(b.grid), dim3(BB), 0, s, b.args, L);
+    else hipLaunchKernelGGL(k_batch_level<15>, dim3(b.grid), dim3(BB), 0, s, b.args, L);
+  }
 }
 
 void batch_run(Batch& b, std::vector<BatchQueryResult>& out, double& device_ms, uint64_t& arena_words) {

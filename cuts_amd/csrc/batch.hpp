@@ -29,6 +29,7 @@ struct BatchArgs {
   const uint32_t *off_out, *adj_out, *off_in, *adj_in, *sig;
   uint32_t V;
   const int32_t* hub_id;
+  const uint2 *nsig_out, *nsig_in;
   const uint64_t* bits;
   uint32_t words;
   const KPlan* plans;

@@ -47,6 +47,10 @@ struct DeviceGraph {
   uint32_t max_deg = 0;
   uint32_t *off_out = nullptr, *adj_out = nullptr, *off_in = nullptr, *adj_in = nullptr;
   uint32_t* sig = nullptr;                 // 2V: in, out
+  // signature of every list entry, parallel to adj_out / adj_in (the in-side
+  // aliases the out-side on symmetric graphs): a candidate's signature is
+  // read at its list index, coalesced with the list word itself
+  uint2 *nsig_out = nullptr, *nsig_in = nullptr;
   std::map<uint32_t, uint32_t*> cnt_tables; // degree threshold -> #neighbours meeting it
   // adjacency bitmaps of hub vertices (symmetric graphs): hub_id[v] = row of
   // v in `bits` or -1; a row is `words` uint64 covering all V vertices

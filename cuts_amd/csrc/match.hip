@@ -57,12 +57,14 @@ struct GView {
   const uint32_t *off_out, *adj_out, *off_in, *adj_in, *sig;
   uint32_t V;
   const int32_t* hub_id;  // null when the graph has no hub bitmaps
+  const uint2 *nsig_out, *nsig_in;  // signature of every list entry
   const uint64_t* bits;
   uint32_t words;
 };
 
 GView view_of(const DeviceGraph& dg) {
-  return GView{dg.off_out, dg.adj_out, dg.off_in, dg.adj_in, dg.sig, dg.V, dg.hub_id, dg.bits, dg.words};
+  return GView{dg.off_out, dg.adj_out, dg.off_in, dg.adj_in, dg.sig, dg.V, dg.hub_id, dg.nsig_out, dg.nsig_in, dg.bits,
+               dg.words};
 }
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
@@ -500,7 +502,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
         for (int j = 0; j < G; ++j) v[j] = in[j] ? ((jc[j] >> 7) ? g.adj_in : g.adj_out)[lb[j] + idx[j]] : 0u;
         uint2 sg[G];
 #pragma unroll
-        for (int j = 0; j < G; ++j) sg[j] = *reinterpret_cast<const uint2*>(g.sig + 2 * (uint64_t)v[j]);
+        for (int j = 0; j < G; ++j) sg[j] = in[j] ? ((jc[j] >> 7) ? g.nsig_in : g.nsig_out)[lb[j] + idx[j]] : make_uint2(0u, 0u);
 #pragma unroll
         for (int j = 0; j < G; ++j) {
           ok[j] = in[j] && sg[j].x >= tin && sg[j].y >= tout;
@@ -649,12 +651,14 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
           if (l == (int)(jc & 31)) u = row[l];
         const uint32_t* off = (jc >> 7) ? g.off_in : g.off_out;
         const uint32_t* adj = (jc >> 7) ? g.adj_in : g.adj_out;
-        v = adj[off[u] + idx];
+        const uint32_t li = off[u] + idx;
+        v = adj[li];
+        const uint2 sgv = ((jc >> 7) ? g.nsig_in : g.nsig_out)[li];
         // demand bytes: the candidate word and its signature; once per
         // prefix its row, selector, scan entry and constraint offsets
         bytes += 12;
         if (idx == 0) bytes += 4 * L + 9 + 8 * nj;
-        ok = g.sig[2 * (uint64_t)v] >= tin && g.sig[2 * (uint64_t)v + 1] >= tout;
+        ok = sgv.x >= tin && sgv.y >= tout;
 #pragma unroll
         for (int l = 0; l < L; ++l) ok = ok && v != row[l];
         for (uint32_t c = 0; c < nj && ok; ++c) {
@@ -1204,9 +1208,11 @@ DeviceGraph::~DeviceGraph() {
   (void)hipGetDevice(&prev);
   (void)hipSetDevice(device);
   for (uint32_t* p : {off_out, adj_out, sig}) (void)hipFree(p);
+  if (nsig_out) (void)hipFree(nsig_out);
   if (!symmetric) {
     (void)hipFree(off_in);
     (void)hipFree(adj_in);
+    if (nsig_in) (void)hipFree(nsig_in);
   }
   for (auto& kv : cnt_tables) (void)hipFree(kv.second);
   if (hub_id) (void)hipFree(hub_id);
@@ -1275,6 +1281,16 @@ void upload_graph(Context& c, const Graph& g, DeviceGraph& dg) {
     up(dg.off_in, g.in_off);
     up(dg.adj_in, g.in_adj);
   }
+  auto up_nsig = [&](uint2*& d, const std::vector<uint32_t>& adj) {
+    std::vector<uint2> h(adj.size());
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)adj.size(); ++i) h[i] = make_uint2(g.sig[2 * (uint64_t)adj[i]], g.sig[2 * (uint64_t)adj[i] + 1]);
+    dev_alloc(d, std::max<size_t>(h.size(), 1));
+    if (!h.empty()) HIPCHECK(hipMemcpy(d, h.data(), h.size() * sizeof(uint2), hipMemcpyHostToDevice));
+  };
+  up_nsig(dg.nsig_out, g.out_adj);
+  if (g.symmetric) dg.nsig_in = dg.nsig_out;
+  else up_nsig(dg.nsig_in, g.in_adj);
   // Hub bitmaps: the lowest power-of-two degree threshold (>= 32) whose
   // bitmaps fit the budget (1/32 of free HBM, at most 4 GiB).
   if (g.symmetric && g.V >= 64) {

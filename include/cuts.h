@@ -182,7 +182,7 @@ int cuts_log_tables(const char *path, const char *label, const cuts_plan *plan, 
 typedef struct cuts_batch cuts_batch;
 typedef struct {
   uint32_t flags;         /* CUTS_FLAG_NO_SHORTCUT / CUTS_FLAG_NO_FUSE / CUTS_FLAG_NO_BITMAP */
-  uint32_t blocks_per_cu; /* grid of each level launch = CUs x this, 1..8; 0 = 2 */
+  uint32_t blocks_per_cu; /* grid of each level launch = CUs x this, 1..8; 0 = 6 */
   uint32_t vchunk;        /* level-0 vertices per initial item; 0 = 4096 */
   uint32_t no_graph;      /* 1: plain launches instead of a hipGraph */
   uint64_t queue_cap;     /* work items per level; 0 = 4M */
