@@ -22,8 +22,9 @@ algorithm (oracle/, test infrastructure) on a bounded sample of level 0.
 Other workloads (one JSON line each; the default above is the headline):
     --workload chunglu4k   triangle + 2-edge tail on Chung-Lu n=4,096 (full count)
     --workload rmat20      5-cycle + chord + 2 tails on R-MAT scale 20: the full
-                           count exceeds 2^64, so the run is time-limited and
-                           reports the exact count of a root prefix
+                           count exceeds 2^64, so the query is matched exactly
+                           on a nested root sample of position 0 sized to 2^36
+                           candidates
     --workload batch256    256 random-walk queries, batched level queue in one hipGraph
                            launch, learned orders, autotuned launch shape
     --workload learned     train the table-size network on R-MAT scale 18,
@@ -117,24 +118,41 @@ def run_chunglu4k(args):
                    {"count": counts[0], "p50_ms_per_query": statistics.median(ms)}))
 
 
-def run_rmat20(args, limit_s=20.0):
+def run_rmat20(args, max_work=1 << 36):
+    """Config 3: the full count exceeds 2^64, so the query is matched exactly
+    on the densest nested root sample of position 0 (v = s-1 mod s, s a
+    power of two) whose run stays within max_work candidates -- found by a
+    binary search of capped runs, then timed."""
     import cuts_amd
     from cuts_amd import workloads as W
     g = W.config3_graph()
     ctx = cuts_amd.Context(0)
     dg = ctx.upload(g)
     p = cuts_amd.Plan(W.query_graph(W.C5CHORD_2TAIL))
-    roots = (1 << 19, 1 << 20, 97)
-    log(f"R-MAT 20 uploaded ({g.arcs} arcs); roots {roots}, {limit_s}s limit, one root per trunk")
-    r = ctx.match(dg, p, roots=roots, trunks=1 << 20, time_limit_ms=limit_s * 1e3, flags=cuts_amd.FLAG_PROFILE)
+    log(f"R-MAT 20 uploaded ({g.arcs} arcs)")
+    lo, hi = 0, 20
+    while lo < hi:  # smallest stride exponent whose sample fits (nested samples: monotone)
+        mid = (lo + hi) // 2
+        r = ctx.match(dg, p, roots=W.root_sample(g.V, 1 << mid), max_work=max_work)
+        log(f"stride 2^{mid}: {'over the cap' if r.work_capped else f'{r.work} candidates'}")
+        if r.work_capped:
+            lo = mid + 1
+        else:
+            hi = mid
+    roots = W.root_sample(g.V, 1 << hi)
+    for _ in range(args.warmup):
+        ctx.match(dg, p, roots=roots)
+    runs = [ctx.match(dg, p, roots=roots, flags=cuts_amd.FLAG_PROFILE) for _ in range(max(1, args.steps))]
+    r = runs[-1]
+    ms = statistics.median(x.device_ms for x in runs)
     dname, (dms, dn, dbytes) = max(r.kernels.items(), key=lambda kv: kv[1][0])
     ach = r.merge_bytes[dname] / dn / (dms / dn / 1e3) / 1e9 if dn else 0.0
-    line = base_line("exact embeddings/sec (time-limited exact root prefix)", r.count / (r.device_ms / 1e3),
-                     "embeddings/s", 1, 0, r.device_ms,
-                     "rmat scale=20 ef=16 a,b,c=.57,.19,.19 seed=0; 5-cycle+chord+2-tails; heuristic order",
-                     {"roots": list(roots), "roots_done": r.roots_done, "last_root": r.last_root,
-                      "count_of_root_prefix": r.count, "complete": r.complete, "level_sizes": r.level_sizes,
-                      "time_limit_s": limit_s})
+    line = base_line("exact embeddings/sec + p50 time/query", r.count / (ms / 1e3), "embeddings/s",
+                     len(runs), args.warmup, ms,
+                     "rmat scale=20 ef=16 a,b,c=.57,.19,.19 seed=0; 5-cycle+chord+2-tails; score order; exact "
+                     "count on the densest nested root sample of position 0 within 2^36 candidates",
+                     {"roots": list(roots), "count": r.count, "level_sizes": r.level_sizes, "work": r.work,
+                      "p50_ms_per_query": ms})
     line["roofline"] = {"bound": "hbm", "kernel": dname, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": ach / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": dms / max(dn, 1),
                         "launches": dn, "bytes_model": "merge model",
@@ -167,6 +185,23 @@ def run_batch256(args):
     counts = [r.count for r in single]
     assert counts == [q["count"] for q in fx["queries"]], "single-query counts differ from the fixture"
     log(f"single heuristic: p50 {statistics.median(s_ms):.3f} ms, total {sum(s_ms):.1f} ms")
+    if args.score_only:
+        # score orders only, default launch shape (for profiling the level kernel)
+        bh = cuts_amd.Batch(ctx, dg, heur, arena_bytes=BATCH_ARENA, queue_cap=BATCH_QUEUE, roots=roots,
+                            use_graph=not args.plain_launches)
+        for _ in range(args.warmup):
+            bh.run()
+        runs = [bh.run() for _ in range(args.steps)]
+        assert all(r.counts == counts for r in runs), "batched counts differ from single-query counts"
+        ms = statistics.median(r.device_ms for r in runs)
+        emit(base_line("exact embeddings/sec + p50 time/query", sum(counts) / (ms / 1e3), "embeddings/s",
+                       args.steps, args.warmup, ms,
+                       "256 random-walk queries (k 4-12 in equal quotas) on chung-lu n=36692, each exact on its "
+                       "admitted root sample of position 0; score orders; level-synchronous batch queue",
+                       {"queries": len(qs), "sum_count": sum(counts), "batch_ms": ms,
+                        "amortised_ms_per_query": ms / len(qs), "p50_ms_single_heuristic": statistics.median(s_ms),
+                        "plain_launches": bool(args.plain_launches)}))
+        return
     # learned orders from a network trained on tables logged on other queries
     # of the same graph (same admission rule, another seed)
     train, tried = W.admit(ctx, dg, g, min(args.train_queries, 1024), seed=1)
@@ -356,6 +391,8 @@ def main():
     ap.add_argument("--train-queries", type=int, default=2000)
     ap.add_argument("--eval-queries", type=int, default=500)
     ap.add_argument("--save-weights", default="")
+    ap.add_argument("--score-only", action="store_true", help="batch256: score orders only, no training")
+    ap.add_argument("--plain-launches", action="store_true", help="batch256 --score-only: no hipGraph")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: only one GPU is used by this build (asked for --gpus {args.gpus}); "

@@ -67,6 +67,12 @@ GView view_of(const DeviceGraph& dg) {
                dg.words};
 }
 
+__device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ uint64_t uni64(uint64_t x) {
+  return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(x >> 32)) << 32) |
+         __builtin_amdgcn_readfirstlane((uint32_t)x);
+}
+
 __device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
@@ -412,27 +418,32 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
   // the tiles (neighbouring prefixes share adjacency lists in that XCD's L2)
   // and, once that share is drained, takes tiles from the other shares.
   // One atomic per 1,024-item tile; no static imbalance between blocks.
+  // The fetch runs in wave 0 under a wave-uniform branch; lane 0's atomic
+  // is nested inside it and its result broadcast, so the branches at the
+  // edges of this barrier loop are all wave-uniform.
   __shared__ uint64_t s_tile;
   const uint32_t nx = a.xcd ? 8u : 1u;
   const uint32_t myx = a.xcd ? (blockIdx.x & 7u) : 0u;
   uint32_t tries = 0;
   while (true) {
-    if (tid == 0) {
+    if (uni(tid >> 6) == 0) {
       uint64_t t = ~0ull;
       while (tries < nx) {
         const uint32_t xx = (myx + tries) % nx;
         const uint64_t lo = a.ntiles * xx / nx, hi = a.ntiles * (xx + 1) / nx;
-        const uint64_t i = atomicAdd(&a.tilectr[xx], 1u);
+        uint32_t got = 0;
+        if ((tid & 63) == 0) got = atomicAdd(&a.tilectr[xx], 1u);  // inside the wave-uniform branch
+        const uint64_t i = uni(got);
         if (lo + i < hi) {
           t = lo + i;
           break;
         }
         ++tries;
       }
-      s_tile = t;
+      s_tile = t;  // same value from every lane of wave 0
     }
     __syncthreads();
-    const uint64_t tile = s_tile;
+    const uint64_t tile = uni64(s_tile);
     if (tile == ~0ull) break;
     {
     const uint64_t w0 = a.w_begin + tile * TILE;
