@@ -417,7 +417,7 @@ def main():
             ctx.match(dg, p)
         print(f"bench: warmup step {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
 
-    per_query_ms, counts = [], []
+    per_query_ms, counts, work = [], [], []
     kern = {}
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -425,6 +425,7 @@ def main():
             r = ctx.match(dg, p, flags=cuts_amd.FLAG_PROFILE)
             per_query_ms.append(r.device_ms)
             counts.append(r.count)
+            work.append(r.work)
             for name, (ms, n, b) in r.kernels.items():
                 k = kern.setdefault(name, [0.0, 0, 0.0, 0.0])
                 k[0] += ms
@@ -459,6 +460,8 @@ def main():
             "counts": counts[: len(plans)],
             "p50_ms_per_query": statistics.median(per_query_ms),
             "p99_ms_per_query": sorted(per_query_ms)[min(len(per_query_ms) - 1, int(0.99 * len(per_query_ms)))],
+            "candidates_per_step": sum(work) / args.steps,
+            "candidates_per_s": sum(work) / (total_ms / 1e3),
             "wall_s": wall,
         },
         "roofline": {
@@ -475,6 +478,9 @@ def main():
             "launches": dn,
             "demand_bytes_per_launch": dbytes / dn,
             "demand_GBs": dbytes / dn / avg_launch_s / 1e9,
+            "demand_frac": dbytes / dn / avg_launch_s / 1e9 / HBM_PEAK_GBS,
+            "note": "achieved/frac use the merge model, which charges list reads that the closed form never makes, "
+                    "so frac > 1 here; demand_GBs is what the kernel loads, traffic what reaches HBM (rocprofv3)",
         },
         "merge_bytes_total": sum(v[3] for v in kern.values()),
         "kernels_ms": {k: v[0] for k, v in kern.items() if v[1]},
