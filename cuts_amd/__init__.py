@@ -29,9 +29,11 @@ FLAG_NO_XCD = 8
 FLAG_PROFILE = 16
 FLAG_NO_BITMAP = 32
 FLAG_NO_PREFIX_BASE = 64
+FLAG_NO_PAIR = 128
 MAX_QUERY = 16
 KERNEL_CLASSES = ["level0", "plan", "scan", "tile_index", "expand_write", "expand_count", "expand_fused",
-                  "closed_form"]
+                  "closed_form", "expand_pair", "pair_common"]
+_NCLASS = len(KERNEL_CLASSES)
 
 _u32p = C.POINTER(C.c_uint32)
 
@@ -52,9 +54,10 @@ class _Opts(C.Structure):
 class _Result(C.Structure):
     _fields_ = [("count", C.c_uint64), ("level_sizes", C.c_uint64 * 16), ("device_ms", C.c_double),
                 ("emb_written", C.c_uint64), ("chunks", C.c_uint32), ("complete", C.c_uint32),
-                ("kernel_ms", C.c_double * 8), ("kernel_launches", C.c_uint64 * 8), ("kernel_bytes", C.c_double * 8),
+                ("kernel_ms", C.c_double * _NCLASS), ("kernel_launches", C.c_uint64 * _NCLASS),
+                ("kernel_bytes", C.c_double * _NCLASS),
                 ("roots_done", C.c_uint64), ("last_root", C.c_uint32), ("work_capped", C.c_uint32),
-                ("work", C.c_uint64), ("merge_bytes", C.c_double * 8)]
+                ("work", C.c_uint64), ("merge_bytes", C.c_double * _NCLASS)]
 
 
 class _BatchOpts(C.Structure):
@@ -333,8 +336,8 @@ class Context:
             res.rows = rows[: r.emb_written].copy()
         if o.flags & FLAG_PROFILE:
             res.kernels = {KERNEL_CLASSES[i]: (float(r.kernel_ms[i]), int(r.kernel_launches[i]),
-                                               float(r.kernel_bytes[i])) for i in range(8)}
-            res.merge_bytes = {KERNEL_CLASSES[i]: float(r.merge_bytes[i]) for i in range(8)}
+                                               float(r.kernel_bytes[i])) for i in range(_NCLASS)}
+            res.merge_bytes = {KERNEL_CLASSES[i]: float(r.merge_bytes[i]) for i in range(_NCLASS)}
         return res
 
 

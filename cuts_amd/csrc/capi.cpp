@@ -357,7 +357,7 @@ int cuts_match(cuts_context* ctx, const cuts_device_graph* dg, const cuts_plan* 
     res->last_root = r.last_root;
     res->work_capped = r.work_capped;
     res->work = r.work;
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < 10; ++i) {
       res->kernel_ms[i] = r.kernel_ms[i];
       res->kernel_launches[i] = r.kernel_launches[i];
       res->kernel_bytes[i] = r.kernel_bytes[i];

@@ -35,6 +35,16 @@ struct KPlan {
   uint32_t sc_pos;     // position whose list is counted
   uint32_t sc_adjmask; // positions certainly inside that list
   uint32_t sc_thr;     // degree threshold of the last position
+  // last two positions counted together (symmetric graphs): when neither is
+  // a backward neighbour of the other and each has one backward neighbour,
+  // at positions pr_a and pr_b older than position k-3, a prefix of
+  // positions 0..k-3 has |A| * |B| - |A & B| completions, with
+  // A = N(row[pr_a]) and B = N(row[pr_b]) minus the prefix, filtered by
+  // degree thresholds
+  uint32_t pr_ok;
+  uint32_t pr_a, pr_b;
+  uint32_t pr_ta, pr_tb, pr_tab;  // degree thresholds of A, B and A & B
+  uint32_t pr_ma, pr_mb;          // positions < k-2 certainly inside N(row[pr_a]) / N(row[pr_b])
 };
 
 KPlan make_kplan(const Plan& p, bool symmetric_data);
@@ -69,6 +79,8 @@ struct LevelBuf {
   uint8_t* minsel = nullptr;    // cap
   uint64_t* work_off = nullptr; // cap+1
   uint32_t* cbase = nullptr;    // cap: per-prefix closed-form base (fused levels)
+  uint4* pbase = nullptr;       // cap: per-prefix |A|, |B|, |A & B| bases (pair level)
+  uint32_t* nsurv = nullptr;    // cap: survivors per prefix (pair level)
   uint64_t* bsum = nullptr;     // scan partials
   uint64_t bsum_cap = 0;
   uint32_t* tile_p = nullptr;   // tiles+1
@@ -82,8 +94,10 @@ struct Context {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   uint64_t default_budget = 0;
   LevelBuf lv[kMaxQuery];
-  // [0] embeddings, [1] fused level size, [2] write count, [3] demand bytes,
-  // [4..7] eight uint32 tile counters, [8] merge-model bytes
+  // [0] embeddings, [1] fused level size (level k-3 with the pair level),
+  // [2] level k-2 size of the pair level, [3] demand bytes, [4..7] eight
+  // uint32 tile counters, [8] merge-model bytes, [10] rows written by one
+  // expansion
   unsigned long long* counters = nullptr;
   hipEvent_t kev[2] = {nullptr, nullptr};
   uint64_t* host_pinned = nullptr;        // small pinned staging
@@ -112,10 +126,10 @@ struct MatchResult {
   uint32_t last_root = 0;
   uint32_t work_capped = 0;
   uint64_t work = 0;
-  double kernel_ms[8] = {0};
-  uint64_t kernel_launches[8] = {0};
-  double kernel_bytes[8] = {0};
-  double merge_bytes[8] = {0};
+  double kernel_ms[10] = {0};
+  uint64_t kernel_launches[10] = {0};
+  double kernel_bytes[10] = {0};
+  double merge_bytes[10] = {0};
 };
 
 void context_init(Context& c, int device);

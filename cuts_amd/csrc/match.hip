@@ -159,6 +159,16 @@ __global__ __launch_bounds__(BLOCK) void k_level0_scatter(const uint32_t* __rest
 }
 
 // ------------------------------------------------------------ planning
+// Degree-threshold tables of the pair level: |{x in N(u) : deg(x) >= t}| for
+// the thresholds of A, B and A & B (null: t <= 1, the plain degree).
+struct PairTables {
+  const uint32_t *a, *b, *ab;
+};
+
+__device__ __forceinline__ uint32_t tbl_or_deg(const GView& g, const uint32_t* tbl, uint32_t u) {
+  return tbl ? tbl[u] : g.off_out[u + 1] - g.off_out[u];
+}
+
 // Per prefix: the constraint whose list is shortest (upper bound on the
 // prefix's children).  Ties keep the lowest constraint index.
 // With merge (profiling) it adds the merge-model bytes of expanding the
@@ -167,10 +177,17 @@ __global__ __launch_bounds__(BLOCK) void k_level0_scatter(const uint32_t* __rest
 // With cbase (fused levels whose closed-form vertex u is an older position)
 // it also stores the part of the closed form that only depends on the
 // prefix: cnt(u) - |{i < L : r_i in N(u), deg(r_i) >= t}|.
+// With pbase (the pair level, L = k-3) it stores the prefix's parts of |A|,
+// |B| and |A & B|: the degree-filtered list sizes of u = r[pr_a] and
+// w = r[pr_b] minus the prefix vertices inside them, and for |A & B| (as a
+// signed word) the prefix vertices inside both, negated, plus -- when u = w --
+// the filtered size of N(u); for u != w the size of N(u) & N(w) is counted
+// afterwards by k_pair_common, only for prefixes with survivors.
 template <int L>
 __global__ __launch_bounds__(BLOCK) void k_plan(GView g, KPlan P, const uint32_t* __restrict__ rows, uint64_t n,
                                                uint32_t* __restrict__ minlen, uint8_t* __restrict__ minsel,
                                                uint32_t* __restrict__ cbase, const uint32_t* __restrict__ tbl,
+                                               uint4* __restrict__ pbase, PairTables pt,
                                                unsigned long long* __restrict__ merge) {
   const uint32_t nj = P.nj[L];
   uint64_t mb = 0;
@@ -208,6 +225,27 @@ __global__ __launch_bounds__(BLOCK) void k_plan(GView g, KPlan P, const uint32_t
         if (in && (P.sc_thr <= 1 || g.sig[2 * (uint64_t)row[l]] >= P.sc_thr)) --c;
       }
       cbase[p] = (uint32_t)c;
+    }
+    if (pbase) {
+      uint32_t u = 0, w = 0;
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        if (l == (int)P.pr_a) u = row[l];
+        if (l == (int)P.pr_b) w = row[l];
+      }
+      int64_t ca = tbl_or_deg(g, pt.a, u), cb = tbl_or_deg(g, pt.b, w);
+      int64_t cab = P.pr_a == P.pr_b ? (int64_t)tbl_or_deg(g, pt.ab, u) : 0;
+      uint64_t dummy = 0;
+#pragma unroll
+      for (int l = 0; l < L; ++l) {
+        const bool ina = ((P.pr_ma >> l) & 1) || adj_has(g, u, row[l], dummy);
+        const bool inb = ((P.pr_mb >> l) & 1) || adj_has(g, w, row[l], dummy);
+        const uint32_t d = g.sig[2 * (uint64_t)row[l]];
+        ca -= ina && d >= P.pr_ta;
+        cb -= inb && d >= P.pr_tb;
+        cab -= ina && inb && d >= P.pr_tab;
+      }
+      pbase[p] = make_uint4((uint32_t)ca, (uint32_t)cb, (uint32_t)(int32_t)cab, 0u);
     }
   }
   if (merge) add_wave(merge, mb);
@@ -387,7 +425,26 @@ __global__ __launch_bounds__(BLOCK) void k_cnt_table(GView g, uint32_t thr, uint
 }
 
 // ------------------------------------------------------------ expansion
-enum { M_WRITE = 0, M_COUNT = 1, M_FUSED = 2 };
+// Adds to cnt[p] the number of this wave's lanes with `ok` set whose item
+// belongs to prefix p.  Lanes hold consecutive work items, so the items of a
+// prefix are one run of lanes; the first lane of each run adds the run's
+// count (one atomic per prefix and wave instead of one per survivor).
+// Called by all lanes of the wave.
+__device__ __forceinline__ void add_runs(uint32_t* __restrict__ cnt, uint64_t p, bool ok) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t okm = __ballot(ok);
+  if (okm == 0) return;
+  const uint32_t pl = (uint32_t)p;
+  const uint32_t prev = __shfl_up(pl, 1);
+  const bool head = lane == 0 || prev != pl;
+  const uint64_t hm = __ballot(head);
+  const uint64_t above = lane == 63 ? 0ull : (hm >> (lane + 1)) << (lane + 1);
+  const uint64_t upto = above ? (above & (~above + 1)) - 1 : ~0ull;
+  const uint64_t mine = okm & upto & (~0ull << lane);
+  if (head && mine) atomicAdd(&cnt[p], (uint32_t)__popcll(mine));
+}
+
+enum { M_WRITE = 0, M_COUNT = 1, M_FUSED = 2, M_PAIR = 3 };
 
 struct ExpandArgs {
   const uint32_t* rows;      // n rows of L words
@@ -400,6 +457,8 @@ struct ExpandArgs {
   unsigned long long* counters;
   const uint32_t* tbl;       // M_FUSED closed-form table (null: plain degree)
   const uint32_t* cbase;     // M_FUSED with an older closed-form vertex: per-prefix part
+  const uint4* pbase;        // M_PAIR: per-prefix parts of |A|, |B|, |A & B|
+  uint32_t* nsurv;           // M_PAIR with u != w: survivors per prefix (zero at launch)
   uint32_t xcd;              // 1: XCD-contiguous tile mapping
   unsigned int* tilectr;     // 8 tile counters, zero at launch
   uint32_t stats;            // 1: accumulate demand bytes into counters[3]
@@ -412,7 +471,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
   const uint32_t tid = threadIdx.x, wave = tid >> 6;
   const uint32_t nj = P.nj[L];
   const uint32_t tin = P.sig_in[L], tout = P.sig_out[L];
-  uint64_t acc = 0, valid = 0, bytes = 0, mb = 0;
+  uint64_t acc = 0, valid = 0, pairs = 0, bytes = 0, mb = 0;
   // Dynamic, XCD-aware tile fetching: blocks are dispatched round-robin over
   // the 8 XCDs, so block b works first on XCD (b mod 8)'s contiguous share of
   // the tiles (neighbouring prefixes share adjacency lists in that XCD's L2)
@@ -461,7 +520,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
       // independent and in flight together instead of one dependent chain
       // after another.  Items past w1 are clamped to the tile's last prefix
       // and masked; their list word is never read.
-      constexpr int G = (MODE == M_FUSED) ? ((L <= 6) ? 2 : 1) : ((L <= 6) ? 4 : (L <= 10 ? 2 : 1));
+      constexpr int G = (MODE != M_COUNT) ? ((L <= 6) ? 2 : 1) : ((L <= 6) ? 4 : (L <= 10 ? 2 : 1));
       for (int g0 = 0; g0 < IPT; g0 += G) {
         uint64_t w[G];
         uint32_t pi[G];
@@ -484,6 +543,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
           n -= half;
         }
         uint32_t row[G][L], sel[G], jc[G], idx[G], lb[G], v[G], cb[G];
+        uint4 pb[G];
         bool ok[G];
 #pragma unroll
         for (int j = 0; j < G; ++j) {
@@ -495,6 +555,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
           for (int l = 0; l < L; ++l) row[j][l] = a.rows[p * L + l];
           sel[j] = a.minsel[p];
           cb[j] = (MODE == M_FUSED && a.cbase) ? a.cbase[p] : 0u;
+          pb[j] = MODE == M_PAIR ? a.pbase[p] : make_uint4(0u, 0u, 0u, 0u);
           if (a.stats) {
             bytes += 12;
             if (in[j] && idx[j] == 0) bytes += 4 * L + 9 + 8 * nj;
@@ -565,6 +626,76 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
         if (MODE == M_COUNT) {
 #pragma unroll
           for (int j = 0; j < G; ++j) acc += ok[j];
+        } else if (MODE == M_PAIR) {
+          // a survivor v completes the prefix of positions 0..k-3; its
+          // |A| * |B| - |A & B| completions take the prefix's bases minus v's
+          // own membership (certain when v's position is adjacent to pr_a /
+          // pr_b in the query, a bit test or a search otherwise)
+          const bool ca = (P.pr_ma >> L) & 1, cbb = (P.pr_mb >> L) & 1;
+          uint32_t ua[G], ub[G];
+          int32_t ha[G], hb[G];
+          uint64_t bwa[G], bwb[G];
+#pragma unroll
+          for (int j = 0; j < G; ++j) {
+            ua[j] = 0;
+            ub[j] = 0;
+#pragma unroll
+            for (int l = 0; l < L; ++l) {
+              if (l == (int)P.pr_a) ua[j] = row[j][l];
+              if (l == (int)P.pr_b) ub[j] = row[j][l];
+            }
+            ha[j] = (!ca && g.hub_id) ? g.hub_id[ua[j]] : -1;
+            hb[j] = (!cbb && g.hub_id) ? g.hub_id[ub[j]] : -1;
+          }
+#pragma unroll
+          for (int j = 0; j < G; ++j) {
+            bwa[j] = ha[j] >= 0 ? g.bits[(uint64_t)ha[j] * g.words + (v[j] >> 6)] : 0;
+            bwb[j] = hb[j] >= 0 ? g.bits[(uint64_t)hb[j] * g.words + (v[j] >> 6)] : 0;
+          }
+#pragma unroll
+          for (int j = 0; j < G; ++j) {
+            if (!ok[j]) continue;
+            ++valid;
+            bool ina = ca, inb = cbb;
+            if (!ina) {
+              if (ha[j] >= 0) {
+                ina = (bwa[j] >> (v[j] & 63)) & 1ull;
+              } else {
+                const uint32_t lo2 = g.off_out[ua[j]], hi2 = g.off_out[ua[j] + 1];
+                ina = member_safe(g.adj_out, lo2, hi2, v[j]);
+                if (a.stats) bytes += 8 + bs_bytes(hi2 - lo2);
+              }
+            }
+            if (!inb) {
+              if (hb[j] >= 0) {
+                inb = (bwb[j] >> (v[j] & 63)) & 1ull;
+              } else {
+                const uint32_t lo2 = g.off_out[ub[j]], hi2 = g.off_out[ub[j] + 1];
+                inb = member_safe(g.adj_out, lo2, hi2, v[j]);
+                if (a.stats) bytes += 8 + bs_bytes(hi2 - lo2);
+              }
+            }
+            const uint32_t dv = sg[j].x;
+            const uint64_t na = pb[j].x - (uint32_t)(ina && dv >= P.pr_ta);
+            const uint64_t nb = pb[j].y - (uint32_t)(inb && dv >= P.pr_tb);
+            // |A & B| without N(u) & N(w) itself when u != w (added per
+            // prefix by k_pair_common); modulo-2^64 sums are exact
+            const int64_t nab = (int64_t)(int32_t)pb[j].z - (int64_t)(ina && inb && dv >= P.pr_tab);
+            acc += na * nb - (uint64_t)nab;
+            pairs += na;
+            if (a.stats) {
+              bytes += 16 + ((ha[j] >= 0) ? 12 : 0) + ((hb[j] >= 0) ? 12 : 0);
+              // merge model: the survivor's node and its expansion by the
+              // list of pr_a; each of its na children's node and expansion
+              // by the list of pr_b
+              mb += 16 + 8 + 4ull * (g.off_out[ua[j] + 1] - g.off_out[ua[j]]) +
+                    na * (16 + 8 + 4ull * (g.off_out[ub[j] + 1] - g.off_out[ub[j]]));
+            }
+          }
+          if (a.nsurv) {
+#pragma unroll
+            for (int j = 0; j < G; ++j) add_runs(a.nsurv, plo + min(pi[j], span - 1), ok[j]);
+          }
         } else if (a.cbase) {
           // closed form = per-prefix base - [v in N(u), deg(v) >= t]; v is
           // certainly in N(u) when it was drawn from u's own list
@@ -750,12 +881,105 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
   }
   if (MODE != M_WRITE) {
     add_wave(&a.counters[0], acc);
-    if (MODE == M_FUSED) add_wave(&a.counters[1], valid);
+    if (MODE == M_FUSED || MODE == M_PAIR) add_wave(&a.counters[1], valid);
+    if (MODE == M_PAIR) add_wave(&a.counters[2], pairs);
   }
   if (a.stats) {
     add_wave(&a.counters[3], bytes);
-    if (MODE == M_FUSED) add_wave(&a.counters[8], mb);
+    if (MODE == M_FUSED || MODE == M_PAIR) add_wave(&a.counters[8], mb);
   }
+}
+
+// ------------------------------------------ pair level: |N(u) & N(w)|
+// For the pair level with u = r[pr_a] != w = r[pr_b]: the work of counting
+// N(u) & N(w) for each prefix that has survivors -- the shorter of the two
+// lists (minsel 0: u's, 1: w's); zero for prefixes without survivors.
+template <int L>
+__global__ __launch_bounds__(BLOCK) void k_pair_prep(GView g, KPlan P, const uint32_t* __restrict__ rows, uint64_t n,
+                                                    const uint32_t* __restrict__ nsurv, uint32_t* __restrict__ minlen,
+                                                    uint8_t* __restrict__ minsel) {
+  for (uint64_t p = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (uint64_t)gridDim.x * BLOCK) {
+    uint32_t len = 0, sel = 0;
+    if (nsurv[p]) {
+      const uint32_t u = rows[p * L + P.pr_a], w = rows[p * L + P.pr_b];
+      const uint32_t du = g.off_out[u + 1] - g.off_out[u], dw = g.off_out[w + 1] - g.off_out[w];
+      len = min(du, dw);
+      sel = du <= dw ? 0u : 1u;
+    }
+    minlen[p] = len;
+    minsel[p] = (uint8_t)sel;
+  }
+}
+
+// One work item per element x of a prefix's shorter list: when deg(x) meets
+// the threshold of A & B and x lies in the other list, x belongs to
+// N(u) & N(w) and every survivor of the prefix loses one completion, so the
+// count drops by the prefix's survivor number.  Tiles are fetched as in
+// k_expand (dynamic, XCD-aware, wave-uniform loop edges).
+template <int L>
+__global__ __launch_bounds__(BLOCK) void k_pair_common(GView g, KPlan P, ExpandArgs a) {
+  __shared__ uint64_t s_off[LDS_OFF + 1];
+  __shared__ uint64_t s_tile;
+  const uint32_t tid = threadIdx.x;
+  uint64_t acc = 0, bytes = 0;
+  const uint32_t nx = a.xcd ? 8u : 1u;
+  const uint32_t myx = a.xcd ? (blockIdx.x & 7u) : 0u;
+  uint32_t tries = 0;
+  while (true) {
+    if (uni(tid >> 6) == 0) {
+      uint64_t t = ~0ull;
+      while (tries < nx) {
+        const uint32_t xx = (myx + tries) % nx;
+        const uint64_t lo = a.ntiles * xx / nx, hi = a.ntiles * (xx + 1) / nx;
+        uint32_t got = 0;
+        if ((tid & 63) == 0) got = atomicAdd(&a.tilectr[xx], 1u);  // inside the wave-uniform branch
+        const uint64_t i = uni(got);
+        if (lo + i < hi) {
+          t = lo + i;
+          break;
+        }
+        ++tries;
+      }
+      s_tile = t;
+    }
+    __syncthreads();
+    const uint64_t tile = uni64(s_tile);
+    if (tile == ~0ull) break;
+    const uint64_t w0 = a.w_begin + tile * TILE;
+    const uint64_t w1 = min(w0 + (uint64_t)TILE, a.w_end);
+    const uint32_t plo = a.tile_p[tile], phi = a.tile_p[tile + 1];
+    const uint32_t span = phi - plo + 1;
+    const bool lds = span <= LDS_OFF;
+    if (lds)
+      for (uint32_t i = tid; i <= span; i += BLOCK) s_off[i] = a.work_off[plo + i];
+    __syncthreads();
+    for (int it = 0; it < IPT; ++it) {
+      const uint64_t w = w0 + (uint64_t)it * BLOCK + tid;
+      if (w >= w1) continue;
+      uint32_t lo = 0, hi = span + 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint64_t o = lds ? s_off[mid] : a.work_off[plo + mid];
+        if (o <= w) lo = mid + 1;
+        else hi = mid;
+      }
+      const uint32_t pi = lo - 1;
+      const uint64_t p = (uint64_t)plo + pi;
+      const uint64_t pbase = lds ? s_off[pi] : a.work_off[p];
+      const uint32_t idx = (uint32_t)(w - pbase);
+      const uint32_t u = a.rows[p * L + P.pr_a], wv = a.rows[p * L + P.pr_b];
+      const bool sel = a.minsel[p] != 0;
+      const uint32_t own = sel ? wv : u, oth = sel ? u : wv;
+      const uint32_t li = g.off_out[own] + idx;
+      const uint32_t x = g.adj_out[li];
+      if (a.stats) bytes += 12;
+      if (P.pr_tab > 1 && g.nsig_out[li].x < P.pr_tab) continue;
+      if (adj_has(g, oth, x, bytes)) acc -= a.nsurv[p];
+    }
+    __syncthreads();
+  }
+  add_wave(&a.counters[0], acc);
+  if (a.stats) add_wave(&a.counters[3], bytes);
 }
 
 // ---------------------------------------------------------- dispatch
@@ -776,6 +1000,25 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
     case 13: CALL(13); break;    \
     case 14: CALL(14); break;    \
     case 15: CALL(15); break;    \
+    default: throw Error(-3, "query position out of range"); \
+  }
+
+// the pair level is position k-3 <= 13
+#define CUTS_DISPATCH_PAIR(L, CALL) \
+  switch (L) {                      \
+    case 1: CALL(1); break;         \
+    case 2: CALL(2); break;         \
+    case 3: CALL(3); break;         \
+    case 4: CALL(4); break;         \
+    case 5: CALL(5); break;         \
+    case 6: CALL(6); break;         \
+    case 7: CALL(7); break;         \
+    case 8: CALL(8); break;         \
+    case 9: CALL(9); break;         \
+    case 10: CALL(10); break;       \
+    case 11: CALL(11); break;       \
+    case 12: CALL(12); break;       \
+    case 13: CALL(13); break;       \
     default: throw Error(-3, "query position out of range"); \
   }
 
@@ -807,6 +1050,8 @@ void ensure_rows(LevelBuf& b, uint64_t rows, uint32_t words) {
   dev_alloc(b.minsel, cap);
   dev_alloc(b.work_off, cap + 1);
   dev_alloc(b.cbase, cap);
+  dev_alloc(b.pbase, cap);
+  dev_alloc(b.nsurv, cap);
   b.cap = cap;
 }
 void ensure_rows_exact(LevelBuf& b, uint64_t rows, uint32_t words) {
@@ -817,6 +1062,8 @@ void ensure_rows_exact(LevelBuf& b, uint64_t rows, uint32_t words) {
   dev_alloc(b.minsel, std::max<uint64_t>(rows, 1));
   dev_alloc(b.work_off, std::max<uint64_t>(rows, 1) + 1);
   dev_alloc(b.cbase, std::max<uint64_t>(rows, 1));
+  dev_alloc(b.pbase, std::max<uint64_t>(rows, 1));
+  dev_alloc(b.nsurv, std::max<uint64_t>(rows, 1));
   b.cap = rows;
 }
 void ensure_bsum(LevelBuf& b, uint64_t nb) {
@@ -839,8 +1086,9 @@ struct Matcher {
   KPlan kp;
   GView g;
   uint32_t k;
-  bool closed_last = false, fuse = false, write = false;
+  bool closed_last = false, fuse = false, write = false, pair = false;
   const uint32_t* tbl = nullptr;
+  PairTables ptbl{};
   uint64_t budget = 0;
   uint64_t level_cap[kMaxQuery] = {0};
   uint32_t grid_cap = 0;
@@ -927,15 +1175,16 @@ struct Matcher {
     // plan + scan over the input rows; arrays live with the input level
     const bool use_cbase = fuse && L == k - 2 && kp.sc_pos < L && !(o.flags & 64u);
     int mode = M_WRITE;
-    if (L == k - 1 && !write) mode = M_COUNT;
+    if (pair && L == k - 3) mode = M_PAIR;
+    else if (L == k - 1 && !write) mode = M_COUNT;
     else if (L == k - 2 && fuse) mode = M_FUSED;
-    const int mclass = mode == M_WRITE ? 4 : (mode == M_COUNT ? 5 : 6);
+    const int mclass = mode == M_WRITE ? 4 : (mode == M_COUNT ? 5 : (mode == M_FUSED ? 6 : 8));
     {
       const uint32_t grid = grid_for(n, BLOCK, grid_cap);
       unsigned long long* merge = prof ? c.counters + 8 : nullptr;
       if (merge) HIPCHECK(hipMemsetAsync(merge, 0, 8, c.stream));
       pbegin(false);
-#define PL(LL) hipLaunchKernelGGL((k_plan<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.minlen, in.minsel, use_cbase ? in.cbase : nullptr, tbl, merge)
+#define PL(LL) hipLaunchKernelGGL((k_plan<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.minlen, in.minsel, use_cbase ? in.cbase : nullptr, tbl, mode == M_PAIR ? in.pbase : nullptr, ptbl, merge)
       CUTS_DISPATCH_L(L, PL)
 #undef PL
       HIPCHECK(hipGetLastError());
@@ -954,6 +1203,8 @@ struct Matcher {
       ensure_rows_exact(out, std::min<uint64_t>(chunk, W), L + 1);
       chunk = std::min<uint64_t>(chunk, out.cap);
     }
+    const bool pair_common = mode == M_PAIR && kp.pr_a != kp.pr_b;
+    if (pair_common) HIPCHECK(hipMemsetAsync(in.nsurv, 0, n * sizeof(uint32_t), c.stream));
     for (uint64_t w0 = 0; w0 < W; w0 += chunk) {
       if (past_deadline()) return;
       const uint64_t w1 = std::min(W, w0 + chunk);
@@ -974,6 +1225,8 @@ struct Matcher {
       a.counters = c.counters;
       a.tbl = tbl;
       a.cbase = use_cbase ? in.cbase : nullptr;
+      a.pbase = mode == M_PAIR ? in.pbase : nullptr;
+      a.nsurv = pair_common ? in.nsurv : nullptr;
       a.xcd = (o.flags & 8u) ? 0 : 1;
       a.tilectr = (unsigned int*)(c.counters + 4);  // 8 x uint32 in counters[4..7]
       HIPCHECK(hipMemsetAsync(a.tilectr, 0, 8 * sizeof(unsigned int), c.stream));
@@ -982,15 +1235,15 @@ struct Matcher {
       r.chunks++;
       if (mode == M_WRITE) {
         a.out_rows = out.rows;
-        a.out_count = (uint32_t*)(c.counters + 2);
-        HIPCHECK(hipMemsetAsync(c.counters + 2, 0, 8, c.stream));
+        a.out_count = (uint32_t*)(c.counters + 10);
+        HIPCHECK(hipMemsetAsync(c.counters + 10, 0, 8, c.stream));
         pbegin(true);
 #define EW(LL) hipLaunchKernelGGL((k_expand<LL, M_WRITE>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, a)
         CUTS_DISPATCH_L(L, EW)
 #undef EW
         HIPCHECK(hipGetLastError());
         pend(4, 0, true);
-        const uint64_t m = read_u32(c.counters + 2);
+        const uint64_t m = read_u32(c.counters + 10);
         r.level[L] += m;
         if (prof && L < k - 1) r.merge_bytes[4] += 16.0 * (double)m;  // trie nodes written and read back
         if (L == k - 1) {
@@ -1015,14 +1268,71 @@ struct Matcher {
 #undef EC
         HIPCHECK(hipGetLastError());
         pend(5, 0, true);
-      } else {
+      } else if (mode == M_FUSED) {
         pbegin(true);
 #define EF(LL) hipLaunchKernelGGL((k_expand<LL, M_FUSED>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, a)
         CUTS_DISPATCH_L(L, EF)
 #undef EF
         HIPCHECK(hipGetLastError());
         pend(6, 0, true);
+      } else {
+        pbegin(true);
+#define EP(LL) hipLaunchKernelGGL((k_expand<LL, M_PAIR>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, a)
+        CUTS_DISPATCH_PAIR(L, EP)
+#undef EP
+        HIPCHECK(hipGetLastError());
+        pend(8, 0, true);
       }
+    }
+    if (pair_common && !aborted) pair_intersections(L, rows, n, in);
+  }
+
+  // Second pass of the pair level with u != w: subtracts, for every prefix
+  // with survivors, survivors x |N(u) & N(w)| (degree-filtered).  The
+  // planning arrays of the input level are reused once the expansion is
+  // done.
+  void pair_intersections(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in) {
+    {
+      const uint32_t grid = grid_for(n, BLOCK, grid_cap);
+      pbegin(false);
+#define PP(LL) hipLaunchKernelGGL((k_pair_prep<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.nsurv, in.minlen, in.minsel)
+      CUTS_DISPATCH_PAIR(L, PP)
+#undef PP
+      HIPCHECK(hipGetLastError());
+      pend(1, (double)n * (4.0 * 2 + 16.0 + 5.0), false);
+    }
+    const uint64_t W = scan(in, n);
+    if (W == 0 || add_work(W)) return;
+    for (uint64_t w0 = 0; w0 < W; w0 += MAX_CHUNK) {
+      if (past_deadline()) return;
+      const uint64_t w1 = std::min(W, w0 + MAX_CHUNK);
+      const uint64_t nt = (w1 - w0 + TILE - 1) / TILE;
+      ensure_tiles(in, nt);
+      pbegin(false);
+      hipLaunchKernelGGL(k_tile_index, dim3(grid_for(nt + 1, BLOCK, 65535)), dim3(BLOCK), 0, c.stream, in.work_off,
+                         n, w0, w1, nt, in.tile_p);
+      pend(3, (double)(nt + 1) * (8.0 * (64 - __builtin_clzll(n + 2)) + 4.0), false);
+      ExpandArgs a{};
+      a.rows = rows;
+      a.work_off = in.work_off;
+      a.minsel = in.minsel;
+      a.tile_p = in.tile_p;
+      a.w_begin = w0;
+      a.w_end = w1;
+      a.ntiles = nt;
+      a.counters = c.counters;
+      a.nsurv = in.nsurv;
+      a.xcd = (o.flags & 8u) ? 0 : 1;
+      a.tilectr = (unsigned int*)(c.counters + 4);
+      HIPCHECK(hipMemsetAsync(a.tilectr, 0, 8 * sizeof(unsigned int), c.stream));
+      a.stats = prof ? 1 : 0;
+      r.chunks++;
+      pbegin(true);
+#define PC(LL) hipLaunchKernelGGL((k_pair_common<LL>), dim3(grid_for(nt, 1, grid_cap)), dim3(BLOCK), 0, c.stream, g, kp, a)
+      CUTS_DISPATCH_PAIR(L, PC)
+#undef PC
+      HIPCHECK(hipGetLastError());
+      pend(9, 0, true);
     }
   }
 
@@ -1036,6 +1346,7 @@ struct Matcher {
     prof = (o.flags & 16u) != 0;
     closed_last = !write && !(o.flags & 1u) && kp.sc_ok && k >= 2;
     fuse = closed_last && !(o.flags & 2u) && k >= 3;
+    pair = fuse && kp.pr_ok && !(o.flags & 128u);
     grid_cap = (uint32_t)c.cus * (o.blocks_per_cu ? o.blocks_per_cu : 8);
     budget = o.budget ? o.budget : c.default_budget;
     // split the budget evenly over the materialised levels 1..k-2 (or k-1
@@ -1049,6 +1360,11 @@ struct Matcher {
       level_cap[l] = std::max<uint64_t>(per / ((l + 1) * 4 + 13), 1024);
     }
     if (closed_last && kp.sc_thr > 1) tbl = cnt_table(c, dg, kp.sc_thr);
+    if (pair) {
+      ptbl.a = kp.pr_ta > 1 ? cnt_table(c, dg, kp.pr_ta) : nullptr;
+      ptbl.b = kp.pr_tb > 1 ? cnt_table(c, dg, kp.pr_tb) : nullptr;
+      ptbl.ab = kp.pr_tab > 1 ? cnt_table(c, dg, kp.pr_tab) : nullptr;
+    }
     // level 0
     const uint32_t rb = o.root_stride ? o.root_begin : 0;
     const uint32_t re = o.root_stride ? std::min(o.root_end, dg.V) : dg.V;
@@ -1089,21 +1405,23 @@ struct Matcher {
       }
       for (uint64_t s = 0; s < n0; s += ts) {
         // snapshot so that a trunk interrupted by the deadline can be undone
-        uint64_t snap[2] = {0, 0};
+        uint64_t snap[3] = {0, 0, 0};
         uint64_t snap_level[kMaxQuery];
         std::memcpy(snap_level, r.level, sizeof(snap_level));
         if (has_deadline && !write) {
           if (past_deadline()) break;
-          HIPCHECK(hipMemcpyAsync(c.host_pinned, c.counters, 16, hipMemcpyDeviceToHost, c.stream));
+          HIPCHECK(hipMemcpyAsync(c.host_pinned, c.counters, 24, hipMemcpyDeviceToHost, c.stream));
           HIPCHECK(hipStreamSynchronize(c.stream));
           snap[0] = c.host_pinned[0];
           snap[1] = c.host_pinned[1];
+          snap[2] = c.host_pinned[2];
         }
         level(1, l0.rows + s, std::min(ts, n0 - s));
         if (aborted) {
           c.host_pinned[0] = snap[0];
           c.host_pinned[1] = snap[1];
-          HIPCHECK(hipMemcpyAsync(c.counters, c.host_pinned, 16, hipMemcpyHostToDevice, c.stream));
+          c.host_pinned[2] = snap[2];
+          HIPCHECK(hipMemcpyAsync(c.counters, c.host_pinned, 24, hipMemcpyHostToDevice, c.stream));
           HIPCHECK(hipStreamSynchronize(c.stream));
           std::memcpy(r.level, snap_level, sizeof(snap_level));
           break;
@@ -1112,11 +1430,16 @@ struct Matcher {
       }
       if (aborted || r.roots_done < n0) r.complete = 0;
       if (!write) {
-        HIPCHECK(hipMemcpyAsync(c.host_pinned, c.counters, 16, hipMemcpyDeviceToHost, c.stream));
+        HIPCHECK(hipMemcpyAsync(c.host_pinned, c.counters, 24, hipMemcpyDeviceToHost, c.stream));
         HIPCHECK(hipStreamSynchronize(c.stream));
         r.count = c.host_pinned[0];
         r.level[k - 1] = r.count;
-        if (fuse) r.level[k - 2] = c.host_pinned[1];
+        if (pair) {
+          r.level[k - 3] = c.host_pinned[1];
+          r.level[k - 2] = c.host_pinned[2];
+        } else if (fuse) {
+          r.level[k - 2] = c.host_pinned[1];
+        }
       }
       if (r.work_capped) r.count = 0;
     }
@@ -1211,6 +1534,28 @@ KPlan make_kplan(const Plan& p, bool symmetric) {
     }
     kp.sc_adjmask = adj;
   }
+  // the last two positions together: a = k-2 and b = k-1 not adjacent, one
+  // backward neighbour each, both older than k-3 (so |A & B| depends on the
+  // rows of level k-4 only)
+  if (kp.sc_ok && p.k >= 4 && kp.nj[last - 1] == 1) {
+    const uint32_t pa = kp.jc[last - 1][0] & 31, pb = kp.sc_pos;
+    if (pb != last - 1 && pa + 3 < p.k && pb + 3 < p.k) {
+      kp.pr_ok = 1;
+      kp.pr_a = pa;
+      kp.pr_b = pb;
+      kp.pr_ta = std::max(p.sig_in[last - 1], p.sig_out[last - 1]);
+      kp.pr_tb = kp.sc_thr;
+      kp.pr_tab = std::max(kp.pr_ta, kp.pr_tb);
+      auto adjacent = [&](uint32_t i, uint32_t j) {
+        const uint32_t a = std::max(i, j), b = std::min(i, j);
+        return i != j && (((p.pmask[a] | p.cmask[a]) >> b) & 1);
+      };
+      for (uint32_t i = 0; i + 2 < p.k; ++i) {
+        if (adjacent(i, pa)) kp.pr_ma |= 1u << i;
+        if (adjacent(i, pb)) kp.pr_mb |= 1u << i;
+      }
+    }
+  }
   return kp;
 }
 
@@ -1235,7 +1580,8 @@ Context::~Context() {
   (void)hipSetDevice(device);
   for (auto& b : lv) {
     for (void* p : {(void*)b.rows, (void*)b.minlen, (void*)b.minsel, (void*)b.work_off, (void*)b.bsum,
-                    (void*)b.tile_p, (void*)b.cbase})
+                    (void*)b.tile_p, (void*)b.cbase, (void*)b.pbase,
+                    (void*)b.nsurv})
       if (p) (void)hipFree(p);
   }
   if (counters) (void)hipFree(counters);

@@ -91,7 +91,8 @@ enum {
   CUTS_FLAG_NO_XCD = 1u << 3,      /* plain blockIdx -> tile mapping */
   CUTS_FLAG_PROFILE = 1u << 4,     /* time every kernel class with HIP events */
   CUTS_FLAG_NO_BITMAP = 1u << 5,   /* lower-bound searches even in hub lists */
-  CUTS_FLAG_NO_PREFIX_BASE = 1u << 6 /* fused closed form fully per work item */
+  CUTS_FLAG_NO_PREFIX_BASE = 1u << 6, /* fused closed form fully per work item */
+  CUTS_FLAG_NO_PAIR = 1u << 7         /* never count the last two positions together */
 };
 
 /* Kernel classes reported by CUTS_FLAG_PROFILE. */
@@ -104,7 +105,9 @@ enum {
   CUTS_K_COUNT = 5,    /* expansion counting the last level */
   CUTS_K_FUSED = 6,    /* expansion + closed-form last level */
   CUTS_K_CLOSED = 7,   /* closed-form last level over stored rows */
-  CUTS_K_CLASSES = 8
+  CUTS_K_PAIR = 8,     /* expansion + the last two levels counted together */
+  CUTS_K_PAIR_COMMON = 9, /* |N(u) & N(w)| of the pair level's prefixes */
+  CUTS_K_CLASSES = 10
 };
 
 typedef struct {
@@ -140,9 +143,9 @@ typedef struct {
   uint32_t complete;        /* 0 if time_limit_ms stopped the run early */
   /* CUTS_FLAG_PROFILE: per kernel class, summed HIP-event time, launches and
    * algorithmic (demand) bytes as defined next to each kernel */
-  double kernel_ms[8];
-  uint64_t kernel_launches[8];
-  double kernel_bytes[8];
+  double kernel_ms[10];
+  uint64_t kernel_launches[10];
+  double kernel_bytes[10];
   uint64_t roots_done;      /* level-0 vertices whose subtrees are counted */
   uint32_t last_root;       /* data vertex of the last of them */
   uint32_t work_capped;     /* 1 if max_work stopped the run (count invalid) */
@@ -154,7 +157,7 @@ typedef struct {
    * levels below k-1 (written and read back), in the class that performs the
    * expansion.  The sum does not depend on kernels or modes: it equals the
    * CPU trie walk's figure for the same graph, query and order. */
-  double merge_bytes[8];
+  double merge_bytes[10];
 } cuts_match_result;
 
 void cuts_match_options_default(cuts_match_options *o);

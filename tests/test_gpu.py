@@ -3,8 +3,8 @@
 Exact, bit-for-bit equality of uint64 counts and of embedding sets:
 networkx-pinned cases, random graphs vs the backtracking oracle under every
 execution mode (closed-form last level on/off, fused last two levels on/off,
-XCD tile mapping on/off, manual trunks, tiny HBM budgets that force many
-chunks), per-level table sizes vs the CPU trie baseline, the full config-1
+last two positions counted together on/off, XCD tile mapping on/off, manual
+trunks, tiny HBM budgets that force many chunks), per-level table sizes vs the CPU trie baseline, the full config-1
 count, root-sampled config-2/3 counts, and the reference-compatible CLI.
 """
 import json
@@ -39,6 +39,7 @@ MODES = [
     dict(flags=cuts_amd.FLAG_NO_BITMAP),
     dict(flags=cuts_amd.FLAG_NO_PREFIX_BASE | cuts_amd.FLAG_NO_BITMAP),
     dict(trunks=3),
+    dict(flags=cuts_amd.FLAG_NO_PAIR),
     dict(budget=4096),  # a few hundred rows per level: many chunks
 ]
 

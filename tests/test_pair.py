@@ -1,0 +1,119 @@
+"""The pair level: the last two query positions counted together.
+
+When positions k-2 and k-1 are not adjacent in the query and each has one
+backward neighbour older than position k-3 (pendants, typically), a prefix
+of positions 0..k-3 has |A| * |B| - |A & B| completions, A and B being the
+degree-filtered neighbour lists of those backward neighbours minus the
+prefix; the expansion of position k-3 adds that number per survivor and
+levels k-2 and k-1 are never expanded.  Counts must equal the oracle's and
+the same run without the pair level; level sizes and merge-model bytes must
+equal the CPU trie walk's, which materialises every level.
+"""
+import random
+
+import pytest
+
+import cuts_amd
+import oracle
+from cuts_amd import workloads as W
+
+SHAPES = {
+    "star3": [(0, 1), (0, 2), (0, 3)],
+    "star4": [(0, 1), (0, 2), (0, 3), (0, 4)],
+    "triangle+2pend-same": [(0, 1), (1, 2), (2, 0), (0, 3), (0, 4)],
+    "triangle+2pend-two": [(0, 1), (1, 2), (2, 0), (0, 3), (1, 4)],
+    "4cycle+2pend": list(W.C4_2PEND.edges),
+    "diamond+2pend": [(0, 1), (0, 2), (1, 2), (1, 3), (2, 3), (0, 4), (3, 5)],
+    "k4+2pend": [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3), (1, 4), (2, 5)],
+    "5cycle+3pend": [(0, 1), (1, 2), (2, 3), (3, 4), (4, 0), (0, 5), (2, 6), (3, 7)],
+    # not eligible: the last two positions are adjacent, or the second
+    # pendant hangs on position k-3
+    "triangle+tail2": list(W.TRI_TAIL2.edges),
+    "k4e+tail2": list(W.K4E_TAIL2.edges),
+    "5cycle+chord+2tails": list(W.C5CHORD_2TAIL.edges),
+}
+
+
+def backward(p):
+    """Backward positions of every order position (undirected query)."""
+    po, pa = list(p.parents_offset), list(p.parents)
+    co, ch = list(p.children_offset), list(p.children)
+    return [set(pa[po[i]:po[i + 1]]) | set(ch[co[i]:co[i + 1]]) for i in range(p.k)]
+
+
+def pair_eligible(p):
+    k = p.k
+    if k < 4:
+        return False
+    b = backward(p)
+    if len(b[k - 1]) != 1 or len(b[k - 2]) != 1:
+        return False
+    (pa,), (pb,) = b[k - 2], b[k - 1]
+    return pb != k - 2 and pa <= k - 4 and pb <= k - 4
+
+
+def graph_for(seed):
+    rng = random.Random(seed)
+    n = rng.randint(50, 140)
+    return cuts_amd.Graph.chung_lu(n, rng.randint(2 * n, 4 * n), rng.uniform(2.2, 2.8), 0, seed)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", list(SHAPES))
+@pytest.mark.parametrize("seed", range(3))
+def test_pair_level_vs_oracle_and_cpu_trie(ctx, name, seed):
+    edges = SHAPES[name]
+    k = 1 + max(max(e) for e in edges)
+    g = graph_for(7000 + 13 * seed + k)
+    og = oracle.OGraph.from_view(g.view())
+    qa = oracle.qmatrix(k, oracle.undirected(edges))
+    want = oracle.count(og, qa)
+    p = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges))
+    cpu = oracle.trie_cpu(og, qa, p.order, nthreads=4)
+    assert cpu.count == want
+    dg = ctx.upload(g)
+    prof = cuts_amd.FLAG_PROFILE
+    for flags in (0, cuts_amd.FLAG_NO_BITMAP, cuts_amd.FLAG_NO_PAIR):
+        r = ctx.match(dg, p, flags=flags | prof)
+        assert r.count == want, (name, flags)
+        assert r.level_sizes == [cpu.level[i] for i in range(k)], (name, flags)
+        assert sum(r.merge_bytes.values()) == cpu.bytes, (name, flags, r.merge_bytes)
+        used = r.kernels["expand_pair"][1] > 0
+        expect = pair_eligible(p) and not flags & cuts_amd.FLAG_NO_PAIR
+        assert not used or expect, (name, flags, p.order)
+        assert used or not expect or r.level_sizes[k - 3] == 0, (name, flags, p.order)
+    for m in (dict(trunks=5), dict(budget=2048), dict(flags=cuts_amd.FLAG_NO_XCD)):
+        assert ctx.match(dg, p, **m).count == want, (name, m)
+
+
+def test_pair_shapes_cover_both_cases():
+    """The shape table holds eligible shapes with one and with two distinct
+    list owners, and ineligible ones (plans only, no GPU work)."""
+    kinds = set()
+    for name, edges in SHAPES.items():
+        k = 1 + max(max(e) for e in edges)
+        p = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges))
+        if pair_eligible(p):
+            b = backward(p)
+            kinds.add("same" if b[k - 1] == b[k - 2] else "two")
+        else:
+            kinds.add("no")
+    assert kinds == {"same", "two", "no"}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", range(12))
+def test_pair_level_random_queries(ctx, seed):
+    """Random-walk queries on random graphs: eligible or not, every mode
+    counts the oracle's number."""
+    rng = random.Random(9100 + seed)
+    g = graph_for(9100 + seed)
+    k = rng.randint(4, 7)
+    e, _ = g.walk_query(k, rng.randint(k - 1, k + 2), seed + 5)
+    edges = e.tolist()
+    og = oracle.OGraph.from_view(g.view())
+    want = oracle.count(og, oracle.qmatrix(k, oracle.undirected(edges)))
+    p = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges))
+    dg = ctx.upload(g)
+    for flags in (0, cuts_amd.FLAG_NO_PAIR, cuts_amd.FLAG_NO_BITMAP):
+        assert ctx.match(dg, p, flags=flags).count == want, (edges, flags)
