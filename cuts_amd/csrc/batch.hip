@@ -57,9 +57,12 @@ struct GV {
 };
 
 __device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+// readfirstlane returns int: both halves are taken as uint32 before they
+// are widened, or a low word >= 2^31 would be sign-extended into the high one
 __device__ __forceinline__ uint64_t uni64(uint64_t x) {
-  return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(x >> 32)) << 32) |
-         __builtin_amdgcn_readfirstlane((uint32_t)x);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)x);
+  return ((uint64_t)hi << 32) | lo;
 }
 __device__ __forceinline__ uint32_t wave_id() { return uni(threadIdx.x >> 6); }
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63; }
@@ -253,6 +256,26 @@ __device__ void stage_append(const BatchArgs& a, Shared<WROW>& s, uint32_t q, bo
   __syncthreads();
 }
 
+// Exclusive block scan of one length per row (thread tid < n holds row
+// tid's) into s.off[0..n]; returns the total.  Whole block.
+template <int WROW>
+__device__ uint64_t block_scan(Shared<WROW>& s, uint32_t n, uint64_t len) {
+  const uint32_t tid = threadIdx.x;
+  const uint64_t incl = wave_incl_scan(len);
+  if ((tid & 63) == 63) s.wwork[tid >> 6] = incl;
+  __syncthreads();
+  uint64_t before = 0, W = 0;
+  for (uint32_t w = 0; w < (uint32_t)NW; ++w) {
+    const uint64_t x = uni64(s.wwork[w]);
+    if (w < wave_id()) before += x;
+    W += x;
+  }
+  if (tid < n) s.off[tid] = before + incl - len;
+  s.off[n] = W;  // same value from every thread; no thread writes index n above
+  __syncthreads();
+  return W;
+}
+
 template <int WROW>
 __device__ void process(const BatchArgs& a, Shared<WROW>& s, uint32_t L) {
   const GV g{a.off_out, a.adj_out, a.off_in, a.adj_in, a.sig, a.V, a.hub_id, a.nsig_out, a.nsig_in, a.bits, a.words};
@@ -264,10 +287,11 @@ __device__ void process(const BatchArgs& a, Shared<WROW>& s, uint32_t L) {
   const KPlan& P = a.plans[q];
   const uint32_t k = uni(P.k);
   const uint32_t tid = threadIdx.x;
-  const uint32_t* tbl = a.tbls[q];
+  const uint32_t* tbl = a.tbls[3 * q];
   const bool closed = (a.flags & 1u) == 0 && uni(P.sc_ok);
   const bool fuse = closed && (a.flags & 2u) == 0 && k >= 3;
-  uint64_t cnt = 0, surv = 0;
+  const bool pair = fuse && uni(P.pr_ok) && (a.flags & 8u) == 0;
+  uint64_t cnt = 0, surv = 0, sna = 0;
 
   if (L == 0) {
     // level-0 generation over roots off .. off+n-1 of the query's root sequence
@@ -300,14 +324,47 @@ __device__ void process(const BatchArgs& a, Shared<WROW>& s, uint32_t L) {
 
   // shortest constraint list per row (one row per thread), then a block scan
   const uint32_t nj = uni(P.nj[L]);
-  const int mode = (L == k - 1) ? 1 : ((L == k - 2 && fuse) ? 2 : 0);
+  const int mode = (L == k - 1) ? 1 : ((pair && L == k - 3) ? 3 : ((L == k - 2 && fuse) ? 2 : 0));
   // fused level with an older closed-form vertex u: the closed form splits
   // into a per-prefix base, cnt(u) - |{i < L : r_i in N(u), deg(r_i) >= t}|,
   // and one term for the candidate itself
   const uint32_t sc_pos = uni(P.sc_pos), sc_thr = uni(P.sc_thr);
   const bool use_cbase = mode == 2 && sc_pos < L && (a.flags & 4u) == 0;
+  // pair level (L = k-3): a survivor v completes the prefix and has
+  // |A| * |B| - |A & B| completions (A = N(row[pr_a]), B = N(row[pr_b]) minus
+  // the prefix, degree-filtered).  Per row: the prefix parts of |A|, |B| and
+  // of |A & B| (signed: for pr_a != pr_b the size of N(u) & N(w) is
+  // subtracted afterwards, per survivor, by a second pass over the rows), and
+  // the row's survivor count.  The stage is empty on this level (nothing is
+  // flushed), so these live there.
+  const uint32_t pa = uni(P.pr_a), pb = uni(P.pr_b);
+  uint32_t* const pxa = s.stage;
+  uint32_t* const pxb = s.stage + RMAX;
+  uint32_t* const pxz = s.stage + 2 * RMAX;
+  uint32_t* const pns = s.stage + 3 * RMAX;
   uint64_t len = 0;
   if (tid < n) {
+    if (mode == 3) {
+      const uint32_t* row = &s.rows[tid * WROW];
+      const uint32_t u = row[pa], w = row[pb];
+      const uint32_t* ta_tbl = a.tbls[3 * q + 1];
+      const uint32_t* tab_tbl = a.tbls[3 * q + 2];
+      int64_t ca = ta_tbl ? (int64_t)ta_tbl[u] : (int64_t)(g.off_out[u + 1] - g.off_out[u]);
+      int64_t cb = tbl ? (int64_t)tbl[w] : (int64_t)(g.off_out[w + 1] - g.off_out[w]);
+      int64_t cab = pa != pb ? 0 : (tab_tbl ? (int64_t)tab_tbl[u] : (int64_t)(g.off_out[u + 1] - g.off_out[u]));
+      for (uint32_t l = 0; l < L; ++l) {
+        const bool ina = ((P.pr_ma >> l) & 1) || has_arc(g, u, row[l]);
+        const bool inb = ((P.pr_mb >> l) & 1) || has_arc(g, w, row[l]);
+        const uint32_t d = g.sig[2 * (uint64_t)row[l]];
+        ca -= ina && d >= P.pr_ta;
+        cb -= inb && d >= P.pr_tb;
+        cab -= ina && inb && d >= P.pr_tab;
+      }
+      pxa[tid] = (uint32_t)ca;
+      pxb[tid] = (uint32_t)cb;
+      pxz[tid] = (uint32_t)(int32_t)cab;
+      pns[tid] = 0;
+    }
     if (use_cbase) {
       const uint32_t* row = &s.rows[tid * WROW];
       const uint32_t u = row[sc_pos];
@@ -332,18 +389,7 @@ __device__ void process(const BatchArgs& a, Shared<WROW>& s, uint32_t L) {
     s.minsel[tid] = (uint8_t)sel;
     len = best;
   }
-  const uint64_t incl = wave_incl_scan(len);
-  if ((tid & 63) == 63) s.wwork[tid >> 6] = incl;
-  __syncthreads();
-  uint64_t before = 0, W = 0;
-  for (uint32_t w = 0; w < (uint32_t)NW; ++w) {
-    const uint64_t x = uni64(s.wwork[w]);
-    if (w < wave_id()) before += x;
-    W += x;
-  }
-  if (tid < n) s.off[tid] = before + incl - len;
-  s.off[n] = W;  // same value from every thread; no thread writes index n above
-  __syncthreads();
+  const uint64_t W = block_scan(s, n, len);
 
   const uint32_t tin = P.sig_in[L], tout = P.sig_out[L];
   for (uint64_t w0 = 0; w0 < W; w0 += BB) {
@@ -383,6 +429,16 @@ __device__ void process(const BatchArgs& a, Shared<WROW>& s, uint32_t L) {
           ok = bmember(adj2, off2[u2], off2[u2 + 1], v);
         }
       }
+      if (mode == 3 && ok) {
+        const bool ina = ((P.pr_ma >> L) & 1) || has_arc(g, row[pa], v);
+        const bool inb = ((P.pr_mb >> L) & 1) || has_arc(g, row[pb], v);
+        const uint64_t na = pxa[r] - (uint32_t)(ina && sgv.x >= P.pr_ta);
+        const uint64_t nb = pxb[r] - (uint32_t)(inb && sgv.x >= P.pr_tb);
+        const int64_t nab = (int64_t)(int32_t)pxz[r] - (int64_t)(ina && inb && sgv.x >= P.pr_tab);
+        fc = na * nb - (uint64_t)nab;  // modulo-2^64 sums are exact
+        sna += na;
+        if (pa != pb) atomicAdd(&pns[r], 1u);
+      }
       if (mode == 2 && ok) {
         if (use_cbase) {
           // v is certainly in N(u) when it was drawn from u's own list
@@ -397,7 +453,7 @@ __device__ void process(const BatchArgs& a, Shared<WROW>& s, uint32_t L) {
     const uint32_t* row = &s.rows[r * WROW];
     if (mode == 1) {
       cnt += ok;
-    } else if (mode == 2) {
+    } else if (mode >= 2) {
       if (ok) {
         ++surv;
         cnt += fc;
@@ -408,7 +464,44 @@ __device__ void process(const BatchArgs& a, Shared<WROW>& s, uint32_t L) {
     }
   }
   if (mode == 0) flush_stage(a, s, q, L + 1);
-  if (mode == 1) {
+  if (mode == 3 && pa != pb) {
+    // second pass: every x in N(u) & N(w) (degree-filtered) takes one
+    // completion from each survivor of its row; per row with survivors the
+    // shorter of the two lists is walked and x looked up in the other
+    __syncthreads();
+    uint64_t len2 = 0;
+    if (tid < n && pns[tid]) {
+      const uint32_t u = s.rows[tid * WROW + pa], w = s.rows[tid * WROW + pb];
+      const uint32_t du = g.off_out[u + 1] - g.off_out[u], dw = g.off_out[w + 1] - g.off_out[w];
+      len2 = min(du, dw);
+      s.minsel[tid] = du <= dw ? 0 : 1;
+    }
+    const uint64_t W2 = block_scan(s, n, len2);
+    for (uint64_t w0 = 0; w0 < W2; w0 += BB) {
+      const uint64_t w = w0 + tid;
+      if (w < W2) {
+        uint32_t lo = 0, hi = n + 1;
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (s.off[mid] <= w) lo = mid + 1;
+          else hi = mid;
+        }
+        const uint32_t r = lo - 1;
+        const uint32_t idx = (uint32_t)(w - s.off[r]);
+        const uint32_t u = s.rows[r * WROW + pa], wv = s.rows[r * WROW + pb];
+        const bool sel = s.minsel[r] != 0;
+        const uint32_t li = g.off_out[sel ? wv : u] + idx;
+        const uint32_t x = g.adj_out[li];
+        if ((P.pr_tab <= 1 || g.nsig_out[li].x >= P.pr_tab) && has_arc(g, sel ? u : wv, x)) cnt -= pns[r];
+      }
+    }
+  }
+  if (mode == 3) {
+    add_block(&a.counts[q], cnt);
+    add_block(&a.levels[q * kMaxQuery + L], surv);
+    add_block(&a.levels[q * kMaxQuery + L + 1], sna);
+    add_block(&a.levels[q * kMaxQuery + L + 2], cnt);
+  } else if (mode == 1) {
     add_block(&a.counts[q], cnt);
     add_block(&a.levels[q * kMaxQuery + L], cnt);
   } else if (mode == 2) {
@@ -568,14 +661,18 @@ void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& p
   const uint64_t half_words = (o.arena_bytes ? o.arena_bytes : (8ull << 30)) / 8;
   if (half_words < 16) throw Error(-1, "batch: arena too small");
   std::vector<KPlan> kp(b.nq);
-  std::vector<const uint32_t*> tb(b.nq, nullptr);
+  std::vector<const uint32_t*> tb(3 * (size_t)b.nq, nullptr);
   b.kmax = 0;
   for (uint32_t q = 0; q < b.nq; ++q) {
     if (plans[q]->k == 0 || plans[q]->k > kMaxQuery) throw Error(-3, "batch: query size out of range");
     kp[q] = make_kplan(*plans[q], dg.symmetric);
     b.k.push_back(plans[q]->k);
     b.kmax = std::max(b.kmax, plans[q]->k);
-    if (kp[q].sc_ok && kp[q].sc_thr > 1 && !(o.flags & 1u)) tb[q] = cnt_table(c, dg, kp[q].sc_thr);
+    if (kp[q].sc_ok && kp[q].sc_thr > 1 && !(o.flags & 1u)) tb[3 * q] = cnt_table(c, dg, kp[q].sc_thr);
+    if (kp[q].pr_ok && !(o.flags & (1u | 2u | 128u))) {
+      if (kp[q].pr_ta > 1) tb[3 * q + 1] = cnt_table(c, dg, kp[q].pr_ta);
+      if (kp[q].pr_tab > 1) tb[3 * q + 2] = cnt_table(c, dg, kp[q].pr_tab);
+    }
   }
   BatchArgs& a = b.args;
   a.off_out = dg.off_out;
@@ -592,8 +689,8 @@ void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& p
   KPlan* dplans = balloc<KPlan>(b, 0, b.nq);
   HIPCHECK(hipMemcpy(dplans, kp.data(), b.nq * sizeof(KPlan), hipMemcpyHostToDevice));
   a.plans = dplans;
-  const uint32_t** dtb = balloc<const uint32_t*>(b, 1, b.nq);
-  HIPCHECK(hipMemcpy(dtb, tb.data(), b.nq * sizeof(void*), hipMemcpyHostToDevice));
+  const uint32_t** dtb = balloc<const uint32_t*>(b, 1, tb.size());
+  HIPCHECK(hipMemcpy(dtb, tb.data(), tb.size() * sizeof(void*), hipMemcpyHostToDevice));
   a.tbls = dtb;
   a.st = balloc<BState>(b, 2, 1);
   HIPCHECK(hipMemset(a.st, 0, sizeof(BState)));
@@ -619,7 +716,7 @@ void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& p
   if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c.device) == hipSuccess && khz > 0)
     b.tick_khz = khz;
   a.item_work = o.item_work ? o.item_work : 16384;
-  a.flags = (o.flags & 3u) | ((o.flags & 64u) ? 4u : 0u);
+  a.flags = (o.flags & 3u) | ((o.flags & 64u) ? 4u : 0u) | ((o.flags & 128u) ? 8u : 0u);
   a.nq = b.nq;
 }
 

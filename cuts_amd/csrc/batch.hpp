@@ -33,7 +33,8 @@ struct BatchArgs {
   const uint64_t* bits;
   uint32_t words;
   const KPlan* plans;
-  const uint32_t* const* tbls;  // per query: closed-form table or null
+  const uint32_t* const* tbls;  // per query: 3 degree-threshold tables or null -- the closed form's
+                                // (= B's of the pair level), A's and A & B's
   BState* st;
   Item* items;                  // 2 halves of qcap items; level L uses half L & 1
   uint64_t qcap;
@@ -45,7 +46,7 @@ struct BatchArgs {
   const Item* init_items;       // level-0 items, copied into the queue by every replay
   uint64_t n_init;
   uint64_t item_work;           // candidate work at which a staged item is flushed
-  uint32_t flags;               // 1: no closed form, 2: no fusion, 4: no per-prefix base
+  uint32_t flags;               // 1: no closed form, 2: no fusion, 4: no per-prefix base, 8: no pair level
   uint32_t nq;                  // queries in the batch
 };
 

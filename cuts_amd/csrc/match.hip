@@ -68,9 +68,12 @@ GView view_of(const DeviceGraph& dg) {
 }
 
 __device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+// readfirstlane returns int: both halves are taken as uint32 before they
+// are widened, or a low word >= 2^31 would be sign-extended into the high one
 __device__ __forceinline__ uint64_t uni64(uint64_t x) {
-  return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(x >> 32)) << 32) |
-         __builtin_amdgcn_readfirstlane((uint32_t)x);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)x);
+  return ((uint64_t)hi << 32) | lo;
 }
 
 __device__ __forceinline__ uint32_t lane_rank(uint64_t m) {

@@ -52,7 +52,8 @@ def test_level_batch_equals_single_and_oracle(ctx, seed):
         assert ctx.match(dg, plans[i]).count == want, i
     singles = [ctx.match(dg, p) for p in plans]
     for flags, kw in ((0, {}), (cuts_amd.FLAG_NO_FUSE, {}), (cuts_amd.FLAG_NO_SHORTCUT, {}),
-                      (cuts_amd.FLAG_NO_BITMAP, {}), (0, dict(item_work=64, blocks_per_cu=1)),
+                      (cuts_amd.FLAG_NO_BITMAP, {}), (cuts_amd.FLAG_NO_PAIR, {}),
+                      (0, dict(item_work=64, blocks_per_cu=1)),
                       (0, dict(vchunk=7, item_work=1))):
         print(f"batch seed={seed} flags={flags} {kw}", flush=True)
         b = cuts_amd.Batch(ctx, dg, plans, flags=flags, vchunk=kw.pop("vchunk", 64), **kw)
@@ -60,6 +61,49 @@ def test_level_batch_equals_single_and_oracle(ctx, seed):
         for i in range(len(qs)):
             assert r.counts[i] == singles[i].count, (flags, i)
             assert r.level_sizes[i] == singles[i].level_sizes, (flags, i)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_level_batch_pair_level_shapes(ctx, seed):
+    """Shapes whose last two positions are counted together (and some whose
+    are not) in one batch: per-query counts and level sizes equal the
+    single-query matcher's with and without the pair level, and the
+    oracle's."""
+    from test_pair import SHAPES
+    g = cuts_amd.Graph.chung_lu(120, 420, 2.4, 0, 300 + seed)
+    dg = ctx.upload(g)
+    og = oracle.OGraph.from_view(g.view())
+    qs = [(1 + max(max(e) for e in edges), edges) for edges in SHAPES.values()]
+    plans = [cuts_amd.Plan(cuts_amd.Graph.from_edges(k, e)) for k, e in qs]
+    singles = [ctx.match(dg, p) for p in plans]
+    for (k, e), x in zip(qs, singles):
+        assert x.count == oracle.count(og, oracle.qmatrix(k, oracle.undirected(e))), e
+        assert ctx.match(dg, cuts_amd.Plan(cuts_amd.Graph.from_edges(k, e)), flags=cuts_amd.FLAG_NO_PAIR).level_sizes \
+            == x.level_sizes, e
+    for flags, kw in ((0, {}), (cuts_amd.FLAG_NO_PAIR, {}), (cuts_amd.FLAG_NO_BITMAP, {}),
+                      (0, dict(item_work=32, vchunk=5))):
+        print(f"batch seed={seed} flags={flags} {kw}", flush=True)
+        r = cuts_amd.Batch(ctx, dg, plans, flags=flags, **kw).run()
+        assert r.counts == [x.count for x in singles], flags
+        assert r.level_sizes == [x.level_sizes for x in singles], flags
+
+
+def test_level_batch_wave_sums_beyond_32_bits(ctx):
+    """Per-wave partial counts far above 2^31 (a 4-vertex star plus tail
+    on a 4,096-vertex power-law graph): a 64-bit wave broadcast that
+    sign-extended its low word once lost multiples of 2^32 here."""
+    g = cuts_amd.Graph.chung_lu(4096, 32768, 2.3, 0, 5)
+    dg = ctx.upload(g)
+    edges = [(0, 1), (1, 2), (2, 3), (2, 4)]
+    plans = [cuts_amd.Plan(cuts_amd.Graph.from_edges(5, edges), o) for o in ([2, 1, 0, 3, 4], [2, 3, 4, 1, 0])]
+    plans.append(cuts_amd.Plan(cuts_amd.Graph.from_edges(4, [(0, 1), (0, 2), (0, 3)])))
+    singles = [ctx.match(dg, p) for p in plans]
+    assert singles[0].count == singles[1].count > 1 << 36
+    for flags in (0, cuts_amd.FLAG_NO_PAIR, cuts_amd.FLAG_NO_PAIR | cuts_amd.FLAG_NO_PREFIX_BASE):
+        print(f"batch flags={flags}", flush=True)
+        r = cuts_amd.Batch(ctx, dg, plans, flags=flags, arena_bytes=4 << 30).run()
+        assert r.counts == [x.count for x in singles], flags
+        assert r.level_sizes == [x.level_sizes for x in singles], flags
 
 
 def test_level_batch_mixed_sizes_and_one_vertex_queries(ctx):

@@ -106,3 +106,18 @@ def test_expand_tile_loops_are_wave_uniform(tmp_path):
     for L in (1, 3):
         name = f"_ZN4cuts12_GLOBAL__N_113k_pair_commonILi{L}EEEvNS0_5GViewENS_5KPlanENS0_10ExpandArgsE"
         check_block_loop(function_body(asm, name), f"k_pair_common<{L}>")
+
+
+@pytest.mark.parametrize("src", ["batch.hip", "match.hip"])
+def test_no_sign_extended_wave_broadcasts(tmp_path, src):
+    """`__builtin_amdgcn_readfirstlane` returns int; a 64-bit broadcast that
+    widens its low half without a uint32 cast sign-extends it into the high
+    word (seen as per-wave count sums off by multiples of 2^32).  In the
+    compiler's IR that is a 32-bit shift left followed by an exact
+    arithmetic shift right of an i64; no kernel may contain one."""
+    out = tmp_path / (src + ".ll")
+    subprocess.run([HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+                    "-packed-fp32-ops", "--cuda-device-only", "-S", "-emit-llvm", src, "-o", str(out)],
+                   check=True, capture_output=True, cwd=CSRC)
+    ir = out.read_text()
+    assert re.search(r"ashr exact i64 %\w+, 32", ir) is None
