@@ -30,9 +30,10 @@ FLAG_PROFILE = 16
 FLAG_NO_BITMAP = 32
 FLAG_NO_PREFIX_BASE = 64
 FLAG_NO_PAIR = 128
+FLAG_NO_CHAIN = 256
 MAX_QUERY = 16
 KERNEL_CLASSES = ["level0", "plan", "scan", "tile_index", "expand_write", "expand_count", "expand_fused",
-                  "closed_form", "expand_pair", "pair_common"]
+                  "closed_form", "last_two", "common"]
 _NCLASS = len(KERNEL_CLASSES)
 
 _u32p = C.POINTER(C.c_uint32)

@@ -89,5 +89,7 @@ void batch_run(Batch& b, std::vector<BatchQueryResult>& out, double& device_ms, 
 
 // Degree-threshold count table of the closed form (built once per threshold).
 const uint32_t* cnt_table(Context& c, DeviceGraph& dg, uint32_t thr);
+// Chain-level vertex tables for thresholds (ta, tb), 2V uint64 (built once).
+const uint64_t* chain_table(Context& c, DeviceGraph& dg, uint32_t ta, uint32_t tb);
 
 }  // namespace cuts

@@ -45,6 +45,19 @@ struct KPlan {
   uint32_t pr_a, pr_b;
   uint32_t pr_ta, pr_tb, pr_tab;  // degree thresholds of A, B and A & B
   uint32_t pr_ma, pr_mb;          // positions < k-2 certainly inside N(row[pr_a]) / N(row[pr_b])
+  // last two positions as a chain (symmetric graphs): k-1's one backward
+  // neighbour is k-2, whose one backward neighbour ch_u is older than k-3.
+  // A prefix P of positions 0..k-3 with u = row[ch_u] and
+  // A = {x in N(u) : deg(x) >= ch_ta} - P has
+  //   sum_{x in A} |{y in N(x) : deg(y) >= ch_tb} - P - {x}|
+  // completions, which take the vertex tables H(u) = sum over N(u) of the
+  // degree-filtered neighbour counts and one intersection N(u) & N(r_i) per
+  // other prefix vertex -- those of positions < k-3 shared by all rows with
+  // the same parent prefix
+  uint32_t ch_ok;
+  uint32_t ch_u;
+  uint32_t ch_ta, ch_tb;
+  uint32_t qadj[kMaxQuery];       // bit j of qadj[i]: positions i and j adjacent in the query
 };
 
 KPlan make_kplan(const Plan& p, bool symmetric_data);
@@ -62,6 +75,9 @@ struct DeviceGraph {
   // read at its list index, coalesced with the list word itself
   uint2 *nsig_out = nullptr, *nsig_in = nullptr;
   std::map<uint32_t, uint32_t*> cnt_tables; // degree threshold -> #neighbours meeting it
+  // (ta, tb) -> 2V uint64: per vertex u, sum over x in N(u) with deg(x) >= ta
+  // of |{y in N(x) : deg(y) >= tb}|, then of deg(x)
+  std::map<std::pair<uint32_t, uint32_t>, uint64_t*> chain_tables;
   // adjacency bitmaps of hub vertices (symmetric graphs): hub_id[v] = row of
   // v in `bits` or -1; a row is `words` uint64 covering all V vertices
   int32_t* hub_id = nullptr;
@@ -80,7 +96,9 @@ struct LevelBuf {
   uint64_t* work_off = nullptr; // cap+1
   uint32_t* cbase = nullptr;    // cap: per-prefix closed-form base (fused levels)
   uint4* pbase = nullptr;       // cap: per-prefix |A|, |B|, |A & B| bases (pair level)
-  uint32_t* nsurv = nullptr;    // cap: survivors per prefix (pair level)
+  uint32_t* nsurv = nullptr;    // cap: per prefix, survivors (pair level) or the shared
+                                // intersections of its children (chain level)
+  uint32_t* parent = nullptr;   // cap: parent prefix of every row (rows of the chain level)
   uint64_t* bsum = nullptr;     // scan partials
   uint64_t bsum_cap = 0;
   uint32_t* tile_p = nullptr;   // tiles+1

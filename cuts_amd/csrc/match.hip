@@ -462,6 +462,7 @@ struct ExpandArgs {
   const uint32_t* cbase;     // M_FUSED with an older closed-form vertex: per-prefix part
   const uint4* pbase;        // M_PAIR: per-prefix parts of |A|, |B|, |A & B|
   uint32_t* nsurv;           // M_PAIR with u != w: survivors per prefix (zero at launch)
+  uint32_t* out_parent;      // M_WRITE: parent prefix of every row written (or null)
   uint32_t xcd;              // 1: XCD-contiguous tile mapping
   unsigned int* tilectr;     // 8 tile counters, zero at launch
   uint32_t stats;            // 1: accumulate demand bytes into counters[3]
@@ -762,7 +763,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
     for (int it = 0; it < IPT; ++it) {
       const uint64_t w = w0 + (uint64_t)it * BLOCK + tid;
       bool ok = false;
-      uint32_t v = 0;
+      uint32_t v = 0, par = 0;
       uint64_t fused_contrib = 0;
       uint32_t row[L];
 #pragma unroll
@@ -784,6 +785,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
         }
         const uint32_t pi = lo - 1;
         const uint64_t p = (uint64_t)plo + pi;
+        par = (uint32_t)p;
         const uint64_t pbase = lds ? s_off[pi] : a.work_off[p];
         const uint32_t idx = (uint32_t)(w - pbase);
 #pragma unroll
@@ -848,10 +850,12 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
         __syncthreads();
         if (ok) {
           bytes += 4 * (L + 1);
-          uint32_t* dst = a.out_rows + (uint64_t)(s_base + s_pre[wave] + lane_rank(m)) * (L + 1);
+          const uint64_t o = (uint64_t)s_base + s_pre[wave] + lane_rank(m);
+          uint32_t* dst = a.out_rows + o * (L + 1);
 #pragma unroll
           for (int l = 0; l < L; ++l) dst[l] = row[l];
           dst[L] = v;
+          if (a.out_parent) a.out_parent[o] = par;
         }
       } else if (MODE == M_COUNT) {
         acc += ok;
@@ -893,10 +897,18 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
   }
 }
 
-// ------------------------------------------ pair level: |N(u) & N(w)|
-// For the pair level with u = r[pr_a] != w = r[pr_b]: the work of counting
-// N(u) & N(w) for each prefix that has survivors -- the shorter of the two
-// lists (minsel 0: u's, 1: w's); zero for prefixes without survivors.
+// ------------------------------------- intersections |N(u) & N(w)|
+// Levels that count the last two positions together need, per row, the
+// size of N(u) & N(w) for two of the row's vertices u = r[pa], w = r[pb]
+// (degree-filtered).  minlen/minsel give each row's work: the shorter of
+// the two lists (minsel 0: u's, 1: w's), or 0 for rows that need none.
+struct CommonArgs {
+  uint32_t pa, pb;  // row positions of u and w
+  uint32_t thr;     // degree threshold of a common neighbour
+  uint32_t mode;    // 0: count -= weight[p] per hit; 1: count -= 1 per hit; 2: weight[p] += hits
+};
+
+// Pair level: the work of the rows (prefixes) that have survivors.
 template <int L>
 __global__ __launch_bounds__(BLOCK) void k_pair_prep(GView g, KPlan P, const uint32_t* __restrict__ rows, uint64_t n,
                                                     const uint32_t* __restrict__ nsurv, uint32_t* __restrict__ minlen,
@@ -914,13 +926,29 @@ __global__ __launch_bounds__(BLOCK) void k_pair_prep(GView g, KPlan P, const uin
   }
 }
 
-// One work item per element x of a prefix's shorter list: when deg(x) meets
-// the threshold of A & B and x lies in the other list, x belongs to
-// N(u) & N(w) and every survivor of the prefix loses one completion, so the
-// count drops by the prefix's survivor number.  Tiles are fetched as in
-// k_expand (dynamic, XCD-aware, wave-uniform loop edges).
+// Chain level, shared part: the work of N(u) & N(r_i) for one older
+// position i of every parent prefix (rows of L words, u = r[ch_u]); zero
+// when deg(r_i) is below the last position's threshold (such r_i take no
+// completions away).
 template <int L>
-__global__ __launch_bounds__(BLOCK) void k_pair_common(GView g, KPlan P, ExpandArgs a) {
+__global__ __launch_bounds__(BLOCK) void k_chain_prep(GView g, KPlan P, const uint32_t* __restrict__ rows, uint64_t n,
+                                                     uint32_t i, uint32_t* __restrict__ minlen,
+                                                     uint8_t* __restrict__ minsel) {
+  for (uint64_t p = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (uint64_t)gridDim.x * BLOCK) {
+    const uint32_t u = rows[p * L + P.ch_u], w = rows[p * L + i];
+    const uint32_t du = g.off_out[u + 1] - g.off_out[u], dw = g.off_out[w + 1] - g.off_out[w];
+    const bool need = P.ch_tb <= 1 || g.sig[2 * (uint64_t)w] >= P.ch_tb;
+    minlen[p] = need ? min(du, dw) : 0u;
+    minsel[p] = (uint8_t)(du <= dw ? 0u : 1u);
+  }
+}
+
+// One work item per element x of a row's shorter list: x is a common
+// neighbour when deg(x) meets the threshold and x lies in the other list.
+// Tiles are fetched as in k_expand (dynamic, XCD-aware, wave-uniform loop
+// edges).
+template <int L>
+__global__ __launch_bounds__(BLOCK) void k_common(GView g, KPlan P, ExpandArgs a, CommonArgs c) {
   __shared__ uint64_t s_off[LDS_OFF + 1];
   __shared__ uint64_t s_tile;
   const uint32_t tid = threadIdx.x;
@@ -958,31 +986,146 @@ __global__ __launch_bounds__(BLOCK) void k_pair_common(GView g, KPlan P, ExpandA
     __syncthreads();
     for (int it = 0; it < IPT; ++it) {
       const uint64_t w = w0 + (uint64_t)it * BLOCK + tid;
-      if (w >= w1) continue;
+      const bool in = w < w1;
       uint32_t lo = 0, hi = span + 1;
       while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
         const uint64_t o = lds ? s_off[mid] : a.work_off[plo + mid];
-        if (o <= w) lo = mid + 1;
+        if (o <= (in ? w : w1 - 1)) lo = mid + 1;
         else hi = mid;
       }
       const uint32_t pi = lo - 1;
       const uint64_t p = (uint64_t)plo + pi;
-      const uint64_t pbase = lds ? s_off[pi] : a.work_off[p];
-      const uint32_t idx = (uint32_t)(w - pbase);
-      const uint32_t u = a.rows[p * L + P.pr_a], wv = a.rows[p * L + P.pr_b];
-      const bool sel = a.minsel[p] != 0;
-      const uint32_t own = sel ? wv : u, oth = sel ? u : wv;
-      const uint32_t li = g.off_out[own] + idx;
-      const uint32_t x = g.adj_out[li];
-      if (a.stats) bytes += 12;
-      if (P.pr_tab > 1 && g.nsig_out[li].x < P.pr_tab) continue;
-      if (adj_has(g, oth, x, bytes)) acc -= a.nsurv[p];
+      bool hit = false;
+      if (in) {
+        const uint64_t pbase = lds ? s_off[pi] : a.work_off[p];
+        const uint32_t idx = (uint32_t)(w - pbase);
+        const uint32_t u = a.rows[p * L + c.pa], wv = a.rows[p * L + c.pb];
+        const bool sel = a.minsel[p] != 0;
+        const uint32_t li = g.off_out[sel ? wv : u] + idx;
+        const uint32_t x = g.adj_out[li];
+        if (a.stats) bytes += 12;
+        hit = (c.thr <= 1 || g.nsig_out[li].x >= c.thr) && adj_has(g, sel ? u : wv, x, bytes);
+      }
+      if (c.mode == 2) {
+        add_runs(a.nsurv, p, hit);  // all lanes; items past w1 never hit
+      } else if (hit) {
+        acc -= c.mode == 0 ? (uint64_t)a.nsurv[p] : 1ull;
+      }
     }
     __syncthreads();
   }
-  add_wave(&a.counters[0], acc);
+  if (c.mode != 2) add_wave(&a.counters[0], acc);
   if (a.stats) add_wave(&a.counters[3], bytes);
+}
+
+// Chain tables for thresholds (ta, tb): H[u] = sum over x in N(u) with
+// deg(x) >= ta of |{y in N(x) : deg(y) >= tb}|, H[V + u] = sum of those
+// deg(x).  One wave per vertex.
+__global__ __launch_bounds__(BLOCK) void k_chain_table(GView g, uint32_t ta, const uint32_t* __restrict__ tb_tbl,
+                                                      uint64_t* __restrict__ H) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t nw = (uint64_t)gridDim.x * (BLOCK / 64);
+  for (uint64_t u = (uint64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6); u < g.V; u += nw) {
+    const uint32_t lo = g.off_out[u], hi = g.off_out[u + 1];
+    uint64_t h = 0, hd = 0;
+    for (uint32_t i = lo + lane; i < hi; i += 64) {
+      const uint32_t x = g.adj_out[i];
+      const uint32_t dx = g.off_out[x + 1] - g.off_out[x];
+      if (dx >= ta) {
+        h += tb_tbl ? tb_tbl[x] : dx;
+        hd += dx;
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      h += ((uint64_t)__shfl_xor((uint32_t)(h >> 32), o) << 32) | __shfl_xor((uint32_t)h, o);
+      hd += ((uint64_t)__shfl_xor((uint32_t)(hd >> 32), o) << 32) | __shfl_xor((uint32_t)hd, o);
+    }
+    if (lane == 0) {
+      H[u] = h;
+      H[g.V + u] = hd;
+    }
+  }
+}
+
+struct ChainTables {
+  const uint64_t* H;           // 2V, k_chain_table
+  const uint32_t *ta, *tb;     // degree-threshold tables (null: plain degree)
+};
+
+// Chain level over rows of L = k-2 words (positions 0..k-3), u = r[ch_u],
+// v = r[L-1] (the newest position):
+//   count(P) = H(u) - sum_{r_i in N(u), deg >= ta} cnt_tb(r_i) - [deg u >= tb] |A|
+//              - sum_{i != ch_u, deg(r_i) >= tb} (|N(u) & N(r_i)|_ta - c_i)
+// with |A| = cnt_ta(u) - |{i : r_i in N(u), deg(r_i) >= ta}| and c_i the
+// prefix vertices inside N(u) & N(r_i) meeting ta.  Here everything except
+// the intersections is added; those of older positions were summed per
+// parent prefix (shared[parent]) and the one with v is left to k_common
+// (minlen/minsel set here).  Level k-2 size += |A|.
+template <int L>
+__global__ __launch_bounds__(BLOCK) void k_chain_row(GView g, KPlan P, const uint32_t* __restrict__ rows, uint64_t n,
+                                                    const uint32_t* __restrict__ parent,
+                                                    const uint32_t* __restrict__ shared, ChainTables t,
+                                                    uint32_t* __restrict__ minlen, uint8_t* __restrict__ minsel,
+                                                    unsigned long long* __restrict__ counters, int stats) {
+  uint64_t acc = 0, na_sum = 0, mb = 0;
+  uint64_t dummy = 0;
+  for (uint64_t p = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (uint64_t)gridDim.x * BLOCK) {
+    uint32_t r[L], d[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) r[l] = rows[p * L + l];
+    uint32_t u = 0;
+#pragma unroll
+    for (int l = 0; l < L; ++l)
+      if (l == (int)P.ch_u) u = r[l];
+#pragma unroll
+    for (int l = 0; l < L; ++l) d[l] = g.sig[2 * (uint64_t)r[l]];
+    const uint32_t du = g.off_out[u + 1] - g.off_out[u];
+    // prefix vertices inside N(u) that meet ta
+    uint32_t inu = 0;
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      const bool in = l != (int)P.ch_u && (((P.qadj[P.ch_u] >> l) & 1) || adj_has(g, u, r[l], dummy));
+      if (in && d[l] >= P.ch_ta) inu |= 1u << l;
+    }
+    const int64_t na = (int64_t)(t.ta ? t.ta[u] : du) - __popc(inu);
+    int64_t c = (int64_t)t.H[u];
+    uint64_t dsum = 0;
+#pragma unroll
+    for (int l = 0; l < L; ++l)
+      if ((inu >> l) & 1) {
+        c -= t.tb ? t.tb[r[l]] : g.off_out[r[l] + 1] - g.off_out[r[l]];
+        dsum += g.off_out[r[l] + 1] - g.off_out[r[l]];
+      }
+    if (P.ch_tb <= 1 || du >= P.ch_tb) c -= na;
+    // c_i: prefix vertices r_j inside N(u) & N(r_i), for every r_i that
+    // takes completions away
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+      if (i == (int)P.ch_u || d[i] < P.ch_tb) continue;
+#pragma unroll
+      for (int j = 0; j < L; ++j) {
+        if (j == i || !((inu >> j) & 1)) continue;
+        if (((P.qadj[i] >> j) & 1) || adj_has(g, r[i], r[j], dummy)) ++c;
+      }
+    }
+    c -= (int64_t)shared[parent[p]];
+    acc += (uint64_t)c;
+    na_sum += (uint64_t)na;
+    const uint32_t v = r[L - 1];
+    const uint32_t dv = g.off_out[v + 1] - g.off_out[v];
+    const bool need = P.ch_tb <= 1 || d[L - 1] >= P.ch_tb;
+    minlen[p] = need ? min(du, dv) : 0u;
+    minsel[p] = (uint8_t)(du <= dv ? 0u : 1u);
+    if (stats) {
+      // merge model: the row's expansion by u's list, each of its na
+      // children's node and expansion by the child's own list
+      mb += 8 + 4ull * du + 24ull * (uint64_t)na + 4ull * (t.H[g.V + u] - dsum);
+    }
+  }
+  add_wave(&counters[0], acc);
+  add_wave(&counters[1], na_sum);
+  if (stats) add_wave(&counters[8], mb);
 }
 
 // ---------------------------------------------------------- dispatch
@@ -1055,6 +1198,7 @@ void ensure_rows(LevelBuf& b, uint64_t rows, uint32_t words) {
   dev_alloc(b.cbase, cap);
   dev_alloc(b.pbase, cap);
   dev_alloc(b.nsurv, cap);
+  dev_alloc(b.parent, cap);
   b.cap = cap;
 }
 void ensure_rows_exact(LevelBuf& b, uint64_t rows, uint32_t words) {
@@ -1067,6 +1211,7 @@ void ensure_rows_exact(LevelBuf& b, uint64_t rows, uint32_t words) {
   dev_alloc(b.cbase, std::max<uint64_t>(rows, 1));
   dev_alloc(b.pbase, std::max<uint64_t>(rows, 1));
   dev_alloc(b.nsurv, std::max<uint64_t>(rows, 1));
+  dev_alloc(b.parent, std::max<uint64_t>(rows, 1));
   b.cap = rows;
 }
 void ensure_bsum(LevelBuf& b, uint64_t nb) {
@@ -1089,9 +1234,10 @@ struct Matcher {
   KPlan kp;
   GView g;
   uint32_t k;
-  bool closed_last = false, fuse = false, write = false, pair = false;
+  bool closed_last = false, fuse = false, write = false, pair = false, chain = false;
   const uint32_t* tbl = nullptr;
   PairTables ptbl{};
+  ChainTables ctab{};
   uint64_t budget = 0;
   uint64_t level_cap[kMaxQuery] = {0};
   uint32_t grid_cap = 0;
@@ -1164,6 +1310,14 @@ struct Matcher {
   void level(uint32_t L, const uint32_t* rows, uint64_t n) {
     if (n == 0) return;
     LevelBuf& in = c.lv[L - 1];
+    if (chain && L == k - 2) {
+      chain_rows(L, rows, n);
+      return;
+    }
+    if (chain && L == k - 3) {
+      chain_shared(L, rows, n, in);
+      if (aborted) return;
+    }
     if (L == k - 1 && closed_last) {
       if (add_work(n)) return;
       const uint32_t grid = grid_for(n, BLOCK, grid_cap);
@@ -1230,6 +1384,7 @@ struct Matcher {
       a.cbase = use_cbase ? in.cbase : nullptr;
       a.pbase = mode == M_PAIR ? in.pbase : nullptr;
       a.nsurv = pair_common ? in.nsurv : nullptr;
+      a.out_parent = (chain && mode == M_WRITE && L == k - 3) ? out.parent : nullptr;
       a.xcd = (o.flags & 8u) ? 0 : 1;
       a.tilectr = (unsigned int*)(c.counters + 4);  // 8 x uint32 in counters[4..7]
       HIPCHECK(hipMemsetAsync(a.tilectr, 0, 8 * sizeof(unsigned int), c.stream));
@@ -1295,15 +1450,19 @@ struct Matcher {
   // planning arrays of the input level are reused once the expansion is
   // done.
   void pair_intersections(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in) {
-    {
-      const uint32_t grid = grid_for(n, BLOCK, grid_cap);
-      pbegin(false);
+    const uint32_t grid = grid_for(n, BLOCK, grid_cap);
+    pbegin(false);
 #define PP(LL) hipLaunchKernelGGL((k_pair_prep<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.nsurv, in.minlen, in.minsel)
-      CUTS_DISPATCH_PAIR(L, PP)
+    CUTS_DISPATCH_PAIR(L, PP)
 #undef PP
-      HIPCHECK(hipGetLastError());
-      pend(1, (double)n * (4.0 * 2 + 16.0 + 5.0), false);
-    }
+    HIPCHECK(hipGetLastError());
+    pend(1, (double)n * (4.0 * 2 + 16.0 + 5.0), false);
+    common_pass(L, rows, n, in, CommonArgs{kp.pr_a, kp.pr_b, kp.pr_tab, 0u}, in.nsurv);
+  }
+
+  // Counts common neighbours over the work planned in in.minlen / in.minsel
+  // (k_common; mode and weights as in ca), in chunks of the work space.
+  void common_pass(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in, CommonArgs ca, uint32_t* weight) {
     const uint64_t W = scan(in, n);
     if (W == 0 || add_work(W)) return;
     for (uint64_t w0 = 0; w0 < W; w0 += MAX_CHUNK) {
@@ -1324,19 +1483,52 @@ struct Matcher {
       a.w_end = w1;
       a.ntiles = nt;
       a.counters = c.counters;
-      a.nsurv = in.nsurv;
+      a.nsurv = weight;
       a.xcd = (o.flags & 8u) ? 0 : 1;
       a.tilectr = (unsigned int*)(c.counters + 4);
       HIPCHECK(hipMemsetAsync(a.tilectr, 0, 8 * sizeof(unsigned int), c.stream));
       a.stats = prof ? 1 : 0;
       r.chunks++;
       pbegin(true);
-#define PC(LL) hipLaunchKernelGGL((k_pair_common<LL>), dim3(grid_for(nt, 1, grid_cap)), dim3(BLOCK), 0, c.stream, g, kp, a)
-      CUTS_DISPATCH_PAIR(L, PC)
+#define PC(LL) hipLaunchKernelGGL((k_common<LL>), dim3(grid_for(nt, 1, grid_cap)), dim3(BLOCK), 0, c.stream, g, kp, a, ca)
+      CUTS_DISPATCH_L(L, PC)
 #undef PC
       HIPCHECK(hipGetLastError());
       pend(9, 0, true);
     }
+  }
+
+  // Chain level, before the rows of level k-3 are written from the rows of
+  // level k-4 (L = k-3 words): per parent row, the sum over its older
+  // positions i of |N(u) & N(r_i)| (those whose degree takes completions
+  // away), shared by all its children; into in.nsurv.
+  void chain_shared(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in) {
+    HIPCHECK(hipMemsetAsync(in.nsurv, 0, n * sizeof(uint32_t), c.stream));
+    for (uint32_t i = 0; i < L && !aborted; ++i) {
+      if (i == kp.ch_u) continue;
+      const uint32_t grid = grid_for(n, BLOCK, grid_cap);
+      pbegin(false);
+#define CP(LL) hipLaunchKernelGGL((k_chain_prep<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, i, in.minlen, in.minsel)
+      CUTS_DISPATCH_PAIR(L, CP)
+#undef CP
+      HIPCHECK(hipGetLastError());
+      pend(1, (double)n * (4.0 * 2 + 16.0 + 5.0), false);
+      common_pass(L, rows, n, in, CommonArgs{kp.ch_u, i, kp.ch_ta, 2u}, in.nsurv);
+    }
+  }
+
+  // Chain level over the rows of level k-3 (L = k-2 words): closed form per
+  // row (k_chain_row), then the intersection with the newest vertex.
+  void chain_rows(uint32_t L, const uint32_t* rows, uint64_t n) {
+    LevelBuf& in = c.lv[L - 1];
+    const uint32_t grid = grid_for(n, BLOCK, grid_cap);
+    pbegin(true);
+#define CR(LL) hipLaunchKernelGGL((k_chain_row<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.parent, c.lv[L - 2].nsurv, ctab, in.minlen, in.minsel, c.counters, (int)prof)
+    CUTS_DISPATCH_L(L, CR)
+#undef CR
+    HIPCHECK(hipGetLastError());
+    pend(8, 0, true);
+    common_pass(L, rows, n, in, CommonArgs{kp.ch_u, L - 1, kp.ch_ta, 1u}, nullptr);
   }
 
   void run() {
@@ -1350,6 +1542,7 @@ struct Matcher {
     closed_last = !write && !(o.flags & 1u) && kp.sc_ok && k >= 2;
     fuse = closed_last && !(o.flags & 2u) && k >= 3;
     pair = fuse && kp.pr_ok && !(o.flags & 128u);
+    chain = fuse && !pair && kp.ch_ok && !(o.flags & 256u);
     grid_cap = (uint32_t)c.cus * (o.blocks_per_cu ? o.blocks_per_cu : 8);
     budget = o.budget ? o.budget : c.default_budget;
     // split the budget evenly over the materialised levels 1..k-2 (or k-1
@@ -1363,6 +1556,11 @@ struct Matcher {
       level_cap[l] = std::max<uint64_t>(per / ((l + 1) * 4 + 13), 1024);
     }
     if (closed_last && kp.sc_thr > 1) tbl = cnt_table(c, dg, kp.sc_thr);
+    if (chain) {
+      ctab.ta = kp.ch_ta > 1 ? cnt_table(c, dg, kp.ch_ta) : nullptr;
+      ctab.tb = kp.ch_tb > 1 ? cnt_table(c, dg, kp.ch_tb) : nullptr;
+      ctab.H = chain_table(c, dg, kp.ch_ta, kp.ch_tb);
+    }
     if (pair) {
       ptbl.a = kp.pr_ta > 1 ? cnt_table(c, dg, kp.pr_ta) : nullptr;
       ptbl.b = kp.pr_tb > 1 ? cnt_table(c, dg, kp.pr_tb) : nullptr;
@@ -1484,6 +1682,22 @@ const uint32_t* cnt_table(Context& c, DeviceGraph& dg, uint32_t thr) {
   return it->second;
 }
 
+const uint64_t* chain_table(Context& c, DeviceGraph& dg, uint32_t ta, uint32_t tb) {
+  const auto key = std::make_pair(ta, tb);
+  auto it = dg.chain_tables.find(key);
+  if (it == dg.chain_tables.end()) {
+    uint64_t* t = nullptr;
+    dev_alloc(t, 2 * (uint64_t)dg.V + 1);
+    const GView g = view_of(dg);
+    const uint32_t* tbt = tb > 1 ? cnt_table(c, dg, tb) : nullptr;
+    hipLaunchKernelGGL(k_chain_table, dim3(grid_for(dg.V, BLOCK / 64, 65535)), dim3(BLOCK), 0, c.stream, g, ta, tbt, t);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(c.stream));
+    it = dg.chain_tables.emplace(key, t).first;
+  }
+  return it->second;
+}
+
 double copy_probe(Context& c, uint64_t bytes, uint32_t iters) {
   HIPCHECK(hipSetDevice(c.device));
   const uint64_t n = std::max<uint64_t>(bytes / 16, 1);
@@ -1559,6 +1773,20 @@ KPlan make_kplan(const Plan& p, bool symmetric) {
       }
     }
   }
+  for (uint32_t i = 0; i < p.k; ++i)
+    for (uint32_t j = 0; j < p.k; ++j)
+      if (i != j && ((p.pmask[std::max(i, j)] | p.cmask[std::max(i, j)]) >> std::min(i, j) & 1)) kp.qadj[i] |= 1u << j;
+  // the last two positions as a chain: k-1 hangs on k-2 only, k-2 on one
+  // position older than k-3 only
+  if (kp.sc_ok && p.k >= 4 && kp.sc_pos == last - 1 && kp.nj[last - 1] == 1) {
+    const uint32_t u = kp.jc[last - 1][0] & 31;
+    if (u + 3 < p.k) {
+      kp.ch_ok = 1;
+      kp.ch_u = u;
+      kp.ch_ta = std::max(p.sig_in[last - 1], p.sig_out[last - 1]);
+      kp.ch_tb = kp.sc_thr;
+    }
+  }
   return kp;
 }
 
@@ -1574,6 +1802,7 @@ DeviceGraph::~DeviceGraph() {
     if (nsig_in) (void)hipFree(nsig_in);
   }
   for (auto& kv : cnt_tables) (void)hipFree(kv.second);
+  for (auto& kv : chain_tables) (void)hipFree(kv.second);
   if (hub_id) (void)hipFree(hub_id);
   if (bits) (void)hipFree(bits);
   (void)hipSetDevice(prev);
@@ -1584,7 +1813,7 @@ Context::~Context() {
   for (auto& b : lv) {
     for (void* p : {(void*)b.rows, (void*)b.minlen, (void*)b.minsel, (void*)b.work_off, (void*)b.bsum,
                     (void*)b.tile_p, (void*)b.cbase, (void*)b.pbase,
-                    (void*)b.nsurv})
+                    (void*)b.nsurv, (void*)b.parent})
       if (p) (void)hipFree(p);
   }
   if (counters) (void)hipFree(counters);

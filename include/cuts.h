@@ -92,7 +92,8 @@ enum {
   CUTS_FLAG_PROFILE = 1u << 4,     /* time every kernel class with HIP events */
   CUTS_FLAG_NO_BITMAP = 1u << 5,   /* lower-bound searches even in hub lists */
   CUTS_FLAG_NO_PREFIX_BASE = 1u << 6, /* fused closed form fully per work item */
-  CUTS_FLAG_NO_PAIR = 1u << 7         /* never count the last two positions together */
+  CUTS_FLAG_NO_PAIR = 1u << 7,        /* never count the last two positions together */
+  CUTS_FLAG_NO_CHAIN = 1u << 8        /* not as a chain either (k-1 hanging on k-2) */
 };
 
 /* Kernel classes reported by CUTS_FLAG_PROFILE. */
@@ -105,8 +106,8 @@ enum {
   CUTS_K_COUNT = 5,    /* expansion counting the last level */
   CUTS_K_FUSED = 6,    /* expansion + closed-form last level */
   CUTS_K_CLOSED = 7,   /* closed-form last level over stored rows */
-  CUTS_K_PAIR = 8,     /* expansion + the last two levels counted together */
-  CUTS_K_PAIR_COMMON = 9, /* |N(u) & N(w)| of the pair level's prefixes */
+  CUTS_K_LAST2 = 8,    /* the last two positions counted together (pair / chain levels) */
+  CUTS_K_COMMON = 9,   /* intersection sizes |N(u) & N(w)| those levels need */
   CUTS_K_CLASSES = 10
 };
 

@@ -103,9 +103,9 @@ def test_expand_tile_loops_are_wave_uniform(tmp_path):
     for L, mode in ((4, 2), (3, 0), (5, 1), (3, 3)):
         name = f"_ZN4cuts12_GLOBAL__N_18k_expandILi{L}ELi{mode}EEEvNS0_5GViewENS_5KPlanENS0_10ExpandArgsE"
         check_block_loop(function_body(asm, name), f"k_expand<{L}, {mode}>")
-    for L in (1, 3):
-        name = f"_ZN4cuts12_GLOBAL__N_113k_pair_commonILi{L}EEEvNS0_5GViewENS_5KPlanENS0_10ExpandArgsE"
-        check_block_loop(function_body(asm, name), f"k_pair_common<{L}>")
+    for L in (1, 3, 4):
+        name = f"_ZN4cuts12_GLOBAL__N_18k_commonILi{L}EEEvNS0_5GViewENS_5KPlanENS0_10ExpandArgsENS0_10CommonArgsE"
+        check_block_loop(function_body(asm, name), f"k_common<{L}>")
 
 
 @pytest.mark.parametrize("src", ["batch.hip", "match.hip"])

@@ -1,13 +1,21 @@
-"""The pair level: the last two query positions counted together.
+"""The last two query positions counted together.
 
-When positions k-2 and k-1 are not adjacent in the query and each has one
-backward neighbour older than position k-3 (pendants, typically), a prefix
-of positions 0..k-3 has |A| * |B| - |A & B| completions, A and B being the
-degree-filtered neighbour lists of those backward neighbours minus the
-prefix; the expansion of position k-3 adds that number per survivor and
-levels k-2 and k-1 are never expanded.  Counts must equal the oracle's and
-the same run without the pair level; level sizes and merge-model bytes must
-equal the CPU trie walk's, which materialises every level.
+Pair level: when positions k-2 and k-1 are not adjacent in the query and
+each has one backward neighbour older than position k-3 (pendants,
+typically), a prefix of positions 0..k-3 has |A| * |B| - |A & B|
+completions, A and B being the degree-filtered neighbour lists of those
+backward neighbours minus the prefix; the expansion of position k-3 adds
+that number per survivor and levels k-2 and k-1 are never expanded.
+
+Chain level: when k-1 hangs on k-2 alone and k-2 on one position u older
+than k-3 alone (a 2-edge tail), a prefix of positions 0..k-3 has
+sum_{x in A} |N(x) - prefix - x| completions, computed from per-vertex
+tables and one intersection N(u) & N(r_i) per other prefix vertex (those of
+older positions shared by the rows of one parent prefix).
+
+Counts must equal the oracle's and the same run without these levels; level
+sizes and merge-model bytes must equal the CPU trie walk's, which
+materialises every level.
 """
 import random
 
@@ -26,11 +34,16 @@ SHAPES = {
     "diamond+2pend": [(0, 1), (0, 2), (1, 2), (1, 3), (2, 3), (0, 4), (3, 5)],
     "k4+2pend": [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3), (1, 4), (2, 5)],
     "5cycle+3pend": [(0, 1), (1, 2), (2, 3), (3, 4), (4, 0), (0, 5), (2, 6), (3, 7)],
-    # not eligible: the last two positions are adjacent, or the second
-    # pendant hangs on position k-3
+    # chain level: a 2-edge tail on a vertex older than position k-3
     "triangle+tail2": list(W.TRI_TAIL2.edges),
     "k4e+tail2": list(W.K4E_TAIL2.edges),
+    "path5": [(0, 1), (1, 2), (2, 3), (3, 4)],
+    "4cycle+tail2": [(0, 1), (1, 2), (2, 3), (3, 0), (0, 4), (4, 5)],
+    "star3+tail2": [(0, 1), (0, 2), (0, 3), (3, 4), (4, 5)],
+    "k4+tail2": [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3), (3, 4), (4, 5)],
+    # neither: the second pendant hangs on position k-3, or the tail is longer
     "5cycle+chord+2tails": list(W.C5CHORD_2TAIL.edges),
+    "triangle+tail3": [(0, 1), (1, 2), (2, 0), (2, 3), (3, 4), (4, 5)],
 }
 
 
@@ -50,6 +63,17 @@ def pair_eligible(p):
         return False
     (pa,), (pb,) = b[k - 2], b[k - 1]
     return pb != k - 2 and pa <= k - 4 and pb <= k - 4
+
+
+def chain_eligible(p):
+    k = p.k
+    if k < 4:
+        return False
+    b = backward(p)
+    if b[k - 1] != {k - 2} or len(b[k - 2]) != 1:
+        return False
+    (u,) = b[k - 2]
+    return u <= k - 4
 
 
 def graph_for(seed):
@@ -73,32 +97,38 @@ def test_pair_level_vs_oracle_and_cpu_trie(ctx, name, seed):
     assert cpu.count == want
     dg = ctx.upload(g)
     prof = cuts_amd.FLAG_PROFILE
-    for flags in (0, cuts_amd.FLAG_NO_BITMAP, cuts_amd.FLAG_NO_PAIR):
+    for flags in (0, cuts_amd.FLAG_NO_BITMAP, cuts_amd.FLAG_NO_PAIR, cuts_amd.FLAG_NO_CHAIN,
+                  cuts_amd.FLAG_NO_PAIR | cuts_amd.FLAG_NO_CHAIN):
         r = ctx.match(dg, p, flags=flags | prof)
         assert r.count == want, (name, flags)
         assert r.level_sizes == [cpu.level[i] for i in range(k)], (name, flags)
         assert sum(r.merge_bytes.values()) == cpu.bytes, (name, flags, r.merge_bytes)
-        used = r.kernels["expand_pair"][1] > 0
-        expect = pair_eligible(p) and not flags & cuts_amd.FLAG_NO_PAIR
+        used = r.kernels["last_two"][1] > 0
+        expect = (pair_eligible(p) and not flags & cuts_amd.FLAG_NO_PAIR) or \
+            (chain_eligible(p) and not flags & cuts_amd.FLAG_NO_CHAIN)
         assert not used or expect, (name, flags, p.order)
         assert used or not expect or r.level_sizes[k - 3] == 0, (name, flags, p.order)
     for m in (dict(trunks=5), dict(budget=2048), dict(flags=cuts_amd.FLAG_NO_XCD)):
         assert ctx.match(dg, p, **m).count == want, (name, m)
 
 
-def test_pair_shapes_cover_both_cases():
-    """The shape table holds eligible shapes with one and with two distinct
-    list owners, and ineligible ones (plans only, no GPU work)."""
+def test_shapes_cover_every_case():
+    """The shape table holds pair shapes with one and with two distinct list
+    owners, chain shapes with k = 5 and k >= 6, and shapes that are neither
+    (plans only, no GPU work)."""
     kinds = set()
     for name, edges in SHAPES.items():
         k = 1 + max(max(e) for e in edges)
         p = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges))
+        assert not (pair_eligible(p) and chain_eligible(p))
         if pair_eligible(p):
             b = backward(p)
-            kinds.add("same" if b[k - 1] == b[k - 2] else "two")
+            kinds.add("pair-same" if b[k - 1] == b[k - 2] else "pair-two")
+        elif chain_eligible(p):
+            kinds.add("chain-5" if k == 5 else "chain-6+")
         else:
-            kinds.add("no")
-    assert kinds == {"same", "two", "no"}
+            kinds.add("neither")
+    assert kinds == {"pair-same", "pair-two", "chain-5", "chain-6+", "neither"}
 
 
 @pytest.mark.gpu
@@ -115,5 +145,5 @@ def test_pair_level_random_queries(ctx, seed):
     want = oracle.count(og, oracle.qmatrix(k, oracle.undirected(edges)))
     p = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges))
     dg = ctx.upload(g)
-    for flags in (0, cuts_amd.FLAG_NO_PAIR, cuts_amd.FLAG_NO_BITMAP):
+    for flags in (0, cuts_amd.FLAG_NO_PAIR | cuts_amd.FLAG_NO_CHAIN, cuts_amd.FLAG_NO_BITMAP):
         assert ctx.match(dg, p, flags=flags).count == want, (edges, flags)
