@@ -409,7 +409,9 @@ def main():
     g = W.config2_graph()
     shapes = [W.C4_2PEND, W.K4E_TAIL2]
     ctx = cuts_amd.Context(0)
+    t_up = time.perf_counter()
     dg = ctx.upload(g)
+    upload_s = time.perf_counter() - t_up
     plans = [cuts_amd.Plan(W.query_graph(s)) for s in shapes]
 
     for i in range(args.warmup):
@@ -439,7 +441,26 @@ def main():
     dom = max(kern.items(), key=lambda kv: kv[1][0])
     dname, (dms, dn, dbytes, dmerge) = dom
     avg_launch_s = dms / dn / 1e3
-    bytes_per_launch = dmerge / dn
+    if dmerge:
+        # a trie-expansion kernel: the merge model's bytes for the levels it
+        # performs
+        bytes_per_launch = dmerge / dn
+        model = "merge model (8+4*len per distinct backward list per prefix, 16 per trie node)"
+        note = ("achieved/frac use the merge model, which charges list reads that the closed form never makes, "
+                "so frac > 1 here; demand_GBs is what the kernel loads, traffic what reaches HBM (rocprofv3)")
+    else:
+        # an intersection kernel (no trie level of its own): per element of
+        # the shorter list, the element and one membership probe of the
+        # other list -- a hub's id and bitmap word, or the words of a
+        # lower-bound search -- plus the element's degree where a threshold
+        # above 2 needs it; counted on the device
+        bytes_per_launch = dbytes / dn
+        model = ("intersection model: per element of the shorter list 4 B for the element, 12 B for a hub "
+                 "membership probe (id + bitmap word) or 8 B + the lower-bound search's words, 8 B degree "
+                 "where needed")
+        note = ("the dominant kernel counts |N(u) & N(w)| for the closed forms of the last two positions; the graph "
+                "(1.6 MB CSR + bitmaps) is L2-resident, so the bound is L2 latency, not HBM; "
+                "merge_model_GBs is the whole path's merge-model bytes over its device time")
     achieved = bytes_per_launch / avg_launch_s / 1e9
     line = {
         "metric": "exact embeddings/sec + p50 time/query; learned vs heuristic order, same GPU",
@@ -463,6 +484,7 @@ def main():
             "candidates_per_step": sum(work) / args.steps,
             "candidates_per_s": sum(work) / (total_ms / 1e3),
             "wall_s": wall,
+            "upload_s": upload_s,
         },
         "roofline": {
             "bound": "hbm",
@@ -473,16 +495,16 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic_for(dname),
             "algorithmic_bytes_per_launch": bytes_per_launch,
-            "bytes_model": "merge model (8+4*len per distinct backward list per prefix, 16 per trie node)",
+            "bytes_model": model,
             "avg_launch_ms": avg_launch_s * 1e3,
             "launches": dn,
             "demand_bytes_per_launch": dbytes / dn,
             "demand_GBs": dbytes / dn / avg_launch_s / 1e9,
             "demand_frac": dbytes / dn / avg_launch_s / 1e9 / HBM_PEAK_GBS,
-            "note": "achieved/frac use the merge model, which charges list reads that the closed form never makes, "
-                    "so frac > 1 here; demand_GBs is what the kernel loads, traffic what reaches HBM (rocprofv3)",
+            "note": note,
         },
         "merge_bytes_total": sum(v[3] for v in kern.values()),
+        "merge_model_GBs": sum(v[3] for v in kern.values()) / (total_ms / 1e3) / 1e9,
         "kernels_ms": {k: v[0] for k, v in kern.items() if v[1]},
     }
     if args.copy_probe:

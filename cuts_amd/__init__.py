@@ -31,9 +31,10 @@ FLAG_NO_BITMAP = 32
 FLAG_NO_PREFIX_BASE = 64
 FLAG_NO_PAIR = 128
 FLAG_NO_CHAIN = 256
+FLAG_NO_TWO_HOP = 512
 MAX_QUERY = 16
 KERNEL_CLASSES = ["level0", "plan", "scan", "tile_index", "expand_write", "expand_count", "expand_fused",
-                  "closed_form", "last_two", "common"]
+                  "closed_form", "last_two", "common", "chain_common"]
 _NCLASS = len(KERNEL_CLASSES)
 
 _u32p = C.POINTER(C.c_uint32)
@@ -90,6 +91,8 @@ def lib():
             "cuts_graph_from_arcs": (C.c_int, [C.c_uint32, _u32p, _u32p, C.c_uint64, C.c_int, pp]),
             "cuts_graph_save": (C.c_int, [vp, C.c_char_p]),
             "cuts_graph_view_of": (C.c_int, [vp, C.POINTER(_GraphView)]),
+            "cuts_graph_two_hop": (C.c_int, [vp, C.POINTER(C.c_uint64), _u32p, _u32p, C.c_uint64,
+                                             C.POINTER(C.c_uint64)]),
             "cuts_graph_free": (None, [vp]),
             "cuts_gen_chung_lu": (C.c_int, [C.c_uint32, C.c_uint64, C.c_double, C.c_uint32, C.c_uint64, pp]),
             "cuts_gen_rmat": (C.c_int, [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
@@ -201,6 +204,20 @@ class Graph:
         return dict(V=n, arcs=a, symmetric=bool(v.symmetric), out_off=arr(v.out_off, n + 1),
                     out_adj=arr(v.out_adj, a), in_off=arr(v.in_off, n + 1), in_adj=arr(v.in_adj, a),
                     sig=arr(v.sig, 2 * n))
+
+    def two_hop(self):
+        """The two-hop index a device upload builds (symmetric graphs):
+        (off[V+1], v, c) -- for every vertex u, the vertices v != u sharing a
+        neighbour with u, ascending, and the number of shared neighbours."""
+        n = C.c_uint64()
+        _check(lib().cuts_graph_two_hop(self._h, None, None, None, 0, C.byref(n)))
+        V = self.V
+        off = np.zeros(V + 1, dtype=np.uint64)
+        v = np.zeros(max(n.value, 1), dtype=np.uint32)
+        c = np.zeros(max(n.value, 1), dtype=np.uint32)
+        _check(lib().cuts_graph_two_hop(self._h, off.ctypes.data_as(C.POINTER(C.c_uint64)), _ptr(v), _ptr(c),
+                                        n.value, C.byref(n)))
+        return off, v[: n.value], c[: n.value]
 
     @property
     def V(self) -> int:

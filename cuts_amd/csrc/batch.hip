@@ -54,6 +54,8 @@ struct GV {
   const uint2 *nsig_out, *nsig_in;  // signature of every list entry
   const uint64_t* bits;
   uint32_t words;
+  const uint64_t* h2_off;  // two-hop index or null
+  const uint32_t *h2_v, *h2_c;
 };
 
 __device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -88,6 +90,19 @@ __device__ __forceinline__ bool has_arc(const GV& g, uint32_t u, uint32_t x) {
     if (h >= 0) return (g.bits[(uint64_t)h * g.words + (x >> 6)] >> (x & 63)) & 1ull;
   }
   return bmember(g.adj_out, g.off_out[u], g.off_out[u + 1], x);
+}
+
+// |N(u) & N(w)| for u != w from the two-hop index (lower-bound search of w
+// in u's list; common neighbours of distinct vertices have degree >= 2)
+__device__ __forceinline__ uint32_t two_hop(const GV& g, uint32_t u, uint32_t w) {
+  uint64_t lo = g.h2_off[u], hi = g.h2_off[u + 1];
+  const uint64_t end = hi;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (g.h2_v[mid] < w) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < end && g.h2_v[lo] == w) ? g.h2_c[lo] : 0u;
 }
 
 // Closed-form count of the last position given positions r[0..n) (LDS row)
@@ -278,7 +293,8 @@ __device__ uint64_t block_scan(Shared<WROW>& s, uint32_t n, uint64_t len) {
 
 template <int WROW>
 __device__ void process(const BatchArgs& a, Shared<WROW>& s, uint32_t L) {
-  const GV g{a.off_out, a.adj_out, a.off_in, a.adj_in, a.sig, a.V, a.hub_id, a.nsig_out, a.nsig_in, a.bits, a.words};
+  const GV g{a.off_out, a.adj_out, a.off_in, a.adj_in, a.sig, a.V, a.hub_id, a.nsig_out, a.nsig_in, a.bits, a.words,
+             a.h2_off, a.h2_v, a.h2_c};
   Item it;
   it.qL = uni(s.item.qL);
   it.n = uni(s.item.n);
@@ -338,6 +354,8 @@ __device__ void process(const BatchArgs& a, Shared<WROW>& s, uint32_t L) {
   // the row's survivor count.  The stage is empty on this level (nothing is
   // flushed), so these live there.
   const uint32_t pa = uni(P.pr_a), pb = uni(P.pr_b);
+  // |N(u) & N(w)| from the two-hop index where it serves the threshold
+  const bool h2 = mode == 3 && pa != pb && g.h2_off != nullptr && uni(P.pr_tab) <= 2;
   uint32_t* const pxa = s.stage;
   uint32_t* const pxb = s.stage + RMAX;
   uint32_t* const pxz = s.stage + 2 * RMAX;
@@ -351,7 +369,8 @@ __device__ void process(const BatchArgs& a, Shared<WROW>& s, uint32_t L) {
       const uint32_t* tab_tbl = a.tbls[3 * q + 2];
       int64_t ca = ta_tbl ? (int64_t)ta_tbl[u] : (int64_t)(g.off_out[u + 1] - g.off_out[u]);
       int64_t cb = tbl ? (int64_t)tbl[w] : (int64_t)(g.off_out[w + 1] - g.off_out[w]);
-      int64_t cab = pa != pb ? 0 : (tab_tbl ? (int64_t)tab_tbl[u] : (int64_t)(g.off_out[u + 1] - g.off_out[u]));
+      int64_t cab = pa != pb ? (h2 ? (int64_t)two_hop(g, u, w) : 0)
+                             : (tab_tbl ? (int64_t)tab_tbl[u] : (int64_t)(g.off_out[u + 1] - g.off_out[u]));
       for (uint32_t l = 0; l < L; ++l) {
         const bool ina = ((P.pr_ma >> l) & 1) || has_arc(g, u, row[l]);
         const bool inb = ((P.pr_mb >> l) & 1) || has_arc(g, w, row[l]);
@@ -437,7 +456,7 @@ __device__ void process(const BatchArgs& a, Shared<WROW>& s, uint32_t L) {
         const int64_t nab = (int64_t)(int32_t)pxz[r] - (int64_t)(ina && inb && sgv.x >= P.pr_tab);
         fc = na * nb - (uint64_t)nab;  // modulo-2^64 sums are exact
         sna += na;
-        if (pa != pb) atomicAdd(&pns[r], 1u);
+        if (pa != pb && !h2) atomicAdd(&pns[r], 1u);
       }
       if (mode == 2 && ok) {
         if (use_cbase) {
@@ -464,7 +483,7 @@ __device__ void process(const BatchArgs& a, Shared<WROW>& s, uint32_t L) {
     }
   }
   if (mode == 0) flush_stage(a, s, q, L + 1);
-  if (mode == 3 && pa != pb) {
+  if (mode == 3 && pa != pb && !h2) {
     // second pass: every x in N(u) & N(w) (degree-filtered) takes one
     // completion from each survivor of its row; per row with survivors the
     // shorter of the two lists is walked and x looked up in the other
@@ -686,6 +705,9 @@ void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& p
   a.nsig_in = dg.nsig_in;
   a.bits = dg.bits;
   a.words = dg.words;
+  a.h2_off = (o.flags & 512u) ? nullptr : dg.h2_off;
+  a.h2_v = dg.h2_v;
+  a.h2_c = dg.h2_c;
   KPlan* dplans = balloc<KPlan>(b, 0, b.nq);
   HIPCHECK(hipMemcpy(dplans, kp.data(), b.nq * sizeof(KPlan), hipMemcpyHostToDevice));
   a.plans = dplans;

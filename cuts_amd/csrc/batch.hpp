@@ -32,6 +32,8 @@ struct BatchArgs {
   const uint2 *nsig_out, *nsig_in;
   const uint64_t* bits;
   uint32_t words;
+  const uint64_t* h2_off;       // two-hop index or null
+  const uint32_t *h2_v, *h2_c;
   const KPlan* plans;
   const uint32_t* const* tbls;  // per query: 3 degree-threshold tables or null -- the closed form's
                                 // (= B's of the pair level), A's and A & B's

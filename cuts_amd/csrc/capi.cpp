@@ -221,6 +221,27 @@ int cuts_graph_save(const cuts_graph* g, const char* path) {
   });
 }
 
+int cuts_graph_two_hop(const cuts_graph* g, uint64_t* off, uint32_t* v, uint32_t* c, uint64_t cap,
+                       uint64_t* entries) {
+  if (!g || !entries) return fail(CUTS_E_ARG, "null argument");
+  if (!g->g.symmetric) return fail(CUTS_E_ARG, "two-hop index: the graph is not symmetric");
+  try {
+    std::vector<uint64_t> o;
+    std::vector<uint32_t> hv, hc;
+    cuts::build_two_hop(g->g, o, hv, hc);
+    *entries = o.back();
+    if (cap == 0 && !off) return CUTS_OK;
+    if (!off || (o.back() && (!v || !c))) return fail(CUTS_E_ARG, "null output buffer");
+    if (o.back() > cap) return fail(CUTS_E_ARG, "two-hop index: " + std::to_string(o.back()) + " entries exceed cap");
+    std::copy(o.begin(), o.end(), off);
+    std::copy(hv.begin(), hv.begin() + o.back(), v);
+    std::copy(hc.begin(), hc.begin() + o.back(), c);
+    return CUTS_OK;
+  } catch (const std::exception& e) {
+    return fail(CUTS_E_ARG, e.what());
+  }
+}
+
 int cuts_graph_view_of(const cuts_graph* g, cuts_graph_view* v) {
   if (!g || !v) return fail(CUTS_E_ARG, "null argument");
   v->V = g->g.V;
@@ -357,7 +378,7 @@ int cuts_match(cuts_context* ctx, const cuts_device_graph* dg, const cuts_plan* 
     res->last_root = r.last_root;
     res->work_capped = r.work_capped;
     res->work = r.work;
-    for (int i = 0; i < 10; ++i) {
+    for (int i = 0; i < 11; ++i) {
       res->kernel_ms[i] = r.kernel_ms[i];
       res->kernel_launches[i] = r.kernel_launches[i];
       res->kernel_bytes[i] = r.kernel_bytes[i];

@@ -83,6 +83,13 @@ struct DeviceGraph {
   int32_t* hub_id = nullptr;
   uint64_t* bits = nullptr;
   uint32_t words = 0, hubs = 0, hub_min_deg = 0;
+  // two-hop index (symmetric graphs whose wedges fit the budget): per vertex
+  // u, the vertices v != u with a common neighbour, ascending (h2_v over
+  // h2_off[u] .. h2_off[u+1]), and |N(u) & N(v)| for each (h2_c)
+  uint64_t* h2_off = nullptr;
+  uint32_t *h2_v = nullptr, *h2_c = nullptr;
+  uint64_t h2_entries = 0;
+  double h2_build_s = 0;
   ~DeviceGraph();
 };
 
@@ -112,8 +119,8 @@ struct Context {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   uint64_t default_budget = 0;
   LevelBuf lv[kMaxQuery];
-  // [0] embeddings, [1] fused level size (level k-3 with the pair level),
-  // [2] level k-2 size of the pair level, [3] demand bytes, [4..7] eight
+  // [0] embeddings, [1] level k-2 size of the fused or chain level (level
+  // k-3 with the pair level), [2] level k-2 size of the pair level, [3] demand bytes, [4..7] eight
   // uint32 tile counters, [8] merge-model bytes, [10] rows written by one
   // expansion
   unsigned long long* counters = nullptr;
@@ -144,10 +151,10 @@ struct MatchResult {
   uint32_t last_root = 0;
   uint32_t work_capped = 0;
   uint64_t work = 0;
-  double kernel_ms[10] = {0};
-  uint64_t kernel_launches[10] = {0};
-  double kernel_bytes[10] = {0};
-  double merge_bytes[10] = {0};
+  double kernel_ms[11] = {0};
+  uint64_t kernel_launches[11] = {0};
+  double kernel_bytes[11] = {0};
+  double merge_bytes[11] = {0};
 };
 
 void context_init(Context& c, int device);

@@ -233,3 +233,59 @@ std::string save_graph(const std::string& path, const Graph& g) {
 }
 
 }  // namespace cuts
+
+namespace cuts {
+
+void build_two_hop(const Graph& g, std::vector<uint64_t>& off, std::vector<uint32_t>& hv, std::vector<uint32_t>& hc) {
+  const uint32_t V = g.V;
+  off.assign((size_t)V + 1, 0);
+  // pass 1: distinct two-hop vertices per u (a stamp per vertex, u + 1)
+#pragma omp parallel
+  {
+    std::vector<uint32_t> mark(V, 0);
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t u = 0; u < (int64_t)V; ++u) {
+      uint64_t cnt = 0;
+      for (uint32_t i = g.out_off[u]; i < g.out_off[u + 1]; ++i) {
+        const uint32_t x = g.out_adj[i];
+        for (uint32_t j = g.out_off[x]; j < g.out_off[x + 1]; ++j) {
+          const uint32_t v = g.out_adj[j];
+          if (v != (uint32_t)u && mark[v] != (uint32_t)u + 1) {
+            mark[v] = (uint32_t)u + 1;
+            ++cnt;
+          }
+        }
+      }
+      off[u + 1] = cnt;
+    }
+  }
+  for (uint32_t u = 0; u < V; ++u) off[u + 1] += off[u];
+  hv.assign(std::max<uint64_t>(off[V], 1), 0);
+  hc.assign(std::max<uint64_t>(off[V], 1), 0);
+  // pass 2: the vertices, ascending, with their common-neighbour counts
+#pragma omp parallel
+  {
+    std::vector<uint32_t> cnt(V, 0), touched;
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t u = 0; u < (int64_t)V; ++u) {
+      touched.clear();
+      for (uint32_t i = g.out_off[u]; i < g.out_off[u + 1]; ++i) {
+        const uint32_t x = g.out_adj[i];
+        for (uint32_t j = g.out_off[x]; j < g.out_off[x + 1]; ++j) {
+          const uint32_t v = g.out_adj[j];
+          if (v != (uint32_t)u && cnt[v]++ == 0) touched.push_back(v);
+        }
+      }
+      std::sort(touched.begin(), touched.end());
+      uint64_t o = off[u];
+      for (uint32_t v : touched) {
+        hv[o] = v;
+        hc[o] = cnt[v];
+        cnt[v] = 0;
+        ++o;
+      }
+    }
+  }
+}
+
+}  // namespace cuts

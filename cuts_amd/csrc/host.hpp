@@ -34,6 +34,11 @@ std::string load_graph(const std::string& path, Graph& g);
 // Writes a graph in the same edge-list format (header V, then "s\tt").
 std::string save_graph(const std::string& path, const Graph& g);
 
+// Two-hop index of a symmetric graph: for every vertex u, the vertices
+// v != u that share a neighbour with u, ascending (v over off[u] ..
+// off[u+1]), and |N(u) & N(v)| for each (c).  OpenMP over u.
+void build_two_hop(const Graph& g, std::vector<uint64_t>& off, std::vector<uint32_t>& v, std::vector<uint32_t>& c);
+
 // A query plan: the matching order and, per order position i, the earlier
 // positions j whose image constrains position i's candidates
 // (src/graph.cpp:94-133).

@@ -60,11 +60,13 @@ struct GView {
   const uint2 *nsig_out, *nsig_in;  // signature of every list entry
   const uint64_t* bits;
   uint32_t words;
+  const uint64_t* h2_off;  // two-hop index or null
+  const uint32_t *h2_v, *h2_c;
 };
 
 GView view_of(const DeviceGraph& dg) {
   return GView{dg.off_out, dg.adj_out, dg.off_in, dg.adj_in, dg.sig, dg.V, dg.hub_id, dg.nsig_out, dg.nsig_in, dg.bits,
-               dg.words};
+               dg.words, dg.h2_off, dg.h2_v, dg.h2_c};
 }
 
 __device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -110,6 +112,22 @@ __device__ __forceinline__ bool adj_has(const GView& g, uint32_t u, uint32_t x, 
   const uint32_t lo = g.off_out[u], hi = g.off_out[u + 1];
   bytes += 8 + bs_bytes(hi - lo);
   return member_safe(g.adj_out, lo, hi, x);
+}
+
+// |N(u) & N(w)| for u != w from the two-hop index: a lower-bound search of
+// w in u's two-hop list (0 when w is not there).  Every common neighbour of
+// two distinct vertices has degree >= 2, so the count serves degree
+// thresholds up to 2.
+__device__ __forceinline__ uint32_t two_hop(const GView& g, uint32_t u, uint32_t w, uint64_t& bytes) {
+  uint64_t lo = g.h2_off[u], hi = g.h2_off[u + 1];
+  const uint64_t end = hi;
+  bytes += 16 + 4 * ((64 - __clzll(hi - lo)) + 1);
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (g.h2_v[mid] < w) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < end && g.h2_v[lo] == w) ? g.h2_c[lo] : 0u;
 }
 
 // One block per hub: sets the bits of its neighbours.
@@ -230,6 +248,7 @@ __global__ __launch_bounds__(BLOCK) void k_plan(GView g, KPlan P, const uint32_t
       cbase[p] = (uint32_t)c;
     }
     if (pbase) {
+      uint64_t dummy0 = 0;
       uint32_t u = 0, w = 0;
 #pragma unroll
       for (int l = 0; l < L; ++l) {
@@ -237,7 +256,9 @@ __global__ __launch_bounds__(BLOCK) void k_plan(GView g, KPlan P, const uint32_t
         if (l == (int)P.pr_b) w = row[l];
       }
       int64_t ca = tbl_or_deg(g, pt.a, u), cb = tbl_or_deg(g, pt.b, w);
-      int64_t cab = P.pr_a == P.pr_b ? (int64_t)tbl_or_deg(g, pt.ab, u) : 0;
+      int64_t cab = 0;
+      if (P.pr_a == P.pr_b) cab = (int64_t)tbl_or_deg(g, pt.ab, u);
+      else if (g.h2_off && P.pr_tab <= 2) cab = (int64_t)two_hop(g, u, w, dummy0);
       uint64_t dummy = 0;
 #pragma unroll
       for (int l = 0; l < L; ++l) {
@@ -984,39 +1005,116 @@ __global__ __launch_bounds__(BLOCK) void k_common(GView g, KPlan P, ExpandArgs a
     if (lds)
       for (uint32_t i = tid; i <= span; i += BLOCK) s_off[i] = a.work_off[plo + i];
     __syncthreads();
-    for (int it = 0; it < IPT; ++it) {
-      const uint64_t w = w0 + (uint64_t)it * BLOCK + tid;
-      const bool in = w < w1;
-      uint32_t lo = 0, hi = span + 1;
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        const uint64_t o = lds ? s_off[mid] : a.work_off[plo + mid];
-        if (o <= (in ? w : w1 - 1)) lo = mid + 1;
-        else hi = mid;
+    // the IPT items of a thread in lock-step stages (prefix search, row,
+    // list word, membership) so that their loads are in flight together
+    uint64_t w[IPT];
+    uint32_t pi[IPT];
+    bool in[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      w[j] = w0 + (uint64_t)j * BLOCK + tid;
+      in[j] = w[j] < w1;
+      pi[j] = 0;
+    }
+    // largest pi with off(pi) <= w: a fixed number of halvings for the whole
+    // block (off(0) <= w0 <= w and off(span) >= w1)
+    for (uint32_t n = span + 1; n > 1;) {
+      const uint32_t half = n >> 1;
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const uint64_t o = lds ? s_off[pi[j] + half] : a.work_off[plo + pi[j] + half];
+        pi[j] = (o <= w[j]) ? pi[j] + half : pi[j];
       }
-      const uint32_t pi = lo - 1;
-      const uint64_t p = (uint64_t)plo + pi;
-      bool hit = false;
-      if (in) {
-        const uint64_t pbase = lds ? s_off[pi] : a.work_off[p];
-        const uint32_t idx = (uint32_t)(w - pbase);
-        const uint32_t u = a.rows[p * L + c.pa], wv = a.rows[p * L + c.pb];
-        const bool sel = a.minsel[p] != 0;
-        const uint32_t li = g.off_out[sel ? wv : u] + idx;
-        const uint32_t x = g.adj_out[li];
-        if (a.stats) bytes += 12;
-        hit = (c.thr <= 1 || g.nsig_out[li].x >= c.thr) && adj_has(g, sel ? u : wv, x, bytes);
+      n -= half;
+    }
+    uint64_t p[IPT];
+    uint32_t own[IPT], oth[IPT], li[IPT], x[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      const uint32_t pj = min(pi[j], span - 1);
+      p[j] = (uint64_t)plo + pj;
+      const uint64_t pbase = lds ? s_off[pj] : a.work_off[p[j]];
+      const uint32_t u = a.rows[p[j] * L + c.pa], wv = a.rows[p[j] * L + c.pb];
+      const bool sel = a.minsel[p[j]] != 0;
+      own[j] = sel ? wv : u;
+      oth[j] = sel ? u : wv;
+      li[j] = g.off_out[own[j]] + (uint32_t)(w[j] - pbase);
+    }
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) x[j] = in[j] ? g.adj_out[li[j]] : 0u;
+    bool hit[IPT];
+    if (g.hub_id) {
+      int32_t h[IPT];
+      uint64_t bw[IPT];
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) h[j] = g.hub_id[oth[j]];
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) bw[j] = (in[j] && h[j] >= 0) ? g.bits[(uint64_t)h[j] * g.words + (x[j] >> 6)] : 0;
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        if (!in[j]) {
+          hit[j] = false;
+        } else if (h[j] >= 0) {
+          hit[j] = (bw[j] >> (x[j] & 63)) & 1ull;
+          if (a.stats) bytes += 12;
+        } else {
+          const uint32_t lo2 = g.off_out[oth[j]], hi2 = g.off_out[oth[j] + 1];
+          hit[j] = member_safe(g.adj_out, lo2, hi2, x[j]);
+          if (a.stats) bytes += 12 + bs_bytes(hi2 - lo2);
+        }
       }
-      if (c.mode == 2) {
-        add_runs(a.nsurv, p, hit);  // all lanes; items past w1 never hit
-      } else if (hit) {
-        acc -= c.mode == 0 ? (uint64_t)a.nsurv[p] : 1ull;
+    } else {
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        hit[j] = false;
+        if (!in[j]) continue;
+        const uint32_t lo2 = g.off_out[oth[j]], hi2 = g.off_out[oth[j] + 1];
+        hit[j] = member_safe(g.adj_out, lo2, hi2, x[j]);
+        if (a.stats) bytes += 8 + bs_bytes(hi2 - lo2);
       }
+    }
+    // a common neighbour of two distinct vertices has degree >= 2, so only
+    // higher thresholds need the degree
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+      if (a.stats && in[j]) bytes += 4;
+      if (c.thr > 2 && hit[j]) {
+        hit[j] = g.nsig_out[li[j]].x >= c.thr;
+        if (a.stats) bytes += 8;
+      }
+    }
+    if (c.mode == 2) {
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) add_runs(a.nsurv, p[j], hit[j]);  // all lanes; items past w1 never hit
+    } else {
+#pragma unroll
+      for (int j = 0; j < IPT; ++j)
+        if (hit[j]) acc -= c.mode == 0 ? (uint64_t)a.nsurv[p[j]] : 1ull;
     }
     __syncthreads();
   }
   if (c.mode != 2) add_wave(&a.counters[0], acc);
   if (a.stats) add_wave(&a.counters[3], bytes);
+}
+
+// Chain level, shared part from the two-hop index: per parent prefix (rows
+// of L words), the sum of |N(u) & N(r_i)| over its positions i != ch_u
+// whose degree takes completions away.
+template <int L>
+__global__ __launch_bounds__(BLOCK) void k_chain_shared_h2(GView g, KPlan P, const uint32_t* __restrict__ rows,
+                                                          uint64_t n, uint32_t* __restrict__ shared) {
+  uint64_t dummy = 0;
+  for (uint64_t p = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (uint64_t)gridDim.x * BLOCK) {
+    const uint32_t u = rows[p * L + P.ch_u];
+    uint32_t t = 0;
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+      if (i == (int)P.ch_u) continue;
+      const uint32_t w = rows[p * L + i];
+      if (P.ch_tb <= 1 || g.sig[2 * (uint64_t)w] >= P.ch_tb) t += two_hop(g, u, w, dummy);
+    }
+    shared[p] = t;
+  }
 }
 
 // Chain tables for thresholds (ta, tb): H[u] = sum over x in N(u) with
@@ -1115,7 +1213,13 @@ __global__ __launch_bounds__(BLOCK) void k_chain_row(GView g, KPlan P, const uin
     const uint32_t v = r[L - 1];
     const uint32_t dv = g.off_out[v + 1] - g.off_out[v];
     const bool need = P.ch_tb <= 1 || d[L - 1] >= P.ch_tb;
-    minlen[p] = need ? min(du, dv) : 0u;
+    if (g.h2_off && P.ch_ta <= 2) {
+      // the intersection with v from the two-hop index; no second pass
+      if (need) acc -= two_hop(g, u, v, dummy);
+      minlen[p] = 0u;
+    } else {
+      minlen[p] = need ? min(du, dv) : 0u;
+    }
     minsel[p] = (uint8_t)(du <= dv ? 0u : 1u);
     if (stats) {
       // merge model: the row's expansion by u's list, each of its na
@@ -1360,7 +1464,7 @@ struct Matcher {
       ensure_rows_exact(out, std::min<uint64_t>(chunk, W), L + 1);
       chunk = std::min<uint64_t>(chunk, out.cap);
     }
-    const bool pair_common = mode == M_PAIR && kp.pr_a != kp.pr_b;
+    const bool pair_common = mode == M_PAIR && kp.pr_a != kp.pr_b && !(g.h2_off && kp.pr_tab <= 2);
     if (pair_common) HIPCHECK(hipMemsetAsync(in.nsurv, 0, n * sizeof(uint32_t), c.stream));
     for (uint64_t w0 = 0; w0 < W; w0 += chunk) {
       if (past_deadline()) return;
@@ -1462,7 +1566,8 @@ struct Matcher {
 
   // Counts common neighbours over the work planned in in.minlen / in.minsel
   // (k_common; mode and weights as in ca), in chunks of the work space.
-  void common_pass(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in, CommonArgs ca, uint32_t* weight) {
+  void common_pass(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in, CommonArgs ca, uint32_t* weight,
+                   int cls = 9) {
     const uint64_t W = scan(in, n);
     if (W == 0 || add_work(W)) return;
     for (uint64_t w0 = 0; w0 < W; w0 += MAX_CHUNK) {
@@ -1494,7 +1599,7 @@ struct Matcher {
       CUTS_DISPATCH_L(L, PC)
 #undef PC
       HIPCHECK(hipGetLastError());
-      pend(9, 0, true);
+      pend(cls, 0, true);
     }
   }
 
@@ -1503,6 +1608,16 @@ struct Matcher {
   // positions i of |N(u) & N(r_i)| (those whose degree takes completions
   // away), shared by all its children; into in.nsurv.
   void chain_shared(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in) {
+    if (g.h2_off && kp.ch_ta <= 2) {
+      const uint32_t grid = grid_for(n, BLOCK, grid_cap);
+      pbegin(false);
+#define CS(LL) hipLaunchKernelGGL((k_chain_shared_h2<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.nsurv)
+      CUTS_DISPATCH_PAIR(L, CS)
+#undef CS
+      HIPCHECK(hipGetLastError());
+      pend(1, (double)n * (4.0 * L + 4.0), false);
+      return;
+    }
     HIPCHECK(hipMemsetAsync(in.nsurv, 0, n * sizeof(uint32_t), c.stream));
     for (uint32_t i = 0; i < L && !aborted; ++i) {
       if (i == kp.ch_u) continue;
@@ -1528,7 +1643,7 @@ struct Matcher {
 #undef CR
     HIPCHECK(hipGetLastError());
     pend(8, 0, true);
-    common_pass(L, rows, n, in, CommonArgs{kp.ch_u, L - 1, kp.ch_ta, 1u}, nullptr);
+    if (!(g.h2_off && kp.ch_ta <= 2)) common_pass(L, rows, n, in, CommonArgs{kp.ch_u, L - 1, kp.ch_ta, 1u}, nullptr, 10);
   }
 
   void run() {
@@ -1537,6 +1652,7 @@ struct Matcher {
     kp = make_kplan(plan, dg.symmetric);
     g = view_of(dg);
     if (o.flags & 32u) g.hub_id = nullptr;  // CUTS_FLAG_NO_BITMAP
+    if (o.flags & 512u) g.h2_off = nullptr;  // CUTS_FLAG_NO_TWO_HOP
     write = (o.flags & 4u) != 0;
     prof = (o.flags & 16u) != 0;
     closed_last = !write && !(o.flags & 1u) && kp.sc_ok && k >= 2;
@@ -1805,6 +1921,8 @@ DeviceGraph::~DeviceGraph() {
   for (auto& kv : chain_tables) (void)hipFree(kv.second);
   if (hub_id) (void)hipFree(hub_id);
   if (bits) (void)hipFree(bits);
+  for (void* p : {(void*)h2_off, (void*)h2_v, (void*)h2_c})
+    if (p) (void)hipFree(p);
   (void)hipSetDevice(prev);
 }
 
@@ -1918,6 +2036,33 @@ void upload_graph(Context& c, const Graph& g, DeviceGraph& dg) {
       HIPCHECK(hipGetLastError());
       HIPCHECK(hipStreamSynchronize(c.stream));
       (void)hipFree(dh);
+    }
+  }
+  // Two-hop index: built on the host when the wedges (ordered neighbour
+  // pairs of every vertex, an upper bound on the entries) fit 1/16 of free
+  // HBM at 8 bytes each, and at most 2^28 of them.
+  if (g.symmetric && g.V >= 2) {
+    const auto t0 = std::chrono::steady_clock::now();
+    uint64_t wedges = 0;
+    for (uint32_t v = 0; v < g.V; ++v) {
+      const uint64_t d = g.out_off[v + 1] - g.out_off[v];
+      wedges += d * (d ? d - 1 : 0);
+    }
+    size_t fr = 0, tot = 0;
+    HIPCHECK(hipMemGetInfo(&fr, &tot));
+    if (wedges <= std::min<uint64_t>(fr / 16 / 8, 1ull << 28)) {
+      const uint32_t V = g.V;
+      std::vector<uint64_t> off;
+      std::vector<uint32_t> hv, hc;
+      build_two_hop(g, off, hv, hc);
+      dg.h2_entries = off[V];
+      dev_alloc(dg.h2_off, V + 1);
+      HIPCHECK(hipMemcpy(dg.h2_off, off.data(), (V + 1) * 8, hipMemcpyHostToDevice));
+      dev_alloc(dg.h2_v, hv.size());
+      HIPCHECK(hipMemcpy(dg.h2_v, hv.data(), hv.size() * 4, hipMemcpyHostToDevice));
+      dev_alloc(dg.h2_c, hc.size());
+      HIPCHECK(hipMemcpy(dg.h2_c, hc.data(), hc.size() * 4, hipMemcpyHostToDevice));
+      dg.h2_build_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
   }
 }

@@ -347,18 +347,20 @@ class Judged:
 
 
 def judge(ctx: Context, dg, queries: Sequence[Tuple[int, List[Tuple[int, int]]]], heuristic_plans,
-          heuristic_results, choices: Sequence[OrderChoice], roots=None) -> List[Judged]:
+          heuristic_results, choices: Sequence[OrderChoice], roots=None, flags: int = 0) -> List[Judged]:
     """The score order judges every learned proposal: a proposal is kept
     only if its own run, capped at the score order's work, finishes with
     fewer candidates examined.  A kept order's count must equal the score
-    order's (order never changes a count); anything else raises."""
+    order's (order never changes a count); anything else raises.  `flags`
+    are the match flags the score order's results were produced with (work
+    depends on the execution mode)."""
     out = []
     for i, ((k, edges), hp, hr, ch) in enumerate(zip(queries, heuristic_plans, heuristic_results, choices)):
         rt = None if roots is None else roots[i]
         kept = None
         if ch.learned and list(ch.order) != list(hp.order):
             lp = Plan(Graph.from_edges(k, edges), ch.order)
-            lr = ctx.match(dg, lp, roots=rt, max_work=hr.work)
+            lr = ctx.match(dg, lp, roots=rt, max_work=hr.work, flags=flags)
             if not lr.work_capped and lr.work < hr.work:
                 if lr.count != hr.count:
                     raise AssertionError(f"query {i}: learned order counted {lr.count}, score order {hr.count}")

@@ -15,7 +15,7 @@ from typing import Dict, List, Tuple
 
 import numpy as np
 
-from . import Graph
+from . import FLAG_NO_CHAIN, FLAG_NO_PAIR, Graph
 
 Edge = Tuple[int, int]
 
@@ -150,6 +150,10 @@ def quotas(n: int, kmin: int = KMIN, kmax: int = KMAX) -> Dict[int, int]:
     return {k: n // m + (1 if k - kmin < n % m else 0) for k in range(kmin, kmax + 1)}
 
 
+# execution mode of the admission rule (see admit)
+ADMIT_FLAGS = FLAG_NO_PAIR | FLAG_NO_CHAIN
+
+
 def admit(ctx, dg, g: Graph, n: int, seed: int = 0, max_work: int = MAX_WORK, max_rows: int = MAX_ROWS,
           kmin: int = KMIN, kmax: int = KMAX, max_tries_per_k: int = 3000, log=None):
     """Per size k, the first quotas(n)[k] candidates of size k from
@@ -164,7 +168,10 @@ def admit(ctx, dg, g: Graph, n: int, seed: int = 0, max_work: int = MAX_WORK, ma
     out = []
 
     def run(p, j):
-        r = ctx.match(dg, p, roots=root_sample(g.V, 1 << j), max_work=max_work)
+        # work (candidates examined) depends on the execution mode; the rule
+        # is applied with the last two levels expanded level by level, so the
+        # admitted set does not move when faster modes are added
+        r = ctx.match(dg, p, roots=root_sample(g.V, 1 << j), max_work=max_work, flags=ADMIT_FLAGS)
         return (not r.work_capped and materialised_rows(r.level_sizes) <= max_rows), r
 
     for i, k, ne, edges in batch_candidates(g, seed):

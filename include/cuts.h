@@ -53,6 +53,14 @@ typedef struct {
 } cuts_graph_view;
 int cuts_graph_view_of(const cuts_graph *g, cuts_graph_view *view);
 void cuts_graph_free(cuts_graph *g);
+/* Two-hop index (no reference counterpart): the table a device upload of a
+ * symmetric graph builds when its wedges fit the budget.  For every vertex
+ * u, the vertices v != u sharing a neighbour with u, ascending
+ * (v[off[u] .. off[u+1]]), and |N(u) & N(v)| for each (c).  off holds V+1
+ * entries; v and c hold cap entries.  *entries receives the table size;
+ * CUTS_E_ARG when it exceeds cap (call with cap = 0 to size the buffers). */
+int cuts_graph_two_hop(const cuts_graph *g, uint64_t *off, uint32_t *v, uint32_t *c, uint64_t cap,
+                       uint64_t *entries);
 
 /* ---- synthetic inputs (no reference counterpart) ---------------------- */
 int cuts_gen_chung_lu(uint32_t n, uint64_t m, double gamma, uint32_t max_deg, uint64_t seed,
@@ -93,7 +101,8 @@ enum {
   CUTS_FLAG_NO_BITMAP = 1u << 5,   /* lower-bound searches even in hub lists */
   CUTS_FLAG_NO_PREFIX_BASE = 1u << 6, /* fused closed form fully per work item */
   CUTS_FLAG_NO_PAIR = 1u << 7,        /* never count the last two positions together */
-  CUTS_FLAG_NO_CHAIN = 1u << 8        /* not as a chain either (k-1 hanging on k-2) */
+  CUTS_FLAG_NO_CHAIN = 1u << 8,       /* not as a chain either (k-1 hanging on k-2) */
+  CUTS_FLAG_NO_TWO_HOP = 1u << 9      /* intersections by list walks even where the two-hop index exists */
 };
 
 /* Kernel classes reported by CUTS_FLAG_PROFILE. */
@@ -108,7 +117,8 @@ enum {
   CUTS_K_CLOSED = 7,   /* closed-form last level over stored rows */
   CUTS_K_LAST2 = 8,    /* the last two positions counted together (pair / chain levels) */
   CUTS_K_COMMON = 9,   /* intersection sizes |N(u) & N(w)| those levels need */
-  CUTS_K_CLASSES = 10
+  CUTS_K_CHAIN_COMMON = 10, /* the chain level's per-row intersection with its newest vertex */
+  CUTS_K_CLASSES = 11
 };
 
 typedef struct {
@@ -144,9 +154,9 @@ typedef struct {
   uint32_t complete;        /* 0 if time_limit_ms stopped the run early */
   /* CUTS_FLAG_PROFILE: per kernel class, summed HIP-event time, launches and
    * algorithmic (demand) bytes as defined next to each kernel */
-  double kernel_ms[10];
-  uint64_t kernel_launches[10];
-  double kernel_bytes[10];
+  double kernel_ms[11];
+  uint64_t kernel_launches[11];
+  double kernel_bytes[11];
   uint64_t roots_done;      /* level-0 vertices whose subtrees are counted */
   uint32_t last_root;       /* data vertex of the last of them */
   uint32_t work_capped;     /* 1 if max_work stopped the run (count invalid) */
@@ -158,7 +168,7 @@ typedef struct {
    * levels below k-1 (written and read back), in the class that performs the
    * expansion.  The sum does not depend on kernels or modes: it equals the
    * CPU trie walk's figure for the same graph, query and order. */
-  double merge_bytes[10];
+  double merge_bytes[11];
 } cuts_match_result;
 
 void cuts_match_options_default(cuts_match_options *o);

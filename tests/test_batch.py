@@ -81,7 +81,7 @@ def test_level_batch_pair_level_shapes(ctx, seed):
         assert ctx.match(dg, cuts_amd.Plan(cuts_amd.Graph.from_edges(k, e)), flags=cuts_amd.FLAG_NO_PAIR).level_sizes \
             == x.level_sizes, e
     for flags, kw in ((0, {}), (cuts_amd.FLAG_NO_PAIR, {}), (cuts_amd.FLAG_NO_BITMAP, {}),
-                      (0, dict(item_work=32, vchunk=5))):
+                      (cuts_amd.FLAG_NO_TWO_HOP, {}), (0, dict(item_work=32, vchunk=5))):
         print(f"batch seed={seed} flags={flags} {kw}", flush=True)
         r = cuts_amd.Batch(ctx, dg, plans, flags=flags, **kw).run()
         assert r.counts == [x.count for x in singles], flags
@@ -99,7 +99,8 @@ def test_level_batch_wave_sums_beyond_32_bits(ctx):
     plans.append(cuts_amd.Plan(cuts_amd.Graph.from_edges(4, [(0, 1), (0, 2), (0, 3)])))
     singles = [ctx.match(dg, p) for p in plans]
     assert singles[0].count == singles[1].count > 1 << 36
-    for flags in (0, cuts_amd.FLAG_NO_PAIR, cuts_amd.FLAG_NO_PAIR | cuts_amd.FLAG_NO_PREFIX_BASE):
+    for flags in (0, cuts_amd.FLAG_NO_TWO_HOP, cuts_amd.FLAG_NO_PAIR,
+                  cuts_amd.FLAG_NO_PAIR | cuts_amd.FLAG_NO_PREFIX_BASE):
         print(f"batch flags={flags}", flush=True)
         r = cuts_amd.Batch(ctx, dg, plans, flags=flags, arena_bytes=4 << 30).run()
         assert r.counts == [x.count for x in singles], flags

@@ -40,6 +40,7 @@ MODES = [
     dict(flags=cuts_amd.FLAG_NO_PREFIX_BASE | cuts_amd.FLAG_NO_BITMAP),
     dict(trunks=3),
     dict(flags=cuts_amd.FLAG_NO_PAIR),
+    dict(flags=cuts_amd.FLAG_NO_TWO_HOP),
     dict(budget=4096),  # a few hundred rows per level: many chunks
 ]
 

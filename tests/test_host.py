@@ -170,3 +170,44 @@ def test_product_order_equals_oracle_restatement():
         q = cuts_amd.Graph.from_edges(k, edges)
         qa = oracle.qmatrix(k, oracle.undirected(edges))
         assert cuts_amd.Plan(q).order == oracle.heuristic_order(qa)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_two_hop_index_equals_adjacency_square(seed):
+    """The two-hop index (built at device upload; inspected here on the
+    host) equals the off-diagonal support and entries of A @ A: for every
+    vertex u, the vertices sharing a neighbour with u, ascending, with the
+    number shared."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(2, 300))
+    g = cuts_amd.Graph.chung_lu(n, int(rng.integers(n, 4 * n)), float(rng.uniform(2.1, 2.9)), 0, seed)
+    v = g.view()
+    off, adj = v["out_off"].astype(np.int64), v["out_adj"].astype(np.int64)
+    A = sp.csr_matrix((np.ones(len(adj), dtype=np.int64), (np.repeat(np.arange(v["V"]), np.diff(off)), adj)),
+                      shape=(v["V"], v["V"]))
+    A2 = (A @ A).tolil()
+    A2.setdiag(0)
+    A2 = A2.tocsr()
+    A2.eliminate_zeros()
+    A2.sort_indices()
+    h_off, h_v, h_c = g.two_hop()
+    assert np.array_equal(h_off.astype(np.int64), A2.indptr.astype(np.int64))
+    assert np.array_equal(h_v.astype(np.int64), A2.indices.astype(np.int64))
+    assert np.array_equal(h_c.astype(np.int64), A2.data.astype(np.int64))
+
+
+def test_two_hop_index_small_cases():
+    """Closed forms: a star's leaves share exactly the centre; in K4 every
+    pair shares the two other vertices; a path of three has one two-hop
+    pair."""
+    star = cuts_amd.Graph.from_edges(5, [(0, 1), (0, 2), (0, 3), (0, 4)])
+    off, v, c = star.two_hop()
+    assert off.tolist() == [0, 0, 3, 6, 9, 12]
+    assert v[:3].tolist() == [2, 3, 4] and set(c.tolist()) == {1}
+    k4 = cuts_amd.Graph.from_edges(4, [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)])
+    off, v, c = k4.two_hop()
+    assert off.tolist() == [0, 3, 6, 9, 12] and set(c.tolist()) == {2}
+    path = cuts_amd.Graph.from_edges(3, [(0, 1), (1, 2)])
+    off, v, c = path.two_hop()
+    assert off.tolist() == [0, 1, 1, 2] and v.tolist() == [2, 0] and c.tolist() == [1, 1]

@@ -13,9 +13,11 @@ sum_{x in A} |N(x) - prefix - x| completions, computed from per-vertex
 tables and one intersection N(u) & N(r_i) per other prefix vertex (those of
 older positions shared by the rows of one parent prefix).
 
-Counts must equal the oracle's and the same run without these levels; level
-sizes and merge-model bytes must equal the CPU trie walk's, which
-materialises every level.
+The intersections come from the two-hop index built at upload (small
+graphs) or from flattened list walks (FLAG_NO_TWO_HOP, and graphs too large
+for the index).  Counts must equal the oracle's and the same run without
+these levels; level sizes and merge-model bytes must equal the CPU trie
+walk's, which materialises every level.
 """
 import random
 
@@ -97,8 +99,8 @@ def test_pair_level_vs_oracle_and_cpu_trie(ctx, name, seed):
     assert cpu.count == want
     dg = ctx.upload(g)
     prof = cuts_amd.FLAG_PROFILE
-    for flags in (0, cuts_amd.FLAG_NO_BITMAP, cuts_amd.FLAG_NO_PAIR, cuts_amd.FLAG_NO_CHAIN,
-                  cuts_amd.FLAG_NO_PAIR | cuts_amd.FLAG_NO_CHAIN):
+    for flags in (0, cuts_amd.FLAG_NO_BITMAP, cuts_amd.FLAG_NO_TWO_HOP, cuts_amd.FLAG_NO_TWO_HOP | cuts_amd.FLAG_NO_BITMAP,
+                  cuts_amd.FLAG_NO_PAIR, cuts_amd.FLAG_NO_CHAIN, cuts_amd.FLAG_NO_PAIR | cuts_amd.FLAG_NO_CHAIN):
         r = ctx.match(dg, p, flags=flags | prof)
         assert r.count == want, (name, flags)
         assert r.level_sizes == [cpu.level[i] for i in range(k)], (name, flags)
@@ -108,7 +110,8 @@ def test_pair_level_vs_oracle_and_cpu_trie(ctx, name, seed):
             (chain_eligible(p) and not flags & cuts_amd.FLAG_NO_CHAIN)
         assert not used or expect, (name, flags, p.order)
         assert used or not expect or r.level_sizes[k - 3] == 0, (name, flags, p.order)
-    for m in (dict(trunks=5), dict(budget=2048), dict(flags=cuts_amd.FLAG_NO_XCD)):
+    for m in (dict(trunks=5), dict(budget=2048), dict(flags=cuts_amd.FLAG_NO_XCD),
+              dict(budget=2048, flags=cuts_amd.FLAG_NO_TWO_HOP)):
         assert ctx.match(dg, p, **m).count == want, (name, m)
 
 
@@ -145,5 +148,5 @@ def test_pair_level_random_queries(ctx, seed):
     want = oracle.count(og, oracle.qmatrix(k, oracle.undirected(edges)))
     p = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges))
     dg = ctx.upload(g)
-    for flags in (0, cuts_amd.FLAG_NO_PAIR | cuts_amd.FLAG_NO_CHAIN, cuts_amd.FLAG_NO_BITMAP):
+    for flags in (0, cuts_amd.FLAG_NO_PAIR | cuts_amd.FLAG_NO_CHAIN, cuts_amd.FLAG_NO_BITMAP, cuts_amd.FLAG_NO_TWO_HOP):
         assert ctx.match(dg, p, flags=flags).count == want, (edges, flags)
