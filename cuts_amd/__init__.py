@@ -34,7 +34,7 @@ FLAG_NO_CHAIN = 256
 FLAG_NO_TWO_HOP = 512
 MAX_QUERY = 16
 KERNEL_CLASSES = ["level0", "plan", "scan", "tile_index", "expand_write", "expand_count", "expand_fused",
-                  "closed_form", "last_two", "common", "chain_common"]
+                  "closed_form", "pair", "common", "chain"]
 _NCLASS = len(KERNEL_CLASSES)
 
 _u32p = C.POINTER(C.c_uint32)

@@ -33,6 +33,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "batch.hpp"
@@ -448,6 +449,69 @@ __global__ __launch_bounds__(BLOCK) void k_cnt_table(GView g, uint32_t thr, uint
   }
 }
 
+// ------------------------------------------------------------ chain level
+struct ChainTables {
+  const uint64_t* H;           // 2V, k_chain_table
+  const uint32_t *ta, *tb;     // degree-threshold tables (null: plain degree)
+};
+
+// Completions of the last two positions as a chain for the prefix r[0..N)
+// of positions 0..k-3 (N = k-2), u = r[ch_u]:
+//   H(u) - sum_{r_i in N(u), deg >= ta} cnt_tb(r_i) - [deg u >= tb] |A|
+//        - sum_{i != ch_u, deg(r_i) >= tb} (|N(u) & N(r_i)| - c_i)
+// with |A| = cnt_ta(u) - |{i : r_i in N(u), deg(r_i) >= ta}| (returned in
+// na) and c_i the prefix vertices inside N(u) & N(r_i) meeting ta.  The
+// intersections of positions < N-1 come summed in `shared`; the one with
+// r[N-1] is looked up in the two-hop index when h2, else left out (a later
+// pass subtracts it).  Adds the demand bytes of its loads to bytes and, with
+// stats, to mb the merge-model bytes of expanding the prefix by u's list and
+// each of its |A| children by its own list.
+template <int N>
+__device__ __forceinline__ int64_t chain_closed(const GView& g, const KPlan& P, const uint32_t (&r)[N],
+                                                const ChainTables& t, uint32_t shared, bool h2, int64_t& na,
+                                                bool stats, uint64_t& mb, uint64_t& bytes) {
+  uint64_t& dummy = bytes;
+  bytes += 4 * N + 8 + 8 + (t.ta ? 4 : 0);  // degrees of the row, u's offsets, H(u), cnt_ta(u)
+  uint32_t u = 0, d[N];
+#pragma unroll
+  for (int l = 0; l < N; ++l) {
+    if (l == (int)P.ch_u) u = r[l];
+    d[l] = g.sig[2 * (uint64_t)r[l]];
+  }
+  const uint32_t du = g.off_out[u + 1] - g.off_out[u];
+  // prefix vertices inside N(u) that meet ta
+  uint32_t inu = 0;
+#pragma unroll
+  for (int l = 0; l < N; ++l) {
+    const bool in = l != (int)P.ch_u && (((P.qadj[P.ch_u] >> l) & 1) || adj_has(g, u, r[l], dummy));
+    if (in && d[l] >= P.ch_ta) inu |= 1u << l;
+  }
+  na = (int64_t)(t.ta ? t.ta[u] : du) - __popc(inu);
+  int64_t c = (int64_t)t.H[u];
+  uint64_t dsum = 0;
+#pragma unroll
+  for (int l = 0; l < N; ++l)
+    if ((inu >> l) & 1) {
+      c -= t.tb ? t.tb[r[l]] : g.off_out[r[l] + 1] - g.off_out[r[l]];
+      dsum += g.off_out[r[l] + 1] - g.off_out[r[l]];
+      bytes += t.tb ? 4 : 8;
+    }
+  if (P.ch_tb <= 1 || du >= P.ch_tb) c -= na;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    if (i == (int)P.ch_u || d[i] < P.ch_tb) continue;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      if (j == i || !((inu >> j) & 1)) continue;
+      if (((P.qadj[i] >> j) & 1) || adj_has(g, r[i], r[j], dummy)) ++c;
+    }
+  }
+  c -= (int64_t)shared;
+  if (h2 && (P.ch_tb <= 1 || d[N - 1] >= P.ch_tb)) c -= two_hop(g, u, r[N - 1], dummy);
+  if (stats) mb += 8 + 4ull * du + 24ull * (uint64_t)na + 4ull * (t.H[g.V + u] - dsum);
+  return c;
+}
+
 // ------------------------------------------------------------ expansion
 // Adds to cnt[p] the number of this wave's lanes with `ok` set whose item
 // belongs to prefix p.  Lanes hold consecutive work items, so the items of a
@@ -468,7 +532,7 @@ __device__ __forceinline__ void add_runs(uint32_t* __restrict__ cnt, uint64_t p,
   if (head && mine) atomicAdd(&cnt[p], (uint32_t)__popcll(mine));
 }
 
-enum { M_WRITE = 0, M_COUNT = 1, M_FUSED = 2, M_PAIR = 3 };
+enum { M_WRITE = 0, M_COUNT = 1, M_FUSED = 2, M_PAIR = 3, M_CHAIN = 4 };
 
 struct ExpandArgs {
   const uint32_t* rows;      // n rows of L words
@@ -484,6 +548,8 @@ struct ExpandArgs {
   const uint4* pbase;        // M_PAIR: per-prefix parts of |A|, |B|, |A & B|
   uint32_t* nsurv;           // M_PAIR with u != w: survivors per prefix (zero at launch)
   uint32_t* out_parent;      // M_WRITE: parent prefix of every row written (or null)
+  const uint32_t* shared;    // M_CHAIN: per prefix, its older positions' intersections with u
+  ChainTables ctab;          // M_CHAIN
   uint32_t xcd;              // 1: XCD-contiguous tile mapping
   unsigned int* tilectr;     // 8 tile counters, zero at launch
   uint32_t stats;            // 1: accumulate demand bytes into counters[3]
@@ -651,6 +717,23 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
         if (MODE == M_COUNT) {
 #pragma unroll
           for (int j = 0; j < G; ++j) acc += ok[j];
+        } else if (MODE == M_CHAIN) {
+          // a survivor v completes the prefix of positions 0..k-3, whose
+          // chain of the last two positions is counted in closed form
+#pragma unroll
+          for (int j = 0; j < G; ++j) {
+            if (!ok[j]) continue;
+            ++valid;
+            uint32_t r[L + 1];
+#pragma unroll
+            for (int l = 0; l < L; ++l) r[l] = row[j][l];
+            r[L] = v[j];
+            const uint32_t sh = a.shared[plo + min(pi[j], span - 1)];
+            int64_t na = 0;
+            acc += (uint64_t)chain_closed<L + 1>(g, P, r, a.ctab, sh, true, na, a.stats != 0, mb, bytes);
+            pairs += (uint64_t)na;
+            if (a.stats) mb += 16;  // the survivor's node, written and read back
+          }
         } else if (MODE == M_PAIR) {
           // a survivor v completes the prefix of positions 0..k-3; its
           // |A| * |B| - |A & B| completions take the prefix's bases minus v's
@@ -909,12 +992,12 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
   }
   if (MODE != M_WRITE) {
     add_wave(&a.counters[0], acc);
-    if (MODE == M_FUSED || MODE == M_PAIR) add_wave(&a.counters[1], valid);
-    if (MODE == M_PAIR) add_wave(&a.counters[2], pairs);
+    if (MODE == M_FUSED || MODE == M_PAIR || MODE == M_CHAIN) add_wave(&a.counters[1], valid);
+    if (MODE == M_PAIR || MODE == M_CHAIN) add_wave(&a.counters[2], pairs);
   }
   if (a.stats) {
     add_wave(&a.counters[3], bytes);
-    if (MODE == M_FUSED || MODE == M_PAIR) add_wave(&a.counters[8], mb);
+    if (MODE == M_FUSED || MODE == M_PAIR || MODE == M_CHAIN) add_wave(&a.counters[8], mb);
   }
 }
 
@@ -1146,90 +1229,42 @@ __global__ __launch_bounds__(BLOCK) void k_chain_table(GView g, uint32_t ta, con
   }
 }
 
-struct ChainTables {
-  const uint64_t* H;           // 2V, k_chain_table
-  const uint32_t *ta, *tb;     // degree-threshold tables (null: plain degree)
-};
-
-// Chain level over rows of L = k-2 words (positions 0..k-3), u = r[ch_u],
-// v = r[L-1] (the newest position):
-//   count(P) = H(u) - sum_{r_i in N(u), deg >= ta} cnt_tb(r_i) - [deg u >= tb] |A|
-//              - sum_{i != ch_u, deg(r_i) >= tb} (|N(u) & N(r_i)|_ta - c_i)
-// with |A| = cnt_ta(u) - |{i : r_i in N(u), deg(r_i) >= ta}| and c_i the
-// prefix vertices inside N(u) & N(r_i) meeting ta.  Here everything except
-// the intersections is added; those of older positions were summed per
-// parent prefix (shared[parent]) and the one with v is left to k_common
-// (minlen/minsel set here).  Level k-2 size += |A|.
+// Chain level over stored rows of L = k-2 words (positions 0..k-3):
+// chain_closed per row; the intersection with the newest vertex v = r[L-1]
+// from the two-hop index, or left to k_common (minlen/minsel set here).
+// Level k-2 size += |A|.
 template <int L>
 __global__ __launch_bounds__(BLOCK) void k_chain_row(GView g, KPlan P, const uint32_t* __restrict__ rows, uint64_t n,
                                                     const uint32_t* __restrict__ parent,
                                                     const uint32_t* __restrict__ shared, ChainTables t,
                                                     uint32_t* __restrict__ minlen, uint8_t* __restrict__ minsel,
                                                     unsigned long long* __restrict__ counters, int stats) {
-  uint64_t acc = 0, na_sum = 0, mb = 0;
-  uint64_t dummy = 0;
+  uint64_t acc = 0, na_sum = 0, mb = 0, bytes = 0;
+  const bool h2 = g.h2_off && P.ch_ta <= 2;
   for (uint64_t p = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (uint64_t)gridDim.x * BLOCK) {
-    uint32_t r[L], d[L];
+    uint32_t r[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) r[l] = rows[p * L + l];
+    int64_t na = 0;
+    acc += (uint64_t)chain_closed<L>(g, P, r, t, shared[parent[p]], h2, na, stats != 0, mb, bytes);
+    bytes += 4 * L + 8;  // the row and its parent's shared sum
+    na_sum += (uint64_t)na;
     uint32_t u = 0;
 #pragma unroll
     for (int l = 0; l < L; ++l)
       if (l == (int)P.ch_u) u = r[l];
-#pragma unroll
-    for (int l = 0; l < L; ++l) d[l] = g.sig[2 * (uint64_t)r[l]];
-    const uint32_t du = g.off_out[u + 1] - g.off_out[u];
-    // prefix vertices inside N(u) that meet ta
-    uint32_t inu = 0;
-#pragma unroll
-    for (int l = 0; l < L; ++l) {
-      const bool in = l != (int)P.ch_u && (((P.qadj[P.ch_u] >> l) & 1) || adj_has(g, u, r[l], dummy));
-      if (in && d[l] >= P.ch_ta) inu |= 1u << l;
-    }
-    const int64_t na = (int64_t)(t.ta ? t.ta[u] : du) - __popc(inu);
-    int64_t c = (int64_t)t.H[u];
-    uint64_t dsum = 0;
-#pragma unroll
-    for (int l = 0; l < L; ++l)
-      if ((inu >> l) & 1) {
-        c -= t.tb ? t.tb[r[l]] : g.off_out[r[l] + 1] - g.off_out[r[l]];
-        dsum += g.off_out[r[l] + 1] - g.off_out[r[l]];
-      }
-    if (P.ch_tb <= 1 || du >= P.ch_tb) c -= na;
-    // c_i: prefix vertices r_j inside N(u) & N(r_i), for every r_i that
-    // takes completions away
-#pragma unroll
-    for (int i = 0; i < L; ++i) {
-      if (i == (int)P.ch_u || d[i] < P.ch_tb) continue;
-#pragma unroll
-      for (int j = 0; j < L; ++j) {
-        if (j == i || !((inu >> j) & 1)) continue;
-        if (((P.qadj[i] >> j) & 1) || adj_has(g, r[i], r[j], dummy)) ++c;
-      }
-    }
-    c -= (int64_t)shared[parent[p]];
-    acc += (uint64_t)c;
-    na_sum += (uint64_t)na;
     const uint32_t v = r[L - 1];
-    const uint32_t dv = g.off_out[v + 1] - g.off_out[v];
-    const bool need = P.ch_tb <= 1 || d[L - 1] >= P.ch_tb;
-    if (g.h2_off && P.ch_ta <= 2) {
-      // the intersection with v from the two-hop index; no second pass
-      if (need) acc -= two_hop(g, u, v, dummy);
-      minlen[p] = 0u;
-    } else {
-      minlen[p] = need ? min(du, dv) : 0u;
-    }
+    const uint32_t du = g.off_out[u + 1] - g.off_out[u], dv = g.off_out[v + 1] - g.off_out[v];
+    const bool need = P.ch_tb <= 1 || g.sig[2 * (uint64_t)v] >= P.ch_tb;
+    minlen[p] = (!h2 && need) ? min(du, dv) : 0u;
     minsel[p] = (uint8_t)(du <= dv ? 0u : 1u);
-    if (stats) {
-      // merge model: the row's expansion by u's list, each of its na
-      // children's node and expansion by the child's own list
-      mb += 8 + 4ull * du + 24ull * (uint64_t)na + 4ull * (t.H[g.V + u] - dsum);
-    }
   }
   add_wave(&counters[0], acc);
   add_wave(&counters[1], na_sum);
-  if (stats) add_wave(&counters[8], mb);
+  if (stats) {
+    add_wave(&counters[8], mb);
+    add_wave(&counters[3], bytes);
+  }
 }
 
 // ---------------------------------------------------------- dispatch
@@ -1338,7 +1373,7 @@ struct Matcher {
   KPlan kp;
   GView g;
   uint32_t k;
-  bool closed_last = false, fuse = false, write = false, pair = false, chain = false;
+  bool closed_last = false, fuse = false, write = false, pair = false, chain = false, chain_fused = false;
   const uint32_t* tbl = nullptr;
   PairTables ptbl{};
   ChainTables ctab{};
@@ -1414,7 +1449,7 @@ struct Matcher {
   void level(uint32_t L, const uint32_t* rows, uint64_t n) {
     if (n == 0) return;
     LevelBuf& in = c.lv[L - 1];
-    if (chain && L == k - 2) {
+    if (chain && !chain_fused && L == k - 2) {
       chain_rows(L, rows, n);
       return;
     }
@@ -1437,9 +1472,10 @@ struct Matcher {
     const bool use_cbase = fuse && L == k - 2 && kp.sc_pos < L && !(o.flags & 64u);
     int mode = M_WRITE;
     if (pair && L == k - 3) mode = M_PAIR;
+    else if (chain_fused && L == k - 3) mode = M_CHAIN;
     else if (L == k - 1 && !write) mode = M_COUNT;
     else if (L == k - 2 && fuse) mode = M_FUSED;
-    const int mclass = mode == M_WRITE ? 4 : (mode == M_COUNT ? 5 : (mode == M_FUSED ? 6 : 8));
+    const int mclass = mode == M_WRITE ? 4 : mode == M_COUNT ? 5 : mode == M_FUSED ? 6 : mode == M_PAIR ? 8 : 10;
     {
       const uint32_t grid = grid_for(n, BLOCK, grid_cap);
       unsigned long long* merge = prof ? c.counters + 8 : nullptr;
@@ -1489,6 +1525,8 @@ struct Matcher {
       a.pbase = mode == M_PAIR ? in.pbase : nullptr;
       a.nsurv = pair_common ? in.nsurv : nullptr;
       a.out_parent = (chain && mode == M_WRITE && L == k - 3) ? out.parent : nullptr;
+      a.shared = in.nsurv;
+      a.ctab = ctab;
       a.xcd = (o.flags & 8u) ? 0 : 1;
       a.tilectr = (unsigned int*)(c.counters + 4);  // 8 x uint32 in counters[4..7]
       HIPCHECK(hipMemsetAsync(a.tilectr, 0, 8 * sizeof(unsigned int), c.stream));
@@ -1530,6 +1568,13 @@ struct Matcher {
 #undef EC
         HIPCHECK(hipGetLastError());
         pend(5, 0, true);
+      } else if (mode == M_CHAIN) {
+        pbegin(true);
+#define EC2(LL) hipLaunchKernelGGL((k_expand<LL, M_CHAIN>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, a)
+        CUTS_DISPATCH_PAIR(L, EC2)
+#undef EC2
+        HIPCHECK(hipGetLastError());
+        pend(10, 0, true);
       } else if (mode == M_FUSED) {
         pbegin(true);
 #define EF(LL) hipLaunchKernelGGL((k_expand<LL, M_FUSED>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, a)
@@ -1566,8 +1611,7 @@ struct Matcher {
 
   // Counts common neighbours over the work planned in in.minlen / in.minsel
   // (k_common; mode and weights as in ca), in chunks of the work space.
-  void common_pass(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in, CommonArgs ca, uint32_t* weight,
-                   int cls = 9) {
+  void common_pass(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in, CommonArgs ca, uint32_t* weight) {
     const uint64_t W = scan(in, n);
     if (W == 0 || add_work(W)) return;
     for (uint64_t w0 = 0; w0 < W; w0 += MAX_CHUNK) {
@@ -1599,7 +1643,7 @@ struct Matcher {
       CUTS_DISPATCH_L(L, PC)
 #undef PC
       HIPCHECK(hipGetLastError());
-      pend(cls, 0, true);
+      pend(9, 0, true);
     }
   }
 
@@ -1642,8 +1686,8 @@ struct Matcher {
     CUTS_DISPATCH_L(L, CR)
 #undef CR
     HIPCHECK(hipGetLastError());
-    pend(8, 0, true);
-    if (!(g.h2_off && kp.ch_ta <= 2)) common_pass(L, rows, n, in, CommonArgs{kp.ch_u, L - 1, kp.ch_ta, 1u}, nullptr, 10);
+    pend(10, 0, true);
+    if (!(g.h2_off && kp.ch_ta <= 2)) common_pass(L, rows, n, in, CommonArgs{kp.ch_u, L - 1, kp.ch_ta, 1u}, nullptr);
   }
 
   void run() {
@@ -1672,6 +1716,9 @@ struct Matcher {
       level_cap[l] = std::max<uint64_t>(per / ((l + 1) * 4 + 13), 1024);
     }
     if (closed_last && kp.sc_thr > 1) tbl = cnt_table(c, dg, kp.sc_thr);
+    // with the two-hop index the chain is counted inside the expansion of
+    // position k-3 (no rows of level k-3 are stored)
+    chain_fused = chain && g.h2_off && kp.ch_ta <= 2;
     if (chain) {
       ctab.ta = kp.ch_ta > 1 ? cnt_table(c, dg, kp.ch_ta) : nullptr;
       ctab.tb = kp.ch_tb > 1 ? cnt_table(c, dg, kp.ch_tb) : nullptr;
@@ -1751,7 +1798,7 @@ struct Matcher {
         HIPCHECK(hipStreamSynchronize(c.stream));
         r.count = c.host_pinned[0];
         r.level[k - 1] = r.count;
-        if (pair) {
+        if (pair || chain_fused) {
           r.level[k - 3] = c.host_pinned[1];
           r.level[k - 2] = c.host_pinned[2];
         } else if (fuse) {
@@ -2008,6 +2055,9 @@ void upload_graph(Context& c, const Graph& g, DeviceGraph& dg) {
     std::vector<uint32_t> deg(g.V);
     for (uint32_t v = 0; v < g.V; ++v) deg[v] = g.out_off[v + 1] - g.out_off[v];
     uint32_t t = 32;
+    // CUTS_HUB_MIN_DEG raises the starting threshold (fewer, larger hubs:
+    // a bitmap set small enough to stay in L2)
+    if (const char* e = std::getenv("CUTS_HUB_MIN_DEG")) t = std::max<uint32_t>(t, (uint32_t)std::strtoul(e, nullptr, 10));
     for (; t <= (1u << 30); t *= 2) {
       uint64_t nh = 0;
       for (uint32_t d : deg) nh += d >= t;

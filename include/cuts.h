@@ -115,9 +115,9 @@ enum {
   CUTS_K_COUNT = 5,    /* expansion counting the last level */
   CUTS_K_FUSED = 6,    /* expansion + closed-form last level */
   CUTS_K_CLOSED = 7,   /* closed-form last level over stored rows */
-  CUTS_K_LAST2 = 8,    /* the last two positions counted together (pair / chain levels) */
-  CUTS_K_COMMON = 9,   /* intersection sizes |N(u) & N(w)| those levels need */
-  CUTS_K_CHAIN_COMMON = 10, /* the chain level's per-row intersection with its newest vertex */
+  CUTS_K_PAIR = 8,     /* expansion + the last two positions counted as a pair */
+  CUTS_K_COMMON = 9,   /* intersection sizes |N(u) & N(w)| by list walks */
+  CUTS_K_CHAIN = 10,   /* the last two positions counted as a chain */
   CUTS_K_CLASSES = 11
 };
 
