@@ -105,6 +105,48 @@ __device__ __forceinline__ uint32_t two_hop(const GV& g, uint32_t u, uint32_t w)
   return (lo < end && g.h2_v[lo] == w) ? g.h2_c[lo] : 0u;
 }
 
+// Chain of the last two positions (k-1 on k-2 alone, k-2 on ch_u alone)
+// for the prefix row[0..L) + v of positions 0..k-3, u = row[ch_u]:
+//   H(u) - sum_{r_i in N(u), deg >= ta} cnt_tb(r_i) - [deg u >= tb] |A|
+//        - sum_{i != ch_u, deg(r_i) >= tb} (|N(u) & N(r_i)| - c_i)
+// with |A| = cnt_ta(u) - |{i : r_i in N(u), deg(r_i) >= ta}| (into na) and
+// c_i the prefix vertices inside N(u) & N(r_i) meeting ta.  The
+// intersections of the row's own positions come summed in `shared`; the one
+// with v is looked up in the two-hop index.
+__device__ int64_t chain_closed_rt(const GV& g, const KPlan& P, const uint32_t* row, uint32_t L, uint32_t v,
+                                   const uint64_t* __restrict__ H, const uint32_t* __restrict__ ta_tbl,
+                                   const uint32_t* __restrict__ tb_tbl, uint32_t shared, int64_t& na) {
+  const uint32_t N = L + 1;
+  const uint32_t cu = P.ch_u, ta = P.ch_ta, tb = P.ch_tb;
+  const uint32_t u = row[cu];
+  const uint32_t du = g.off_out[u + 1] - g.off_out[u];
+  uint32_t inu = 0;
+  for (uint32_t l = 0; l < N; ++l) {
+    const uint32_t x = l == L ? v : row[l];
+    const bool in = l != cu && (((P.qadj[cu] >> l) & 1) || has_arc(g, u, x));
+    if (in && g.sig[2 * (uint64_t)x] >= ta) inu |= 1u << l;
+  }
+  na = (int64_t)(ta_tbl ? ta_tbl[u] : du) - __popc(inu);
+  int64_t c = (int64_t)H[u];
+  for (uint32_t l = 0; l < N; ++l)
+    if ((inu >> l) & 1) {
+      const uint32_t x = l == L ? v : row[l];
+      c -= tb_tbl ? tb_tbl[x] : g.off_out[x + 1] - g.off_out[x];
+    }
+  if (tb <= 1 || du >= tb) c -= na;
+  for (uint32_t i = 0; i < N; ++i) {
+    const uint32_t xi = i == L ? v : row[i];
+    if (i == cu || g.sig[2 * (uint64_t)xi] < tb) continue;
+    for (uint32_t j = 0; j < N; ++j) {
+      if (j == i || !((inu >> j) & 1)) continue;
+      if (((P.qadj[i] >> j) & 1) || has_arc(g, xi, j == L ? v : row[j])) ++c;
+    }
+  }
+  c -= (int64_t)shared;
+  if (tb <= 1 || g.sig[2 * (uint64_t)v] >= tb) c -= two_hop(g, u, v);
+  return c;
+}
+
 // Closed-form count of the last position given positions r[0..n) (LDS row)
 // plus, when has_v, the candidate v at position n.
 __device__ uint64_t closed_form_rt(const GV& g, const KPlan& P, const uint32_t* r, uint32_t n, bool has_v, uint32_t v,
@@ -307,6 +349,8 @@ __device__ void process(const BatchArgs& a, Shared<WROW>& s, uint32_t L) {
   const bool closed = (a.flags & 1u) == 0 && uni(P.sc_ok);
   const bool fuse = closed && (a.flags & 2u) == 0 && k >= 3;
   const bool pair = fuse && uni(P.pr_ok) && (a.flags & 8u) == 0;
+  // the chain level needs the two-hop index (threshold of A at most 2)
+  const bool chain = fuse && !pair && uni(P.ch_ok) && (a.flags & 16u) == 0 && g.h2_off != nullptr && uni(P.ch_ta) <= 2;
   uint64_t cnt = 0, surv = 0, sna = 0;
 
   if (L == 0) {
@@ -340,7 +384,11 @@ __device__ void process(const BatchArgs& a, Shared<WROW>& s, uint32_t L) {
 
   // shortest constraint list per row (one row per thread), then a block scan
   const uint32_t nj = uni(P.nj[L]);
-  const int mode = (L == k - 1) ? 1 : ((pair && L == k - 3) ? 3 : ((L == k - 2 && fuse) ? 2 : 0));
+  const int mode = (L == k - 1) ? 1
+                   : (pair && L == k - 3)  ? 3
+                   : (chain && L == k - 3) ? 4
+                   : (L == k - 2 && fuse)  ? 2
+                                           : 0;
   // fused level with an older closed-form vertex u: the closed form splits
   // into a per-prefix base, cnt(u) - |{i < L : r_i in N(u), deg(r_i) >= t}|,
   // and one term for the candidate itself
@@ -383,6 +431,17 @@ __device__ void process(const BatchArgs& a, Shared<WROW>& s, uint32_t L) {
       pxb[tid] = (uint32_t)cb;
       pxz[tid] = (uint32_t)(int32_t)cab;
       pns[tid] = 0;
+    }
+    if (mode == 4) {
+      // chain level: the intersections of N(u) with the row's own positions
+      const uint32_t* row = &s.rows[tid * WROW];
+      const uint32_t cu = P.ch_u, u = row[cu];
+      uint32_t t = 0;
+      for (uint32_t i = 0; i < L; ++i) {
+        if (i == cu) continue;
+        if (P.ch_tb <= 1 || g.sig[2 * (uint64_t)row[i]] >= P.ch_tb) t += two_hop(g, u, row[i]);
+      }
+      pxa[tid] = t;
     }
     if (use_cbase) {
       const uint32_t* row = &s.rows[tid * WROW];
@@ -458,6 +517,11 @@ __device__ void process(const BatchArgs& a, Shared<WROW>& s, uint32_t L) {
         sna += na;
         if (pa != pb && !h2) atomicAdd(&pns[r], 1u);
       }
+      if (mode == 4 && ok) {
+        int64_t na = 0;
+        fc = (uint64_t)chain_closed_rt(g, P, row, L, v, a.chH[q], a.tbls[3 * q + 1], tbl, pxa[r], na);
+        sna += (uint64_t)na;
+      }
       if (mode == 2 && ok) {
         if (use_cbase) {
           // v is certainly in N(u) when it was drawn from u's own list
@@ -515,7 +579,7 @@ __device__ void process(const BatchArgs& a, Shared<WROW>& s, uint32_t L) {
       }
     }
   }
-  if (mode == 3) {
+  if (mode == 3 || mode == 4) {
     add_block(&a.counts[q], cnt);
     add_block(&a.levels[q * kMaxQuery + L], surv);
     add_block(&a.levels[q * kMaxQuery + L + 1], sna);
@@ -681,6 +745,7 @@ void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& p
   if (half_words < 16) throw Error(-1, "batch: arena too small");
   std::vector<KPlan> kp(b.nq);
   std::vector<const uint32_t*> tb(3 * (size_t)b.nq, nullptr);
+  std::vector<const uint64_t*> chh(b.nq, nullptr);
   b.kmax = 0;
   for (uint32_t q = 0; q < b.nq; ++q) {
     if (plans[q]->k == 0 || plans[q]->k > kMaxQuery) throw Error(-3, "batch: query size out of range");
@@ -691,6 +756,11 @@ void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& p
     if (kp[q].pr_ok && !(o.flags & (1u | 2u | 128u))) {
       if (kp[q].pr_ta > 1) tb[3 * q + 1] = cnt_table(c, dg, kp[q].pr_ta);
       if (kp[q].pr_tab > 1) tb[3 * q + 2] = cnt_table(c, dg, kp[q].pr_tab);
+    }
+    // chain level (never together with the pair level): A's table in slot 1
+    if (!kp[q].pr_ok && kp[q].ch_ok && dg.h2_off && kp[q].ch_ta <= 2 && !(o.flags & (1u | 2u | 256u | 512u))) {
+      if (kp[q].ch_ta > 1) tb[3 * q + 1] = cnt_table(c, dg, kp[q].ch_ta);
+      chh[q] = chain_table(c, dg, kp[q].ch_ta, kp[q].ch_tb);
     }
   }
   BatchArgs& a = b.args;
@@ -714,6 +784,9 @@ void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& p
   const uint32_t** dtb = balloc<const uint32_t*>(b, 1, tb.size());
   HIPCHECK(hipMemcpy(dtb, tb.data(), tb.size() * sizeof(void*), hipMemcpyHostToDevice));
   a.tbls = dtb;
+  const uint64_t** dchh = balloc<const uint64_t*>(b, 12, b.nq);
+  HIPCHECK(hipMemcpy(dchh, chh.data(), b.nq * sizeof(void*), hipMemcpyHostToDevice));
+  a.chH = dchh;
   a.st = balloc<BState>(b, 2, 1);
   HIPCHECK(hipMemset(a.st, 0, sizeof(BState)));
   a.items = balloc<Item>(b, 3, 2 * qcap);
@@ -738,7 +811,7 @@ void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& p
   if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c.device) == hipSuccess && khz > 0)
     b.tick_khz = khz;
   a.item_work = o.item_work ? o.item_work : 16384;
-  a.flags = (o.flags & 3u) | ((o.flags & 64u) ? 4u : 0u) | ((o.flags & 128u) ? 8u : 0u);
+  a.flags = (o.flags & 3u) | ((o.flags & 64u) ? 4u : 0u) | ((o.flags & 128u) ? 8u : 0u) | ((o.flags & 256u) ? 16u : 0u);
   a.nq = b.nq;
 }
 

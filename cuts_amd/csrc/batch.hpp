@@ -36,7 +36,8 @@ struct BatchArgs {
   const uint32_t *h2_v, *h2_c;
   const KPlan* plans;
   const uint32_t* const* tbls;  // per query: 3 degree-threshold tables or null -- the closed form's
-                                // (= B's of the pair level), A's and A & B's
+                                // (= B's of the pair level), A's and A & B's (A's of the chain level)
+  const uint64_t* const* chH;   // per query: chain-level vertex tables or null
   BState* st;
   Item* items;                  // 2 halves of qcap items; level L uses half L & 1
   uint64_t qcap;
@@ -48,7 +49,8 @@ struct BatchArgs {
   const Item* init_items;       // level-0 items, copied into the queue by every replay
   uint64_t n_init;
   uint64_t item_work;           // candidate work at which a staged item is flushed
-  uint32_t flags;               // 1: no closed form, 2: no fusion, 4: no per-prefix base, 8: no pair level
+  uint32_t flags;               // 1: no closed form, 2: no fusion, 4: no per-prefix base, 8: no pair level,
+                                // 16: no chain level
   uint32_t nq;                  // queries in the batch
 };
 
@@ -76,7 +78,7 @@ struct Batch {
   uint32_t nq = 0, vchunk = 0, grid = 0, kmax = 0;
   BatchArgs args{};
   std::vector<uint32_t> k;   // query sizes
-  void* allocs[12] = {nullptr};
+  void* allocs[13] = {nullptr};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool use_graph = true;

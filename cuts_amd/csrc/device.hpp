@@ -102,10 +102,12 @@ struct LevelBuf {
   uint8_t* minsel = nullptr;    // cap
   uint64_t* work_off = nullptr; // cap+1
   uint32_t* cbase = nullptr;    // cap: per-prefix closed-form base (fused levels)
-  uint4* pbase = nullptr;       // cap: per-prefix |A|, |B|, |A & B| bases (pair level)
-  uint32_t* nsurv = nullptr;    // cap: per prefix, survivors (pair level) or the shared
+  // allocated only on the levels that use them (pbase_cap etc. rows each)
+  uint4* pbase = nullptr;       // per-prefix |A|, |B|, |A & B| bases (pair level)
+  uint32_t* nsurv = nullptr;    // per prefix, survivors (pair level) or the shared
                                 // intersections of its children (chain level)
-  uint32_t* parent = nullptr;   // cap: parent prefix of every row (rows of the chain level)
+  uint32_t* parent = nullptr;   // parent prefix of every row (rows of the chain level)
+  uint64_t pbase_cap = 0, nsurv_cap = 0, parent_cap = 0;
   uint64_t* bsum = nullptr;     // scan partials
   uint64_t bsum_cap = 0;
   uint32_t* tile_p = nullptr;   // tiles+1

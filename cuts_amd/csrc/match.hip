@@ -1335,9 +1335,6 @@ void ensure_rows(LevelBuf& b, uint64_t rows, uint32_t words) {
   dev_alloc(b.minsel, cap);
   dev_alloc(b.work_off, cap + 1);
   dev_alloc(b.cbase, cap);
-  dev_alloc(b.pbase, cap);
-  dev_alloc(b.nsurv, cap);
-  dev_alloc(b.parent, cap);
   b.cap = cap;
 }
 void ensure_rows_exact(LevelBuf& b, uint64_t rows, uint32_t words) {
@@ -1348,11 +1345,17 @@ void ensure_rows_exact(LevelBuf& b, uint64_t rows, uint32_t words) {
   dev_alloc(b.minsel, std::max<uint64_t>(rows, 1));
   dev_alloc(b.work_off, std::max<uint64_t>(rows, 1) + 1);
   dev_alloc(b.cbase, std::max<uint64_t>(rows, 1));
-  dev_alloc(b.pbase, std::max<uint64_t>(rows, 1));
-  dev_alloc(b.nsurv, std::max<uint64_t>(rows, 1));
-  dev_alloc(b.parent, std::max<uint64_t>(rows, 1));
   b.cap = rows;
 }
+// The per-row arrays of the pair and chain levels, allocated only for the
+// levels that use them (each grows to the level's row capacity).
+template <typename T>
+void ensure_aux(T*& p, uint64_t& cap, uint64_t rows) {
+  if (cap >= rows && p) return;
+  dev_alloc(p, std::max<uint64_t>(rows, 1));
+  cap = std::max<uint64_t>(rows, 1);
+}
+
 void ensure_bsum(LevelBuf& b, uint64_t nb) {
   if (b.bsum_cap >= nb + 1 && b.bsum) return;
   dev_alloc(b.bsum, nb + 1);
@@ -1476,6 +1479,7 @@ struct Matcher {
     else if (L == k - 1 && !write) mode = M_COUNT;
     else if (L == k - 2 && fuse) mode = M_FUSED;
     const int mclass = mode == M_WRITE ? 4 : mode == M_COUNT ? 5 : mode == M_FUSED ? 6 : mode == M_PAIR ? 8 : 10;
+    if (mode == M_PAIR) ensure_aux(in.pbase, in.pbase_cap, n);
     {
       const uint32_t grid = grid_for(n, BLOCK, grid_cap);
       unsigned long long* merge = prof ? c.counters + 8 : nullptr;
@@ -1501,7 +1505,10 @@ struct Matcher {
       chunk = std::min<uint64_t>(chunk, out.cap);
     }
     const bool pair_common = mode == M_PAIR && kp.pr_a != kp.pr_b && !(g.h2_off && kp.pr_tab <= 2);
-    if (pair_common) HIPCHECK(hipMemsetAsync(in.nsurv, 0, n * sizeof(uint32_t), c.stream));
+    if (pair_common) {
+      ensure_aux(in.nsurv, in.nsurv_cap, n);
+      HIPCHECK(hipMemsetAsync(in.nsurv, 0, n * sizeof(uint32_t), c.stream));
+    }
     for (uint64_t w0 = 0; w0 < W; w0 += chunk) {
       if (past_deadline()) return;
       const uint64_t w1 = std::min(W, w0 + chunk);
@@ -1524,7 +1531,11 @@ struct Matcher {
       a.cbase = use_cbase ? in.cbase : nullptr;
       a.pbase = mode == M_PAIR ? in.pbase : nullptr;
       a.nsurv = pair_common ? in.nsurv : nullptr;
-      a.out_parent = (chain && mode == M_WRITE && L == k - 3) ? out.parent : nullptr;
+      a.out_parent = nullptr;
+      if (chain && !chain_fused && mode == M_WRITE && L == k - 3) {
+        ensure_aux(out.parent, out.parent_cap, out.cap);
+        a.out_parent = out.parent;
+      }
       a.shared = in.nsurv;
       a.ctab = ctab;
       a.xcd = (o.flags & 8u) ? 0 : 1;
@@ -1652,6 +1663,7 @@ struct Matcher {
   // positions i of |N(u) & N(r_i)| (those whose degree takes completions
   // away), shared by all its children; into in.nsurv.
   void chain_shared(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in) {
+    ensure_aux(in.nsurv, in.nsurv_cap, n);
     if (g.h2_off && kp.ch_ta <= 2) {
       const uint32_t grid = grid_for(n, BLOCK, grid_cap);
       pbegin(false);
@@ -1713,7 +1725,10 @@ struct Matcher {
       if (l < k - 1 || write) ++nlev;
     for (uint32_t l = 1; l < k; ++l) {
       const uint64_t per = nlev ? budget / nlev : budget;
-      level_cap[l] = std::max<uint64_t>(per / ((l + 1) * 4 + 13), 1024);
+      // rows, planning arrays (minlen, minsel, scan entry, closed-form base)
+      // and, for the levels that feed the pair / chain level, their bases,
+      // survivor sums and parent indices
+      level_cap[l] = std::max<uint64_t>(per / ((l + 1) * 4 + 17 + 24), 1024);
     }
     if (closed_last && kp.sc_thr > 1) tbl = cnt_table(c, dg, kp.sc_thr);
     // with the two-hop index the chain is counted inside the expansion of

@@ -65,8 +65,8 @@ def test_level_batch_equals_single_and_oracle(ctx, seed):
 
 @pytest.mark.parametrize("seed", [0, 1])
 def test_level_batch_pair_level_shapes(ctx, seed):
-    """Shapes whose last two positions are counted together (and some whose
-    are not) in one batch: per-query counts and level sizes equal the
+    """Shapes whose last two positions are counted together as a pair or a
+    chain (and some whose are not) in one batch: per-query counts and level sizes equal the
     single-query matcher's with and without the pair level, and the
     oracle's."""
     from test_pair import SHAPES
@@ -80,7 +80,7 @@ def test_level_batch_pair_level_shapes(ctx, seed):
         assert x.count == oracle.count(og, oracle.qmatrix(k, oracle.undirected(e))), e
         assert ctx.match(dg, cuts_amd.Plan(cuts_amd.Graph.from_edges(k, e)), flags=cuts_amd.FLAG_NO_PAIR).level_sizes \
             == x.level_sizes, e
-    for flags, kw in ((0, {}), (cuts_amd.FLAG_NO_PAIR, {}), (cuts_amd.FLAG_NO_BITMAP, {}),
+    for flags, kw in ((0, {}), (cuts_amd.FLAG_NO_PAIR, {}), (cuts_amd.FLAG_NO_CHAIN, {}), (cuts_amd.FLAG_NO_BITMAP, {}),
                       (cuts_amd.FLAG_NO_TWO_HOP, {}), (0, dict(item_work=32, vchunk=5))):
         print(f"batch seed={seed} flags={flags} {kw}", flush=True)
         r = cuts_amd.Batch(ctx, dg, plans, flags=flags, **kw).run()
