@@ -147,7 +147,7 @@ def run_rmat20(args, max_work=1 << 36):
     r = runs[-1]
     ms = statistics.median(x.device_ms for x in runs)
     dname, (dms, dn, dbytes) = max(r.kernels.items(), key=lambda kv: kv[1][0])
-    ach = r.merge_bytes[dname] / dn / (dms / dn / 1e3) / 1e9 if dn else 0.0
+    ach = dbytes / dn / (dms / dn / 1e3) / 1e9 if dn else 0.0  # demand model, as for the headline
     line = base_line("exact embeddings/sec + p50 time/query", r.count / (ms / 1e3), "embeddings/s",
                      len(runs), args.warmup, ms,
                      "rmat scale=20 ef=16 a,b,c=.57,.19,.19 seed=0; 5-cycle+chord+2-tails; score order; exact "
@@ -156,8 +156,8 @@ def run_rmat20(args, max_work=1 << 36):
                       "p50_ms_per_query": ms})
     line["roofline"] = {"bound": "hbm", "kernel": dname, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": ach / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": dms / max(dn, 1),
-                        "launches": dn, "bytes_model": "merge model",
-                        "demand_GBs": dbytes / dn / (dms / dn / 1e3) / 1e9 if dn else 0.0}
+                        "launches": dn, "bytes_model": "demand model (loads the kernel's algorithm issues)",
+                        "merge_model_GBs": r.merge_bytes[dname] / dn / (dms / dn / 1e3) / 1e9 if dn else 0.0}
     emit(line)
 
 
