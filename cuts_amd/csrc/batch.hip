@@ -409,6 +409,7 @@ __device__ void process(const BatchArgs& a, Shared<WROW>& s, uint32_t L) {
   uint32_t* const pxz = s.stage + 2 * RMAX;
   uint32_t* const pns = s.stage + 3 * RMAX;
   uint64_t len = 0;
+  bool collapsed = false;
   if (tid < n) {
     if (mode == 3) {
       const uint32_t* row = &s.rows[tid * WROW];
@@ -431,6 +432,15 @@ __device__ void process(const BatchArgs& a, Shared<WROW>& s, uint32_t L) {
       pxb[tid] = (uint32_t)cb;
       pxz[tid] = (uint32_t)(int32_t)cab;
       pns[tid] = 0;
+      if (h2 && P.pr_collapse) {
+        // position k-3's survivors are N(u) & N(w) minus the prefix, each
+        // adding the same number: no candidate is expanded
+        const int64_t ns = cab;
+        cnt += (uint64_t)(ns * ((ca - 1) * (cb - 1) - (cab - 1)));
+        surv += (uint64_t)ns;
+        sna += (uint64_t)(ns * (ca - 1));
+        collapsed = true;
+      }
     }
     if (mode == 4) {
       // chain level: the intersections of N(u) with the row's own positions
@@ -465,7 +475,7 @@ __device__ void process(const BatchArgs& a, Shared<WROW>& s, uint32_t L) {
       }
     }
     s.minsel[tid] = (uint8_t)sel;
-    len = best;
+    len = collapsed ? 0 : best;
   }
   const uint64_t W = block_scan(s, n, len);
 

@@ -45,6 +45,11 @@ struct KPlan {
   uint32_t pr_a, pr_b;
   uint32_t pr_ta, pr_tb, pr_tab;  // degree thresholds of A, B and A & B
   uint32_t pr_ma, pr_mb;          // positions < k-2 certainly inside N(row[pr_a]) / N(row[pr_b])
+  // pair level whose position k-3 has exactly the constraints pr_a, pr_b
+  // (thresholds <= 2): every survivor v of a prefix adds the same
+  // (|A|-1)(|B|-1) - (|A & B|-1), and the survivors are N(r_a) & N(r_b) minus
+  // the prefix, so the prefix's count is closed-form from one two-hop lookup
+  uint32_t pr_collapse;
   // last two positions as a chain (symmetric graphs): k-1's one backward
   // neighbour is k-2, whose one backward neighbour ch_u is older than k-3.
   // A prefix P of positions 0..k-3 with u = row[ch_u] and

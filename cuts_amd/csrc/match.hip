@@ -210,9 +210,11 @@ __global__ __launch_bounds__(BLOCK) void k_plan(GView g, KPlan P, const uint32_t
                                                uint32_t* __restrict__ minlen, uint8_t* __restrict__ minsel,
                                                uint32_t* __restrict__ cbase, const uint32_t* __restrict__ tbl,
                                                uint4* __restrict__ pbase, PairTables pt,
-                                               unsigned long long* __restrict__ merge) {
+                                               unsigned long long* __restrict__ merge,
+                                               unsigned long long* __restrict__ counters) {
   const uint32_t nj = P.nj[L];
-  uint64_t mb = 0;
+  const bool collapse = pbase && P.pr_collapse && g.h2_off;
+  uint64_t mb = 0, acc = 0, surv = 0, sna = 0;
   for (uint64_t p = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (uint64_t)gridDim.x * BLOCK) {
     uint32_t row[L];
 #pragma unroll
@@ -271,9 +273,28 @@ __global__ __launch_bounds__(BLOCK) void k_plan(GView g, KPlan P, const uint32_t
         cab -= ina && inb && d >= P.pr_tab;
       }
       pbase[p] = make_uint4((uint32_t)ca, (uint32_t)cb, (uint32_t)(int32_t)cab, 0u);
+      if (collapse) {
+        // position k-3's survivors are N(u) & N(w) minus the prefix (cab of
+        // them: with thresholds <= 2 every common neighbour qualifies), each
+        // inside both lists: the prefix's count without expanding it
+        const int64_t ns = cab;
+        acc += (uint64_t)(ns * ((ca - 1) * (cb - 1) - (cab - 1)));
+        surv += (uint64_t)ns;
+        sna += (uint64_t)(ns * (ca - 1));
+        if (merge) {
+          const uint64_t du = g.off_out[u + 1] - g.off_out[u], dw = g.off_out[w + 1] - g.off_out[w];
+          mb += (uint64_t)ns * (24 + 4 * du + (uint64_t)(ca - 1) * (24 + 4 * dw));
+        }
+        minlen[p] = 0;
+      }
     }
   }
   if (merge) add_wave(merge, mb);
+  if (collapse) {
+    add_wave(&counters[0], acc);
+    add_wave(&counters[1], surv);
+    add_wave(&counters[2], sna);
+  }
 }
 
 // -------------------------------------------------- exclusive scan (u32->u64)
@@ -1485,7 +1506,7 @@ struct Matcher {
       unsigned long long* merge = prof ? c.counters + 8 : nullptr;
       if (merge) HIPCHECK(hipMemsetAsync(merge, 0, 8, c.stream));
       pbegin(false);
-#define PL(LL) hipLaunchKernelGGL((k_plan<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.minlen, in.minsel, use_cbase ? in.cbase : nullptr, tbl, mode == M_PAIR ? in.pbase : nullptr, ptbl, merge)
+#define PL(LL) hipLaunchKernelGGL((k_plan<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.minlen, in.minsel, use_cbase ? in.cbase : nullptr, tbl, mode == M_PAIR ? in.pbase : nullptr, ptbl, merge, c.counters)
       CUTS_DISPATCH_L(L, PL)
 #undef PL
       HIPCHECK(hipGetLastError());
@@ -1948,6 +1969,11 @@ KPlan make_kplan(const Plan& p, bool symmetric) {
       for (uint32_t i = 0; i + 2 < p.k; ++i) {
         if (adjacent(i, pa)) kp.pr_ma |= 1u << i;
         if (adjacent(i, pb)) kp.pr_mb |= 1u << i;
+      }
+      const uint32_t c = last - 2;
+      if (pa != pb && kp.nj[c] == 2 && std::max(p.sig_in[c], p.sig_out[c]) <= 2 && kp.pr_tab <= 2) {
+        const uint32_t j0 = kp.jc[c][0] & 31, j1 = kp.jc[c][1] & 31;
+        kp.pr_collapse = (j0 == pa && j1 == pb) || (j0 == pb && j1 == pa);
       }
     }
   }

@@ -67,6 +67,19 @@ def pair_eligible(p):
     return pb != k - 2 and pa <= k - 4 and pb <= k - 4
 
 
+def collapse_eligible(p):
+    """Pair level whose position k-3 has exactly the pair's two list owners
+    as constraints, all thresholds <= 2: counted per prefix of level k-4
+    from the two-hop index, with no expansion at all."""
+    if not pair_eligible(p):
+        return False
+    k = p.k
+    b = backward(p)
+    deg = [len(b[i]) + sum(1 for j in range(i + 1, k) if i in b[j]) for i in range(k)]
+    owners = b[k - 2] | b[k - 1]
+    return len(owners) == 2 and b[k - 3] == owners and max(deg[k - 3], deg[k - 2], deg[k - 1]) <= 2
+
+
 def chain_eligible(p):
     k = p.k
     if k < 4:
@@ -109,7 +122,8 @@ def test_pair_level_vs_oracle_and_cpu_trie(ctx, name, seed):
         expect = (pair_eligible(p) and not flags & cuts_amd.FLAG_NO_PAIR) or \
             (chain_eligible(p) and not flags & cuts_amd.FLAG_NO_CHAIN)
         assert not used or expect, (name, flags, p.order)
-        assert used or not expect or r.level_sizes[k - 3] == 0, (name, flags, p.order)
+        collapsed = collapse_eligible(p) and not flags & (cuts_amd.FLAG_NO_PAIR | cuts_amd.FLAG_NO_TWO_HOP)
+        assert used or not expect or collapsed or r.level_sizes[k - 3] == 0, (name, flags, p.order)
     for m in (dict(trunks=5), dict(budget=2048), dict(flags=cuts_amd.FLAG_NO_XCD),
               dict(budget=2048, flags=cuts_amd.FLAG_NO_TWO_HOP)):
         assert ctx.match(dg, p, **m).count == want, (name, m)
@@ -117,8 +131,8 @@ def test_pair_level_vs_oracle_and_cpu_trie(ctx, name, seed):
 
 def test_shapes_cover_every_case():
     """The shape table holds pair shapes with one and with two distinct list
-    owners, chain shapes with k = 5 and k >= 6, and shapes that are neither
-    (plans only, no GPU work)."""
+    owners (some of the latter collapsing position k-3), chain shapes with
+    k = 5 and k >= 6, and shapes that are neither (plans only, no GPU work)."""
     kinds = set()
     for name, edges in SHAPES.items():
         k = 1 + max(max(e) for e in edges)
@@ -127,11 +141,13 @@ def test_shapes_cover_every_case():
         if pair_eligible(p):
             b = backward(p)
             kinds.add("pair-same" if b[k - 1] == b[k - 2] else "pair-two")
+            if collapse_eligible(p):
+                kinds.add("pair-collapse")
         elif chain_eligible(p):
             kinds.add("chain-5" if k == 5 else "chain-6+")
         else:
             kinds.add("neither")
-    assert kinds == {"pair-same", "pair-two", "chain-5", "chain-6+", "neither"}
+    assert kinds == {"pair-same", "pair-two", "pair-collapse", "chain-5", "chain-6+", "neither"}
 
 
 @pytest.mark.gpu
