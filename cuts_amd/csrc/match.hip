@@ -27,7 +27,14 @@
 //     its candidates are counted in closed form (|N(u)| meeting the degree
 //     threshold, minus prefix members of N(u)), and that count is fused into
 //     the expansion of the level before, so the largest level is never
-//     materialised.
+//     materialised;
+//   * when the last two positions are two pendants on older vertices (a
+//     pair) or a 2-edge tail on an older vertex (a chain), both are counted
+//     in closed form inside the expansion of position k-3, from degree
+//     tables and |N(u) & N(w)| looked up in a two-hop index built at upload
+//     (or counted by flattened list walks without it); a pair whose position
+//     k-3 is exactly the common neighbours of its two list owners is counted
+//     per prefix of level k-4 without expanding position k-3 at all.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -203,8 +210,10 @@ __device__ __forceinline__ uint32_t tbl_or_deg(const GView& g, const uint32_t* t
 // |B| and |A & B|: the degree-filtered list sizes of u = r[pr_a] and
 // w = r[pr_b] minus the prefix vertices inside them, and for |A & B| (as a
 // signed word) the prefix vertices inside both, negated, plus -- when u = w --
-// the filtered size of N(u); for u != w the size of N(u) & N(w) is counted
-// afterwards by k_pair_common, only for prefixes with survivors.
+// the filtered size of N(u), else |N(u) & N(w)| from the two-hop index
+// (without it, k_common counts that afterwards, only for prefixes with
+// survivors).  When the pair level collapses (pr_collapse) it also counts the
+// prefix's completions into counters[0..2] and gives the prefix no work.
 template <int L>
 __global__ __launch_bounds__(BLOCK) void k_plan(GView g, KPlan P, const uint32_t* __restrict__ rows, uint64_t n,
                                                uint32_t* __restrict__ minlen, uint8_t* __restrict__ minsel,
@@ -807,8 +816,9 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
             const uint32_t dv = sg[j].x;
             const uint64_t na = pb[j].x - (uint32_t)(ina && dv >= P.pr_ta);
             const uint64_t nb = pb[j].y - (uint32_t)(inb && dv >= P.pr_tb);
-            // |A & B| without N(u) & N(w) itself when u != w (added per
-            // prefix by k_pair_common); modulo-2^64 sums are exact
+            // |A & B|: without the two-hop index and u != w, N(u) & N(w)
+            // itself is subtracted per prefix afterwards by k_common;
+            // modulo-2^64 sums are exact
             const int64_t nab = (int64_t)(int32_t)pb[j].z - (int64_t)(ina && inb && dv >= P.pr_tab);
             acc += na * nb - (uint64_t)nab;
             pairs += na;
