@@ -56,6 +56,8 @@ struct GV {
   uint32_t words;
   const uint64_t* h2_off;  // two-hop index or null
   const uint32_t *h2_v, *h2_c;
+  const int32_t* h2_did;   // dense rows of the index or null
+  const uint16_t* h2_dense;
 };
 
 __device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -95,6 +97,10 @@ __device__ __forceinline__ bool has_arc(const GV& g, uint32_t u, uint32_t x) {
 // |N(u) & N(w)| for u != w from the two-hop index (lower-bound search of w
 // in u's list; common neighbours of distinct vertices have degree >= 2)
 __device__ __forceinline__ uint32_t two_hop(const GV& g, uint32_t u, uint32_t w) {
+  if (g.h2_did) {
+    const int32_t dr = g.h2_did[u];
+    if (dr >= 0) return g.h2_dense[(uint64_t)dr * g.V + w];
+  }
   uint64_t lo = g.h2_off[u], hi = g.h2_off[u + 1];
   const uint64_t end = hi;
   while (lo < hi) {
@@ -336,7 +342,7 @@ __device__ uint64_t block_scan(Shared<WROW>& s, uint32_t n, uint64_t len) {
 template <int WROW>
 __device__ void process(const BatchArgs& a, Shared<WROW>& s, uint32_t L) {
   const GV g{a.off_out, a.adj_out, a.off_in, a.adj_in, a.sig, a.V, a.hub_id, a.nsig_out, a.nsig_in, a.bits, a.words,
-             a.h2_off, a.h2_v, a.h2_c};
+             a.h2_off, a.h2_v, a.h2_c, a.h2_did, a.h2_dense};
   Item it;
   it.qL = uni(s.item.qL);
   it.n = uni(s.item.n);
@@ -788,6 +794,8 @@ void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& p
   a.h2_off = (o.flags & 512u) ? nullptr : dg.h2_off;
   a.h2_v = dg.h2_v;
   a.h2_c = dg.h2_c;
+  a.h2_did = (o.flags & 512u) ? nullptr : dg.h2_did;
+  a.h2_dense = dg.h2_dense;
   KPlan* dplans = balloc<KPlan>(b, 0, b.nq);
   HIPCHECK(hipMemcpy(dplans, kp.data(), b.nq * sizeof(KPlan), hipMemcpyHostToDevice));
   a.plans = dplans;

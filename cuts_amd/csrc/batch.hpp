@@ -34,6 +34,8 @@ struct BatchArgs {
   uint32_t words;
   const uint64_t* h2_off;       // two-hop index or null
   const uint32_t *h2_v, *h2_c;
+  const int32_t* h2_did;        // dense rows of the index or null
+  const uint16_t* h2_dense;
   const KPlan* plans;
   const uint32_t* const* tbls;  // per query: 3 degree-threshold tables or null -- the closed form's
                                 // (= B's of the pair level), A's and A & B's (A's of the chain level)

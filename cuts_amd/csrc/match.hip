@@ -70,11 +70,13 @@ struct GView {
   uint32_t words;
   const uint64_t* h2_off;  // two-hop index or null
   const uint32_t *h2_v, *h2_c;
+  const int32_t* h2_did;   // dense rows of the index or null
+  const uint16_t* h2_dense;
 };
 
 GView view_of(const DeviceGraph& dg) {
   return GView{dg.off_out, dg.adj_out, dg.off_in, dg.adj_in, dg.sig, dg.V, dg.hub_id, dg.nsig_out, dg.nsig_in, dg.bits,
-               dg.words, dg.h2_off, dg.h2_v, dg.h2_c};
+               dg.words, dg.h2_off, dg.h2_v, dg.h2_c, dg.h2_did, dg.h2_dense};
 }
 
 __device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -127,6 +129,13 @@ __device__ __forceinline__ bool adj_has(const GView& g, uint32_t u, uint32_t x, 
 // two distinct vertices has degree >= 2, so the count serves degree
 // thresholds up to 2.
 __device__ __forceinline__ uint32_t two_hop(const GView& g, uint32_t u, uint32_t w, uint64_t& bytes) {
+  if (g.h2_did) {
+    const int32_t dr = g.h2_did[u];
+    if (dr >= 0) {
+      bytes += 6;
+      return g.h2_dense[(uint64_t)dr * g.V + w];
+    }
+  }
   uint64_t lo = g.h2_off[u], hi = g.h2_off[u + 1];
   const uint64_t end = hi;
   bytes += 16 + 4 * ((64 - __clzll(hi - lo)) + 1);
@@ -1740,6 +1749,7 @@ struct Matcher {
     g = view_of(dg);
     if (o.flags & 32u) g.hub_id = nullptr;  // CUTS_FLAG_NO_BITMAP
     if (o.flags & 512u) g.h2_off = nullptr;  // CUTS_FLAG_NO_TWO_HOP
+    if (o.flags & 512u) g.h2_did = nullptr;
     write = (o.flags & 4u) != 0;
     prof = (o.flags & 16u) != 0;
     closed_last = !write && !(o.flags & 1u) && kp.sc_ok && k >= 2;
@@ -2019,7 +2029,7 @@ DeviceGraph::~DeviceGraph() {
   for (auto& kv : chain_tables) (void)hipFree(kv.second);
   if (hub_id) (void)hipFree(hub_id);
   if (bits) (void)hipFree(bits);
-  for (void* p : {(void*)h2_off, (void*)h2_v, (void*)h2_c})
+  for (void* p : {(void*)h2_off, (void*)h2_v, (void*)h2_c, (void*)h2_did, (void*)h2_dense})
     if (p) (void)hipFree(p);
   (void)hipSetDevice(prev);
 }
@@ -2157,6 +2167,34 @@ void upload_graph(Context& c, const Graph& g, DeviceGraph& dg) {
       std::vector<uint32_t> hv, hc;
       build_two_hop(g, off, hv, hc);
       dg.h2_entries = off[V];
+      // dense rows for the longest lists (at least V/4 entries), while their
+      // bytes stay within 1/32 of free HBM; counts fit uint16 below degree 2^16
+      if (dg.max_deg < 65536) {
+        std::vector<uint32_t> cand;
+        for (uint32_t u = 0; u < V; ++u)
+          if ((off[u + 1] - off[u]) * 4 >= (uint64_t)V) cand.push_back(u);
+        std::sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) {
+          return off[a + 1] - off[a] > off[b + 1] - off[b] || (off[a + 1] - off[a] == off[b + 1] - off[b] && a < b);
+        });
+        const uint64_t row_bytes = 2ull * V;
+        const uint64_t rows = std::min<uint64_t>(cand.size(), fr / 32 / std::max<uint64_t>(row_bytes, 1));
+        if (rows) {
+          std::vector<int32_t> did(V, -1);
+          std::vector<uint16_t> dense(rows * V, 0);
+#pragma omp parallel for schedule(dynamic, 4)
+          for (int64_t r = 0; r < (int64_t)rows; ++r) {
+            const uint32_t u = cand[r];
+            uint16_t* row = dense.data() + (uint64_t)r * V;
+            for (uint64_t i = off[u]; i < off[u + 1]; ++i) row[hv[i]] = (uint16_t)hc[i];
+          }
+          for (uint64_t r = 0; r < rows; ++r) did[cand[r]] = (int32_t)r;
+          dev_alloc(dg.h2_did, V);
+          HIPCHECK(hipMemcpy(dg.h2_did, did.data(), (uint64_t)V * 4, hipMemcpyHostToDevice));
+          dev_alloc(dg.h2_dense, rows * V);
+          HIPCHECK(hipMemcpy(dg.h2_dense, dense.data(), rows * row_bytes, hipMemcpyHostToDevice));
+          dg.h2_dense_rows = (uint32_t)rows;
+        }
+      }
       dev_alloc(dg.h2_off, V + 1);
       HIPCHECK(hipMemcpy(dg.h2_off, off.data(), (V + 1) * 8, hipMemcpyHostToDevice));
       dev_alloc(dg.h2_v, hv.size());
