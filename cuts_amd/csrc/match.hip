@@ -207,6 +207,40 @@ __device__ __forceinline__ uint32_t tbl_or_deg(const GView& g, const uint32_t* t
   return tbl ? tbl[u] : g.off_out[u + 1] - g.off_out[u];
 }
 
+// The pair level's prefix parts for the prefix r[0..N) of positions
+// 0..k-4: |A|, |B| without the candidate of position k-3 (the degree
+// tables of u = r[pr_a], w = r[pr_b] minus the prefix vertices inside), and
+// for |A & B| the prefix vertices inside both, negated, plus the filtered
+// size of N(u) when u = w, else |N(u) & N(w)| from the two-hop index (0
+// without it).
+template <int N>
+__device__ __forceinline__ void pair_bases(const GView& g, const KPlan& P, const uint32_t (&r)[N], const PairTables& pt,
+                                           uint32_t& u, uint32_t& w, int64_t& ca, int64_t& cb, int64_t& cab,
+                                           uint64_t& bytes) {
+  u = 0;
+  w = 0;
+#pragma unroll
+  for (int l = 0; l < N; ++l) {
+    if (l == (int)P.pr_a) u = r[l];
+    if (l == (int)P.pr_b) w = r[l];
+  }
+  ca = tbl_or_deg(g, pt.a, u);
+  cb = tbl_or_deg(g, pt.b, w);
+  cab = 0;
+  if (P.pr_a == P.pr_b) cab = (int64_t)tbl_or_deg(g, pt.ab, u);
+  else if (g.h2_off && P.pr_tab <= 2) cab = (int64_t)two_hop(g, u, w, bytes);
+  bytes += 16 + 4 * N;
+#pragma unroll
+  for (int l = 0; l < N; ++l) {
+    const bool ina = ((P.pr_ma >> l) & 1) || adj_has(g, u, r[l], bytes);
+    const bool inb = ((P.pr_mb >> l) & 1) || adj_has(g, w, r[l], bytes);
+    const uint32_t d = g.sig[2 * (uint64_t)r[l]];
+    ca -= ina && d >= P.pr_ta;
+    cb -= inb && d >= P.pr_tb;
+    cab -= ina && inb && d >= P.pr_tab;
+  }
+}
+
 // Per prefix: the constraint whose list is shortest (upper bound on the
 // prefix's children).  Ties keep the lowest constraint index.
 // With merge (profiling) it adds the merge-model bytes of expanding the
@@ -269,27 +303,10 @@ __global__ __launch_bounds__(BLOCK) void k_plan(GView g, KPlan P, const uint32_t
       cbase[p] = (uint32_t)c;
     }
     if (pbase) {
-      uint64_t dummy0 = 0;
       uint32_t u = 0, w = 0;
-#pragma unroll
-      for (int l = 0; l < L; ++l) {
-        if (l == (int)P.pr_a) u = row[l];
-        if (l == (int)P.pr_b) w = row[l];
-      }
-      int64_t ca = tbl_or_deg(g, pt.a, u), cb = tbl_or_deg(g, pt.b, w);
-      int64_t cab = 0;
-      if (P.pr_a == P.pr_b) cab = (int64_t)tbl_or_deg(g, pt.ab, u);
-      else if (g.h2_off && P.pr_tab <= 2) cab = (int64_t)two_hop(g, u, w, dummy0);
+      int64_t ca = 0, cb = 0, cab = 0;
       uint64_t dummy = 0;
-#pragma unroll
-      for (int l = 0; l < L; ++l) {
-        const bool ina = ((P.pr_ma >> l) & 1) || adj_has(g, u, row[l], dummy);
-        const bool inb = ((P.pr_mb >> l) & 1) || adj_has(g, w, row[l], dummy);
-        const uint32_t d = g.sig[2 * (uint64_t)row[l]];
-        ca -= ina && d >= P.pr_ta;
-        cb -= inb && d >= P.pr_tb;
-        cab -= ina && inb && d >= P.pr_tab;
-      }
+      pair_bases<L>(g, P, row, pt, u, w, ca, cb, cab, dummy);
       pbase[p] = make_uint4((uint32_t)ca, (uint32_t)cb, (uint32_t)(int32_t)cab, 0u);
       if (collapse) {
         // position k-3's survivors are N(u) & N(w) minus the prefix (cab of
@@ -571,7 +588,7 @@ __device__ __forceinline__ void add_runs(uint32_t* __restrict__ cnt, uint64_t p,
   if (head && mine) atomicAdd(&cnt[p], (uint32_t)__popcll(mine));
 }
 
-enum { M_WRITE = 0, M_COUNT = 1, M_FUSED = 2, M_PAIR = 3, M_CHAIN = 4 };
+enum { M_WRITE = 0, M_COUNT = 1, M_FUSED = 2, M_PAIR = 3, M_CHAIN = 4, M_COLLAPSE = 5 };
 
 struct ExpandArgs {
   const uint32_t* rows;      // n rows of L words
@@ -589,6 +606,7 @@ struct ExpandArgs {
   uint32_t* out_parent;      // M_WRITE: parent prefix of every row written (or null)
   const uint32_t* shared;    // M_CHAIN: per prefix, its older positions' intersections with u
   ChainTables ctab;          // M_CHAIN
+  PairTables ptab;           // M_COLLAPSE
   uint32_t xcd;              // 1: XCD-contiguous tile mapping
   unsigned int* tilectr;     // 8 tile counters, zero at launch
   uint32_t stats;            // 1: accumulate demand bytes into counters[3]
@@ -601,7 +619,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
   const uint32_t tid = threadIdx.x, wave = tid >> 6;
   const uint32_t nj = P.nj[L];
   const uint32_t tin = P.sig_in[L], tout = P.sig_out[L];
-  uint64_t acc = 0, valid = 0, pairs = 0, bytes = 0, mb = 0;
+  uint64_t acc = 0, valid = 0, pairs = 0, trips = 0, bytes = 0, mb = 0;
   // Dynamic, XCD-aware tile fetching: blocks are dispatched round-robin over
   // the 8 XCDs, so block b works first on XCD (b mod 8)'s contiguous share of
   // the tiles (neighbouring prefixes share adjacency lists in that XCD's L2)
@@ -756,6 +774,33 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
         if (MODE == M_COUNT) {
 #pragma unroll
           for (int j = 0; j < G; ++j) acc += ok[j];
+        } else if (MODE == M_COLLAPSE) {
+          // a survivor v completes the prefix of positions 0..k-4 of a
+          // collapsing pair level: its |A & B| candidates of position k-3
+          // each add (|A|-1)(|B|-1) - (|A & B|-1)
+#pragma unroll
+          for (int j = 0; j < G; ++j) {
+            if (!ok[j]) continue;
+            ++valid;
+            uint32_t r[L + 1];
+#pragma unroll
+            for (int l = 0; l < L; ++l) r[l] = row[j][l];
+            r[L] = v[j];
+            uint32_t u, w;
+            int64_t ca, cb, cab;
+            pair_bases<L + 1>(g, P, r, a.ptab, u, w, ca, cb, cab, bytes);
+            acc += (uint64_t)(cab * ((ca - 1) * (cb - 1) - (cab - 1)));
+            pairs += (uint64_t)cab;
+            trips += (uint64_t)(cab * (ca - 1));
+            if (a.stats) {
+              // merge model: the survivor's node and its expansion by the
+              // lists of u and w; each of its cab children's node and
+              // expansion by u's list; each of their (ca-1) children's node
+              // and expansion by w's list
+              const uint64_t du = g.off_out[u + 1] - g.off_out[u], dw = g.off_out[w + 1] - g.off_out[w];
+              mb += 16 + 16 + 4 * (du + dw) + (uint64_t)cab * (24 + 4 * du + (uint64_t)(ca - 1) * (24 + 4 * dw));
+            }
+          }
         } else if (MODE == M_CHAIN) {
           // a survivor v completes the prefix of positions 0..k-3, whose
           // chain of the last two positions is counted in closed form
@@ -1032,12 +1077,13 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
   }
   if (MODE != M_WRITE) {
     add_wave(&a.counters[0], acc);
-    if (MODE == M_FUSED || MODE == M_PAIR || MODE == M_CHAIN) add_wave(&a.counters[1], valid);
-    if (MODE == M_PAIR || MODE == M_CHAIN) add_wave(&a.counters[2], pairs);
+    if (MODE == M_FUSED || MODE == M_PAIR || MODE == M_CHAIN || MODE == M_COLLAPSE) add_wave(&a.counters[1], valid);
+    if (MODE == M_PAIR || MODE == M_CHAIN || MODE == M_COLLAPSE) add_wave(&a.counters[2], pairs);
+    if (MODE == M_COLLAPSE) add_wave(&a.counters[9], trips);
   }
   if (a.stats) {
     add_wave(&a.counters[3], bytes);
-    if (MODE == M_FUSED || MODE == M_PAIR || MODE == M_CHAIN) add_wave(&a.counters[8], mb);
+    if (MODE == M_FUSED || MODE == M_PAIR || MODE == M_CHAIN || MODE == M_COLLAPSE) add_wave(&a.counters[8], mb);
   }
 }
 
@@ -1417,6 +1463,7 @@ struct Matcher {
   GView g;
   uint32_t k;
   bool closed_last = false, fuse = false, write = false, pair = false, chain = false, chain_fused = false;
+  bool collapse_fused = false;
   const uint32_t* tbl = nullptr;
   PairTables ptbl{};
   ChainTables ctab{};
@@ -1514,11 +1561,12 @@ struct Matcher {
     // plan + scan over the input rows; arrays live with the input level
     const bool use_cbase = fuse && L == k - 2 && kp.sc_pos < L && !(o.flags & 64u);
     int mode = M_WRITE;
-    if (pair && L == k - 3) mode = M_PAIR;
+    if (collapse_fused && L == k - 4) mode = M_COLLAPSE;
+    else if (pair && L == k - 3) mode = M_PAIR;
     else if (chain_fused && L == k - 3) mode = M_CHAIN;
     else if (L == k - 1 && !write) mode = M_COUNT;
     else if (L == k - 2 && fuse) mode = M_FUSED;
-    const int mclass = mode == M_WRITE ? 4 : mode == M_COUNT ? 5 : mode == M_FUSED ? 6 : mode == M_PAIR ? 8 : 10;
+    const int mclass = mode == M_WRITE ? 4 : mode == M_COUNT ? 5 : mode == M_FUSED ? 6 : (mode == M_PAIR || mode == M_COLLAPSE) ? 8 : 10;
     if (mode == M_PAIR) ensure_aux(in.pbase, in.pbase_cap, n);
     {
       const uint32_t grid = grid_for(n, BLOCK, grid_cap);
@@ -1578,6 +1626,7 @@ struct Matcher {
       }
       a.shared = in.nsurv;
       a.ctab = ctab;
+      a.ptab = ptbl;
       a.xcd = (o.flags & 8u) ? 0 : 1;
       a.tilectr = (unsigned int*)(c.counters + 4);  // 8 x uint32 in counters[4..7]
       HIPCHECK(hipMemsetAsync(a.tilectr, 0, 8 * sizeof(unsigned int), c.stream));
@@ -1619,6 +1668,13 @@ struct Matcher {
 #undef EC
         HIPCHECK(hipGetLastError());
         pend(5, 0, true);
+      } else if (mode == M_COLLAPSE) {
+        pbegin(true);
+#define EK(LL) hipLaunchKernelGGL((k_expand<LL, M_COLLAPSE>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, a)
+        CUTS_DISPATCH_PAIR(L, EK)
+#undef EK
+        HIPCHECK(hipGetLastError());
+        pend(8, 0, true);
       } else if (mode == M_CHAIN) {
         pbegin(true);
 #define EC2(LL) hipLaunchKernelGGL((k_expand<LL, M_CHAIN>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, a)
@@ -1775,6 +1831,10 @@ struct Matcher {
     // with the two-hop index the chain is counted inside the expansion of
     // position k-3 (no rows of level k-3 are stored)
     chain_fused = chain && g.h2_off && kp.ch_ta <= 2;
+    // a collapsing pair level is counted inside the expansion of position
+    // k-4 (no rows of level k-4 are stored); with a deadline, in the planner
+    // of level k-3 instead (the rolled-back counters are those of 0..2)
+    collapse_fused = pair && kp.pr_collapse && g.h2_off && k >= 5 && !(o.time_limit_ms > 0);
     if (chain) {
       ctab.ta = kp.ch_ta > 1 ? cnt_table(c, dg, kp.ch_ta) : nullptr;
       ctab.tb = kp.ch_tb > 1 ? cnt_table(c, dg, kp.ch_tb) : nullptr;
@@ -1793,6 +1853,7 @@ struct Matcher {
     LevelBuf& l0 = c.lv[0];
     ensure_rows(l0, std::max<uint32_t>(n0c, 1), 1);
     HIPCHECK(hipMemsetAsync(c.counters, 0, 4 * 8, c.stream));
+    HIPCHECK(hipMemsetAsync(c.counters + 9, 0, 8, c.stream));
     HIPCHECK(hipEventRecord(c.ev0, c.stream));
     const auto t_start = std::chrono::steady_clock::now();
     uint64_t n0 = 0;
@@ -1854,7 +1915,11 @@ struct Matcher {
         HIPCHECK(hipStreamSynchronize(c.stream));
         r.count = c.host_pinned[0];
         r.level[k - 1] = r.count;
-        if (pair || chain_fused) {
+        if (collapse_fused) {
+          r.level[k - 4] = c.host_pinned[1];
+          r.level[k - 3] = c.host_pinned[2];
+          r.level[k - 2] = read_u64(c.counters + 9);
+        } else if (pair || chain_fused) {
           r.level[k - 3] = c.host_pinned[1];
           r.level[k - 2] = c.host_pinned[2];
         } else if (fuse) {
