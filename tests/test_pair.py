@@ -166,3 +166,27 @@ def test_pair_level_random_queries(ctx, seed):
     dg = ctx.upload(g)
     for flags in (0, cuts_amd.FLAG_NO_PAIR | cuts_amd.FLAG_NO_CHAIN, cuts_amd.FLAG_NO_BITMAP, cuts_amd.FLAG_NO_TWO_HOP):
         assert ctx.match(dg, p, flags=flags).count == want, (edges, flags)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["4cycle+2pend", "triangle+2pend-two", "k4e+tail2", "star4"])
+def test_last_two_levels_under_a_deadline(ctx, name):
+    """With a time limit the run stops between chunks and rolls back a cut
+    trunk; the collapsed pair level is then counted by the planner over
+    stored rows.  Whatever it stopped at, the count equals the count of the
+    first roots_done level-0 vertices, and a generous limit gives the full,
+    oracle-equal count."""
+    edges = SHAPES[name]
+    k = 1 + max(max(e) for e in edges)
+    g = graph_for(7700 + k)
+    og = oracle.OGraph.from_view(g.view())
+    want = oracle.count(og, oracle.qmatrix(k, oracle.undirected(edges)))
+    dg = ctx.upload(g)
+    p = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges))
+    full = ctx.match(dg, p, time_limit_ms=60_000)
+    assert full.complete and full.count == want
+    n0 = full.level_sizes[0]
+    part = ctx.match(dg, p, trunks=n0, time_limit_ms=0.05)
+    if part.roots_done:
+        again = ctx.match(dg, p, roots=(0, part.last_root + 1, 1))
+        assert part.count == again.count and again.level_sizes[0] == part.roots_done
