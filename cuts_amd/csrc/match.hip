@@ -2216,7 +2216,8 @@ void upload_graph(Context& c, const Graph& g, DeviceGraph& dg) {
   }
   // Two-hop index: built on the host when the wedges (ordered neighbour
   // pairs of every vertex, an upper bound on the entries) fit 1/16 of free
-  // HBM at 8 bytes each, and at most 2^28 of them.
+  // HBM at 8 bytes each, at most 2^28 of them, on graphs of at most 2^25
+  // vertices.
   if (g.symmetric && g.V >= 2) {
     const auto t0 = std::chrono::steady_clock::now();
     uint64_t wedges = 0;
@@ -2226,7 +2227,8 @@ void upload_graph(Context& c, const Graph& g, DeviceGraph& dg) {
     }
     size_t fr = 0, tot = 0;
     HIPCHECK(hipMemGetInfo(&fr, &tot));
-    if (wedges <= std::min<uint64_t>(fr / 16 / 8, 1ull << 28)) {
+    // the host build keeps two V-word arrays per thread: bounded V
+    if (wedges <= std::min<uint64_t>(fr / 16 / 8, 1ull << 28) && g.V <= (1u << 25)) {
       const uint32_t V = g.V;
       std::vector<uint64_t> off;
       std::vector<uint32_t> hv, hc;
