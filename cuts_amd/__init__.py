@@ -32,6 +32,7 @@ FLAG_NO_PREFIX_BASE = 64
 FLAG_NO_PAIR = 128
 FLAG_NO_CHAIN = 256
 FLAG_NO_TWO_HOP = 512
+FLAG_NO_SHARED = 1024
 MAX_QUERY = 16
 KERNEL_CLASSES = ["level0", "plan", "scan", "tile_index", "expand_write", "expand_count", "expand_fused",
                   "closed_form", "pair", "common", "chain"]

@@ -62,6 +62,9 @@ struct KPlan {
   uint32_t ch_ok;
   uint32_t ch_u;
   uint32_t ch_ta, ch_tb;
+  // chain level whose position k-3 has no constraint on position k-4: its
+  // candidates depend on the row's parent prefix only (shared by siblings)
+  uint32_t ch_sib;
   uint32_t qadj[kMaxQuery];       // bit j of qadj[i]: positions i and j adjacent in the query
 };
 
@@ -119,6 +122,11 @@ struct LevelBuf {
                                 // intersections of its children (chain level)
   uint32_t* parent = nullptr;   // parent prefix of every row (rows of the chain level)
   uint64_t pbase_cap = 0, nsurv_cap = 0, parent_cap = 0;
+  // chain level with shared candidates: per prefix of this level, the start
+  // of its block in `cand` (n+1) and the candidates of position k-3
+  uint64_t* coff = nullptr;
+  uint32_t* cand = nullptr;
+  uint64_t coff_cap = 0, cand_cap = 0;
   uint64_t* bsum = nullptr;     // scan partials
   uint64_t bsum_cap = 0;
   uint32_t* tile_p = nullptr;   // tiles+1

@@ -568,6 +568,25 @@ __device__ __forceinline__ int64_t chain_closed(const GView& g, const KPlan& P, 
   return c;
 }
 
+// Chain level over rows of L = k-3 words whose position-(k-3) candidates are
+// their parent's block of `cand` (built once per parent, constraints and
+// threshold checked): one work item per (row, candidate); a candidate equal
+// to the row's newest vertex is skipped (injectivity), every other one adds
+// the chain count of the row + candidate.  Tiles as in k_expand.
+struct SibArgs {
+  const uint32_t* rows;
+  const uint32_t* parent;      // parent prefix of every row
+  const uint64_t* coff;        // per parent: start of its block in cand
+  const uint32_t* cand;
+  const uint32_t* shared;      // per row: its own positions' intersections with u
+  const uint64_t* work_off;
+  const uint32_t* tile_p;
+  uint64_t w_begin, w_end, ntiles;
+  unsigned long long* counters;
+  unsigned int* tilectr;
+  uint32_t xcd, stats;
+};
+
 // ------------------------------------------------------------ expansion
 // Adds to cnt[p] the number of this wave's lanes with `ok` set whose item
 // belongs to prefix p.  Lanes hold consecutive work items, so the items of a
@@ -588,7 +607,7 @@ __device__ __forceinline__ void add_runs(uint32_t* __restrict__ cnt, uint64_t p,
   if (head && mine) atomicAdd(&cnt[p], (uint32_t)__popcll(mine));
 }
 
-enum { M_WRITE = 0, M_COUNT = 1, M_FUSED = 2, M_PAIR = 3, M_CHAIN = 4, M_COLLAPSE = 5 };
+enum { M_WRITE = 0, M_COUNT = 1, M_FUSED = 2, M_PAIR = 3, M_CHAIN = 4, M_COLLAPSE = 5, M_GCOUNT = 6, M_GWRITE = 7 };
 
 struct ExpandArgs {
   const uint32_t* rows;      // n rows of L words
@@ -607,6 +626,8 @@ struct ExpandArgs {
   const uint32_t* shared;    // M_CHAIN: per prefix, its older positions' intersections with u
   ChainTables ctab;          // M_CHAIN
   PairTables ptab;           // M_COLLAPSE
+  const uint64_t* coff;      // M_GWRITE: start of every prefix's block in cand
+  uint32_t* cand;            // M_GWRITE: candidates grouped by prefix
   uint32_t xcd;              // 1: XCD-contiguous tile mapping
   unsigned int* tilectr;     // 8 tile counters, zero at launch
   uint32_t stats;            // 1: accumulate demand bytes into counters[3]
@@ -668,7 +689,8 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
       // independent and in flight together instead of one dependent chain
       // after another.  Items past w1 are clamped to the tile's last prefix
       // and masked; their list word is never read.
-      constexpr int G = (MODE != M_COUNT) ? ((L <= 6) ? 2 : 1) : ((L <= 6) ? 4 : (L <= 10 ? 2 : 1));
+      constexpr int G = (MODE != M_COUNT && MODE != M_GCOUNT && MODE != M_GWRITE) ? ((L <= 6) ? 2 : 1)
+                                                                                  : ((L <= 6) ? 4 : (L <= 10 ? 2 : 1));
       for (int g0 = 0; g0 < IPT; g0 += G) {
         uint64_t w[G];
         uint32_t pi[G];
@@ -774,6 +796,19 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
         if (MODE == M_COUNT) {
 #pragma unroll
           for (int j = 0; j < G; ++j) acc += ok[j];
+        } else if (MODE == M_GCOUNT) {
+          // survivors per prefix (all lanes)
+#pragma unroll
+          for (int j = 0; j < G; ++j) add_runs(a.nsurv, plo + min(pi[j], span - 1), ok[j]);
+        } else if (MODE == M_GWRITE) {
+          // survivors into their prefix's block (order within a block free)
+#pragma unroll
+          for (int j = 0; j < G; ++j) {
+            if (!ok[j]) continue;
+            const uint64_t p = (uint64_t)plo + min(pi[j], span - 1);
+            a.cand[a.coff[p] + atomicAdd(&a.nsurv[p], 1u)] = v[j];
+            if (a.stats) bytes += 16;  // block start, slot counter, candidate
+          }
         } else if (MODE == M_COLLAPSE) {
           // a survivor v completes the prefix of positions 0..k-4 of a
           // collapsing pair level: its |A & B| candidates of position k-3
@@ -1087,6 +1122,81 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
   }
 }
 
+template <int L>
+__global__ __launch_bounds__(BLOCK) void k_chain_sib(GView g, KPlan P, SibArgs a, ChainTables t) {
+  __shared__ uint64_t s_off[LDS_OFF + 1];
+  __shared__ uint64_t s_tile;
+  const uint32_t tid = threadIdx.x;
+  uint64_t acc = 0, valid = 0, pairs = 0, bytes = 0, mb = 0;
+  const uint32_t nx = a.xcd ? 8u : 1u;
+  const uint32_t myx = a.xcd ? (blockIdx.x & 7u) : 0u;
+  uint32_t tries = 0;
+  while (true) {
+    if (uni(tid >> 6) == 0) {
+      uint64_t tt = ~0ull;
+      while (tries < nx) {
+        const uint32_t xx = (myx + tries) % nx;
+        const uint64_t lo = a.ntiles * xx / nx, hi = a.ntiles * (xx + 1) / nx;
+        uint32_t got = 0;
+        if ((tid & 63) == 0) got = atomicAdd(&a.tilectr[xx], 1u);  // inside the wave-uniform branch
+        const uint64_t i = uni(got);
+        if (lo + i < hi) {
+          tt = lo + i;
+          break;
+        }
+        ++tries;
+      }
+      s_tile = tt;
+    }
+    __syncthreads();
+    const uint64_t tile = uni64(s_tile);
+    if (tile == ~0ull) break;
+    const uint64_t w0 = a.w_begin + tile * TILE;
+    const uint64_t w1 = min(w0 + (uint64_t)TILE, a.w_end);
+    const uint32_t plo = a.tile_p[tile], phi = a.tile_p[tile + 1];
+    const uint32_t span = phi - plo + 1;
+    const bool lds = span <= LDS_OFF;
+    if (lds)
+      for (uint32_t i = tid; i <= span; i += BLOCK) s_off[i] = a.work_off[plo + i];
+    __syncthreads();
+    for (int it = 0; it < IPT; ++it) {
+      const uint64_t w = w0 + (uint64_t)it * BLOCK + tid;
+      if (w >= w1) continue;
+      uint32_t lo = 0, hi = span + 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint64_t o = lds ? s_off[mid] : a.work_off[plo + mid];
+        if (o <= w) lo = mid + 1;
+        else hi = mid;
+      }
+      const uint32_t pi = lo - 1;
+      const uint64_t q = (uint64_t)plo + pi;
+      const uint64_t qbase = lds ? s_off[pi] : a.work_off[q];
+      const uint32_t par = a.parent[q];
+      const uint32_t v = a.cand[a.coff[par] + (w - qbase)];
+      uint32_t r[L + 1];
+#pragma unroll
+      for (int l = 0; l < L; ++l) r[l] = a.rows[q * L + l];
+      r[L] = v;
+      if (a.stats) bytes += 4 * L + 16;
+      if (v == r[L - 1]) continue;
+      ++valid;
+      int64_t na = 0;
+      acc += (uint64_t)chain_closed<L + 1>(g, P, r, t, a.shared[q], true, na, a.stats != 0, mb, bytes);
+      pairs += (uint64_t)na;
+      if (a.stats) mb += 16;  // the survivor's node, written and read back
+    }
+    __syncthreads();
+  }
+  add_wave(&a.counters[0], acc);
+  add_wave(&a.counters[1], valid);
+  add_wave(&a.counters[2], pairs);
+  if (a.stats) {
+    add_wave(&a.counters[3], bytes);
+    add_wave(&a.counters[8], mb);
+  }
+}
+
 // ------------------------------------- intersections |N(u) & N(w)|
 // Levels that count the last two positions together need, per row, the
 // size of N(u) & N(w) for two of the row's vertices u = r[pa], w = r[pb]
@@ -1286,6 +1396,17 @@ __global__ __launch_bounds__(BLOCK) void k_chain_shared_h2(GView g, KPlan P, con
   }
 }
 
+// Chain level with shared candidates, rows of L = k-3 words (level k-4):
+// every row's work is its parent's block of position-(k-3) candidates.
+template <int L>
+__global__ __launch_bounds__(BLOCK) void k_sib_len(const uint32_t* __restrict__ parent, const uint64_t* __restrict__ coff,
+                                                  uint64_t n, uint32_t* __restrict__ minlen) {
+  for (uint64_t q = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; q < n; q += (uint64_t)gridDim.x * BLOCK) {
+    const uint32_t par = parent[q];
+    minlen[q] = (uint32_t)(coff[par + 1] - coff[par]);
+  }
+}
+
 // Chain tables for thresholds (ta, tb): H[u] = sum over x in N(u) with
 // deg(x) >= ta of |{y in N(x) : deg(y) >= tb}|, H[V + u] = sum of those
 // deg(x).  One wave per vertex.
@@ -1463,7 +1584,9 @@ struct Matcher {
   GView g;
   uint32_t k;
   bool closed_last = false, fuse = false, write = false, pair = false, chain = false, chain_fused = false;
-  bool collapse_fused = false;
+  bool collapse_fused = false, sib = false;
+  bool sib_live = false;      // candidates of the current level-(k-4) input are built
+  uint64_t cand_limit = 0;    // most candidates one build may hold
   const uint32_t* tbl = nullptr;
   PairTables ptbl{};
   ChainTables ctab{};
@@ -1546,6 +1669,14 @@ struct Matcher {
     if (chain && L == k - 3) {
       chain_shared(L, rows, n, in);
       if (aborted) return;
+      if (sib && sib_live) {
+        sib_level(L, rows, n);
+        return;
+      }
+    }
+    if (sib && L == k - 4) {
+      build_candidates(L, rows, n, in);
+      if (aborted) return;
     }
     if (L == k - 1 && closed_last) {
       if (add_work(n)) return;
@@ -1620,7 +1751,7 @@ struct Matcher {
       a.pbase = mode == M_PAIR ? in.pbase : nullptr;
       a.nsurv = pair_common ? in.nsurv : nullptr;
       a.out_parent = nullptr;
-      if (chain && !chain_fused && mode == M_WRITE && L == k - 3) {
+      if ((chain && !chain_fused && mode == M_WRITE && L == k - 3) || (sib && mode == M_WRITE && L == k - 4)) {
         ensure_aux(out.parent, out.parent_cap, out.cap);
         a.out_parent = out.parent;
       }
@@ -1784,6 +1915,149 @@ struct Matcher {
     }
   }
 
+  // Expansion chunks of the rows (L words) under plan pk in a grouping mode
+  // (M_GCOUNT / M_GWRITE), over the work planned in in.minlen / in.minsel.
+  void group_pass(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in, const KPlan& pk, int mode) {
+    const uint64_t W = scan(in, n);
+    if (W == 0 || (mode == M_GCOUNT && add_work(W))) return;  // the write pass repeats the same work
+    for (uint64_t w0 = 0; w0 < W; w0 += MAX_CHUNK) {
+      if (past_deadline()) return;
+      const uint64_t w1 = std::min(W, w0 + MAX_CHUNK);
+      const uint64_t nt = (w1 - w0 + TILE - 1) / TILE;
+      ensure_tiles(in, nt);
+      hipLaunchKernelGGL(k_tile_index, dim3(grid_for(nt + 1, BLOCK, 65535)), dim3(BLOCK), 0, c.stream, in.work_off,
+                         n, w0, w1, nt, in.tile_p);
+      ExpandArgs a{};
+      a.rows = rows;
+      a.work_off = in.work_off;
+      a.minsel = in.minsel;
+      a.tile_p = in.tile_p;
+      a.w_begin = w0;
+      a.w_end = w1;
+      a.ntiles = nt;
+      a.counters = c.counters;
+      a.nsurv = in.nsurv;
+      a.coff = in.coff;
+      a.cand = in.cand;
+      a.xcd = (o.flags & 8u) ? 0 : 1;
+      a.tilectr = (unsigned int*)(c.counters + 4);
+      a.stats = prof ? 1 : 0;
+      HIPCHECK(hipMemsetAsync(a.tilectr, 0, 8 * sizeof(unsigned int), c.stream));
+      r.chunks++;
+      const uint32_t grid = grid_for(nt, 1, grid_cap);
+      pbegin(true);
+      if (mode == M_GCOUNT) {
+#define GC(LL) hipLaunchKernelGGL((k_expand<LL, M_GCOUNT>), dim3(grid), dim3(BLOCK), 0, c.stream, g, pk, a)
+        CUTS_DISPATCH_PAIR(L, GC)
+#undef GC
+      } else {
+#define GW(LL) hipLaunchKernelGGL((k_expand<LL, M_GWRITE>), dim3(grid), dim3(BLOCK), 0, c.stream, g, pk, a)
+        CUTS_DISPATCH_PAIR(L, GW)
+#undef GW
+      }
+      HIPCHECK(hipGetLastError());
+      pend(4, 0, true);  // an expansion that writes (candidates)
+    }
+  }
+
+  // Chain level with shared candidates, before the rows of level k-4 are
+  // written from the rows of level k-5 (L = k-4 words): the candidates of
+  // position k-3 for every row of level k-5 (its constraints and threshold
+  // checked, injective against the row), grouped by row into in.cand with
+  // block starts in in.coff.
+  void build_candidates(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in) {
+    KPlan vk = kp;  // position k-3 evaluated at position k-4
+    vk.nj[L] = kp.nj[L + 1];
+    for (uint32_t i = 0; i < 2 * kMaxQuery; ++i) vk.jc[L][i] = kp.jc[L + 1][i];
+    vk.sig_in[L] = kp.sig_in[L + 1];
+    vk.sig_out[L] = kp.sig_out[L + 1];
+    const uint32_t grid = grid_for(n, BLOCK, grid_cap);
+    auto plan_vk = [&]() {
+      pbegin(false);
+#define PV(LL) hipLaunchKernelGGL((k_plan<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, vk, rows, n, in.minlen, in.minsel, nullptr, nullptr, nullptr, ptbl, nullptr, c.counters)
+      CUTS_DISPATCH_PAIR(L, PV)
+#undef PV
+      HIPCHECK(hipGetLastError());
+      pend(1, (double)n * (4.0 * L + 8.0 * vk.nj[L] + 5.0), false);
+    };
+    sib_live = false;
+    ensure_aux(in.nsurv, in.nsurv_cap, n);
+    ensure_aux(in.coff, in.coff_cap, n + 1);
+    // survivors per row
+    plan_vk();
+    HIPCHECK(hipMemsetAsync(in.nsurv, 0, n * sizeof(uint32_t), c.stream));
+    group_pass(L, rows, n, in, vk, M_GCOUNT);
+    if (aborted) return;
+    // block starts: a scan of the counts
+    HIPCHECK(hipMemcpyAsync(in.minlen, in.nsurv, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c.stream));
+    const uint64_t total = scan(in, n);
+    // beyond the limit, the children of these rows find their candidates
+    // themselves (the fused chain expansion)
+    if (total > cand_limit) return;
+    HIPCHECK(hipMemcpyAsync(in.coff, in.work_off, (n + 1) * sizeof(uint64_t), hipMemcpyDeviceToDevice, c.stream));
+    ensure_aux(in.cand, in.cand_cap, total);
+    // the candidates, each into its row's block
+    plan_vk();
+    HIPCHECK(hipMemsetAsync(in.nsurv, 0, n * sizeof(uint32_t), c.stream));
+    group_pass(L, rows, n, in, vk, M_GWRITE);
+    sib_live = !aborted;
+  }
+
+  // Chain level with shared candidates over the rows of level k-4 (L = k-3
+  // words): every row paired with its parent's candidates (k_chain_sib).
+  void sib_level(uint32_t L, const uint32_t* rows, uint64_t n) {
+    LevelBuf& in = c.lv[L - 1];
+    LevelBuf& par = c.lv[L - 2];
+    const uint32_t grid = grid_for(n, BLOCK, grid_cap);
+    if (prof) {
+      // merge model: the expansion of these rows (the planner's figure)
+      unsigned long long* merge = c.counters + 8;
+      HIPCHECK(hipMemsetAsync(merge, 0, 8, c.stream));
+#define PS(LL) hipLaunchKernelGGL((k_plan<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.minlen, in.minsel, nullptr, nullptr, nullptr, ptbl, merge, c.counters)
+      CUTS_DISPATCH_PAIR(L, PS)
+#undef PS
+      HIPCHECK(hipGetLastError());
+      r.merge_bytes[10] += (double)read_u64(merge);
+    }
+    pbegin(false);
+    hipLaunchKernelGGL((k_sib_len<0>), dim3(grid), dim3(BLOCK), 0, c.stream, in.parent, par.coff, n, in.minlen);
+    HIPCHECK(hipGetLastError());
+    pend(1, (double)n * 20.0, false);
+    const uint64_t W = scan(in, n);
+    if (W == 0 || add_work(W)) return;
+    for (uint64_t w0 = 0; w0 < W; w0 += MAX_CHUNK) {
+      if (past_deadline()) return;
+      const uint64_t w1 = std::min(W, w0 + MAX_CHUNK);
+      const uint64_t nt = (w1 - w0 + TILE - 1) / TILE;
+      ensure_tiles(in, nt);
+      hipLaunchKernelGGL(k_tile_index, dim3(grid_for(nt + 1, BLOCK, 65535)), dim3(BLOCK), 0, c.stream, in.work_off,
+                         n, w0, w1, nt, in.tile_p);
+      SibArgs a{};
+      a.rows = rows;
+      a.parent = in.parent;
+      a.coff = par.coff;
+      a.cand = par.cand;
+      a.shared = in.nsurv;
+      a.work_off = in.work_off;
+      a.tile_p = in.tile_p;
+      a.w_begin = w0;
+      a.w_end = w1;
+      a.ntiles = nt;
+      a.counters = c.counters;
+      a.tilectr = (unsigned int*)(c.counters + 4);
+      HIPCHECK(hipMemsetAsync(a.tilectr, 0, 8 * sizeof(unsigned int), c.stream));
+      a.xcd = (o.flags & 8u) ? 0 : 1;
+      a.stats = prof ? 1 : 0;
+      r.chunks++;
+      pbegin(true);
+#define SB(LL) hipLaunchKernelGGL((k_chain_sib<LL>), dim3(grid_for(nt, 1, grid_cap)), dim3(BLOCK), 0, c.stream, g, kp, a, ctab)
+      CUTS_DISPATCH_PAIR(L, SB)
+#undef SB
+      HIPCHECK(hipGetLastError());
+      pend(10, 0, true);
+    }
+  }
+
   // Chain level over the rows of level k-3 (L = k-2 words): closed form per
   // row (k_chain_row), then the intersection with the newest vertex.
   void chain_rows(uint32_t L, const uint32_t* rows, uint64_t n) {
@@ -1827,6 +2101,8 @@ struct Matcher {
       // survivor sums and parent indices
       level_cap[l] = std::max<uint64_t>(per / ((l + 1) * 4 + 17 + 24), 1024);
     }
+    // shared chain candidates: up to one level's share of the budget
+    cand_limit = std::max<uint64_t>((nlev ? budget / nlev : budget) / 4, 1u << 16);
     if (closed_last && kp.sc_thr > 1) tbl = cnt_table(c, dg, kp.sc_thr);
     // with the two-hop index the chain is counted inside the expansion of
     // position k-3 (no rows of level k-3 are stored)
@@ -1835,6 +2111,8 @@ struct Matcher {
     // k-4 (no rows of level k-4 are stored); with a deadline, in the planner
     // of level k-3 instead (the rolled-back counters are those of 0..2)
     collapse_fused = pair && kp.pr_collapse && g.h2_off && k >= 5 && !(o.time_limit_ms > 0);
+    // the fused chain with position k-3's candidates shared by siblings
+    sib = chain_fused && kp.ch_sib && !(o.flags & 1024u);
     if (chain) {
       ctab.ta = kp.ch_ta > 1 ? cnt_table(c, dg, kp.ch_ta) : nullptr;
       ctab.tb = kp.ch_tb > 1 ? cnt_table(c, dg, kp.ch_tb) : nullptr;
@@ -2074,6 +2352,11 @@ KPlan make_kplan(const Plan& p, bool symmetric) {
       kp.ch_u = u;
       kp.ch_ta = std::max(p.sig_in[last - 1], p.sig_out[last - 1]);
       kp.ch_tb = kp.sc_thr;
+      // position k-3's constraints all older than k-4 (and at least one)
+      const uint32_t c = last - 2;
+      bool sib = p.k >= 5 && kp.nj[c] >= 1;
+      for (uint32_t i = 0; i < kp.nj[c]; ++i) sib = sib && (kp.jc[c][i] & 31) + 1 < c;
+      kp.ch_sib = sib;
     }
   }
   return kp;
@@ -2104,7 +2387,7 @@ Context::~Context() {
   for (auto& b : lv) {
     for (void* p : {(void*)b.rows, (void*)b.minlen, (void*)b.minsel, (void*)b.work_off, (void*)b.bsum,
                     (void*)b.tile_p, (void*)b.cbase, (void*)b.pbase,
-                    (void*)b.nsurv, (void*)b.parent})
+                    (void*)b.nsurv, (void*)b.parent, (void*)b.coff, (void*)b.cand})
       if (p) (void)hipFree(p);
   }
   if (counters) (void)hipFree(counters);

@@ -102,7 +102,8 @@ enum {
   CUTS_FLAG_NO_PREFIX_BASE = 1u << 6, /* fused closed form fully per work item */
   CUTS_FLAG_NO_PAIR = 1u << 7,        /* never count the last two positions together */
   CUTS_FLAG_NO_CHAIN = 1u << 8,       /* not as a chain either (k-1 hanging on k-2) */
-  CUTS_FLAG_NO_TWO_HOP = 1u << 9      /* intersections by list walks even where the two-hop index exists */
+  CUTS_FLAG_NO_TWO_HOP = 1u << 9,     /* intersections by list walks even where the two-hop index exists */
+  CUTS_FLAG_NO_SHARED = 1u << 10      /* chain level: position k-3's candidates found per row, not per parent */
 };
 
 /* Kernel classes reported by CUTS_FLAG_PROFILE. */

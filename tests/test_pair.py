@@ -113,7 +113,8 @@ def test_pair_level_vs_oracle_and_cpu_trie(ctx, name, seed):
     dg = ctx.upload(g)
     prof = cuts_amd.FLAG_PROFILE
     for flags in (0, cuts_amd.FLAG_NO_BITMAP, cuts_amd.FLAG_NO_TWO_HOP, cuts_amd.FLAG_NO_TWO_HOP | cuts_amd.FLAG_NO_BITMAP,
-                  cuts_amd.FLAG_NO_PAIR, cuts_amd.FLAG_NO_CHAIN, cuts_amd.FLAG_NO_PAIR | cuts_amd.FLAG_NO_CHAIN):
+                  cuts_amd.FLAG_NO_PAIR, cuts_amd.FLAG_NO_CHAIN, cuts_amd.FLAG_NO_PAIR | cuts_amd.FLAG_NO_CHAIN,
+                  cuts_amd.FLAG_NO_SHARED, cuts_amd.FLAG_NO_SHARED | cuts_amd.FLAG_NO_BITMAP):
         r = ctx.match(dg, p, flags=flags | prof)
         assert r.count == want, (name, flags)
         assert r.level_sizes == [cpu.level[i] for i in range(k)], (name, flags)
@@ -164,7 +165,8 @@ def test_pair_level_random_queries(ctx, seed):
     want = oracle.count(og, oracle.qmatrix(k, oracle.undirected(edges)))
     p = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges))
     dg = ctx.upload(g)
-    for flags in (0, cuts_amd.FLAG_NO_PAIR | cuts_amd.FLAG_NO_CHAIN, cuts_amd.FLAG_NO_BITMAP, cuts_amd.FLAG_NO_TWO_HOP):
+    for flags in (0, cuts_amd.FLAG_NO_PAIR | cuts_amd.FLAG_NO_CHAIN, cuts_amd.FLAG_NO_BITMAP, cuts_amd.FLAG_NO_TWO_HOP,
+                  cuts_amd.FLAG_NO_SHARED):
         assert ctx.match(dg, p, flags=flags).count == want, (edges, flags)
 
 
