@@ -2102,7 +2102,7 @@ struct Matcher {
       level_cap[l] = std::max<uint64_t>(per / ((l + 1) * 4 + 17 + 24), 1024);
     }
     // shared chain candidates: up to one level's share of the budget
-    cand_limit = std::max<uint64_t>((nlev ? budget / nlev : budget) / 4, 1u << 16);
+    cand_limit = std::max<uint64_t>((nlev ? budget / nlev : budget) / 4, 1u << 10);
     if (closed_last && kp.sc_thr > 1) tbl = cnt_table(c, dg, kp.sc_thr);
     // with the two-hop index the chain is counted inside the expansion of
     // position k-3 (no rows of level k-3 are stored)
