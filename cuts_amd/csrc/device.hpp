@@ -97,9 +97,9 @@ struct DeviceGraph {
   uint64_t* h2_off = nullptr;
   uint32_t *h2_v = nullptr, *h2_c = nullptr;
   uint64_t h2_entries = 0;
-  // dense rows of the index for vertices whose two-hop list covers at least
-  // a quarter of the graph: h2_did[u] = row or -1, a row = V uint16 counts
-  // (a lookup is one load; no larger than the list it replaces)
+  // dense rows of the index for vertices whose two-hop list holds at least
+  // V/64 entries: h2_did[u] = row or -1, a row = V uint16 counts (a lookup
+  // is one load instead of a lower-bound search)
   int32_t* h2_did = nullptr;
   uint16_t* h2_dense = nullptr;
   uint32_t h2_dense_rows = 0;

@@ -2517,12 +2517,17 @@ void upload_graph(Context& c, const Graph& g, DeviceGraph& dg) {
       std::vector<uint32_t> hv, hc;
       build_two_hop(g, off, hv, hc);
       dg.h2_entries = off[V];
-      // dense rows for the longest lists (at least V/4 entries), while their
-      // bytes stay within 1/32 of free HBM; counts fit uint16 below degree 2^16
+      // dense rows for the longest lists (at least V/64 entries, or
+      // V/CUTS_H2_DENSE_DIV; 1 turns them off), while their bytes stay within
+      // 1/32 of free HBM; counts fit uint16 below degree 2^16.  A lookup in a
+      // row is one load instead of a lower-bound search; below V/64 entries
+      // the search is short and cached and rows gain nothing more
       if (dg.max_deg < 65536) {
+        uint64_t div = 64;
+        if (const char* e = std::getenv("CUTS_H2_DENSE_DIV")) div = std::max<uint64_t>(1, std::strtoull(e, nullptr, 10));
         std::vector<uint32_t> cand;
         for (uint32_t u = 0; u < V; ++u)
-          if ((off[u + 1] - off[u]) * 4 >= (uint64_t)V) cand.push_back(u);
+          if ((off[u + 1] - off[u]) * div >= (uint64_t)V) cand.push_back(u);
         std::sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) {
           return off[a + 1] - off[a] > off[b + 1] - off[b] || (off[a + 1] - off[a] == off[b + 1] - off[b] && a < b);
         });

@@ -171,6 +171,26 @@ def test_pair_level_random_queries(ctx, seed):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("div", ["1", "8", "1000000"])
+def test_two_hop_dense_rows_and_searches_agree(ctx, monkeypatch, div):
+    """Two-hop lookups by lower-bound search only (CUTS_H2_DENSE_DIV=1: no
+    vertex's list reaches V entries), by dense rows for the longest lists, and
+    by dense rows for every vertex: the same counts as the oracle."""
+    monkeypatch.setenv("CUTS_H2_DENSE_DIV", div)
+    for seed, names in ((9301, ("4cycle+2pend", "k4e+tail2", "4cycle+tail2")),
+                        (9302, ("triangle+2pend-two", "k4+tail2", "path5"))):
+        g = graph_for(seed)
+        og = oracle.OGraph.from_view(g.view())
+        dg = ctx.upload(g)
+        for name in names:
+            edges = SHAPES[name]
+            k = 1 + max(max(e) for e in edges)
+            want = oracle.count(og, oracle.qmatrix(k, oracle.undirected(edges)))
+            p = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges))
+            assert ctx.match(dg, p).count == want, (div, seed, name)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("name", ["4cycle+2pend", "triangle+2pend-two", "k4e+tail2", "star4"])
 def test_last_two_levels_under_a_deadline(ctx, name):
     """With a time limit the run stops between chunks and rolls back a cut
