@@ -1398,7 +1398,6 @@ __global__ __launch_bounds__(BLOCK) void k_chain_shared_h2(GView g, KPlan P, con
 
 // Chain level with shared candidates, rows of L = k-3 words (level k-4):
 // every row's work is its parent's block of position-(k-3) candidates.
-template <int L>
 __global__ __launch_bounds__(BLOCK) void k_sib_len(const uint32_t* __restrict__ parent, const uint64_t* __restrict__ coff,
                                                   uint64_t n, uint32_t* __restrict__ minlen) {
   for (uint64_t q = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; q < n; q += (uint64_t)gridDim.x * BLOCK) {
@@ -2020,7 +2019,7 @@ struct Matcher {
       r.merge_bytes[10] += (double)read_u64(merge);
     }
     pbegin(false);
-    hipLaunchKernelGGL((k_sib_len<0>), dim3(grid), dim3(BLOCK), 0, c.stream, in.parent, par.coff, n, in.minlen);
+    hipLaunchKernelGGL(k_sib_len, dim3(grid), dim3(BLOCK), 0, c.stream, in.parent, par.coff, n, in.minlen);
     HIPCHECK(hipGetLastError());
     pend(1, (double)n * 20.0, false);
     const uint64_t W = scan(in, n);
