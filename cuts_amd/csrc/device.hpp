@@ -118,12 +118,14 @@ struct LevelBuf {
   uint32_t* cbase = nullptr;    // cap: per-prefix closed-form base (fused levels)
   // allocated only on the levels that use them (pbase_cap etc. rows each)
   uint4* pbase = nullptr;       // per-prefix |A|, |B|, |A & B| bases (pair level)
-  uint32_t* nsurv = nullptr;    // per prefix, survivors (pair level) or the shared
-                                // intersections of its children (chain level)
+  uint32_t* nsurv = nullptr;    // per prefix, survivors (pair level), the shared
+                                // intersections of its children (chain level) or
+                                // the filled length of its block in `cand`
   uint32_t* parent = nullptr;   // parent prefix of every row (rows of the chain level)
   uint64_t pbase_cap = 0, nsurv_cap = 0, parent_cap = 0;
   // chain level with shared candidates: per prefix of this level, the start
-  // of its block in `cand` (n+1) and the candidates of position k-3
+  // of its block in `cand` (n+1; a block holds as many entries as the
+  // prefix's shortest constraint list) and the candidates of position k-3
   uint64_t* coff = nullptr;
   uint32_t* cand = nullptr;
   uint64_t coff_cap = 0, cand_cap = 0;

@@ -607,7 +607,7 @@ __device__ __forceinline__ void add_runs(uint32_t* __restrict__ cnt, uint64_t p,
   if (head && mine) atomicAdd(&cnt[p], (uint32_t)__popcll(mine));
 }
 
-enum { M_WRITE = 0, M_COUNT = 1, M_FUSED = 2, M_PAIR = 3, M_CHAIN = 4, M_COLLAPSE = 5, M_GCOUNT = 6, M_GWRITE = 7 };
+enum { M_WRITE = 0, M_COUNT = 1, M_FUSED = 2, M_PAIR = 3, M_CHAIN = 4, M_COLLAPSE = 5, M_GWRITE = 6 };
 
 struct ExpandArgs {
   const uint32_t* rows;      // n rows of L words
@@ -689,7 +689,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
       // independent and in flight together instead of one dependent chain
       // after another.  Items past w1 are clamped to the tile's last prefix
       // and masked; their list word is never read.
-      constexpr int G = (MODE != M_COUNT && MODE != M_GCOUNT && MODE != M_GWRITE) ? ((L <= 6) ? 2 : 1)
+      constexpr int G = (MODE != M_COUNT && MODE != M_GWRITE) ? ((L <= 6) ? 2 : 1)
                                                                                   : ((L <= 6) ? 4 : (L <= 10 ? 2 : 1));
       for (int g0 = 0; g0 < IPT; g0 += G) {
         uint64_t w[G];
@@ -796,10 +796,6 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
         if (MODE == M_COUNT) {
 #pragma unroll
           for (int j = 0; j < G; ++j) acc += ok[j];
-        } else if (MODE == M_GCOUNT) {
-          // survivors per prefix (all lanes)
-#pragma unroll
-          for (int j = 0; j < G; ++j) add_runs(a.nsurv, plo + min(pi[j], span - 1), ok[j]);
         } else if (MODE == M_GWRITE) {
           // survivors into their prefix's block (order within a block free)
 #pragma unroll
@@ -1397,13 +1393,12 @@ __global__ __launch_bounds__(BLOCK) void k_chain_shared_h2(GView g, KPlan P, con
 }
 
 // Chain level with shared candidates, rows of L = k-3 words (level k-4):
-// every row's work is its parent's block of position-(k-3) candidates.
-__global__ __launch_bounds__(BLOCK) void k_sib_len(const uint32_t* __restrict__ parent, const uint64_t* __restrict__ coff,
+// every row's work is the filled part of its parent's block of
+// position-(k-3) candidates (cnt = the parents' filled lengths).
+__global__ __launch_bounds__(BLOCK) void k_sib_len(const uint32_t* __restrict__ parent, const uint32_t* __restrict__ cnt,
                                                   uint64_t n, uint32_t* __restrict__ minlen) {
-  for (uint64_t q = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; q < n; q += (uint64_t)gridDim.x * BLOCK) {
-    const uint32_t par = parent[q];
-    minlen[q] = (uint32_t)(coff[par + 1] - coff[par]);
-  }
+  for (uint64_t q = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; q < n; q += (uint64_t)gridDim.x * BLOCK)
+    minlen[q] = cnt[parent[q]];
 }
 
 // Chain tables for thresholds (ta, tb): H[u] = sum over x in N(u) with
@@ -1914,11 +1909,12 @@ struct Matcher {
     }
   }
 
-  // Expansion chunks of the rows (L words) under plan pk in a grouping mode
-  // (M_GCOUNT / M_GWRITE), over the work planned in in.minlen / in.minsel.
-  void group_pass(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in, const KPlan& pk, int mode) {
+  // Expansion chunks of the rows (L words) under plan pk whose survivors go
+  // into their row's block of in.cand (M_GWRITE), over the work planned in
+  // in.minlen / in.minsel.
+  void write_candidates(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in, const KPlan& pk) {
     const uint64_t W = scan(in, n);
-    if (W == 0 || (mode == M_GCOUNT && add_work(W))) return;  // the write pass repeats the same work
+    if (W == 0 || add_work(W)) return;
     for (uint64_t w0 = 0; w0 < W; w0 += MAX_CHUNK) {
       if (past_deadline()) return;
       const uint64_t w1 = std::min(W, w0 + MAX_CHUNK);
@@ -1945,15 +1941,9 @@ struct Matcher {
       r.chunks++;
       const uint32_t grid = grid_for(nt, 1, grid_cap);
       pbegin(true);
-      if (mode == M_GCOUNT) {
-#define GC(LL) hipLaunchKernelGGL((k_expand<LL, M_GCOUNT>), dim3(grid), dim3(BLOCK), 0, c.stream, g, pk, a)
-        CUTS_DISPATCH_PAIR(L, GC)
-#undef GC
-      } else {
 #define GW(LL) hipLaunchKernelGGL((k_expand<LL, M_GWRITE>), dim3(grid), dim3(BLOCK), 0, c.stream, g, pk, a)
-        CUTS_DISPATCH_PAIR(L, GW)
+      CUTS_DISPATCH_PAIR(L, GW)
 #undef GW
-      }
       HIPCHECK(hipGetLastError());
       pend(4, 0, true);  // an expansion that writes (candidates)
     }
@@ -1982,23 +1972,18 @@ struct Matcher {
     sib_live = false;
     ensure_aux(in.nsurv, in.nsurv_cap, n);
     ensure_aux(in.coff, in.coff_cap, n + 1);
-    // survivors per row
+    // a row's shortest constraint list bounds its candidates: blocks sized by
+    // the bounds (the scan of the planner's lengths), filled in one pass, with
+    // the filled length of every block in in.nsurv
     plan_vk();
-    HIPCHECK(hipMemsetAsync(in.nsurv, 0, n * sizeof(uint32_t), c.stream));
-    group_pass(L, rows, n, in, vk, M_GCOUNT);
-    if (aborted) return;
-    // block starts: a scan of the counts
-    HIPCHECK(hipMemcpyAsync(in.minlen, in.nsurv, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c.stream));
     const uint64_t total = scan(in, n);
     // beyond the limit, the children of these rows find their candidates
     // themselves (the fused chain expansion)
     if (total > cand_limit) return;
     HIPCHECK(hipMemcpyAsync(in.coff, in.work_off, (n + 1) * sizeof(uint64_t), hipMemcpyDeviceToDevice, c.stream));
     ensure_aux(in.cand, in.cand_cap, total);
-    // the candidates, each into its row's block
-    plan_vk();
     HIPCHECK(hipMemsetAsync(in.nsurv, 0, n * sizeof(uint32_t), c.stream));
-    group_pass(L, rows, n, in, vk, M_GWRITE);
+    write_candidates(L, rows, n, in, vk);
     sib_live = !aborted;
   }
 
@@ -2019,7 +2004,7 @@ struct Matcher {
       r.merge_bytes[10] += (double)read_u64(merge);
     }
     pbegin(false);
-    hipLaunchKernelGGL(k_sib_len, dim3(grid), dim3(BLOCK), 0, c.stream, in.parent, par.coff, n, in.minlen);
+    hipLaunchKernelGGL(k_sib_len, dim3(grid), dim3(BLOCK), 0, c.stream, in.parent, par.nsurv, n, in.minlen);
     HIPCHECK(hipGetLastError());
     pend(1, (double)n * 20.0, false);
     const uint64_t W = scan(in, n);
