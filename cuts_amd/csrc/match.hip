@@ -2070,7 +2070,9 @@ struct Matcher {
     fuse = closed_last && !(o.flags & 2u) && k >= 3;
     pair = fuse && kp.pr_ok && !(o.flags & 128u);
     chain = fuse && !pair && kp.ch_ok && !(o.flags & 256u);
-    grid_cap = (uint32_t)c.cus * (o.blocks_per_cu ? o.blocks_per_cu : 8);
+    // 6 blocks of 4 waves per CU: measured best on the power-law workloads
+    // (the tiles are taken dynamically, so the grid only sets residency)
+    grid_cap = (uint32_t)c.cus * (o.blocks_per_cu ? o.blocks_per_cu : 6);
     budget = o.budget ? o.budget : c.default_budget;
     // split the budget evenly over the materialised levels 1..k-2 (or k-1
     // when writing embeddings); each row of level l costs (l+1)*4 bytes plus

@@ -129,7 +129,7 @@ typedef struct {
   /* level-0 restriction: only data vertices root_begin + i*root_stride
    * (< root_end) may match position 0; all zero = every vertex. */
   uint32_t root_begin, root_end, root_stride;
-  uint32_t blocks_per_cu; /* launch shape for the expansion kernels; 0 = default */
+  uint32_t blocks_per_cu; /* launch shape for the expansion kernels; 0 = default (6 blocks per CU) */
   /* CUTS_FLAG_WRITE: host buffer receiving up to emb_cap rows of k words,
    * row = data vertices in matching-order positions (ans.txt layout). */
   uint32_t *emb_rows;
