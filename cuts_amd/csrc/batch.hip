@@ -765,7 +765,7 @@ void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& p
   b.kmax = 0;
   for (uint32_t q = 0; q < b.nq; ++q) {
     if (plans[q]->k == 0 || plans[q]->k > kMaxQuery) throw Error(-3, "batch: query size out of range");
-    kp[q] = make_kplan(*plans[q], dg.symmetric);
+    kp[q] = make_kplan(*plans[q], dg.symmetric, dg.loops == 0);
     b.k.push_back(plans[q]->k);
     b.kmax = std::max(b.kmax, plans[q]->k);
     if (kp[q].sc_ok && kp[q].sc_thr > 1 && !(o.flags & 1u)) tb[3 * q] = cnt_table(c, dg, kp[q].sc_thr);

@@ -68,7 +68,11 @@ struct KPlan {
   uint32_t qadj[kMaxQuery];       // bit j of qadj[i]: positions i and j adjacent in the query
 };
 
-KPlan make_kplan(const Plan& p, bool symmetric_data);
+// loop_free: the data graph has no self-loop.  The pair, chain and collapsed
+// closed forms assume u not in N(u); on a graph with self-loops they are
+// off and those positions are expanded (the last-level closed form counts
+// u in N(u) itself and stays on).
+KPlan make_kplan(const Plan& p, bool symmetric_data, bool loop_free);
 
 struct DeviceGraph {
   int device = 0;
@@ -76,6 +80,7 @@ struct DeviceGraph {
   uint64_t A = 0;
   bool symmetric = false;
   uint32_t max_deg = 0;
+  uint64_t loops = 0;  // vertices u with u in out(u)
   uint32_t *off_out = nullptr, *adj_out = nullptr, *off_in = nullptr, *adj_in = nullptr;
   uint32_t* sig = nullptr;                 // 2V: in, out
   // signature of every list entry, parallel to adj_out / adj_in (the in-side

@@ -2059,7 +2059,7 @@ struct Matcher {
   void run() {
     k = plan.k;
     if (k == 0 || k > kMaxQuery) throw Error(-3, "query must have between 1 and 16 vertices");
-    kp = make_kplan(plan, dg.symmetric);
+    kp = make_kplan(plan, dg.symmetric, dg.loops == 0);
     g = view_of(dg);
     if (o.flags & 32u) g.hub_id = nullptr;  // CUTS_FLAG_NO_BITMAP
     if (o.flags & 512u) g.h2_off = nullptr;  // CUTS_FLAG_NO_TWO_HOP
@@ -2266,7 +2266,7 @@ double copy_probe(Context& c, uint64_t bytes, uint32_t iters) {
   return (double)iters * 2.0 * (double)n * 16.0 / (ms / 1e3) / 1e9;
 }
 
-KPlan make_kplan(const Plan& p, bool symmetric) {
+KPlan make_kplan(const Plan& p, bool symmetric, bool loop_free) {
   KPlan kp;
   std::memset(&kp, 0, sizeof(kp));
   kp.k = p.k;
@@ -2302,7 +2302,7 @@ KPlan make_kplan(const Plan& p, bool symmetric) {
   // the last two positions together: a = k-2 and b = k-1 not adjacent, one
   // backward neighbour each, both older than k-3 (so |A & B| depends on the
   // rows of level k-4 only)
-  if (kp.sc_ok && p.k >= 4 && kp.nj[last - 1] == 1) {
+  if (kp.sc_ok && loop_free && p.k >= 4 && kp.nj[last - 1] == 1) {
     const uint32_t pa = kp.jc[last - 1][0] & 31, pb = kp.sc_pos;
     if (pb != last - 1 && pa + 3 < p.k && pb + 3 < p.k) {
       kp.pr_ok = 1;
@@ -2331,7 +2331,7 @@ KPlan make_kplan(const Plan& p, bool symmetric) {
       if (i != j && ((p.pmask[std::max(i, j)] | p.cmask[std::max(i, j)]) >> std::min(i, j) & 1)) kp.qadj[i] |= 1u << j;
   // the last two positions as a chain: k-1 hangs on k-2 only, k-2 on one
   // position older than k-3 only
-  if (kp.sc_ok && p.k >= 4 && kp.sc_pos == last - 1 && kp.nj[last - 1] == 1) {
+  if (kp.sc_ok && loop_free && p.k >= 4 && kp.sc_pos == last - 1 && kp.nj[last - 1] == 1) {
     const uint32_t u = kp.jc[last - 1][0] & 31;
     if (u + 3 < p.k) {
       kp.ch_ok = 1;
@@ -2414,8 +2414,13 @@ void upload_graph(Context& c, const Graph& g, DeviceGraph& dg) {
   dg.A = g.arcs();
   dg.symmetric = g.symmetric;
   uint32_t md = 0;
-  for (uint32_t v = 0; v < g.V; ++v) md = std::max(md, g.out_off[v + 1] - g.out_off[v]);
+  uint64_t loops = 0;
+  for (uint32_t v = 0; v < g.V; ++v) {
+    md = std::max(md, g.out_off[v + 1] - g.out_off[v]);
+    loops += std::binary_search(g.out_adj.begin() + g.out_off[v], g.out_adj.begin() + g.out_off[v + 1], v);
+  }
   dg.max_deg = md;
+  dg.loops = loops;
   auto up = [&](uint32_t*& d, const std::vector<uint32_t>& h) {
     dev_alloc(d, h.size());
     HIPCHECK(hipMemcpy(d, h.data(), h.size() * 4, hipMemcpyHostToDevice));

@@ -21,6 +21,7 @@ walk's, which materialises every level.
 """
 import random
 
+import numpy as np
 import pytest
 
 import cuts_amd
@@ -212,3 +213,101 @@ def test_last_two_levels_under_a_deadline(ctx, name):
     if part.roots_done:
         again = ctx.match(dg, p, roots=(0, part.last_root + 1, 1))
         assert part.count == again.count and again.level_sizes[0] == part.roots_done
+
+
+def _chain_formula(N, deg, P, cu, ta, tb):
+    """The chain closed form as the kernels evaluate it (chain_closed in
+    cuts_amd/csrc/match.hip): completions of positions k-2 (on P[cu]) and
+    k-1 (on k-2) for the prefix P, from per-vertex tables and intersections.
+    Valid on loop-free symmetric graphs with ta <= 2."""
+    u = P[cu]
+    cnt = lambda x, t: sum(1 for y in N[x] if deg[y] >= t)
+    H = sum(cnt(x, tb) for x in N[u] if deg[x] >= ta)
+    inu = [l for l in range(len(P)) if l != cu and P[l] in N[u] and deg[P[l]] >= ta]
+    na = cnt(u, ta) - len(inu)
+    c = H - sum(cnt(P[l], tb) for l in inu)
+    if deg[u] >= tb:
+        c -= na
+    for i in range(len(P)):
+        if i == cu or deg[P[i]] < tb:
+            continue
+        c -= len(N[u] & N[P[i]])
+        c += sum(1 for j in inu if j != i and P[j] in N[P[i]])
+    return c
+
+
+def _chain_brute(N, deg, P, cu, ta, tb):
+    u, used = P[cu], set(P)
+    return sum(1 for x in N[u] if deg[x] >= ta and x not in used
+               for y in N[x] if deg[y] >= tb and y not in used and y != x)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_chain_formula_vs_brute_force_loop_free(seed):
+    """CPU pin of the chain closed form on random loop-free symmetric graphs,
+    random injective prefixes (with and without edges among them) and every
+    threshold pair the planner admits (ta <= 2)."""
+    rng = random.Random(4200 + seed)
+    n = rng.randint(12, 40)
+    N = [set() for _ in range(n)]
+    for _ in range(rng.randint(n, 4 * n)):
+        a, b = rng.randrange(n), rng.randrange(n)
+        if a != b:
+            N[a].add(b)
+            N[b].add(a)
+    deg = [len(s) for s in N]
+    for _ in range(40):
+        L = rng.randint(2, 6)
+        P = rng.sample(range(n), L)
+        cu = rng.randrange(L)
+        for ta in (1, 2):
+            for tb in (1, 2, 3, 5):
+                assert _chain_formula(N, deg, P, cu, ta, tb) == _chain_brute(N, deg, P, cu, ta, tb)
+
+
+def _with_loops(g, loops):
+    v = g.view()
+    src = np.repeat(np.arange(v["V"], dtype=np.uint32), np.diff(v["out_off"]).astype(np.int64))
+    dst = v["out_adj"]
+    lo = np.asarray(loops, dtype=np.uint32)
+    return cuts_amd.Graph.from_arcs(v["V"], np.concatenate([src, lo]), np.concatenate([dst, lo]), symmetrize=True)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", range(3))
+def test_symmetric_graph_with_self_loops(ctx, seed):
+    """Symmetric data graphs with self-loops (the loader keeps them; degrees
+    count them): the pair, chain and collapsed closed forms assume u is not
+    in N(u), so the planner turns them off and expands those positions; the
+    last-level closed form counts u in N(u) itself and stays on.  Every mode
+    and the batch kernel give the oracle's count, and no pair or chain kernel
+    runs."""
+    g0 = graph_for(9500 + seed)
+    V = g0.V
+    deg = np.diff(g0.view()["out_off"])
+    rng = random.Random(9500 + seed)
+    hubs = np.argsort(-deg)[:3].tolist()
+    g = _with_loops(g0, hubs + rng.sample(range(V), V // 4))
+    v = g.view()
+    assert v["symmetric"]
+    og = oracle.OGraph.from_view(v)
+    dg = ctx.upload(g)
+    plans, wants = [], []
+    for name in ("triangle+tail2", "k4e+tail2", "4cycle+2pend", "triangle+2pend-two", "star3", "path5",
+                 "4cycle+tail2", "5cycle+chord+2tails"):
+        edges = SHAPES[name]
+        k = 1 + max(max(e) for e in edges)
+        want = oracle.count(og, oracle.qmatrix(k, oracle.undirected(edges)))
+        p = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges))
+        for flags in (0, cuts_amd.FLAG_NO_TWO_HOP, cuts_amd.FLAG_NO_SHARED, cuts_amd.FLAG_NO_BITMAP,
+                      cuts_amd.FLAG_NO_FUSE, cuts_amd.FLAG_NO_SHORTCUT):
+            r = ctx.match(dg, p, flags=flags | cuts_amd.FLAG_PROFILE)
+            assert r.count == want, (name, flags)
+            assert r.kernels["pair"][1] + r.kernels["chain"][1] + r.kernels["common"][1] == 0, (name, flags)
+        plans.append(p)
+        wants.append(want)
+    assert cuts_amd.Batch(ctx, dg, plans).run().counts == wants
+    # the same graph without its loops still takes the closed forms
+    dg0 = ctx.upload(g0)
+    r = ctx.match(dg0, plans[1], flags=cuts_amd.FLAG_PROFILE)
+    assert r.kernels["chain"][1] > 0
