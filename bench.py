@@ -1,35 +1,44 @@
-"""Benchmark: exact embeddings/s and p50 time per query on one MI355X.
+"""Benchmark: exact embeddings/s and p50 time per query on one MI355X,
+learned order against the score (heuristic) order on the same GPU.
 
-Workload: the mid-size power-law Chung-Lu data graph (n=36,692,
-m=183,831, exponent 2.1, degree cap 1,400, seed 0) resident in HBM, and its
-two 6-vertex queries (a 4-cycle with two pendants; K4 minus an edge with a
-2-edge tail).  One step = matching both queries exactly (uint64 counts) with
-the heuristic score order.  value = total embeddings / total device time of
-the timed steps.
+Default workload (the headline): the R-MAT scale-20 data graph (1,048,576
+vertices, edge factor 16, a,b,c = .57,.19,.19, seed 0; 31.4M arcs after
+dedup, loop removal and symmetrisation) resident in HBM, and the queries
+matched against it (tests/golden/rmat20_queries.json):
+  * the 7-vertex 5-cycle + chord + 2 tails query, exact on the root sample
+    v = 127 (mod 128) of order position 0 (its full count exceeds 2^64);
+  * the held-out evaluation set of the learned order: 500 random-walk
+    queries of 6-10 vertices, each exact on its admitted root sample.
+Orders: the score order of src/graph.cpp:62-93 / src/score.cpp:18-99, and
+the learned order -- the table-size network (weights trained on R-MAT
+scale-18 graphs by python -m cuts_amd.train, committed in cuts_amd/data)
+predicts |T| of every connected prefix set, a DP picks the order whose
+expanded levels are cheapest for this engine, and the score order judges
+it by running both (the proposal is kept only if it ran faster).
+One step = every query matched once with the learned orders; the same
+queries with the score orders are timed in the same loop.  value = total
+embeddings / total device time of the learned passes (HIP events on the
+library's stream, first matcher kernel to count readback: the window of
+src/host_funcs.cu:136-162).  Every count is checked against the fixture.
 
-The dominant kernel's roofline comes from HIP events recorded around every
-launch on the library's own stream during the timed steps
-(CUTS_FLAG_PROFILE).  Its algorithmic bytes are the loads its algorithm
-issues, counted on the device per work item (list words, signatures,
-membership probes, table entries, two-hop lookups).  The merge model
-(8 + 4*len per distinct backward list of every prefix, 16 per trie node;
-the sum over all kernels equals the CPU trie walk's figure) is reported for
-the whole path beside it.  The CPU baseline times the OpenMP port of the
-same trie algorithm (oracle/, test infrastructure) on a bounded sample of
-level 0.
+The roofline of the dominant kernel class comes from a profiling pass after
+the timed steps (CUTS_FLAG_PROFILE: HIP events around every launch):
+achieved = HBM bytes per dispatch measured by rocprofv3 --pmc
+(profiles/traffic.json) / the live average launch time.  The loads the
+kernel's algorithm issues (demand bytes, counted on the device; mostly
+L2/MALL hits) and the merge model (8 + 4*len per backward list of every
+prefix, 16 per trie node) are reported beside it under their own names.
+The CPU baseline times the OpenMP port of the same trie algorithm
+(oracle/, test infrastructure) with the score order on a bounded sample.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 
-Other workloads (one JSON line each; the default above is the headline):
+Other workloads (one JSON line each):
+    --workload chunglu36k  the mid-size Chung-Lu graph (n=36,692) and its two 6-vertex queries,
+                           full counts
     --workload chunglu4k   triangle + 2-edge tail on Chung-Lu n=4,096 (full count)
-    --workload rmat20      5-cycle + chord + 2 tails on R-MAT scale 20: the full
-                           count exceeds 2^64, so the query is matched exactly
-                           on a nested root sample of position 0 sized to 2^36
-                           candidates
     --workload batch256    256 random-walk queries, batched level queue in one hipGraph
                            launch, learned orders, autotuned launch shape
-    --workload learned     train the table-size network on R-MAT scale 18,
-                           evaluate learned vs score order on R-MAT scale 20
 """
 import argparse
 import json
@@ -46,41 +55,84 @@ BATCH_ARENA = 48 << 30   # two halves of 24 GiB: the largest trie level of the w
 BATCH_QUEUE = 1 << 26    # work items per level
 
 
-def cpu_baseline(g, shapes, budget_s=20.0):
-    """OpenMP trie baseline (same algorithm, same order) on a level-0 sample:
-    every 64th level-0 vertex, stopped between row groups after
-    budget_s/len(shapes) seconds per query; the rate counts only embeddings
-    of sub-tries that were finished."""
+def cpu_threads():
+    """(threads used, nproc, omp_get_max_threads) for the CPU baseline: the
+    OpenMP pool of the oracle library, at most 16 (the host share of one
+    GPU on the box)."""
+    import oracle
+    omp = oracle.max_threads()
+    return min(16, omp), os.cpu_count() or 1, omp
+
+
+def cpu_baseline(g, jobs, budget_s=20.0):
+    """OpenMP trie baseline (same algorithm, the score order) on a bounded
+    sample: jobs are (label, k, edges, roots, level-0 sample stride); each
+    gets budget_s/len(jobs) seconds, stopped between row groups; the rate
+    counts only embeddings of sub-tries that were finished."""
     import oracle
     import cuts_amd
-    from cuts_amd import workloads as W
     og = oracle.OGraph.from_view(g.view())
-    # the GPU box grants 16 host cores per GPU whatever os.cpu_count() says
-    threads = min(16, os.cpu_count() or 1)
+    threads, nproc, omp = cpu_threads()
     total_emb, total_s, samples = 0, 0.0, []
-    for shape in shapes:
-        qa = oracle.qmatrix(shape.k, W.shape_arcs(shape))
-        order = cuts_amd.Plan(W.query_graph(shape)).order
-        stride = 64
+    for label, k, edges, roots, stride in jobs:
+        qa = oracle.qmatrix(k, oracle.undirected(edges))
+        order = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges)).order
         r = oracle.trie_cpu(og, qa, order, sample_stride=stride, max_rows=1 << 22, nthreads=threads,
-                            max_seconds=budget_s / len(shapes))
+                            max_seconds=budget_s / len(jobs), roots=roots)
         total_emb += r.count
         total_s += r.seconds
-        samples.append(f"{shape.name}: every {stride}th level-0 vertex, {'all' if r.complete else 'time-capped'}; "
-                       f"{r.count} embeddings in {r.seconds:.2f}s")
-    return dict(value=total_emb / total_s if total_s else 0.0, unit="embeddings/s", cores=threads, kind="port",
-                sample="; ".join(samples))
+        samples.append(f"{label}: every {stride}th level-0 vertex of roots {tuple(roots)}, "
+                       f"{'all' if r.complete else 'time-capped'}; {r.count} embeddings in {r.seconds:.2f}s")
+    return dict(value=total_emb / total_s if total_s else 0.0, unit="embeddings/s", cores=threads, nproc=nproc,
+                omp_max_threads=omp, kind="port", sample="; ".join(samples))
 
 
-def traffic_for(kernel):
-    """HBM bytes per dispatch from the committed rocprofv3 --pmc summary."""
+def traffic_for(key):
+    """HBM bytes per dispatch of a kernel class on a workload ("workload:class"),
+    from the committed rocprofv3 --pmc summaries (profiles/traffic.json)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
-    e = t.get(kernel)
-    return None if e is None else e.get("bytes_per_dispatch")
+    e = t.get(key)
+    return None if e is None else e
+
+
+def profile_pass(ctx, dg, plans, roots):
+    """One run of every plan with HIP events around every launch (outside the
+    timed steps): per kernel class [ms, launches, demand bytes, merge bytes]."""
+    import cuts_amd
+    kern = {}
+    for p, rt in zip(plans, roots):
+        r = ctx.match(dg, p, roots=rt, flags=cuts_amd.FLAG_PROFILE)
+        for name, (ms, n, b) in r.kernels.items():
+            k = kern.setdefault(name, [0.0, 0, 0.0, 0.0])
+            k[0] += ms
+            k[1] += n
+            k[2] += b
+            k[3] += r.merge_bytes[name]
+    return kern
+
+
+def roofline(kern, workload):
+    """Roofline of the dominant class: achieved = rocprof HBM bytes per
+    dispatch / live average launch time."""
+    dname, (dms, dn, dbytes, dmerge) = max(kern.items(), key=lambda kv: kv[1][0])
+    avg_s = dms / max(dn, 1) / 1e3
+    t = traffic_for(f"{workload}:{dname}")
+    hbm = t["bytes_per_dispatch"] if t else None
+    ach = hbm / avg_s / 1e9 if (hbm and avg_s) else None
+    return {"bound": "hbm", "kernel": dname, "kernels": t["kernels"] if t else None,
+            "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS if ach is not None else None, "traffic": hbm,
+            "avg_launch_ms": avg_s * 1e3, "launches": dn,
+            "achieved_model": "HBM bytes per dispatch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a plain copy "
+                              "of known size; profiles/traffic.json) / average launch time (HIP events)",
+            "demand_bytes_per_launch": dbytes / max(dn, 1),
+            "demand_GBs": dbytes / max(dn, 1) / avg_s / 1e9 if avg_s else None,
+            "merge_model_bytes_per_launch": dmerge / max(dn, 1),
+            "class_share": dms / sum(v[0] for v in kern.values())}
 
 
 def emit(line):
@@ -117,48 +169,6 @@ def run_chunglu4k(args):
                    args.steps, args.warmup, sum(ms) / args.steps,
                    "chung-lu n=4096 m=32768 gamma=2.3 seed=0; triangle+2-edge-tail; heuristic order",
                    {"count": counts[0], "p50_ms_per_query": statistics.median(ms)}))
-
-
-def run_rmat20(args, max_work=1 << 36):
-    """Config 3: the full count exceeds 2^64, so the query is matched exactly
-    on the densest nested root sample of position 0 (v = s-1 mod s, s a
-    power of two) whose run stays within max_work candidates -- found by a
-    binary search of capped runs, then timed."""
-    import cuts_amd
-    from cuts_amd import workloads as W
-    g = W.config3_graph()
-    ctx = cuts_amd.Context(0)
-    dg = ctx.upload(g)
-    p = cuts_amd.Plan(W.query_graph(W.C5CHORD_2TAIL))
-    log(f"R-MAT 20 uploaded ({g.arcs} arcs)")
-    lo, hi = 0, 20
-    while lo < hi:  # smallest stride exponent whose sample fits (nested samples: monotone)
-        mid = (lo + hi) // 2
-        r = ctx.match(dg, p, roots=W.root_sample(g.V, 1 << mid), max_work=max_work)
-        log(f"stride 2^{mid}: {'over the cap' if r.work_capped else f'{r.work} candidates'}")
-        if r.work_capped:
-            lo = mid + 1
-        else:
-            hi = mid
-    roots = W.root_sample(g.V, 1 << hi)
-    for _ in range(args.warmup):
-        ctx.match(dg, p, roots=roots)
-    runs = [ctx.match(dg, p, roots=roots, flags=cuts_amd.FLAG_PROFILE) for _ in range(max(1, args.steps))]
-    r = runs[-1]
-    ms = statistics.median(x.device_ms for x in runs)
-    dname, (dms, dn, dbytes) = max(r.kernels.items(), key=lambda kv: kv[1][0])
-    ach = dbytes / dn / (dms / dn / 1e3) / 1e9 if dn else 0.0  # demand model, as for the headline
-    line = base_line("exact embeddings/sec + p50 time/query", r.count / (ms / 1e3), "embeddings/s",
-                     len(runs), args.warmup, ms,
-                     "rmat scale=20 ef=16 a,b,c=.57,.19,.19 seed=0; 5-cycle+chord+2-tails; score order; exact "
-                     "count on the densest nested root sample of position 0 within 2^36 candidates",
-                     {"roots": list(roots), "count": r.count, "level_sizes": r.level_sizes, "work": r.work,
-                      "p50_ms_per_query": ms})
-    line["roofline"] = {"bound": "hbm", "kernel": dname, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": ach / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": dms / max(dn, 1),
-                        "launches": dn, "bytes_model": "demand model (loads the kernel's algorithm issues)",
-                        "merge_model_GBs": r.merge_bytes[dname] / dn / (dms / dn / 1e3) / 1e9 if dn else 0.0}
-    emit(line)
 
 
 def load_batch_fixture():
@@ -298,84 +308,209 @@ def train_on_queries(ctx, dg, g, queries, seed, log):
                  "log_s": t1 - t0, "train_s": t2 - t1}
 
 
-def run_learned(args):
-    """Train on R-MAT scale-18 graphs, evaluate on the R-MAT scale-20 graph.
-    Queries of both sets are admitted on nested root samples of position 0
-    (cuts_amd.workloads.admit); learned orders keep the score order's first
-    vertex and are judged by it."""
+WEIGHTS = os.path.join(ROOT, "cuts_amd", "data", "mpnn_rmat18.npy")
+
+
+def load_fixture(name):
+    with open(os.path.join(ROOT, "tests", "golden", name)) as f:
+        return json.load(f)
+
+
+def median(xs):
+    return statistics.median(xs) if xs else 0.0
+
+
+def run_headline(args):
+    """R-MAT scale 20: the config-3 query and the learned order's held-out
+    evaluation set, learned orders against score orders (module docstring)."""
     import numpy as np
     import cuts_amd
     from cuts_amd import learned as LN, workloads as W
-    ctx = cuts_amd.Context(0)
-    samples, targets = [], []
-    rng = np.random.default_rng(0)
-    per_graph = max(1, args.train_queries // 4)
-    for gs in (1, 2, 3, 4):
-        g18 = W.rmat18_graph(gs)
-        dg18 = ctx.upload(g18)
-        gf = LN.graph_features(g18)
-        qs, tried = W.admit(ctx, dg18, g18, per_graph, seed=100 + gs, kmin=5, kmax=10)
-        for q in qs:
-            k, edges, rt = q["k"], q["edges"], tuple(q["roots"])
-            adj = LN.adjacency(k, edges)
-            qg = cuts_amd.Graph.from_edges(k, edges)
-            for order, cap in ((None, 0), (LN.random_connected_order(k, adj, rng, first=q["order"][0]),
-                                           4 * q["work"] + 4096)):
-                p = cuts_amd.Plan(qg, order)
-                r = ctx.match(dg18, p, roots=rt, max_work=cap)
-                if r.work_capped:
-                    continue
-                for kk, a, sm, size in LN.table_samples(k, edges, p.order, r.level_sizes):
-                    x1, A1, m1 = LN.encode([(kk, a, sm)], gf)
-                    samples.append((x1[0], A1[0], m1[0]))
-                    targets.append(LN.target(size * rt[2]))
-        log(f"R-MAT 18 seed {gs}: {len(qs)} queries admitted (tries {tried}), {len(samples)} samples so far")
-        del dg18
-    x = np.stack([s_[0] for s_ in samples])
-    A = np.stack([s_[1] for s_ in samples])
-    meta = np.asarray([s_[2] for s_ in samples], np.uint32)
-    net = LN.Mpnn(ctx, seed=0)
-    hist = net.fit(x, A, meta, np.asarray(targets, np.float32), epochs=60, batch=2048, lr=2e-3)
-    log(f"trained: mse {hist[0]:.4f} -> {hist[-1]:.4f} on {len(meta)} samples")
-    if args.save_weights:
-        np.save(args.save_weights, net.params())
+    fx = load_fixture("rmat20_queries.json")
     g = W.config3_graph()
+    if W.csr_fingerprint(g) != fx["fingerprint"]:
+        raise SystemExit("bench.py: the R-MAT 20 graph differs from the fixture's")
+    ctx = cuts_amd.Context(0)
+    t0 = time.perf_counter()
     dg = ctx.upload(g)
-    gf = LN.graph_features(g)
-    evq, tried = W.admit(ctx, dg, g, args.eval_queries, seed=7, kmin=6, kmax=10)
-    log(f"admitted {len(evq)} evaluation queries (tries {tried})")
-    qs = [(q["k"], [tuple(e) for e in q["edges"]]) for q in evq]
-    roots = [tuple(q["roots"]) for q in evq]
+    upload_s = time.perf_counter() - t0
+    info = dg.info()
+    eng = LN.Engine(two_hop=info["two_hop_entries"] > 0, arc_common=bool(info["arc_common"]))
+    log(f"R-MAT 20 uploaded: {g.arcs} arcs, {upload_s:.2f}s (per-arc table {info['arc_common_build_s']:.2f}s)")
+    items = [fx["config3"]] + fx["queries"][: args.eval_queries]
+    qs = [(q["k"], [tuple(e) for e in q["edges"]]) for q in items]
+    roots = [tuple(q["roots"]) for q in items]
+    want = [q["count"] for q in items]
     heur = [cuts_amd.Plan(cuts_amd.Graph.from_edges(k, e)) for k, e in qs]
-    single = [ctx.match(dg, p, roots=rt) for p, rt in zip(heur, roots)]
+    assert [p.order for p in heur] == [q["order"] for q in items], "score orders differ from the fixture's"
+    hres = [ctx.match(dg, p, roots=rt) for p, rt in zip(heur, roots)]
+    assert [r.count for r in hres] == want, "score-order counts differ from the fixture"
+    # learned orders: network forward over every connected prefix set (HIP),
+    # engine-aware DP, first vertex kept (root samples), judged by time
+    net, winfo = LN.load_weights(ctx, WEIGHTS)
+    gfeat = LN.graph_features(g)
+    t0 = time.perf_counter()
     choices = []
-    for j in range(0, len(qs), 32):
-        choices += LN.choose_orders(net, qs[j:j + 32], gf, [p.order for p in heur[j:j + 32]], keep_first=True)
-    judged = LN.judge(ctx, dg, qs, heur, single, choices, roots=roots)
-    h_ms, l_ms = [], []
-    for p, jd, rt in zip(heur, judged, roots):
-        rh = ctx.match(dg, p, roots=rt)  # timed again, warm
-        h_ms.append(rh.device_ms)
-        l_ms.append(ctx.match(dg, jd.plan, roots=rt).device_ms if jd.learned else rh.device_ms)
-    ratio = [a / b for a, b in zip(h_ms, l_ms) if b > 0]
-    geo = float(np.exp(np.mean(np.log(ratio)))) if ratio else 0.0
-    counts = [r.count for r in single]
+    for i in range(0, len(qs), 64):
+        choices += LN.choose_orders(net, qs[i:i + 64], gfeat, [p.order for p in heur[i:i + 64]], keep_first=True,
+                                    eng=eng)
+    plan_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    judged = LN.judge_by_time(ctx, dg, qs, heur, want, choices, roots=roots, heuristic_work=[r.work for r in hres])
+    judge_s = time.perf_counter() - t0
+    learned = [j.plan for j in judged]
     proposed = sum(1 for c in choices if c.learned)
-    kept = sum(1 for jd in judged if jd.learned)
+    kept = sum(1 for j in judged if j.learned)
+    log(f"learned orders: {proposed} proposed, {kept} kept by the judge ({plan_s:.1f}s planning, {judge_s:.1f}s judging)")
+
+    def one_pass(plans, sink=None):
+        for i, (p, rt) in enumerate(zip(plans, roots)):
+            r = ctx.match(dg, p, roots=rt)
+            if r.count != want[i]:
+                raise AssertionError(f"query {i}: count {r.count}, fixture {want[i]}")
+            if sink is not None:
+                sink[i].append(r.device_ms)
+
+    for i in range(args.warmup):
+        one_pass(learned)
+        one_pass(heur)
+        log(f"warmup step {i + 1}/{args.warmup} done")
+    ms_l = [[] for _ in items]
+    ms_h = [[] for _ in items]
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        one_pass(learned, ms_l)
+        one_pass(heur, ms_h)
+        log(f"timed step {i + 1}/{args.steps}: learned {sum(m[-1] for m in ms_l):.1f} ms, "
+            f"score {sum(m[-1] for m in ms_h):.1f} ms")
+    wall = time.perf_counter() - t0
+    tot_l = sum(sum(m) for m in ms_l)
+    tot_h = sum(sum(m) for m in ms_h)
+    emb = sum(want) * args.steps
+    per_l = [median(m) for m in ms_l]
+    per_h = [median(m) for m in ms_h]
+    ev_l, ev_h = per_l[1:], per_h[1:]
+    ratio = [h / l for h, l in zip(per_h, per_l) if l > 0]
+    kern = profile_pass(ctx, dg, learned, roots)
     line = base_line("exact embeddings/sec + p50 time/query; learned vs heuristic order, same GPU",
-                     sum(counts) / (sum(l_ms) / 1e3), "embeddings/s", 1, 0, sum(l_ms),
-                     "train: 4 R-MAT scale-18 graphs (seeds 1-4), 5-10 vertex walk queries; evaluate: 6-10 vertex "
-                     "walk queries on R-MAT scale 20 (seed 0); every query exact on its admitted root sample of "
-                     "position 0; learned orders keep the first vertex and are judged by the score order",
-                     {"train_samples": int(len(meta)), "mse_first": hist[0], "mse_last": hist[-1],
-                      "eval_queries": len(evq), "tried": tried, "sum_count": sum(counts),
-                      "p50_ms_heuristic": statistics.median(h_ms), "p50_ms_learned": statistics.median(l_ms),
-                      "speedup_p50": statistics.median(h_ms) / statistics.median(l_ms),
-                      "speedup_total": sum(h_ms) / sum(l_ms), "speedup_geomean": geo,
+                     emb / (tot_l / 1e3), "embeddings/s", args.steps, args.warmup, tot_l / args.steps,
+                     "R-MAT scale 20 (config 3 graph: 2^20 vertices, ef 16, a,b,c=.57,.19,.19, seed 0, "
+                     f"{g.arcs} arcs); one step = the 5-cycle+chord+2-tails query (root sample v=127 mod 128) "
+                     f"+ {len(items) - 1} held-out 6-10-vertex walk queries (config 5 evaluation set, each on its "
+                     "admitted root sample), matched with the learned orders; score orders timed alongside",
+                     {"queries": len(items), "sum_count_per_step": sum(want),
+                      "embeddings_per_s_learned": emb / (tot_l / 1e3),
+                      "embeddings_per_s_heuristic": emb / (tot_h / 1e3),
+                      "ms_per_step_learned": tot_l / args.steps, "ms_per_step_heuristic": tot_h / args.steps,
+                      "p50_ms_per_query_learned": median(per_l), "p50_ms_per_query_heuristic": median(per_h),
+                      "p99_ms_per_query_learned": sorted(per_l)[int(0.99 * (len(per_l) - 1))],
+                      "p99_ms_per_query_heuristic": sorted(per_h)[int(0.99 * (len(per_h) - 1))],
+                      "speedup_p50": median(per_h) / median(per_l),
+                      "speedup_total": tot_h / tot_l,
+                      "speedup_geomean": float(np.exp(np.mean(np.log(ratio)))) if ratio else None,
+                      "eval_set": {"p50_ms_learned": median(ev_l), "p50_ms_heuristic": median(ev_h),
+                                   "speedup_p50": median(ev_h) / median(ev_l) if ev_l else None,
+                                   "total_ms_learned": sum(ev_l), "total_ms_heuristic": sum(ev_h)},
+                      "config3_query": {"count": want[0], "roots": list(roots[0]),
+                                        "ms_learned": per_l[0], "ms_heuristic": per_h[0],
+                                        "order_learned": learned[0].order, "order_heuristic": heur[0].order,
+                                        "level_sizes": items[0]["level_sizes"]},
                       "learned_proposed": proposed, "learned_kept": kept,
-                      "fallback_rate": 1.0 - kept / max(1, len(evq)),
-                      "candidates_learned": sum(jd.work for jd in judged),
-                      "candidates_heuristic": sum(r.work for r in single)})
+                      "fallback_rate": 1.0 - kept / len(items),
+                      "order_planning_s": plan_s, "judging_s": judge_s,
+                      "engine": {"two_hop_index": eng.two_hop, "arc_common": eng.arc_common},
+                      "weights": {"file": os.path.relpath(WEIGHTS, ROOT), **{k: winfo.get(k) for k in
+                                  ("graphs", "queries_admitted", "samples", "mse_first", "mse_last")}},
+                      "wall_s_timed_loop": wall, "upload_s": upload_s,
+                      "arc_common_build_s": info["arc_common_build_s"]})
+    line["roofline"] = roofline(kern, "rmat20")
+    line["kernels_ms_profile_pass"] = {k: v[0] for k, v in kern.items() if v[1]}
+    if not args.no_config2:
+        line["config2"] = config2_fields(ctx, steps=5)
+    if not args.no_cpu:
+        log("CPU baseline (about 20 s)")
+        c3 = fx["config3"]
+        jobs = [("5-cycle+chord+2-tails", c3["k"], [tuple(e) for e in c3["edges"]], tuple(c3["roots"]), 1)]
+        jobs += [(f"eval query {i}", q["k"], [tuple(e) for e in q["edges"]], tuple(q["roots"]), 1)
+                 for i, q in enumerate(fx["queries"][:3])]
+        line["cpu_baseline"] = cpu_baseline(g, jobs)
+    emit(line)
+
+
+def config2_fields(ctx, steps):
+    """The mid-size Chung-Lu graph's two 6-vertex queries at full size, score
+    and learned orders (extra fields of the headline line)."""
+    import cuts_amd
+    from cuts_amd import learned as LN, workloads as W
+    g = W.config2_graph()
+    dg = ctx.upload(g)
+    info = dg.info()
+    eng = LN.Engine(two_hop=info["two_hop_entries"] > 0, arc_common=bool(info["arc_common"]))
+    cfg = load_fixture("config_counts.json")
+    shapes = [(W.C4_2PEND, cfg["config2a_full"]["count"]), (W.K4E_TAIL2, cfg["config2b_full"]["count"])]
+    net, _ = LN.load_weights(ctx, WEIGHTS)
+    qs = [(s.k, list(s.edges)) for s, _ in shapes]
+    heur = [cuts_amd.Plan(W.query_graph(s)) for s, _ in shapes]
+    choices = LN.choose_orders(net, qs, LN.graph_features(g), [p.order for p in heur], eng=eng)
+    judged = LN.judge_by_time(ctx, dg, qs, heur, [c for _, c in shapes], choices)
+    out = {"graph": "chung-lu n=36692 m=183831 gamma=2.1 maxdeg=1400 seed=0", "queries": {}}
+    for (s, want), hp, j in zip(shapes, heur, judged):
+        ms = {"heuristic": [], "learned": []}
+        for _ in range(steps):
+            for key, p in (("heuristic", hp), ("learned", j.plan)):
+                r = ctx.match(dg, p)
+                if r.count != want:
+                    raise AssertionError(f"{s.name}: count {r.count}, committed {want}")
+                ms[key].append(r.device_ms)
+        out["queries"][s.name] = {"count": want, "p50_ms_heuristic": median(ms["heuristic"]),
+                                  "p50_ms_learned": median(ms["learned"]), "learned_kept": j.learned,
+                                  "order_learned": j.plan.order, "order_heuristic": hp.order,
+                                  "embeddings_per_s_heuristic": want / (median(ms["heuristic"]) / 1e3)}
+    return out
+
+
+def run_chunglu36k(args):
+    """The mid-size Chung-Lu graph and its two 6-vertex queries at full size,
+    score order; the profiling pass runs after the timed steps."""
+    import cuts_amd
+    from cuts_amd import workloads as W
+    g = W.config2_graph()
+    cfg = load_fixture("config_counts.json")
+    shapes = [(W.C4_2PEND, cfg["config2a_full"]["count"]), (W.K4E_TAIL2, cfg["config2b_full"]["count"])]
+    ctx = cuts_amd.Context(0)
+    t0 = time.perf_counter()
+    dg = ctx.upload(g)
+    upload_s = time.perf_counter() - t0
+    plans = [cuts_amd.Plan(W.query_graph(s)) for s, _ in shapes]
+    for i in range(args.warmup):
+        for p in plans:
+            ctx.match(dg, p)
+        log(f"warmup step {i + 1}/{args.warmup} done")
+    per_query_ms, counts = [], []
+    for i in range(args.steps):
+        for p, (s, want) in zip(plans, shapes):
+            r = ctx.match(dg, p)
+            if r.count != want:
+                raise AssertionError(f"{s.name}: count {r.count}, committed {want}")
+            per_query_ms.append(r.device_ms)
+            counts.append(r.count)
+        log(f"timed step {i + 1}/{args.steps} done, last query {per_query_ms[-1]:.2f} ms")
+    total_ms = sum(per_query_ms)
+    kern = profile_pass(ctx, dg, plans, [None, None])
+    line = base_line("exact embeddings/sec + p50 time/query", sum(counts) / (total_ms / 1e3), "embeddings/s",
+                     args.steps, args.warmup, total_ms / args.steps,
+                     "chung-lu n=36692 m=183831 gamma=2.1 maxdeg=1400 seed=0; queries 4-cycle+2-pendants and "
+                     "K4-e+2-tail, full counts; score order; one step = both queries",
+                     {"counts": [w for _, w in shapes], "p50_ms_per_query": median(per_query_ms),
+                      "upload_s": upload_s, "two_hop_build_s": dg.info()["two_hop_build_s"]})
+    line["roofline"] = roofline(kern, "chunglu36k")
+    line["kernels_ms_profile_pass"] = {k: v[0] for k, v in kern.items() if v[1]}
+    if args.copy_probe:
+        line["copy_probe_GBs"] = ctx.copy_probe(1 << 30, args.copy_probe)
+    if not args.no_cpu:
+        log("CPU baseline (about 20 s)")
+        jobs = [(s.name, s.k, list(s.edges), (0, g.V, 1), 64) for s, _ in shapes]
+        line["cpu_baseline"] = cpu_baseline(g, jobs)
     emit(line)
 
 
@@ -385,13 +520,12 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-config2", action="store_true", help="headline: leave out the mid-size graph's fields")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run N dispatches of a 1 GiB plain copy (byte-counter yardstick)")
-    ap.add_argument("--workload", default="chunglu36k",
-                    choices=["chunglu36k", "chunglu4k", "rmat20", "batch256", "learned"])
-    ap.add_argument("--train-queries", type=int, default=2000)
+    ap.add_argument("--workload", default="rmat20", choices=["rmat20", "chunglu36k", "chunglu4k", "batch256"])
     ap.add_argument("--eval-queries", type=int, default=500)
-    ap.add_argument("--save-weights", default="")
+    ap.add_argument("--train-queries", type=int, default=1024)
     ap.add_argument("--score-only", action="store_true", help="batch256: score orders only, no training")
     ap.add_argument("--plain-launches", action="store_true", help="batch256 --score-only: no hipGraph")
     args = ap.parse_args()
@@ -399,109 +533,8 @@ def main():
         print(f"bench.py: only one GPU is used by this build (asked for --gpus {args.gpus}); "
               "the matcher shards by level-0 chunks, see DESIGN.md", file=sys.stderr)
         sys.exit(2)
-    if args.workload != "chunglu36k":
-        {"chunglu4k": run_chunglu4k, "rmat20": run_rmat20, "batch256": run_batch256,
-         "learned": run_learned}[args.workload](args)
-        return
-
-    import cuts_amd
-    from cuts_amd import workloads as W
-
-    g = W.config2_graph()
-    shapes = [W.C4_2PEND, W.K4E_TAIL2]
-    ctx = cuts_amd.Context(0)
-    t_up = time.perf_counter()
-    dg = ctx.upload(g)
-    upload_s = time.perf_counter() - t_up
-    plans = [cuts_amd.Plan(W.query_graph(s)) for s in shapes]
-
-    for i in range(args.warmup):
-        for p in plans:
-            ctx.match(dg, p)
-        print(f"bench: warmup step {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
-
-    per_query_ms, counts, work = [], [], []
-    kern = {}
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        for p in plans:
-            r = ctx.match(dg, p, flags=cuts_amd.FLAG_PROFILE)
-            per_query_ms.append(r.device_ms)
-            counts.append(r.count)
-            work.append(r.work)
-            for name, (ms, n, b) in r.kernels.items():
-                k = kern.setdefault(name, [0.0, 0, 0.0, 0.0])
-                k[0] += ms
-                k[1] += n
-                k[2] += b
-                k[3] += r.merge_bytes[name]
-        print(f"bench: timed step done, last query {per_query_ms[-1]:.2f} ms", file=sys.stderr, flush=True)
-    wall = time.perf_counter() - t0
-    total_ms = sum(per_query_ms)
-    total_emb = sum(counts)
-    dom = max(kern.items(), key=lambda kv: kv[1][0])
-    dname, (dms, dn, dbytes, dmerge) = dom
-    avg_launch_s = dms / dn / 1e3
-    # algorithmic bytes of the dominant kernel: the loads its algorithm
-    # issues, counted on the device per work item (demand model)
-    bytes_per_launch = dbytes / dn
-    model = ("demand model, counted on the device: per candidate its list word and signature (12 B); per "
-             "membership test a hub's id and bitmap word (12 B) or the offsets and words of a lower-bound search; "
-             "per prefix its row, selector and scan entry; per closed form its table entries; per two-hop lookup "
-             "the offsets and the search's words")
-    note = ("the graph (1.6 MB CSR, signatures, hub bitmaps) and the two-hop index stay largely cache-resident, so "
-            "the kernel is bound by on-chip latency rather than HBM bandwidth (traffic: HBM bytes from rocprofv3); "
-            "merge_model_* restate the whole path in the merge model, whose closed-form levels are never read")
-    achieved = bytes_per_launch / avg_launch_s / 1e9
-    line = {
-        "metric": "exact embeddings/sec + p50 time/query; learned vs heuristic order, same GPU",
-        "value": total_emb / (total_ms / 1e3),
-        "unit": "embeddings/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": total_ms / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u32",
-        "data": "synthetic (seeded Chung-Lu generator; queries located by seeded random walk)",
-        "config": {
-            "workload": "chung-lu n=36692 m=183831 gamma=2.1 maxdeg=1400 seed=0; "
-                        "queries 4-cycle+2-pendants and K4-e+2-tail; heuristic order; one step = both queries",
-            "counts": counts[: len(plans)],
-            "p50_ms_per_query": statistics.median(per_query_ms),
-            "p99_ms_per_query": sorted(per_query_ms)[min(len(per_query_ms) - 1, int(0.99 * len(per_query_ms)))],
-            "candidates_per_step": sum(work) / args.steps,
-            "candidates_per_s": sum(work) / (total_ms / 1e3),
-            "wall_s": wall,
-            "upload_s": upload_s,
-        },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dname,
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_for(dname),
-            "algorithmic_bytes_per_launch": bytes_per_launch,
-            "bytes_model": model,
-            "avg_launch_ms": avg_launch_s * 1e3,
-            "launches": dn,
-            "merge_model_bytes_per_launch": dmerge / dn,
-            "note": note,
-        },
-        "merge_bytes_total": sum(v[3] for v in kern.values()),
-        "merge_model_GBs": sum(v[3] for v in kern.values()) / (total_ms / 1e3) / 1e9,
-        "kernels_ms": {k: v[0] for k, v in kern.items() if v[1]},
-    }
-    if args.copy_probe:
-        line["copy_probe_GBs"] = ctx.copy_probe(1 << 30, args.copy_probe)
-    if not args.no_cpu:
-        print("bench: CPU baseline (about 20 s)", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(g, shapes)
-    print(json.dumps(line), flush=True)
+    {"rmat20": run_headline, "chunglu36k": run_chunglu36k, "chunglu4k": run_chunglu4k,
+     "batch256": run_batch256}[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -33,6 +33,7 @@ FLAG_NO_PAIR = 128
 FLAG_NO_CHAIN = 256
 FLAG_NO_TWO_HOP = 512
 FLAG_NO_SHARED = 1024
+FLAG_NO_ARC_COMMON = 2048
 MAX_QUERY = 16
 KERNEL_CLASSES = ["level0", "plan", "scan", "tile_index", "expand_write", "expand_count", "expand_fused",
                   "closed_form", "pair", "common", "chain"]
@@ -61,6 +62,13 @@ class _Result(C.Structure):
                 ("kernel_bytes", C.c_double * _NCLASS),
                 ("roots_done", C.c_uint64), ("last_root", C.c_uint32), ("work_capped", C.c_uint32),
                 ("work", C.c_uint64), ("merge_bytes", C.c_double * _NCLASS)]
+
+
+class _DGInfo(C.Structure):
+    _fields_ = [("V", C.c_uint32), ("arcs", C.c_uint64), ("symmetric", C.c_int), ("self_loops", C.c_uint64),
+                ("hubs", C.c_uint32), ("hub_min_deg", C.c_uint32), ("two_hop_entries", C.c_uint64),
+                ("two_hop_dense_rows", C.c_uint32), ("arc_common", C.c_int), ("two_hop_build_s", C.c_double),
+                ("arc_common_build_s", C.c_double)]
 
 
 class _BatchOpts(C.Structure):
@@ -107,6 +115,8 @@ def lib():
             "cuts_context_destroy": (None, [vp]),
             "cuts_device_graph_upload": (C.c_int, [vp, vp, pp]),
             "cuts_device_graph_free": (None, [vp]),
+            "cuts_device_graph_query": (C.c_int, [vp, C.POINTER(_DGInfo)]),
+            "cuts_device_graph_arc_common": (C.c_int, [vp, _u32p, C.c_uint64]),
             "cuts_match_options_default": (None, [C.POINTER(_Opts)]),
             "cuts_match": (C.c_int, [vp, vp, vp, C.POINTER(_Opts), C.POINTER(_Result)]),
             "cuts_search": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.POINTER(C.c_uint64)]),
@@ -303,6 +313,19 @@ class DeviceGraph:
         if getattr(self, "_h", None) and self._h.value and _lib is not None:
             _lib.cuts_device_graph_free(self._h)
             self._h = None
+
+    def info(self) -> dict:
+        """What the upload built: hubs, two-hop index, per-arc table, build times."""
+        i = _DGInfo()
+        _check(lib().cuts_device_graph_query(self._h, C.byref(i)))
+        return {name: getattr(i, name) for name, _ in _DGInfo._fields_}
+
+    def arc_common(self) -> np.ndarray:
+        """Per-arc |N(u) & N(w)|, aligned with the graph's out-lists."""
+        n = self.info()["arcs"]
+        out = np.zeros(max(n, 1), dtype=np.uint32)
+        _check(lib().cuts_device_graph_arc_common(self._h, _ptr(out), n))
+        return out[:n]
 
 
 class Context:

@@ -97,5 +97,9 @@ void batch_run(Batch& b, std::vector<BatchQueryResult>& out, double& device_ms, 
 const uint32_t* cnt_table(Context& c, DeviceGraph& dg, uint32_t thr);
 // Chain-level vertex tables for thresholds (ta, tb), 2V uint64 (built once).
 const uint64_t* chain_table(Context& c, DeviceGraph& dg, uint32_t ta, uint32_t tb);
+// The same, built in `arena` for one call (not cached) when the graph has
+// not cached them and arena is non-null.
+const uint32_t* cnt_table(Context& c, DeviceGraph& dg, ScratchArena* arena, uint32_t thr);
+const uint64_t* chain_table(Context& c, DeviceGraph& dg, ScratchArena* arena, uint32_t ta, uint32_t tb);
 
 }  // namespace cuts

@@ -355,8 +355,60 @@ int cuts_device_graph_upload(cuts_context* ctx, const cuts_graph* g, cuts_device
 
 void cuts_device_graph_free(cuts_device_graph* dg) { delete dg; }
 
+int cuts_device_graph_query(const cuts_device_graph* dg, cuts_device_graph_info* info) {
+  return guard([&]() -> int {
+    if (!dg || !info) return fail(CUTS_E_ARG, "null argument");
+    const cuts::DeviceGraph& d = dg->dg;
+    std::memset(info, 0, sizeof(*info));
+    info->V = d.V;
+    info->arcs = d.A;
+    info->symmetric = d.symmetric ? 1 : 0;
+    info->self_loops = d.loops;
+    info->hubs = d.hubs;
+    info->hub_min_deg = d.hub_min_deg;
+    info->two_hop_entries = d.h2_off ? d.h2_entries : 0;
+    info->two_hop_dense_rows = d.h2_dense_rows;
+    info->arc_common = d.arc_cn ? 1 : 0;
+    info->two_hop_build_s = d.h2_build_s;
+    info->arc_common_build_s = d.arc_cn_build_s;
+    return CUTS_OK;
+  });
+}
+
+int cuts_device_graph_arc_common(const cuts_device_graph* dg, uint32_t* out, uint64_t n) {
+  return guard([&]() -> int {
+    if (!dg || !out) return fail(CUTS_E_ARG, "null argument");
+    const cuts::DeviceGraph& d = dg->dg;
+    if (!d.arc_cn) return fail(CUTS_E_ARG, "the device graph has no per-arc common-neighbour table");
+    if (n != d.A) return fail(CUTS_E_ARG, "n must equal the number of arcs");
+    cuts::hip_check(hipSetDevice(d.device), "hipSetDevice");
+    cuts::hip_check(hipMemcpy(out, d.arc_cn, n * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+    return CUTS_OK;
+  });
+}
+
 void cuts_match_options_default(cuts_match_options* o) {
   if (o) std::memset(o, 0, sizeof(*o));
+}
+
+static void fill_result(const cuts::MatchResult& r, cuts_match_result* res) {
+  std::memset(res, 0, sizeof(*res));
+  res->count = r.count;
+  for (uint32_t i = 0; i < cuts::kMaxQuery; ++i) res->level_sizes[i] = r.level[i];
+  res->device_ms = r.device_ms;
+  res->emb_written = r.emb_written;
+  res->chunks = r.chunks;
+  res->complete = r.complete;
+  res->roots_done = r.roots_done;
+  res->last_root = r.last_root;
+  res->work_capped = r.work_capped;
+  res->work = r.work;
+  for (int i = 0; i < 11; ++i) {
+    res->kernel_ms[i] = r.kernel_ms[i];
+    res->kernel_launches[i] = r.kernel_launches[i];
+    res->kernel_bytes[i] = r.kernel_bytes[i];
+    res->merge_bytes[i] = r.merge_bytes[i];
+  }
 }
 
 int cuts_match(cuts_context* ctx, const cuts_device_graph* dg, const cuts_plan* plan, const cuts_match_options* opt,
@@ -367,23 +419,7 @@ int cuts_match(cuts_context* ctx, const cuts_device_graph* dg, const cuts_plan* 
       return fail(CUTS_E_ARG, "CUTS_FLAG_WRITE needs emb_rows");
     cuts::MatchResult r;
     cuts::run_match(ctx->c, const_cast<cuts::DeviceGraph&>(dg->dg), plan->p, to_opts(opt), r);
-    std::memset(res, 0, sizeof(*res));
-    res->count = r.count;
-    for (uint32_t i = 0; i < cuts::kMaxQuery; ++i) res->level_sizes[i] = r.level[i];
-    res->device_ms = r.device_ms;
-    res->emb_written = r.emb_written;
-    res->chunks = r.chunks;
-    res->complete = r.complete;
-    res->roots_done = r.roots_done;
-    res->last_root = r.last_root;
-    res->work_capped = r.work_capped;
-    res->work = r.work;
-    for (int i = 0; i < 11; ++i) {
-      res->kernel_ms[i] = r.kernel_ms[i];
-      res->kernel_launches[i] = r.kernel_launches[i];
-      res->kernel_bytes[i] = r.kernel_bytes[i];
-      res->merge_bytes[i] = r.merge_bytes[i];
-    }
+    fill_result(r, res);
     return CUTS_OK;
   });
 }

@@ -50,6 +50,9 @@ struct KPlan {
   // (|A|-1)(|B|-1) - (|A & B|-1), and the survivors are N(r_a) & N(r_b) minus
   // the prefix, so the prefix's count is closed-form from one two-hop lookup
   uint32_t pr_collapse;
+  // pr_a and pr_b adjacent in the query: |A & B|'s base is the per-arc
+  // common-neighbour count of the arc (row[pr_a], row[pr_b])
+  uint32_t pr_adj;
   // last two positions as a chain (symmetric graphs): k-1's one backward
   // neighbour is k-2, whose one backward neighbour ch_u is older than k-3.
   // A prefix P of positions 0..k-3 with u = row[ch_u] and
@@ -109,6 +112,11 @@ struct DeviceGraph {
   uint16_t* h2_dense = nullptr;
   uint32_t h2_dense_rows = 0;
   double h2_build_s = 0;
+  // per-arc common-neighbour counts (symmetric loop-free graphs): arc_cn[i]
+  // = |N(u) & N(w)| for the arc (u, w) at adj_out[i]; built on the device at
+  // upload
+  uint32_t* arc_cn = nullptr;
+  double arc_cn_build_s = 0;
   ~DeviceGraph();
 };
 
@@ -138,6 +146,15 @@ struct LevelBuf {
   uint64_t bsum_cap = 0;
   uint32_t* tile_p = nullptr;   // tiles+1
   uint64_t tile_cap = 0;
+};
+
+// Bump allocator over a fixed device region (the no-allocation entry
+// point's static scratch array): carve() hands out 256-byte aligned pieces
+// and never frees; it throws CUTS_E_NOMEM when the region is exhausted.
+struct ScratchArena {
+  char* base = nullptr;
+  uint64_t size = 0, used = 0;
+  void* carve(uint64_t bytes);
 };
 
 struct Context {
@@ -188,6 +205,7 @@ struct MatchResult {
 void context_init(Context& c, int device);
 void upload_graph(Context& c, const Graph& g, DeviceGraph& dg);
 void run_match(Context& c, DeviceGraph& dg, const Plan& p, const MatchOpts& o, MatchResult& r);
+
 // Streams `bytes` through a plain copy kernel `iters` times; returns GB/s
 // (read + write).
 double copy_probe(Context& c, uint64_t bytes, uint32_t iters);

@@ -72,11 +72,12 @@ struct GView {
   const uint32_t *h2_v, *h2_c;
   const int32_t* h2_did;   // dense rows of the index or null
   const uint16_t* h2_dense;
+  const uint32_t* arc_cn;  // per arc (u, w) of adj_out: |N(u) & N(w)|, or null
 };
 
 GView view_of(const DeviceGraph& dg) {
   return GView{dg.off_out, dg.adj_out, dg.off_in, dg.adj_in, dg.sig, dg.V, dg.hub_id, dg.nsig_out, dg.nsig_in, dg.bits,
-               dg.words, dg.h2_off, dg.h2_v, dg.h2_c, dg.h2_did, dg.h2_dense};
+               dg.words, dg.h2_off, dg.h2_v, dg.h2_c, dg.h2_did, dg.h2_dense, dg.arc_cn};
 }
 
 __device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -145,6 +146,51 @@ __device__ __forceinline__ uint32_t two_hop(const GView& g, uint32_t u, uint32_t
     else hi = mid;
   }
   return (lo < end && g.h2_v[lo] == w) ? g.h2_c[lo] : 0u;
+}
+
+// |N(u) & N(w)| for adjacent u, w from the per-arc table: w's index in u's
+// sorted list (a lower-bound search), then one load.  Like the two-hop
+// index it serves degree thresholds up to 2.
+__device__ __forceinline__ uint32_t arc_common(const GView& g, uint32_t u, uint32_t w, uint64_t& bytes) {
+  uint32_t lo = g.off_out[u], hi = g.off_out[u + 1];
+  bytes += 8 + bs_bytes(hi - lo) + 4;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (g.adj_out[mid] < w) lo = mid + 1;
+    else hi = mid;
+  }
+  return g.arc_cn[lo];
+}
+
+// Per-arc common-neighbour counts of a symmetric loop-free graph: for every
+// arc (u, w) with u < w, |N(u) & N(w)| by walking the shorter list with a
+// membership test in the longer one (a bit test when it belongs to a hub),
+// stored at (u, w) and at the mirror arc (w, u).  One thread per arc; the
+// arc's source is found by an upper-bound search of the offsets.
+__global__ __launch_bounds__(256) void k_arc_common(GView g, uint64_t A, uint32_t* __restrict__ cn) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < A; i += (uint64_t)gridDim.x * 256) {
+    uint32_t lo = 0, hi = g.V;  // largest u with off[u] <= i
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (g.off_out[mid] <= i) lo = mid;
+      else hi = mid - 1;
+    }
+    const uint32_t u = lo, w = g.adj_out[i];
+    if (u >= w) continue;
+    const uint32_t du = g.off_out[u + 1] - g.off_out[u], dw = g.off_out[w + 1] - g.off_out[w];
+    const uint32_t s = du <= dw ? u : w, o = du <= dw ? w : u;
+    uint32_t c = 0;
+    uint64_t dummy = 0;
+    for (uint32_t j = g.off_out[s]; j < g.off_out[s + 1]; ++j) c += adj_has(g, o, g.adj_out[j], dummy);
+    cn[i] = c;
+    uint32_t a = g.off_out[w], b = g.off_out[w + 1];
+    while (a < b) {
+      const uint32_t mid = (a + b) >> 1;
+      if (g.adj_out[mid] < u) a = mid + 1;
+      else b = mid;
+    }
+    cn[a] = c;
+  }
 }
 
 // One block per hub: sets the bits of its neighbours.
@@ -229,6 +275,7 @@ __device__ __forceinline__ void pair_bases(const GView& g, const KPlan& P, const
   cab = 0;
   if (P.pr_a == P.pr_b) cab = (int64_t)tbl_or_deg(g, pt.ab, u);
   else if (g.h2_off && P.pr_tab <= 2) cab = (int64_t)two_hop(g, u, w, bytes);
+  else if (g.arc_cn && P.pr_adj && P.pr_tab <= 2) cab = (int64_t)arc_common(g, u, w, bytes);
   bytes += 16 + 4 * N;
 #pragma unroll
   for (int l = 0; l < N; ++l) {
@@ -265,7 +312,7 @@ __global__ __launch_bounds__(BLOCK) void k_plan(GView g, KPlan P, const uint32_t
                                                unsigned long long* __restrict__ merge,
                                                unsigned long long* __restrict__ counters) {
   const uint32_t nj = P.nj[L];
-  const bool collapse = pbase && P.pr_collapse && g.h2_off;
+  const bool collapse = pbase && P.pr_collapse && (g.h2_off || (g.arc_cn && P.pr_adj));
   uint64_t mb = 0, acc = 0, surv = 0, sna = 0;
   for (uint64_t p = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (uint64_t)gridDim.x * BLOCK) {
     uint32_t row[L];
@@ -1525,46 +1572,60 @@ void dev_alloc(T*& p, uint64_t n) {
   }
 }
 
+// Device buffer of n elements: hipMalloc'd (the old one freed), or carved
+// from the arena when there is one (nothing freed; sizes are rounded up to a
+// power of two so that regrowth wastes at most half of what was carved).
+template <typename T>
+void dev_alloc(ScratchArena* a, T*& p, uint64_t n) {
+  if (!a) {
+    dev_alloc(p, n);
+    return;
+  }
+  uint64_t m = 1;
+  while (m < std::max<uint64_t>(n, 1)) m <<= 1;
+  p = (T*)a->carve(m * sizeof(T));
+}
+
 // Makes sure level buffer `lv` can hold `rows` rows of `words` words plus
 // the planning arrays for them.
-void ensure_rows(LevelBuf& b, uint64_t rows, uint32_t words) {
+void ensure_rows(ScratchArena* ar, LevelBuf& b, uint64_t rows, uint32_t words) {
   if (b.cap >= rows && b.rows) return;
   uint64_t cap = std::max<uint64_t>(rows, 1024);
   if (b.cap) cap = std::max<uint64_t>(cap, b.cap + b.cap / 2);
-  dev_alloc(b.rows, cap * words);
-  dev_alloc(b.minlen, cap);
-  dev_alloc(b.minsel, cap);
-  dev_alloc(b.work_off, cap + 1);
-  dev_alloc(b.cbase, cap);
+  dev_alloc(ar, b.rows, cap * words);
+  dev_alloc(ar, b.minlen, cap);
+  dev_alloc(ar, b.minsel, cap);
+  dev_alloc(ar, b.work_off, cap + 1);
+  dev_alloc(ar, b.cbase, cap);
   b.cap = cap;
 }
-void ensure_rows_exact(LevelBuf& b, uint64_t rows, uint32_t words) {
+void ensure_rows_exact(ScratchArena* ar, LevelBuf& b, uint64_t rows, uint32_t words) {
   // like ensure_rows but never grows beyond `rows` (budget-bound levels)
   if (b.cap >= rows && b.rows) return;
-  dev_alloc(b.rows, std::max<uint64_t>(rows, 1) * words);
-  dev_alloc(b.minlen, std::max<uint64_t>(rows, 1));
-  dev_alloc(b.minsel, std::max<uint64_t>(rows, 1));
-  dev_alloc(b.work_off, std::max<uint64_t>(rows, 1) + 1);
-  dev_alloc(b.cbase, std::max<uint64_t>(rows, 1));
+  dev_alloc(ar, b.rows, std::max<uint64_t>(rows, 1) * words);
+  dev_alloc(ar, b.minlen, std::max<uint64_t>(rows, 1));
+  dev_alloc(ar, b.minsel, std::max<uint64_t>(rows, 1));
+  dev_alloc(ar, b.work_off, std::max<uint64_t>(rows, 1) + 1);
+  dev_alloc(ar, b.cbase, std::max<uint64_t>(rows, 1));
   b.cap = rows;
 }
 // The per-row arrays of the pair and chain levels, allocated only for the
 // levels that use them (each grows to the level's row capacity).
 template <typename T>
-void ensure_aux(T*& p, uint64_t& cap, uint64_t rows) {
+void ensure_aux(ScratchArena* ar, T*& p, uint64_t& cap, uint64_t rows) {
   if (cap >= rows && p) return;
-  dev_alloc(p, std::max<uint64_t>(rows, 1));
+  dev_alloc(ar, p, std::max<uint64_t>(rows, 1));
   cap = std::max<uint64_t>(rows, 1);
 }
 
-void ensure_bsum(LevelBuf& b, uint64_t nb) {
+void ensure_bsum(ScratchArena* ar, LevelBuf& b, uint64_t nb) {
   if (b.bsum_cap >= nb + 1 && b.bsum) return;
-  dev_alloc(b.bsum, nb + 1);
+  dev_alloc(ar, b.bsum, nb + 1);
   b.bsum_cap = nb + 1;
 }
-void ensure_tiles(LevelBuf& b, uint64_t nt) {
+void ensure_tiles(ScratchArena* ar, LevelBuf& b, uint64_t nt) {
   if (b.tile_cap >= nt + 1 && b.tile_p) return;
-  dev_alloc(b.tile_p, nt + 1);
+  dev_alloc(ar, b.tile_p, nt + 1);
   b.tile_cap = nt + 1;
 }
 
@@ -1588,12 +1649,24 @@ struct Matcher {
   uint64_t level_cap[kMaxQuery] = {0};
   uint32_t grid_cap = 0;
 
+  // where the run's buffers live: the context's level buffers and counters
+  // (hipMalloc'd on demand), or a scratch arena and caller-owned level
+  // buffers (the no-allocation entry point)
+  LevelBuf* lv;
+  unsigned long long* ctr;
+  ScratchArena* arena = nullptr;
+
   Matcher(Context& c_, DeviceGraph& dg_, const Plan& p_, const MatchOpts& o_, MatchResult& r_)
-      : c(c_), dg(dg_), plan(p_), o(o_), r(r_) {}
+      : c(c_), dg(dg_), plan(p_), o(o_), r(r_), lv(c_.lv), ctr(c_.counters) {}
 
   bool prof = false;
   bool has_deadline = false, aborted = false;
   std::chrono::steady_clock::time_point deadline;
+  // the pair level's |N(u) & N(w)| base comes from an index (two-hop, or the
+  // per-arc table when u and w are adjacent) rather than a k_common pass
+  bool pair_cab_known() const {
+    return kp.pr_tab <= 2 && (g.h2_off || (g.arc_cn && kp.pr_adj));
+  }
   bool past_deadline() {
     if (has_deadline && !aborted && std::chrono::steady_clock::now() >= deadline) aborted = true;
     return aborted;
@@ -1610,8 +1683,8 @@ struct Matcher {
   void pbegin(bool dev_bytes) {
     if (!prof) return;
     if (dev_bytes) {
-      HIPCHECK(hipMemsetAsync(c.counters + 3, 0, 8, c.stream));
-      HIPCHECK(hipMemsetAsync(c.counters + 8, 0, 8, c.stream));
+      HIPCHECK(hipMemsetAsync(ctr + 3, 0, 8, c.stream));
+      HIPCHECK(hipMemsetAsync(ctr + 8, 0, 8, c.stream));
     }
     HIPCHECK(hipEventRecord(c.kev[0], c.stream));
   }
@@ -1623,8 +1696,8 @@ struct Matcher {
     HIPCHECK(hipEventElapsedTime(&ms, c.kev[0], c.kev[1]));
     r.kernel_ms[cls] += ms;
     r.kernel_launches[cls]++;
-    r.kernel_bytes[cls] += host_bytes + (dev_bytes ? (double)read_u64(c.counters + 3) : 0.0);
-    if (dev_bytes) r.merge_bytes[cls] += (double)read_u64(c.counters + 8);
+    r.kernel_bytes[cls] += host_bytes + (dev_bytes ? (double)read_u64(ctr + 3) : 0.0);
+    if (dev_bytes) r.merge_bytes[cls] += (double)read_u64(ctr + 8);
   }
 
   uint64_t read_u64(const void* dev) {
@@ -1641,7 +1714,7 @@ struct Matcher {
   // exclusive scan of b.minlen[0..n) into b.work_off[0..n], returns total
   uint64_t scan(LevelBuf& b, uint64_t n) {
     const uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
-    ensure_bsum(b, nb);
+    ensure_bsum(arena, b, nb);
     pbegin(false);
     hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)nb), dim3(BLOCK), 0, c.stream, b.minlen, n, b.bsum);
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(BLOCK), 0, c.stream, b.bsum, nb);
@@ -1655,7 +1728,7 @@ struct Matcher {
   // offset into it); produces position L.
   void level(uint32_t L, const uint32_t* rows, uint64_t n) {
     if (n == 0) return;
-    LevelBuf& in = c.lv[L - 1];
+    LevelBuf& in = lv[L - 1];
     if (chain && !chain_fused && L == k - 2) {
       chain_rows(L, rows, n);
       return;
@@ -1676,7 +1749,7 @@ struct Matcher {
       if (add_work(n)) return;
       const uint32_t grid = grid_for(n, BLOCK, grid_cap);
       pbegin(true);
-#define CF(LL) hipLaunchKernelGGL((k_closed_form<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, tbl, c.counters, (int)prof)
+#define CF(LL) hipLaunchKernelGGL((k_closed_form<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, tbl, ctr, (int)prof)
       CUTS_DISPATCH_L(L, CF)
 #undef CF
       HIPCHECK(hipGetLastError());
@@ -1692,13 +1765,13 @@ struct Matcher {
     else if (L == k - 1 && !write) mode = M_COUNT;
     else if (L == k - 2 && fuse) mode = M_FUSED;
     const int mclass = mode == M_WRITE ? 4 : mode == M_COUNT ? 5 : mode == M_FUSED ? 6 : (mode == M_PAIR || mode == M_COLLAPSE) ? 8 : 10;
-    if (mode == M_PAIR) ensure_aux(in.pbase, in.pbase_cap, n);
+    if (mode == M_PAIR) ensure_aux(arena, in.pbase, in.pbase_cap, n);
     {
       const uint32_t grid = grid_for(n, BLOCK, grid_cap);
-      unsigned long long* merge = prof ? c.counters + 8 : nullptr;
+      unsigned long long* merge = prof ? ctr + 8 : nullptr;
       if (merge) HIPCHECK(hipMemsetAsync(merge, 0, 8, c.stream));
       pbegin(false);
-#define PL(LL) hipLaunchKernelGGL((k_plan<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.minlen, in.minsel, use_cbase ? in.cbase : nullptr, tbl, mode == M_PAIR ? in.pbase : nullptr, ptbl, merge, c.counters)
+#define PL(LL) hipLaunchKernelGGL((k_plan<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.minlen, in.minsel, use_cbase ? in.cbase : nullptr, tbl, mode == M_PAIR ? in.pbase : nullptr, ptbl, merge, ctr)
       CUTS_DISPATCH_L(L, PL)
 #undef PL
       HIPCHECK(hipGetLastError());
@@ -1710,23 +1783,23 @@ struct Matcher {
     if (W == 0) return;
     if (add_work(W)) return;
     uint64_t chunk = MAX_CHUNK;
-    LevelBuf& out = c.lv[L];
+    LevelBuf& out = lv[L];
     if (mode == M_WRITE) {
       const uint64_t cap = level_cap[L];
       chunk = std::min<uint64_t>(chunk, std::max<uint64_t>(cap, 1));
-      ensure_rows_exact(out, std::min<uint64_t>(chunk, W), L + 1);
+      ensure_rows_exact(arena, out, std::min<uint64_t>(chunk, W), L + 1);
       chunk = std::min<uint64_t>(chunk, out.cap);
     }
-    const bool pair_common = mode == M_PAIR && kp.pr_a != kp.pr_b && !(g.h2_off && kp.pr_tab <= 2);
+    const bool pair_common = mode == M_PAIR && kp.pr_a != kp.pr_b && !pair_cab_known();
     if (pair_common) {
-      ensure_aux(in.nsurv, in.nsurv_cap, n);
+      ensure_aux(arena, in.nsurv, in.nsurv_cap, n);
       HIPCHECK(hipMemsetAsync(in.nsurv, 0, n * sizeof(uint32_t), c.stream));
     }
     for (uint64_t w0 = 0; w0 < W; w0 += chunk) {
       if (past_deadline()) return;
       const uint64_t w1 = std::min(W, w0 + chunk);
       const uint64_t nt = (w1 - w0 + TILE - 1) / TILE;
-      ensure_tiles(in, nt);
+      ensure_tiles(arena, in, nt);
       pbegin(false);
       hipLaunchKernelGGL(k_tile_index, dim3(grid_for(nt + 1, BLOCK, 65535)), dim3(BLOCK), 0, c.stream, in.work_off,
                          n, w0, w1, nt, in.tile_p);
@@ -1739,36 +1812,36 @@ struct Matcher {
       a.w_begin = w0;
       a.w_end = w1;
       a.ntiles = nt;
-      a.counters = c.counters;
+      a.counters = ctr;
       a.tbl = tbl;
       a.cbase = use_cbase ? in.cbase : nullptr;
       a.pbase = mode == M_PAIR ? in.pbase : nullptr;
       a.nsurv = pair_common ? in.nsurv : nullptr;
       a.out_parent = nullptr;
       if ((chain && !chain_fused && mode == M_WRITE && L == k - 3) || (sib && mode == M_WRITE && L == k - 4)) {
-        ensure_aux(out.parent, out.parent_cap, out.cap);
+        ensure_aux(arena, out.parent, out.parent_cap, out.cap);
         a.out_parent = out.parent;
       }
       a.shared = in.nsurv;
       a.ctab = ctab;
       a.ptab = ptbl;
       a.xcd = (o.flags & 8u) ? 0 : 1;
-      a.tilectr = (unsigned int*)(c.counters + 4);  // 8 x uint32 in counters[4..7]
+      a.tilectr = (unsigned int*)(ctr + 4);  // 8 x uint32 in counters[4..7]
       HIPCHECK(hipMemsetAsync(a.tilectr, 0, 8 * sizeof(unsigned int), c.stream));
       a.stats = prof ? 1 : 0;
       uint32_t grid = grid_for(nt, 1, grid_cap);
       r.chunks++;
       if (mode == M_WRITE) {
         a.out_rows = out.rows;
-        a.out_count = (uint32_t*)(c.counters + 10);
-        HIPCHECK(hipMemsetAsync(c.counters + 10, 0, 8, c.stream));
+        a.out_count = (uint32_t*)(ctr + 10);
+        HIPCHECK(hipMemsetAsync(ctr + 10, 0, 8, c.stream));
         pbegin(true);
 #define EW(LL) hipLaunchKernelGGL((k_expand<LL, M_WRITE>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, a)
         CUTS_DISPATCH_L(L, EW)
 #undef EW
         HIPCHECK(hipGetLastError());
         pend(4, 0, true);
-        const uint64_t m = read_u32(c.counters + 10);
+        const uint64_t m = read_u32(ctr + 10);
         r.level[L] += m;
         if (prof && L < k - 1) r.merge_bytes[4] += 16.0 * (double)m;  // trie nodes written and read back
         if (L == k - 1) {
@@ -1850,7 +1923,7 @@ struct Matcher {
       if (past_deadline()) return;
       const uint64_t w1 = std::min(W, w0 + MAX_CHUNK);
       const uint64_t nt = (w1 - w0 + TILE - 1) / TILE;
-      ensure_tiles(in, nt);
+      ensure_tiles(arena, in, nt);
       pbegin(false);
       hipLaunchKernelGGL(k_tile_index, dim3(grid_for(nt + 1, BLOCK, 65535)), dim3(BLOCK), 0, c.stream, in.work_off,
                          n, w0, w1, nt, in.tile_p);
@@ -1863,10 +1936,10 @@ struct Matcher {
       a.w_begin = w0;
       a.w_end = w1;
       a.ntiles = nt;
-      a.counters = c.counters;
+      a.counters = ctr;
       a.nsurv = weight;
       a.xcd = (o.flags & 8u) ? 0 : 1;
-      a.tilectr = (unsigned int*)(c.counters + 4);
+      a.tilectr = (unsigned int*)(ctr + 4);
       HIPCHECK(hipMemsetAsync(a.tilectr, 0, 8 * sizeof(unsigned int), c.stream));
       a.stats = prof ? 1 : 0;
       r.chunks++;
@@ -1884,7 +1957,7 @@ struct Matcher {
   // positions i of |N(u) & N(r_i)| (those whose degree takes completions
   // away), shared by all its children; into in.nsurv.
   void chain_shared(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in) {
-    ensure_aux(in.nsurv, in.nsurv_cap, n);
+    ensure_aux(arena, in.nsurv, in.nsurv_cap, n);
     if (g.h2_off && kp.ch_ta <= 2) {
       const uint32_t grid = grid_for(n, BLOCK, grid_cap);
       pbegin(false);
@@ -1919,7 +1992,7 @@ struct Matcher {
       if (past_deadline()) return;
       const uint64_t w1 = std::min(W, w0 + MAX_CHUNK);
       const uint64_t nt = (w1 - w0 + TILE - 1) / TILE;
-      ensure_tiles(in, nt);
+      ensure_tiles(arena, in, nt);
       hipLaunchKernelGGL(k_tile_index, dim3(grid_for(nt + 1, BLOCK, 65535)), dim3(BLOCK), 0, c.stream, in.work_off,
                          n, w0, w1, nt, in.tile_p);
       ExpandArgs a{};
@@ -1930,12 +2003,12 @@ struct Matcher {
       a.w_begin = w0;
       a.w_end = w1;
       a.ntiles = nt;
-      a.counters = c.counters;
+      a.counters = ctr;
       a.nsurv = in.nsurv;
       a.coff = in.coff;
       a.cand = in.cand;
       a.xcd = (o.flags & 8u) ? 0 : 1;
-      a.tilectr = (unsigned int*)(c.counters + 4);
+      a.tilectr = (unsigned int*)(ctr + 4);
       a.stats = prof ? 1 : 0;
       HIPCHECK(hipMemsetAsync(a.tilectr, 0, 8 * sizeof(unsigned int), c.stream));
       r.chunks++;
@@ -1963,15 +2036,15 @@ struct Matcher {
     const uint32_t grid = grid_for(n, BLOCK, grid_cap);
     auto plan_vk = [&]() {
       pbegin(false);
-#define PV(LL) hipLaunchKernelGGL((k_plan<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, vk, rows, n, in.minlen, in.minsel, nullptr, nullptr, nullptr, ptbl, nullptr, c.counters)
+#define PV(LL) hipLaunchKernelGGL((k_plan<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, vk, rows, n, in.minlen, in.minsel, nullptr, nullptr, nullptr, ptbl, nullptr, ctr)
       CUTS_DISPATCH_PAIR(L, PV)
 #undef PV
       HIPCHECK(hipGetLastError());
       pend(1, (double)n * (4.0 * L + 8.0 * vk.nj[L] + 5.0), false);
     };
     sib_live = false;
-    ensure_aux(in.nsurv, in.nsurv_cap, n);
-    ensure_aux(in.coff, in.coff_cap, n + 1);
+    ensure_aux(arena, in.nsurv, in.nsurv_cap, n);
+    ensure_aux(arena, in.coff, in.coff_cap, n + 1);
     // a row's shortest constraint list bounds its candidates: blocks sized by
     // the bounds (the scan of the planner's lengths), filled in one pass, with
     // the filled length of every block in in.nsurv
@@ -1981,7 +2054,7 @@ struct Matcher {
     // themselves (the fused chain expansion)
     if (total > cand_limit) return;
     HIPCHECK(hipMemcpyAsync(in.coff, in.work_off, (n + 1) * sizeof(uint64_t), hipMemcpyDeviceToDevice, c.stream));
-    ensure_aux(in.cand, in.cand_cap, total);
+    ensure_aux(arena, in.cand, in.cand_cap, total);
     HIPCHECK(hipMemsetAsync(in.nsurv, 0, n * sizeof(uint32_t), c.stream));
     write_candidates(L, rows, n, in, vk);
     sib_live = !aborted;
@@ -1990,14 +2063,14 @@ struct Matcher {
   // Chain level with shared candidates over the rows of level k-4 (L = k-3
   // words): every row paired with its parent's candidates (k_chain_sib).
   void sib_level(uint32_t L, const uint32_t* rows, uint64_t n) {
-    LevelBuf& in = c.lv[L - 1];
-    LevelBuf& par = c.lv[L - 2];
+    LevelBuf& in = lv[L - 1];
+    LevelBuf& par = lv[L - 2];
     const uint32_t grid = grid_for(n, BLOCK, grid_cap);
     if (prof) {
       // merge model: the expansion of these rows (the planner's figure)
-      unsigned long long* merge = c.counters + 8;
+      unsigned long long* merge = ctr + 8;
       HIPCHECK(hipMemsetAsync(merge, 0, 8, c.stream));
-#define PS(LL) hipLaunchKernelGGL((k_plan<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.minlen, in.minsel, nullptr, nullptr, nullptr, ptbl, merge, c.counters)
+#define PS(LL) hipLaunchKernelGGL((k_plan<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.minlen, in.minsel, nullptr, nullptr, nullptr, ptbl, merge, ctr)
       CUTS_DISPATCH_PAIR(L, PS)
 #undef PS
       HIPCHECK(hipGetLastError());
@@ -2013,7 +2086,7 @@ struct Matcher {
       if (past_deadline()) return;
       const uint64_t w1 = std::min(W, w0 + MAX_CHUNK);
       const uint64_t nt = (w1 - w0 + TILE - 1) / TILE;
-      ensure_tiles(in, nt);
+      ensure_tiles(arena, in, nt);
       hipLaunchKernelGGL(k_tile_index, dim3(grid_for(nt + 1, BLOCK, 65535)), dim3(BLOCK), 0, c.stream, in.work_off,
                          n, w0, w1, nt, in.tile_p);
       SibArgs a{};
@@ -2027,8 +2100,8 @@ struct Matcher {
       a.w_begin = w0;
       a.w_end = w1;
       a.ntiles = nt;
-      a.counters = c.counters;
-      a.tilectr = (unsigned int*)(c.counters + 4);
+      a.counters = ctr;
+      a.tilectr = (unsigned int*)(ctr + 4);
       HIPCHECK(hipMemsetAsync(a.tilectr, 0, 8 * sizeof(unsigned int), c.stream));
       a.xcd = (o.flags & 8u) ? 0 : 1;
       a.stats = prof ? 1 : 0;
@@ -2045,10 +2118,10 @@ struct Matcher {
   // Chain level over the rows of level k-3 (L = k-2 words): closed form per
   // row (k_chain_row), then the intersection with the newest vertex.
   void chain_rows(uint32_t L, const uint32_t* rows, uint64_t n) {
-    LevelBuf& in = c.lv[L - 1];
+    LevelBuf& in = lv[L - 1];
     const uint32_t grid = grid_for(n, BLOCK, grid_cap);
     pbegin(true);
-#define CR(LL) hipLaunchKernelGGL((k_chain_row<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.parent, c.lv[L - 2].nsurv, ctab, in.minlen, in.minsel, c.counters, (int)prof)
+#define CR(LL) hipLaunchKernelGGL((k_chain_row<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.parent, lv[L - 2].nsurv, ctab, in.minlen, in.minsel, ctr, (int)prof)
     CUTS_DISPATCH_L(L, CR)
 #undef CR
     HIPCHECK(hipGetLastError());
@@ -2064,6 +2137,7 @@ struct Matcher {
     if (o.flags & 32u) g.hub_id = nullptr;  // CUTS_FLAG_NO_BITMAP
     if (o.flags & 512u) g.h2_off = nullptr;  // CUTS_FLAG_NO_TWO_HOP
     if (o.flags & 512u) g.h2_did = nullptr;
+    if (o.flags & 2048u) g.arc_cn = nullptr;  // CUTS_FLAG_NO_ARC_COMMON
     write = (o.flags & 4u) != 0;
     prof = (o.flags & 16u) != 0;
     closed_last = !write && !(o.flags & 1u) && kp.sc_ok && k >= 2;
@@ -2089,35 +2163,35 @@ struct Matcher {
     }
     // shared chain candidates: up to one level's share of the budget
     cand_limit = std::max<uint64_t>((nlev ? budget / nlev : budget) / 4, 1u << 10);
-    if (closed_last && kp.sc_thr > 1) tbl = cnt_table(c, dg, kp.sc_thr);
+    if (closed_last && kp.sc_thr > 1) tbl = cnt_table(c, dg, arena, kp.sc_thr);
     // with the two-hop index the chain is counted inside the expansion of
     // position k-3 (no rows of level k-3 are stored)
     chain_fused = chain && g.h2_off && kp.ch_ta <= 2;
     // a collapsing pair level is counted inside the expansion of position
     // k-4 (no rows of level k-4 are stored); with a deadline, in the planner
     // of level k-3 instead (the rolled-back counters are those of 0..2)
-    collapse_fused = pair && kp.pr_collapse && g.h2_off && k >= 5 && !(o.time_limit_ms > 0);
+    collapse_fused = pair && kp.pr_collapse && pair_cab_known() && k >= 5 && !(o.time_limit_ms > 0);
     // the fused chain with position k-3's candidates shared by siblings
     sib = chain_fused && kp.ch_sib && !(o.flags & 1024u);
     if (chain) {
-      ctab.ta = kp.ch_ta > 1 ? cnt_table(c, dg, kp.ch_ta) : nullptr;
-      ctab.tb = kp.ch_tb > 1 ? cnt_table(c, dg, kp.ch_tb) : nullptr;
-      ctab.H = chain_table(c, dg, kp.ch_ta, kp.ch_tb);
+      ctab.ta = kp.ch_ta > 1 ? cnt_table(c, dg, arena, kp.ch_ta) : nullptr;
+      ctab.tb = kp.ch_tb > 1 ? cnt_table(c, dg, arena, kp.ch_tb) : nullptr;
+      ctab.H = chain_table(c, dg, arena, kp.ch_ta, kp.ch_tb);
     }
     if (pair) {
-      ptbl.a = kp.pr_ta > 1 ? cnt_table(c, dg, kp.pr_ta) : nullptr;
-      ptbl.b = kp.pr_tb > 1 ? cnt_table(c, dg, kp.pr_tb) : nullptr;
-      ptbl.ab = kp.pr_tab > 1 ? cnt_table(c, dg, kp.pr_tab) : nullptr;
+      ptbl.a = kp.pr_ta > 1 ? cnt_table(c, dg, arena, kp.pr_ta) : nullptr;
+      ptbl.b = kp.pr_tb > 1 ? cnt_table(c, dg, arena, kp.pr_tb) : nullptr;
+      ptbl.ab = kp.pr_tab > 1 ? cnt_table(c, dg, arena, kp.pr_tab) : nullptr;
     }
     // level 0
     const uint32_t rb = o.root_stride ? o.root_begin : 0;
     const uint32_t re = o.root_stride ? std::min(o.root_end, dg.V) : dg.V;
     const uint32_t rs = o.root_stride ? o.root_stride : 1;
     const uint32_t n0c = re > rb ? (re - rb + rs - 1) / rs : 0;
-    LevelBuf& l0 = c.lv[0];
-    ensure_rows(l0, std::max<uint32_t>(n0c, 1), 1);
-    HIPCHECK(hipMemsetAsync(c.counters, 0, 4 * 8, c.stream));
-    HIPCHECK(hipMemsetAsync(c.counters + 9, 0, 8, c.stream));
+    LevelBuf& l0 = lv[0];
+    ensure_rows(arena, l0, std::max<uint32_t>(n0c, 1), 1);
+    HIPCHECK(hipMemsetAsync(ctr, 0, 4 * 8, c.stream));
+    HIPCHECK(hipMemsetAsync(ctr + 9, 0, 8, c.stream));
     HIPCHECK(hipEventRecord(c.ev0, c.stream));
     const auto t_start = std::chrono::steady_clock::now();
     uint64_t n0 = 0;
@@ -2155,7 +2229,7 @@ struct Matcher {
         std::memcpy(snap_level, r.level, sizeof(snap_level));
         if (has_deadline && !write) {
           if (past_deadline()) break;
-          HIPCHECK(hipMemcpyAsync(c.host_pinned, c.counters, 24, hipMemcpyDeviceToHost, c.stream));
+          HIPCHECK(hipMemcpyAsync(c.host_pinned, ctr, 24, hipMemcpyDeviceToHost, c.stream));
           HIPCHECK(hipStreamSynchronize(c.stream));
           snap[0] = c.host_pinned[0];
           snap[1] = c.host_pinned[1];
@@ -2166,7 +2240,7 @@ struct Matcher {
           c.host_pinned[0] = snap[0];
           c.host_pinned[1] = snap[1];
           c.host_pinned[2] = snap[2];
-          HIPCHECK(hipMemcpyAsync(c.counters, c.host_pinned, 24, hipMemcpyHostToDevice, c.stream));
+          HIPCHECK(hipMemcpyAsync(ctr, c.host_pinned, 24, hipMemcpyHostToDevice, c.stream));
           HIPCHECK(hipStreamSynchronize(c.stream));
           std::memcpy(r.level, snap_level, sizeof(snap_level));
           break;
@@ -2175,14 +2249,14 @@ struct Matcher {
       }
       if (aborted || r.roots_done < n0) r.complete = 0;
       if (!write) {
-        HIPCHECK(hipMemcpyAsync(c.host_pinned, c.counters, 24, hipMemcpyDeviceToHost, c.stream));
+        HIPCHECK(hipMemcpyAsync(c.host_pinned, ctr, 24, hipMemcpyDeviceToHost, c.stream));
         HIPCHECK(hipStreamSynchronize(c.stream));
         r.count = c.host_pinned[0];
         r.level[k - 1] = r.count;
         if (collapse_fused) {
           r.level[k - 4] = c.host_pinned[1];
           r.level[k - 3] = c.host_pinned[2];
-          r.level[k - 2] = read_u64(c.counters + 9);
+          r.level[k - 2] = read_u64(ctr + 9);
         } else if (pair || chain_fused) {
           r.level[k - 3] = c.host_pinned[1];
           r.level[k - 2] = c.host_pinned[2];
@@ -2216,34 +2290,47 @@ __global__ __launch_bounds__(BLOCK) void k_copy_probe(const uint4* __restrict__ 
 
 }  // namespace
 
-const uint32_t* cnt_table(Context& c, DeviceGraph& dg, uint32_t thr) {
-  auto it = dg.cnt_tables.find(thr);
-  if (it == dg.cnt_tables.end()) {
-    uint32_t* t = nullptr;
-    dev_alloc(t, dg.V + 1);
-    const GView g = view_of(dg);
-    hipLaunchKernelGGL(k_cnt_table, dim3(grid_for(dg.V, BLOCK / 64, 65535)), dim3(BLOCK), 0, c.stream, g, thr, t);
-    HIPCHECK(hipGetLastError());
-    HIPCHECK(hipStreamSynchronize(c.stream));
-    it = dg.cnt_tables.emplace(thr, t).first;
-  }
-  return it->second;
+void* ScratchArena::carve(uint64_t bytes) {
+  const uint64_t at = (used + 255) & ~255ull;
+  if (at + bytes > size)
+    throw Error(-5, "no-allocation scratch of " + std::to_string(size) + " bytes exhausted (" +
+                        std::to_string(at + bytes) + " needed)");
+  used = at + bytes;
+  return base + at;
 }
 
-const uint64_t* chain_table(Context& c, DeviceGraph& dg, uint32_t ta, uint32_t tb) {
+// Degree-threshold table: cached on the device graph, or, with an arena,
+// built in the scratch for this call only (and not cached).
+const uint32_t* cnt_table(Context& c, DeviceGraph& dg, ScratchArena* arena, uint32_t thr) {
+  auto it = dg.cnt_tables.find(thr);
+  if (it != dg.cnt_tables.end()) return it->second;
+  uint32_t* t = nullptr;
+  dev_alloc(arena, t, dg.V + 1);
+  const GView g = view_of(dg);
+  hipLaunchKernelGGL(k_cnt_table, dim3(grid_for(dg.V, BLOCK / 64, 65535)), dim3(BLOCK), 0, c.stream, g, thr, t);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  if (!arena) dg.cnt_tables.emplace(thr, t);
+  return t;
+}
+const uint32_t* cnt_table(Context& c, DeviceGraph& dg, uint32_t thr) { return cnt_table(c, dg, nullptr, thr); }
+
+const uint64_t* chain_table(Context& c, DeviceGraph& dg, ScratchArena* arena, uint32_t ta, uint32_t tb) {
   const auto key = std::make_pair(ta, tb);
   auto it = dg.chain_tables.find(key);
-  if (it == dg.chain_tables.end()) {
-    uint64_t* t = nullptr;
-    dev_alloc(t, 2 * (uint64_t)dg.V + 1);
-    const GView g = view_of(dg);
-    const uint32_t* tbt = tb > 1 ? cnt_table(c, dg, tb) : nullptr;
-    hipLaunchKernelGGL(k_chain_table, dim3(grid_for(dg.V, BLOCK / 64, 65535)), dim3(BLOCK), 0, c.stream, g, ta, tbt, t);
-    HIPCHECK(hipGetLastError());
-    HIPCHECK(hipStreamSynchronize(c.stream));
-    it = dg.chain_tables.emplace(key, t).first;
-  }
-  return it->second;
+  if (it != dg.chain_tables.end()) return it->second;
+  uint64_t* t = nullptr;
+  dev_alloc(arena, t, 2 * (uint64_t)dg.V + 1);
+  const GView g = view_of(dg);
+  const uint32_t* tbt = tb > 1 ? cnt_table(c, dg, arena, tb) : nullptr;
+  hipLaunchKernelGGL(k_chain_table, dim3(grid_for(dg.V, BLOCK / 64, 65535)), dim3(BLOCK), 0, c.stream, g, ta, tbt, t);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  if (!arena) dg.chain_tables.emplace(key, t);
+  return t;
+}
+const uint64_t* chain_table(Context& c, DeviceGraph& dg, uint32_t ta, uint32_t tb) {
+  return chain_table(c, dg, nullptr, ta, tb);
 }
 
 double copy_probe(Context& c, uint64_t bytes, uint32_t iters) {
@@ -2319,6 +2406,7 @@ KPlan make_kplan(const Plan& p, bool symmetric, bool loop_free) {
         if (adjacent(i, pa)) kp.pr_ma |= 1u << i;
         if (adjacent(i, pb)) kp.pr_mb |= 1u << i;
       }
+      kp.pr_adj = adjacent(pa, pb);
       const uint32_t c = last - 2;
       if (pa != pb && kp.nj[c] == 2 && std::max(p.sig_in[c], p.sig_out[c]) <= 2 && kp.pr_tab <= 2) {
         const uint32_t j0 = kp.jc[c][0] & 31, j1 = kp.jc[c][1] & 31;
@@ -2363,6 +2451,7 @@ DeviceGraph::~DeviceGraph() {
   for (auto& kv : chain_tables) (void)hipFree(kv.second);
   if (hub_id) (void)hipFree(hub_id);
   if (bits) (void)hipFree(bits);
+  if (arc_cn) (void)hipFree(arc_cn);
   for (void* p : {(void*)h2_off, (void*)h2_v, (void*)h2_c, (void*)h2_did, (void*)h2_dense})
     if (p) (void)hipFree(p);
   (void)hipSetDevice(prev);
@@ -2486,6 +2575,24 @@ void upload_graph(Context& c, const Graph& g, DeviceGraph& dg) {
       HIPCHECK(hipGetLastError());
       HIPCHECK(hipStreamSynchronize(c.stream));
       (void)hipFree(dh);
+    }
+  }
+  // Per-arc common-neighbour counts (symmetric loop-free graphs), built on
+  // the device with the hub bitmaps above: 4 bytes per arc while they fit
+  // 1/16 of free HBM.
+  if (g.symmetric && dg.loops == 0 && dg.A > 0) {
+    size_t fr = 0, tot = 0;
+    HIPCHECK(hipMemGetInfo(&fr, &tot));
+    if (dg.A * 4 <= fr / 16) {
+      const auto t0 = std::chrono::steady_clock::now();
+      dev_alloc(dg.arc_cn, dg.A);
+      GView gv = view_of(dg);
+      gv.arc_cn = nullptr;
+      const uint32_t grid = (uint32_t)std::min<uint64_t>((dg.A + 255) / 256, (uint64_t)c.cus * 32);
+      hipLaunchKernelGGL(k_arc_common, dim3(grid), dim3(256), 0, c.stream, gv, dg.A, dg.arc_cn);
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipStreamSynchronize(c.stream));
+      dg.arc_cn_build_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
   }
   // Two-hop index: built on the host when the wedges (ordered neighbour

@@ -21,6 +21,7 @@ compared with the score order's.
 from __future__ import annotations
 
 import ctypes as C
+import itertools
 import math
 from dataclasses import dataclass
 from typing import Dict, List, Optional, Sequence, Tuple
@@ -231,35 +232,107 @@ def connected_subsets(k: int, adj: List[int]) -> List[int]:
     return out
 
 
-def order_cost(order: Sequence[int], adj: List[int], f: Dict[int, float]) -> float:
-    """Predicted rows touched: sum of |T| over prefixes of length <= k-1, plus
-    |T(V_Q)| when the last vertex has more than one earlier neighbour."""
+@dataclass(frozen=True)
+class Engine:
+    """What the matcher counts in closed form for a given order
+    (make_kplan and Matcher::run in cuts_amd/csrc/match.hip), so that an
+    order is costed by the levels the engine actually expands:
+
+    * symmetric data graph, last vertex with one earlier neighbour: the last
+      level is counted in closed form inside the expansion of position k-2;
+    * pair_chain: the last two positions counted together, as a pair (two
+      pendants on vertices older than position k-3) or a chain (a 2-edge
+      tail on such a vertex), inside the expansion of position k-3; a pair
+      whose position k-3 is joined exactly to the two pendant owners (degree
+      <= 2) collapses into the expansion of position k-4 when |N(u) & N(w)|
+      has an index: the two-hop index, or the per-arc table when the owners
+      are adjacent;
+    * without that index a pair of distinct owners, and a chain without the
+      two-hop index, pay one more pass over the last expanded level.
+    """
+    symmetric: bool = True
+    pair_chain: bool = True
+    two_hop: bool = False
+    arc_common: bool = False
+
+
+# Engine without pair and chain levels (CUTS_FLAG_NO_PAIR | CUTS_FLAG_NO_CHAIN):
+# only the last level is counted in closed form.
+LAST_LEVEL_ONLY = Engine(pair_chain=False)
+
+
+def _pc(x: int) -> int:
+    return bin(x).count("1")
+
+
+def _tail_mode(adj: List[int], s0: int, tail: Sequence[int], k: int, eng: Engine) -> Tuple[int, bool]:
+    """(E, extra pass) for an order whose first k-len(tail) positions hold
+    the set s0 and whose last positions are `tail`: E is the last position
+    the engine expands (positions after it are counted in closed form)."""
+    T = len(tail)
+    before = [s0]
+    for v in tail:
+        before.append(before[-1] | (1 << v))
+    b = [adj[v] & before[j] for j, v in enumerate(tail)]  # earlier neighbours of each tail position
+    last_b = b[-1]
+    if not eng.symmetric or k < 2 or _pc(last_b) != 1:
+        return k - 1, False
+    if eng.pair_chain and k >= 4 and T == 3:
+        t1, t2, t3 = tail
+        if _pc(b[1]) == 1:
+            a = b[1].bit_length() - 1
+            bb = last_b.bit_length() - 1
+            older = (s0 >> a) & 1 and (s0 >> bb) & 1  # both owners at positions <= k-4
+            if bb != t2 and older:
+                adj_ab = a != bb and (adj[a] >> bb) & 1
+                known = a == bb or eng.two_hop or (eng.arc_common and adj_ab)
+                if (a != bb and b[0] == (1 << a) | (1 << bb) and _pc(adj[t1]) <= 2 and k >= 5 and
+                        (eng.two_hop or (eng.arc_common and adj_ab))):
+                    return k - 4, False
+                return k - 3, not known
+            if last_b == 1 << t2:
+                u = b[1].bit_length() - 1
+                if (s0 >> u) & 1:
+                    return k - 3, not eng.two_hop
+    return (k - 2 if k >= 3 else 0), False
+
+
+def order_cost(order: Sequence[int], adj: List[int], f: Dict[int, float], eng: Engine = LAST_LEVEL_ONLY) -> float:
+    """Predicted trie rows the engine touches: the sum of |T| over the
+    prefixes of positions 0..E, E the last position it expands (plus the last
+    expanded level once more when a pair or chain needs its extra pass)."""
     k = len(order)
+    T = min(3, k - 1)
+    s0 = 0
+    for v in order[:k - T]:
+        s0 |= 1 << v
+    E, extra = _tail_mode(adj, s0, list(order[k - T:]), k, eng)
     sm, cost = 0, 0.0
     for l, v in enumerate(order):
         sm |= 1 << v
-        if l <= k - 2:
+        if l <= E:
             cost += f[sm]
-    last = order[-1]
-    if bin(adj[last] & (sm & ~(1 << last))).count("1") > 1:
-        cost += f[sm]
+        if extra and l == k - 3:
+            cost += f[sm]
     return cost
 
 
-def best_order(k: int, adj: List[int], f: Dict[int, float], first: Optional[int] = None) -> Tuple[List[int], float]:
+def best_order(k: int, adj: List[int], f: Dict[int, float], first: Optional[int] = None,
+               eng: Engine = LAST_LEVEL_ONLY) -> Tuple[List[int], float]:
     """Exact minimum of order_cost over connected orders (every prefix
-    connected), by dynamic programming over connected subsets; with `first`,
-    over the orders that start with that vertex."""
+    connected), by dynamic programming over connected subsets for all but
+    the last three positions and enumeration of those; with `first`, over
+    the orders that start with that vertex."""
     full = (1 << k) - 1
+    if k == 1:
+        return [0], f[full]
     best: Dict[int, Tuple[float, int]] = {}
-    for sm in sorted(f, key=lambda m: bin(m).count("1")):
+    for sm in sorted(f, key=_pc):
         if first is not None and not (sm >> first) & 1:
             continue
-        n = bin(sm).count("1")
-        if n == 1:
-            best[sm] = (f[sm], -1)
-            continue
-        if sm == full:
+        if _pc(sm) == 1:
+            if first is None or sm == 1 << first:
+                best[sm] = (f[sm], -1)
             continue
         cand = None
         m = sm
@@ -273,20 +346,37 @@ def best_order(k: int, adj: List[int], f: Dict[int, float], first: Optional[int]
                     cand = (c, v)
         if cand is not None:
             best[sm] = cand
-    # last vertex: closed form when it has exactly one earlier neighbour
-    if k == 1:
-        return [0], f[full]
+    T = min(3, k - 1)
     top = None
-    for v in range(k):
-        if v == first:
+    for tail in itertools.permutations([v for v in range(k) if v != first], T):
+        s0 = full
+        for v in tail:
+            s0 &= ~(1 << v)
+        if s0 not in best:
             continue
-        rest = full & ~(1 << v)
-        if rest in best and adj[v] & rest:
-            c = best[rest][0] + (f[full] if bin(adj[v] & rest).count("1") > 1 else 0.0)
-            if top is None or c < top[0]:
-                top = (c, v)
-    order = [top[1]]
-    sm = full & ~(1 << top[1])
+        sm, ok = s0, True
+        for v in tail:
+            if not adj[v] & sm:
+                ok = False
+                break
+            sm |= 1 << v
+        if not ok:
+            continue
+        E, extra = _tail_mode(adj, s0, tail, k, eng)
+        c = best[s0][0]
+        sm = s0
+        for j, v in enumerate(tail):
+            sm |= 1 << v
+            pos = k - T + j
+            if pos <= E:
+                c += f[sm]
+            if extra and pos == k - 3:
+                c += f[sm]
+        if top is None or c < top[0]:
+            top = (c, s0, tail)
+    c, s0, tail = top
+    order = list(tail)[::-1]
+    sm = s0
     while sm:
         v = best[sm][1]
         if v < 0:
@@ -295,7 +385,7 @@ def best_order(k: int, adj: List[int], f: Dict[int, float], first: Optional[int]
         order.append(v)
         sm &= ~(1 << v)
     order.reverse()
-    return order, top[0]
+    return order, c
 
 
 @dataclass
@@ -308,9 +398,10 @@ class OrderChoice:
 
 def choose_orders(net: Mpnn, queries: Sequence[Tuple[int, List[Tuple[int, int]]]], gfeat: np.ndarray,
                   heuristic: Sequence[List[int]], margin: float = 0.25,
-                  keep_first: bool = False) -> List[OrderChoice]:
+                  keep_first: bool = False, eng: Engine = LAST_LEVEL_ONLY) -> List[OrderChoice]:
     """One forward launch over every connected subset of every query, then
-    the DP per query.  Falls back to the score order unless the network
+    the DP per query under the engine's cost (`eng`: what the matcher will
+    count in closed form).  Falls back to the score order unless the network
     predicts at least `margin` (relative) fewer rows.  keep_first restricts
     the DP to orders that start with the score order's first vertex (needed
     when position 0 is matched on a sample of the data vertices, so that
@@ -329,8 +420,8 @@ def choose_orders(net: Mpnn, queries: Sequence[Tuple[int, List[Tuple[int, int]]]
     out = []
     for (k, edges), adj, (s0, subs), h in zip(queries, adjs, spans, heuristic):
         f = {sm: float(sizes[s0 + i]) for i, sm in enumerate(subs)}
-        order, c = best_order(k, adj, f, first=h[0] if keep_first else None)
-        ch = order_cost(h, adj, f)
+        order, c = best_order(k, adj, f, first=h[0] if keep_first else None, eng=eng)
+        ch = order_cost(h, adj, f, eng)
         if c < (1.0 - margin) * ch:
             out.append(OrderChoice(order, True, c, ch))
         else:
@@ -369,6 +460,42 @@ def judge(ctx: Context, dg, queries: Sequence[Tuple[int, List[Tuple[int, int]]]]
     return out
 
 
+def judge_by_time(ctx: Context, dg, queries: Sequence[Tuple[int, List[Tuple[int, int]]]], heuristic_plans,
+                  heuristic_counts: Sequence[int], choices: Sequence[OrderChoice], roots=None, flags: int = 0,
+                  heuristic_work: Optional[Sequence[int]] = None, reps: int = 2) -> List["TimedJudged"]:
+    """The score order judges whether a proposal actually runs faster: both
+    orders are run warm `reps` times on the same GPU and the proposal is kept
+    only if its best device time beats the score order's.  The proposal's
+    runs are capped at 4x the score order's candidates (a runaway order is
+    stopped, never waited for).  A kept order's count must equal the score
+    order's (order never changes a count); anything else raises."""
+    out = []
+    for i, ((k, edges), hp, hc, ch) in enumerate(zip(queries, heuristic_plans, heuristic_counts, choices)):
+        rt = None if roots is None else roots[i]
+        th = min(ctx.match(dg, hp, roots=rt, flags=flags).device_ms for _ in range(reps))
+        kept = None
+        if ch.learned and list(ch.order) != list(hp.order):
+            lp = Plan(Graph.from_edges(k, edges), ch.order)
+            cap = 4 * heuristic_work[i] + 4096 if heuristic_work is not None else 0
+            runs = [ctx.match(dg, lp, roots=rt, flags=flags, max_work=cap) for _ in range(reps)]
+            if not any(r.work_capped for r in runs):
+                if any(r.count != hc for r in runs):
+                    raise AssertionError(f"query {i}: learned order counted {runs[0].count}, score order {hc}")
+                tl = min(r.device_ms for r in runs)
+                if tl < th:
+                    kept = TimedJudged(lp, True, tl, th)
+        out.append(kept or TimedJudged(hp, False, th, th))
+    return out
+
+
+@dataclass
+class TimedJudged:
+    plan: Plan
+    learned: bool      # the proposal was kept
+    ms: float          # best judged device time of the kept order
+    heuristic_ms: float
+
+
 # ------------------------------------------------------------- training
 def table_samples(k: int, edges, order: Sequence[int], level_sizes: Sequence[int]):
     """(k, adj, S mask, |T(S)|) for every prefix of a completed run."""
@@ -389,3 +516,53 @@ def random_connected_order(k: int, adj: List[int], rng, first: Optional[int] = N
         order.append(v)
         sm |= 1 << v
     return order
+
+
+# ------------------------------------------------------- weights on disk
+def save_weights(path: str, net: "Mpnn", info: dict):
+    """FP32 master parameters (.npy) plus a JSON description of the training
+    run beside it (path + '.json')."""
+    import json
+    np.save(path, net.params())
+    with open(path + ".json", "w") as f:
+        json.dump(info, f, indent=1, sort_keys=True)
+
+
+def load_weights(ctx: Context, path: str) -> Tuple["Mpnn", dict]:
+    import json
+    import os
+    p = np.load(path, allow_pickle=False)
+    if p.dtype != np.float32 or p.size != num_params():
+        raise ValueError(f"{path}: expected {num_params()} float32 parameters, found {p.size} {p.dtype}")
+    info = {}
+    if os.path.exists(path + ".json"):
+        with open(path + ".json") as f:
+            info = json.load(f)
+    return Mpnn(ctx, p), info
+
+
+def log_tables(ctx: Context, dg, gfeat: np.ndarray, queries, rng, extra_orders: int = 1,
+               flags: int = 0):
+    """Training samples from the solver's own table log: for every admitted
+    query (dict with k, edges, roots, work, order) the score order's run plus
+    `extra_orders` random connected orders with the same first vertex
+    (capped at 4x the score order's work), each prefix's |T| scaled by the
+    root stride so that targets estimate full tables.  Returns (x, A, meta,
+    targets)."""
+    samples, targets = [], []
+    for q in queries:
+        k, edges, rt = q["k"], [tuple(e) for e in q["edges"]], tuple(q["roots"])
+        adj = adjacency(k, edges)
+        qg = Graph.from_edges(k, edges)
+        runs = [(None, 0)] + [(random_connected_order(k, adj, rng, first=q["order"][0]), 4 * q["work"] + 4096)
+                              for _ in range(extra_orders)]
+        for order, cap in runs:
+            p = Plan(qg, order)
+            r = ctx.match(dg, p, roots=rt, max_work=cap, flags=flags)
+            if r.work_capped:
+                continue
+            for kk, a, sm, size in table_samples(k, edges, p.order, r.level_sizes):
+                samples.append((kk, a, sm))
+                targets.append(target(size * rt[2]))
+    x, A, meta = encode(samples, gfeat)
+    return x, A, meta, np.asarray(targets, np.float32)

@@ -180,6 +180,8 @@ def admit(ctx, dg, g: Graph, n: int, seed: int = 0, max_work: int = MAX_WORK, ma
         if k not in want or got[k] >= want[k] or tried[k] >= max_tries_per_k:
             continue
         tried[k] += 1
+        if log and sum(tried.values()) % 200 == 0:
+            log(f"tried {sum(tried.values())} candidates, admitted {len(out)}/{n}")
         p = Plan(Graph.from_edges(k, edges))
         ok, r = run(p, MAX_STRIDE_LOG)
         if not ok:

@@ -91,6 +91,24 @@ void cuts_context_destroy(cuts_context *ctx);
 /* Copies the CSR + signatures to HBM (src/gpu_memory_allocation.cu:26-36). */
 int cuts_device_graph_upload(cuts_context *ctx, const cuts_graph *g, cuts_device_graph **out);
 void cuts_device_graph_free(cuts_device_graph *dg);
+/* What an upload built (no reference counterpart). */
+typedef struct {
+  uint32_t V;
+  uint64_t arcs;
+  int symmetric;
+  uint64_t self_loops;          /* vertices u with u in out(u) */
+  uint32_t hubs, hub_min_deg;   /* hub bitmap rows and their degree threshold */
+  uint64_t two_hop_entries;     /* 0: no two-hop index */
+  uint32_t two_hop_dense_rows;
+  int arc_common;               /* 1: per-arc common-neighbour counts present */
+  double two_hop_build_s, arc_common_build_s;
+} cuts_device_graph_info;
+int cuts_device_graph_query(const cuts_device_graph *dg, cuts_device_graph_info *info);
+/* Per-arc common-neighbour counts (no reference counterpart): built on the
+ * device at upload for symmetric loop-free graphs; out[i] = |N(u) & N(w)|
+ * for the arc (u, w) at position i of the out-lists.  n must equal the
+ * number of arcs. */
+int cuts_device_graph_arc_common(const cuts_device_graph *dg, uint32_t *out, uint64_t n);
 
 enum {
   CUTS_FLAG_NO_SHORTCUT = 1u << 0, /* last level: always scan candidate lists */
@@ -103,7 +121,8 @@ enum {
   CUTS_FLAG_NO_PAIR = 1u << 7,        /* never count the last two positions together */
   CUTS_FLAG_NO_CHAIN = 1u << 8,       /* not as a chain either (k-1 hanging on k-2) */
   CUTS_FLAG_NO_TWO_HOP = 1u << 9,     /* intersections by list walks even where the two-hop index exists */
-  CUTS_FLAG_NO_SHARED = 1u << 10      /* chain level: position k-3's candidates found per row, not per parent */
+  CUTS_FLAG_NO_SHARED = 1u << 10,     /* chain level: position k-3's candidates found per row, not per parent */
+  CUTS_FLAG_NO_ARC_COMMON = 1u << 11  /* ignore the per-arc common-neighbour table */
 };
 
 /* Kernel classes reported by CUTS_FLAG_PROFILE. */

@@ -56,6 +56,12 @@ def lib():
         L.trie_cpu_run_timed.restype = C.c_int
         L.trie_cpu_run_timed.argtypes = [C.POINTER(_OGraph), C.c_uint32, C.POINTER(C.c_uint8), _u32p, C.c_uint32,
                                          C.c_uint64, C.c_int, C.c_double, C.POINTER(TrieResult)]
+        L.trie_cpu_run_roots.restype = C.c_int
+        L.trie_cpu_run_roots.argtypes = [C.POINTER(_OGraph), C.c_uint32, C.POINTER(C.c_uint8), _u32p, C.c_uint32,
+                                         C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, C.c_double,
+                                         C.POINTER(TrieResult)]
+        L.oracle_max_threads.restype = C.c_int
+        L.oracle_max_threads.argtypes = []
         _lib = L
     return _lib
 
@@ -127,6 +133,11 @@ def enumerate_embeddings(g: OGraph, qa: np.ndarray, cap: int = 1 << 20) -> np.nd
     return rows[:n]
 
 
+def max_threads() -> int:
+    """omp_get_max_threads() inside the oracle library."""
+    return int(lib().oracle_max_threads())
+
+
 def heuristic_order(qa: np.ndarray) -> Optional[list]:
     qa, p = _qptr(qa)
     k = qa.shape[0]
@@ -136,12 +147,15 @@ def heuristic_order(qa: np.ndarray) -> Optional[list]:
 
 
 def trie_cpu(g: OGraph, qa: np.ndarray, order: Sequence[int], sample_stride: int = 1, max_rows: int = 0,
-             nthreads: int = 0, max_seconds: float = 0.0) -> TrieResult:
+             nthreads: int = 0, max_seconds: float = 0.0, roots=None) -> TrieResult:
+    """roots: (first, end, stride) restricts level 0 as the matcher's root
+    samples do (before sample_stride applies)."""
     qa, p = _qptr(qa)
     o = np.ascontiguousarray(order, dtype=np.uint32)
     r = TrieResult()
-    rc = lib().trie_cpu_run_timed(C.byref(g.c), qa.shape[0], p, o.ctypes.data_as(_u32p), sample_stride,
-                                  max_rows, nthreads, max_seconds, C.byref(r))
+    rb, re, rs = roots if roots is not None else (0, g.V, 1)
+    rc = lib().trie_cpu_run_roots(C.byref(g.c), qa.shape[0], p, o.ctypes.data_as(_u32p), rb, re, rs,
+                                  sample_stride, max_rows, nthreads, max_seconds, C.byref(r))
     if rc:
         raise ValueError("trie_cpu_run failed")
     return r

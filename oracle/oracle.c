@@ -414,7 +414,13 @@ int trie_cpu_run(const oracle_graph *d, uint32_t k, const uint8_t *qa, const uin
 int trie_cpu_run_timed(const oracle_graph *d, uint32_t k, const uint8_t *qa, const uint32_t *order,
                        uint32_t sample_stride, uint64_t max_rows, int nthreads, double max_seconds,
                        trie_cpu_result *res) {
-  if (k == 0 || k > MAXK) return -1;
+  return trie_cpu_run_roots(d, k, qa, order, 0, d->V, 1, sample_stride, max_rows, nthreads, max_seconds, res);
+}
+
+int trie_cpu_run_roots(const oracle_graph *d, uint32_t k, const uint8_t *qa, const uint32_t *order, uint32_t rb,
+                       uint32_t re, uint32_t rs, uint32_t sample_stride, uint64_t max_rows, int nthreads,
+                       double max_seconds, trie_cpu_result *res) {
+  if (k == 0 || k > MAXK || rs == 0) return -1;
   trie_ctx *c = (trie_ctx *)calloc(1, sizeof(trie_ctx));
   c->d = d;
   plan_from_order(k, qa, order, &c->p, NULL);
@@ -432,7 +438,7 @@ int trie_cpu_run_timed(const oracle_graph *d, uint32_t k, const uint8_t *qa, con
   uint32_t *l0 = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)d->V + 1));
   uint64_t n0 = 0, seen = 0;
   uint32_t dummy[MAXK] = {0};
-  for (uint32_t x = 0; x < d->V; ++x)
+  for (uint32_t x = rb; x < d->V && x < re; x += rs)
     if (accept(d, &c->p, 0, dummy, x, -1)) {
       if (seen++ % sample_stride == 0) l0[n0++] = x;
     }
@@ -454,3 +460,5 @@ int trie_cpu_run_timed(const oracle_graph *d, uint32_t k, const uint8_t *qa, con
   free(c);
   return 0;
 }
+
+int oracle_max_threads(void) { return omp_get_max_threads(); }

@@ -83,6 +83,13 @@ int trie_cpu_run(const oracle_graph *d, uint32_t k, const uint8_t *qa, const uin
 int trie_cpu_run_timed(const oracle_graph *d, uint32_t k, const uint8_t *qa, const uint32_t *order,
                        uint32_t sample_stride, uint64_t max_rows, int nthreads, double max_seconds,
                        trie_cpu_result *res);
+/* As trie_cpu_run_timed, with level 0 restricted to the data vertices
+ * rb, rb+rs, ... below re (the matcher's root samples) before sampling. */
+int trie_cpu_run_roots(const oracle_graph *d, uint32_t k, const uint8_t *qa, const uint32_t *order, uint32_t rb,
+                       uint32_t re, uint32_t rs, uint32_t sample_stride, uint64_t max_rows, int nthreads,
+                       double max_seconds, trie_cpu_result *res);
+/* omp_get_max_threads() of the oracle library (the CPU baseline's pool). */
+int oracle_max_threads(void);
 
 #ifdef __cplusplus
 }

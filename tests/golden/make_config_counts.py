@@ -21,6 +21,13 @@ from cuts_amd import workloads as W  # noqa: E402
 SAMPLES = {
     "config2a": (W.config2_graph, W.C4_2PEND, (2000, 36692, 211)),
     "config2b": (W.config2_graph, W.K4E_TAIL2, (2000, 36692, 211)),
+    # hub roots: vertices 500, 1000, 1500, 2000 -- degrees 71, 42, 28 and 31
+    # against a mean of 10, among the top 6% by degree (Chung-Lu puts the
+    # heaviest weights first); the heaviest vertices (degree ~1,150) have
+    # 1e12+ embeddings each, beyond the oracle, and are pinned on the GPU by
+    # two independent execution modes instead
+    "config2a_hubs": (W.config2_graph, W.C4_2PEND, (500, 2001, 500)),
+    "config2b_hubs": (W.config2_graph, W.K4E_TAIL2, (500, 2001, 500)),
     # v = 511 (mod 512): 2,048 roots of mostly low degree; the full count
     # exceeds 2^64, this sample has 3.3e8 embeddings (the oracle needs ~20 s)
     "config3": (W.config3_graph, W.C5CHORD_2TAIL, (511, 1 << 20, 512)),
@@ -53,4 +60,4 @@ def main(keys):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:] or ["config1", "config2a", "config2b", "config3"])
+    main(sys.argv[1:] or ["config1", "config2a", "config2b", "config2a_hubs", "config2b_hubs", "config3"])

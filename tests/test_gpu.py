@@ -217,6 +217,10 @@ def test_time_limit_counts_exact_root_prefix(ctx, seed):
 
 
 CFG = load("config_counts.json")
+# every closed form above the last level off: pair, chain, the two-hop index,
+# shared chain candidates and the per-arc table
+INDEPENDENT = (cuts_amd.FLAG_NO_PAIR | cuts_amd.FLAG_NO_CHAIN | cuts_amd.FLAG_NO_TWO_HOP | cuts_amd.FLAG_NO_SHARED |
+               cuts_amd.FLAG_NO_ARC_COMMON)
 
 
 def test_config1_full_count(ctx):
@@ -229,17 +233,43 @@ def test_config1_full_count(ctx):
         assert ctx.match(dg, p, **m).count == c["count"], m
 
 
-@pytest.mark.parametrize("key", ["config2a", "config2b", "config3"])
+@pytest.mark.parametrize("key", ["config2a", "config2b", "config2a_hubs", "config2b_hubs", "config3"])
 def test_config_sampled_counts(ctx, key):
+    """Oracle counts on root samples of the benchmark graphs (low-degree
+    roots, and hub roots of config 2), in the default mode and in the mode
+    without pair, chain and index levels."""
     c = CFG[key]
-    g = {"config2a": W.config2_graph, "config2b": W.config2_graph, "config3": W.config3_graph}[key]()
+    cfg = key.split("_")[0]
+    g = {"config2a": W.config2_graph, "config2b": W.config2_graph, "config3": W.config3_graph}[cfg]()
     assert W.csr_fingerprint(g) == c["fingerprint"]
-    shape = {"config2a": W.C4_2PEND, "config2b": W.K4E_TAIL2, "config3": W.C5CHORD_2TAIL}[key]
+    shape = {"config2a": W.C4_2PEND, "config2b": W.K4E_TAIL2, "config3": W.C5CHORD_2TAIL}[cfg]
     p = cuts_amd.Plan(W.query_graph(shape))
     assert p.order[0] == c["root_q"]
     dg = ctx.upload(g)
     r = ctx.match(dg, p, roots=tuple(c["roots"]))
     assert r.count == c["count"]
+    assert ctx.match(dg, p, roots=tuple(c["roots"]), flags=INDEPENDENT).count == c["count"]
+
+
+@pytest.mark.parametrize("key", ["config2a", "config2b"])
+def test_config2_full_counts_in_two_independent_modes(ctx, key):
+    """The mid-size headline counts at full size (1.4e14 and 1.2e13
+    embeddings, far beyond the oracle): the default mode (pair or chain
+    level, two-hop index, shared candidates, dense two-hop rows at the hubs)
+    and a mode that expands every level but the last (counted by the plain
+    last-level closed form, itself oracle-pinned on every random case) must
+    both give the committed count."""
+    c = CFG[key + "_full"]
+    g = W.config2_graph()
+    assert W.csr_fingerprint(g) == c["fingerprint"]
+    shape = {"config2a": W.C4_2PEND, "config2b": W.K4E_TAIL2}[key]
+    p = cuts_amd.Plan(W.query_graph(shape))
+    dg = ctx.upload(g)
+    fast = ctx.match(dg, p)
+    assert fast.count == c["count"]
+    slow = ctx.match(dg, p, flags=INDEPENDENT)
+    assert slow.count == c["count"]
+    assert slow.level_sizes == fast.level_sizes
 
 
 def test_cli_report_line_and_trunks(tmp_path):

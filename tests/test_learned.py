@@ -56,7 +56,7 @@ def test_reference_gradient_matches_finite_differences():
         assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(g[i]), (i, fd, g[i])
 
 
-def brute_best(k, adj, f, first=None):
+def brute_best(k, adj, f, first=None, eng=LN.LAST_LEVEL_ONLY):
     best = None
     for perm in itertools.permutations(range(k)):
         if first is not None and perm[0] != first:
@@ -68,27 +68,74 @@ def brute_best(k, adj, f, first=None):
                 break
             sm |= 1 << v
         if ok:
-            c = LN.order_cost(list(perm), adj, f)
+            c = LN.order_cost(list(perm), adj, f, eng)
             best = c if best is None else min(best, c)
     return best
 
 
-@pytest.mark.parametrize("seed", range(12))
+ENGINES = [LN.LAST_LEVEL_ONLY, LN.Engine(), LN.Engine(two_hop=True), LN.Engine(arc_common=True),
+           LN.Engine(two_hop=True, arc_common=True), LN.Engine(symmetric=False)]
+
+
+@pytest.mark.parametrize("seed", range(24))
 def test_order_dp_is_exact(seed):
+    """The DP (connected subsets + the last three positions enumerated) finds
+    the minimum of the engine-aware cost over all connected orders, with and
+    without a fixed first vertex, for every engine configuration."""
     rng = random.Random(seed)
     k = rng.randint(2, 7)
     edges = [(rng.randrange(v), v) for v in range(1, k)] + [tuple(rng.sample(range(k), 2)) for _ in range(rng.randint(0, 4))]
     adj = LN.adjacency(k, edges)
     f = {sm: rng.uniform(1, 1000) for sm in LN.connected_subsets(k, adj)}
-    order, cost = LN.best_order(k, adj, f)
-    assert sorted(order) == list(range(k))
-    assert math.isclose(cost, LN.order_cost(order, adj, f))
-    assert math.isclose(cost, brute_best(k, adj, f))
-    first = rng.randrange(k)
-    order, cost = LN.best_order(k, adj, f, first=first)
-    assert order[0] == first and sorted(order) == list(range(k))
-    assert math.isclose(cost, LN.order_cost(order, adj, f))
-    assert math.isclose(cost, brute_best(k, adj, f, first))
+    for eng in ENGINES:
+        order, cost = LN.best_order(k, adj, f, eng=eng)
+        assert sorted(order) == list(range(k))
+        assert math.isclose(cost, LN.order_cost(order, adj, f, eng))
+        assert math.isclose(cost, brute_best(k, adj, f, eng=eng))
+        first = rng.randrange(k)
+        order, cost = LN.best_order(k, adj, f, first=first, eng=eng)
+        assert order[0] == first and sorted(order) == list(range(k))
+        assert math.isclose(cost, LN.order_cost(order, adj, f, eng))
+        assert math.isclose(cost, brute_best(k, adj, f, first, eng))
+
+
+def test_engine_modes_follow_the_planner_rules():
+    """The cost model's view of which positions the engine expands agrees with
+    the planner rules restated in tests/test_pair.py (pair, collapse and
+    chain eligibility of a plan), on every connected order of the shapes
+    there and of random walk-like queries."""
+    import test_pair as TP
+    shapes = list(TP.SHAPES.values())
+    rng = random.Random(5)
+    for _ in range(30):
+        k = rng.randint(4, 7)
+        shapes.append([(rng.randrange(v), v) for v in range(1, k)] +
+                      [tuple(rng.sample(range(k), 2)) for _ in range(rng.randint(0, 3))])
+    seen = set()
+    for edges in shapes:
+        k = 1 + max(max(e) for e in edges)
+        adj = LN.adjacency(k, edges)
+        qg = cuts_amd.Graph.from_edges(k, edges)
+        orders = [o for o in itertools.permutations(range(k))
+                  if all(adj[o[i]] & sum(1 << o[j] for j in range(i)) for i in range(1, k))]
+        for o in rng.sample(orders, min(40, len(orders))):
+            p = cuts_amd.Plan(qg, list(o))
+            T = min(3, k - 1)
+            s0 = sum(1 << v for v in o[:k - T])
+            for eng in (LN.Engine(two_hop=True), LN.Engine(), LN.Engine(arc_common=True)):
+                E, extra = LN._tail_mode(adj, s0, list(o[k - T:]), k, eng)
+                b = TP.backward(p)
+                sc = len(b[k - 1]) == 1
+                pair = TP.pair_eligible(p)
+                chain = not pair and TP.chain_eligible(p)
+                coll = pair and TP.collapse_eligible(p)
+                if coll and not eng.two_hop:  # without the two-hop index only adjacent owners collapse
+                    (a,), (bb,) = b[k - 2], b[k - 1]
+                    coll = eng.arc_common and bool(adj[o[a]] >> o[bb] & 1)
+                want = k - 4 if coll else k - 3 if (pair or chain) else k - 2 if sc else k - 1
+                assert E == want, (edges, o, eng, E, want)
+                seen.add(want - k)
+    assert seen == {-4, -3, -2, -1}
 
 
 def test_connected_subset_counts():

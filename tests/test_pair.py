@@ -115,7 +115,8 @@ def test_pair_level_vs_oracle_and_cpu_trie(ctx, name, seed):
     prof = cuts_amd.FLAG_PROFILE
     for flags in (0, cuts_amd.FLAG_NO_BITMAP, cuts_amd.FLAG_NO_TWO_HOP, cuts_amd.FLAG_NO_TWO_HOP | cuts_amd.FLAG_NO_BITMAP,
                   cuts_amd.FLAG_NO_PAIR, cuts_amd.FLAG_NO_CHAIN, cuts_amd.FLAG_NO_PAIR | cuts_amd.FLAG_NO_CHAIN,
-                  cuts_amd.FLAG_NO_SHARED, cuts_amd.FLAG_NO_SHARED | cuts_amd.FLAG_NO_BITMAP):
+                  cuts_amd.FLAG_NO_SHARED, cuts_amd.FLAG_NO_SHARED | cuts_amd.FLAG_NO_BITMAP,
+                  cuts_amd.FLAG_NO_TWO_HOP | cuts_amd.FLAG_NO_ARC_COMMON):
         r = ctx.match(dg, p, flags=flags | prof)
         assert r.count == want, (name, flags)
         assert r.level_sizes == [cpu.level[i] for i in range(k)], (name, flags)
@@ -311,3 +312,56 @@ def test_symmetric_graph_with_self_loops(ctx, seed):
     dg0 = ctx.upload(g0)
     r = ctx.match(dg0, plans[1], flags=cuts_amd.FLAG_PROFILE)
     assert r.kernels["chain"][1] > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", range(3))
+def test_arc_common_table_vs_matrix_product(ctx, seed):
+    """The per-arc common-neighbour table built on the device at upload
+    equals (A @ A)[u, w] at every arc (u, w), with hub bitmaps (power-law
+    graphs whose hubs take bit tests) and without the two-hop index's help."""
+    import scipy.sparse as sp
+    g = graph_for(9700 + seed) if seed < 2 else cuts_amd.Graph.chung_lu(3000, 24000, 2.1, 0, 9702)
+    v = g.view()
+    dg = ctx.upload(g)
+    info = dg.info()
+    assert info["arc_common"] == 1 and info["self_loops"] == 0
+    if seed == 2:
+        assert info["hubs"] > 0
+    V = v["V"]
+    src = np.repeat(np.arange(V), np.diff(v["out_off"]).astype(np.int64))
+    A = sp.csr_matrix((np.ones(len(src), np.int64), (src, v["out_adj"].astype(np.int64))), shape=(V, V))
+    want = np.asarray((A @ A)[src, v["out_adj"].astype(np.int64)]).ravel()
+    assert np.array_equal(dg.arc_common(), want)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", range(3))
+def test_pair_level_from_the_arc_table(ctx, seed):
+    """Pair-level orders whose two pendant owners are adjacent in the query
+    take |N(u) & N(w)| from the per-arc table when there is no two-hop index
+    (FLAG_NO_TWO_HOP stands in for a graph too large for one): no k_common
+    pass runs, and every mode counts the oracle's number.  Covers the
+    5-cycle + chord + 2 tails order with its pendants on positions 2 and 3,
+    and the collapsing triangle + 2 pendants."""
+    g = graph_for(9800 + seed)
+    og = oracle.OGraph.from_view(g.view())
+    dg = ctx.upload(g)
+    F = cuts_amd
+    for name, order in (("5cycle+chord+2tails", [0, 2, 4, 3, 1, 5, 6]),
+                        ("5cycle+chord+2tails", [2, 3, 0, 4, 1, 5, 6]),
+                        ("triangle+2pend-two", None)):
+        edges = SHAPES[name]
+        k = 1 + max(max(e) for e in edges)
+        want = oracle.count(og, oracle.qmatrix(k, oracle.undirected(edges)))
+        p = cuts_amd.Plan(cuts_amd.Graph.from_edges(k, edges), order)
+        assert pair_eligible(p), (name, p.order)
+        r = ctx.match(dg, p, flags=F.FLAG_NO_TWO_HOP | F.FLAG_PROFILE)
+        assert r.count == want, (name, p.order)
+        assert r.kernels["common"][1] == 0 and r.kernels["pair"][1] > 0, (name, r.kernels)
+        r2 = ctx.match(dg, p, flags=F.FLAG_NO_TWO_HOP | F.FLAG_NO_ARC_COMMON | F.FLAG_PROFILE)
+        assert r2.count == want
+        assert r2.level_sizes == r.level_sizes
+        for flags in (0, F.FLAG_NO_TWO_HOP | F.FLAG_NO_BITMAP, F.FLAG_NO_PAIR):
+            assert ctx.match(dg, p, flags=flags).count == want, (name, flags)
+        assert ctx.match(dg, p, flags=F.FLAG_NO_TWO_HOP, budget=2048).count == want
