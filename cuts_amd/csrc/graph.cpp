@@ -148,7 +148,8 @@ std::string load_graph(const std::string& path, Graph& g) {
   }
   const char* p = buf.data();
   const char* end = p + buf.size();
-  const char* nl = (const char*)memchr(p, '\n', buf.size());
+  // (memchr needs a non-null pointer even for an empty file)
+  const char* nl = buf.empty() ? nullptr : (const char*)memchr(p, '\n', buf.size());
   const char* l0e = nl ? nl : end;
   long long Vll;
   if (!stoi_like(p, l0e, Vll) || Vll < 0) return "invalid vertex count on the first line of " + path;
@@ -239,8 +240,11 @@ namespace cuts {
 void build_two_hop(const Graph& g, std::vector<uint64_t>& off, std::vector<uint32_t>& hv, std::vector<uint32_t>& hc) {
   const uint32_t V = g.V;
   off.assign((size_t)V + 1, 0);
+  // every thread keeps one V-word array: at most 4 GiB of them in total
+  const int nt = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)omp_get_max_threads(),
+                                                               (4ull << 30) / (4ull * std::max<uint32_t>(V, 1))));
   // pass 1: distinct two-hop vertices per u (a stamp per vertex, u + 1)
-#pragma omp parallel
+#pragma omp parallel num_threads(nt)
   {
     std::vector<uint32_t> mark(V, 0);
 #pragma omp for schedule(dynamic, 64)
@@ -263,7 +267,7 @@ void build_two_hop(const Graph& g, std::vector<uint64_t>& off, std::vector<uint3
   hv.assign(std::max<uint64_t>(off[V], 1), 0);
   hc.assign(std::max<uint64_t>(off[V], 1), 0);
   // pass 2: the vertices, ascending, with their common-neighbour counts
-#pragma omp parallel
+#pragma omp parallel num_threads(nt)
   {
     std::vector<uint32_t> cnt(V, 0), touched;
 #pragma omp for schedule(dynamic, 64)
