@@ -119,6 +119,8 @@ def lib():
             "cuts_device_graph_arc_common": (C.c_int, [vp, _u32p, C.c_uint64]),
             "cuts_match_options_default": (None, [C.POINTER(_Opts)]),
             "cuts_match": (C.c_int, [vp, vp, vp, C.POINTER(_Opts), C.POINTER(_Result)]),
+            "cuts_match_noalloc": (C.c_int, [vp, vp, vp, C.POINTER(_Opts), C.POINTER(_Result)]),
+            "cuts_noalloc_scratch_bytes": (C.c_uint64, []),
             "cuts_search": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.POINTER(C.c_uint64)]),
             "cuts_search_dfs_bfs_strategy": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_uint32,
                                                        C.POINTER(C.c_uint64)]),
@@ -354,7 +356,10 @@ class Context:
 
     def match(self, dg: DeviceGraph, plan: Plan, trunks: int = 0, flags: int = 0, budget: int = 0,
               roots: Optional[tuple] = None, write: bool = False, emb_cap: int = 0,
-              blocks_per_cu: int = 0, time_limit_ms: float = 0.0, max_work: int = 0) -> MatchResult:
+              blocks_per_cu: int = 0, time_limit_ms: float = 0.0, max_work: int = 0,
+              noalloc: bool = False) -> MatchResult:
+        """One query (cuts_match).  noalloc=True calls cuts_match_noalloc: the
+        run's buffers come from the device's static scratch array."""
         o = _Opts()
         lib().cuts_match_options_default(C.byref(o))
         o.trunks, o.flags, o.budget_bytes, o.blocks_per_cu = trunks, flags, budget, blocks_per_cu
@@ -369,7 +374,8 @@ class Context:
             o.emb_rows = _ptr(rows)
             o.emb_cap = emb_cap
         r = _Result()
-        _check(lib().cuts_match(self._h, dg._h, plan._h, C.byref(o), C.byref(r)))
+        entry = lib().cuts_match_noalloc if noalloc else lib().cuts_match
+        _check(entry(self._h, dg._h, plan._h, C.byref(o), C.byref(r)))
         res = MatchResult(int(r.count), [int(x) for x in r.level_sizes[: plan.k]], float(r.device_ms),
                           int(r.chunks), complete=bool(r.complete), roots_done=int(r.roots_done),
                           last_root=int(r.last_root), work_capped=bool(r.work_capped), work=int(r.work))
@@ -436,6 +442,11 @@ class Batch:
         return BatchResult([int(r.count) for r in res],
                            [[int(x) for x in r.level_sizes[: p.k]] for r, p in zip(res, self.plans)],
                            [float(r.latency_ms) for r in res], float(ms.value), int(ab.value))
+
+
+def noalloc_scratch_bytes() -> int:
+    """Size of the static scratch array behind Context.match(noalloc=True)."""
+    return int(lib().cuts_noalloc_scratch_bytes())
 
 
 def search(query_file: str, data_file: str, write_to_disk: bool = False) -> int:

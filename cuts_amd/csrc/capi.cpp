@@ -534,6 +534,21 @@ int cuts_mpnn_train_step(cuts_mpnn* m, uint32_t n, const float* x, const uint16_
 
 void cuts_mpnn_free(cuts_mpnn* m) { delete m; }
 
+int cuts_match_noalloc(cuts_context* ctx, const cuts_device_graph* dg, const cuts_plan* plan,
+                       const cuts_match_options* opt, cuts_match_result* res) {
+  return guard([&]() -> int {
+    if (!ctx || !dg || !plan || !res) return fail(CUTS_E_ARG, "null argument");
+    if (opt && (opt->flags & CUTS_FLAG_WRITE) && opt->emb_cap && !opt->emb_rows)
+      return fail(CUTS_E_ARG, "CUTS_FLAG_WRITE needs emb_rows");
+    cuts::MatchResult r;
+    cuts::run_match_noalloc(ctx->c, const_cast<cuts::DeviceGraph&>(dg->dg), plan->p, to_opts(opt), r);
+    fill_result(r, res);
+    return CUTS_OK;
+  });
+}
+
+uint64_t cuts_noalloc_scratch_bytes(void) { return cuts::noalloc_scratch_bytes(); }
+
 int cuts_log_tables(const char* path, const char* label, const cuts_plan* plan, const cuts_match_result* r) {
   return guard([&]() -> int {
     if (!path || !plan || !r) return fail(CUTS_E_ARG, "null argument");

@@ -171,6 +171,7 @@ struct Context {
   unsigned long long* counters = nullptr;
   hipEvent_t kev[2] = {nullptr, nullptr};
   uint64_t* host_pinned = nullptr;        // small pinned staging
+  char* scratch = nullptr;                // this device's no-allocation scratch array
   ~Context();
 };
 
@@ -205,7 +206,13 @@ struct MatchResult {
 void context_init(Context& c, int device);
 void upload_graph(Context& c, const Graph& g, DeviceGraph& dg);
 void run_match(Context& c, DeviceGraph& dg, const Plan& p, const MatchOpts& o, MatchResult& r);
-
+// Bytes of the static scratch array behind run_match_noalloc.
+uint64_t noalloc_scratch_bytes();
+// run_match with every buffer of the run carved from the device's static
+// scratch array: no device allocation at all.  Degree tables the graph has
+// not cached yet are built in the scratch too (and not cached).  Calls that
+// overlap on one device share the array and race.
+void run_match_noalloc(Context& c, DeviceGraph& dg, const Plan& p, const MatchOpts& o, MatchResult& r);
 // Streams `bytes` through a plain copy kernel `iters` times; returns GB/s
 // (read + write).
 double copy_probe(Context& c, uint64_t bytes, uint32_t iters);

@@ -43,9 +43,36 @@
 #include <cstdlib>
 #include <cstring>
 
+#include <mutex>
+
 #include "batch.hpp"
 
+// Scratch of the no-allocation entry point (run_match_noalloc): every
+// buffer of such a run -- trie levels, scans, counters, degree tables the
+// graph has not cached -- is carved from this array.  Each device holds its
+// own copy for the life of the process.  Host code never passes this name
+// to a kernel: it is a host-side stand-in; the device address comes from
+// hipGetSymbolAddress (scratch_address below).
+constexpr uint64_t kScratchBytes = 256ull << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace cuts {
+
+// The device address of this device's scratch array, looked up once per
+// device with that device current (context_init calls it).
+char* scratch_address(int device) {
+  static std::mutex mu;
+  static char* addr[64] = {};
+  if (device < 0 || device >= 64) throw Error(-1, "device index out of range for the scratch table");
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[device]) {
+    void* p = nullptr;
+    HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error(-4, "hipGetSymbolAddress returned a null scratch address");
+    addr[device] = (char*)p;
+  }
+  return addr[device];
+}
 
 void hip_check(hipError_t e, const char* what) {
   if (e != hipSuccess) throw Error(-4, std::string(what) + ": " + hipGetErrorString(e));
@@ -2494,6 +2521,7 @@ void context_init(Context& c, int device) {
   HIPCHECK(hipMemGetInfo(&fr, &tot));
   // trie tables may take 40% of the free HBM by default
   c.default_budget = (uint64_t)(fr * 0.4);
+  c.scratch = scratch_address(device);
 }
 
 void upload_graph(Context& c, const Graph& g, DeviceGraph& dg) {
@@ -2663,6 +2691,36 @@ void run_match(Context& c, DeviceGraph& dg, const Plan& p, const MatchOpts& o, M
   HIPCHECK(hipSetDevice(c.device));
   r = MatchResult();
   Matcher m(c, dg, p, o, r);
+  m.run();
+}
+
+uint64_t noalloc_scratch_bytes() { return kScratchBytes; }
+
+void run_match_noalloc(Context& c, DeviceGraph& dg, const Plan& p, const MatchOpts& o, MatchResult& r) {
+  HIPCHECK(hipSetDevice(c.device));
+  if (!c.scratch) throw Error(-4, "the context has no scratch address");
+  // level 0 holds one row per candidate root (at most V) with its planning
+  // arrays, rounded up to powers of two: about 32 bytes per vertex
+  const uint64_t l0 = 32ull * ((uint64_t)dg.V + 1);
+  if (l0 > kScratchBytes / 4)
+    throw Error(-1, "graph of " + std::to_string(dg.V) + " vertices exceeds the no-allocation scratch (" +
+                        std::to_string(kScratchBytes) + " bytes: at most " + std::to_string(kScratchBytes / 128 - 1) +
+                        " vertices)");
+  ScratchArena a;
+  a.base = c.scratch;
+  a.size = kScratchBytes;
+  auto* ctr = (unsigned long long*)a.carve(16 * sizeof(unsigned long long));
+  HIPCHECK(hipMemsetAsync(ctr, 0, 16 * sizeof(unsigned long long), c.stream));
+  LevelBuf lv[kMaxQuery];
+  // trie rows take at most an eighth of the scratch; the scan, tile,
+  // pair/chain arrays and regrowth take the rest
+  MatchOpts o2 = o;
+  o2.budget = o.budget ? std::min<uint64_t>(o.budget, kScratchBytes / 8) : kScratchBytes / 8;
+  r = MatchResult();
+  Matcher m(c, dg, p, o2, r);
+  m.lv = lv;
+  m.ctr = ctr;
+  m.arena = &a;
   m.run();
 }
 

@@ -197,6 +197,22 @@ void cuts_match_options_default(cuts_match_options *o);
 int cuts_match(cuts_context *ctx, const cuts_device_graph *dg, const cuts_plan *plan,
                const cuts_match_options *opt, cuts_match_result *res);
 
+/* No reference counterpart: cuts_match for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, semantics and result;
+ * every buffer of the run (trie levels, scans, counters, degree tables the
+ * device graph has not cached yet) is carved from a static scratch array of
+ * cuts_noalloc_scratch_bytes() bytes that each GPU holds for the life of the
+ * process; nothing is allocated or freed.  The trie levels take at most an
+ * eighth of it (budget_bytes is clamped), so large outputs run in more
+ * chunks.  Fails with CUTS_E_ARG when the data graph has more vertices than
+ * the scratch allows (level 0: about 32 bytes per vertex within a quarter
+ * of it) and with CUTS_E_NOMEM when a run outgrows it.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU race. */
+int cuts_match_noalloc(cuts_context *ctx, const cuts_device_graph *dg, const cuts_plan *plan,
+                       const cuts_match_options *opt, cuts_match_result *res);
+uint64_t cuts_noalloc_scratch_bytes(void);
+
 /* Table log (no reference counterpart; the reference keeps the level sizes
  * in `lengths`, src/host_funcs.cu:95,153, and drops them).  Appends one JSON
  * line to `path`: {"label", "k", "order", "level_sizes", "count", "work",
