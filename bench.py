@@ -356,12 +356,34 @@ def run_headline(args):
                                     eng=eng)
     plan_s = time.perf_counter() - t0
     t0 = time.perf_counter()
-    judged = LN.judge_by_time(ctx, dg, qs, heur, want, choices, roots=roots, heuristic_work=[r.work for r in hres])
+    if args.orders_in:
+        # the kept orders of an earlier run (profiling runs: the same dispatches)
+        with open(args.orders_in) as f:
+            kept_orders = json.load(f)["orders"]
+        learned = [cuts_amd.Plan(cuts_amd.Graph.from_edges(k, e), o) if o else h
+                   for (k, e), o, h in zip(qs, kept_orders, heur)]
+        kept_flags = [bool(o) for o in kept_orders]
+    else:
+        judged = LN.judge_by_time(ctx, dg, qs, heur, want, choices, roots=roots,
+                                  heuristic_work=[r.work for r in hres])
+        learned = [j.plan for j in judged]
+        kept_flags = [j.learned for j in judged]
     judge_s = time.perf_counter() - t0
-    learned = [j.plan for j in judged]
+    if args.orders_out:
+        with open(args.orders_out, "w") as f:
+            json.dump({"orders": [p.order if kf else None for p, kf in zip(learned, kept_flags)]}, f)
     proposed = sum(1 for c in choices if c.learned)
-    kept = sum(1 for j in judged if j.learned)
+    kept = sum(kept_flags)
     log(f"learned orders: {proposed} proposed, {kept} kept by the judge ({plan_s:.1f}s planning, {judge_s:.1f}s judging)")
+    if args.profile_only:
+        # the profiling pass alone (rocprofv3 runs): one run of every learned plan
+        kern = profile_pass(ctx, dg, learned, roots)
+        if args.copy_probe:
+            ctx.copy_probe(1 << 30, args.copy_probe)
+        emit({"profile_only": True, "roofline": roofline(kern, "rmat20"),
+              "kernels_ms_profile_pass": {k: v[0] for k, v in kern.items() if v[1]},
+              "launches": {k: v[1] for k, v in kern.items() if v[1]}})
+        return
 
     def one_pass(plans, sink=None):
         for i, (p, rt) in enumerate(zip(plans, roots)):
@@ -425,6 +447,8 @@ def run_headline(args):
                       "arc_common_build_s": info["arc_common_build_s"]})
     line["roofline"] = roofline(kern, "rmat20")
     line["kernels_ms_profile_pass"] = {k: v[0] for k, v in kern.items() if v[1]}
+    if args.copy_probe:
+        line["copy_probe_GBs"] = ctx.copy_probe(1 << 30, args.copy_probe)
     if not args.no_config2:
         line["config2"] = config2_fields(ctx, steps=5)
     if not args.no_cpu:
@@ -525,6 +549,9 @@ def main():
                     help="also run N dispatches of a 1 GiB plain copy (byte-counter yardstick)")
     ap.add_argument("--workload", default="rmat20", choices=["rmat20", "chunglu36k", "chunglu4k", "batch256"])
     ap.add_argument("--eval-queries", type=int, default=500)
+    ap.add_argument("--orders-out", default="", help="headline: write the judged learned orders to this file")
+    ap.add_argument("--orders-in", default="", help="headline: take the learned orders from this file (no judging)")
+    ap.add_argument("--profile-only", action="store_true", help="headline: only the profiling pass (for rocprofv3)")
     ap.add_argument("--train-queries", type=int, default=1024)
     ap.add_argument("--score-only", action="store_true", help="batch256: score orders only, no training")
     ap.add_argument("--plain-launches", action="store_true", help="batch256 --score-only: no hipGraph")

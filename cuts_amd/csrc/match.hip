@@ -1449,6 +1449,27 @@ __global__ __launch_bounds__(BLOCK) void k_common(GView g, KPlan P, ExpandArgs a
 // Chain level, shared part from the two-hop index: per parent prefix (rows
 // of L words), the sum of |N(u) & N(r_i)| over its positions i != ch_u
 // whose degree takes completions away.
+// Chain level without the two-hop index, shared part for the older
+// positions i in `mask` (adjacent to u's position in the query, so r_i is in
+// N(u)): sum of |N(u) & N(r_i)| from the per-arc table, stored per parent
+// row before the k_common passes of the other positions add theirs.
+template <int L>
+__global__ __launch_bounds__(BLOCK) void k_chain_shared_arc(GView g, KPlan P, const uint32_t* __restrict__ rows,
+                                                           uint64_t n, uint32_t mask, uint32_t* __restrict__ shared) {
+  uint64_t dummy = 0;
+  for (uint64_t p = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (uint64_t)gridDim.x * BLOCK) {
+    const uint32_t u = rows[p * L + P.ch_u];
+    uint32_t t = 0;
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+      if (!((mask >> i) & 1)) continue;
+      const uint32_t w = rows[p * L + i];
+      if (P.ch_tb <= 1 || g.sig[2 * (uint64_t)w] >= P.ch_tb) t += arc_common(g, u, w, dummy);
+    }
+    shared[p] = t;
+  }
+}
+
 template <int L>
 __global__ __launch_bounds__(BLOCK) void k_chain_shared_h2(GView g, KPlan P, const uint32_t* __restrict__ rows,
                                                           uint64_t n, uint32_t* __restrict__ shared) {
@@ -1516,12 +1537,15 @@ __global__ __launch_bounds__(BLOCK) void k_chain_row(GView g, KPlan P, const uin
                                                     unsigned long long* __restrict__ counters, int stats) {
   uint64_t acc = 0, na_sum = 0, mb = 0, bytes = 0;
   const bool h2 = g.h2_off && P.ch_ta <= 2;
+  // without the index, |N(u) & N(r_{L-1})| comes from the per-arc table when
+  // position L-1 is adjacent to u's in the query, else from a k_common pass
+  const bool arc = !h2 && g.arc_cn && ((P.qadj[P.ch_u] >> (L - 1)) & 1) && P.ch_ta <= 2;
   for (uint64_t p = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (uint64_t)gridDim.x * BLOCK) {
     uint32_t r[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) r[l] = rows[p * L + l];
     int64_t na = 0;
-    acc += (uint64_t)chain_closed<L>(g, P, r, t, shared[parent[p]], h2, na, stats != 0, mb, bytes);
+    int64_t c = chain_closed<L>(g, P, r, t, shared[parent[p]], h2, na, stats != 0, mb, bytes);
     bytes += 4 * L + 8;  // the row and its parent's shared sum
     na_sum += (uint64_t)na;
     uint32_t u = 0;
@@ -1531,7 +1555,9 @@ __global__ __launch_bounds__(BLOCK) void k_chain_row(GView g, KPlan P, const uin
     const uint32_t v = r[L - 1];
     const uint32_t du = g.off_out[u + 1] - g.off_out[u], dv = g.off_out[v + 1] - g.off_out[v];
     const bool need = P.ch_tb <= 1 || g.sig[2 * (uint64_t)v] >= P.ch_tb;
-    minlen[p] = (!h2 && need) ? min(du, dv) : 0u;
+    if (arc && need) c -= arc_common(g, u, v, bytes);
+    acc += (uint64_t)c;
+    minlen[p] = (!h2 && !arc && need) ? min(du, dv) : 0u;
     minsel[p] = (uint8_t)(du <= dv ? 0u : 1u);
   }
   add_wave(&counters[0], acc);
@@ -1995,9 +2021,24 @@ struct Matcher {
       pend(1, (double)n * (4.0 * L + 4.0), false);
       return;
     }
-    HIPCHECK(hipMemsetAsync(in.nsurv, 0, n * sizeof(uint32_t), c.stream));
+    // older positions adjacent to u's in the query: per-arc table lookups
+    uint32_t arc_mask = 0;
+    if (g.arc_cn && kp.ch_ta <= 2)
+      for (uint32_t i = 0; i < L; ++i)
+        if (i != kp.ch_u && ((kp.qadj[kp.ch_u] >> i) & 1)) arc_mask |= 1u << i;
+    if (arc_mask) {
+      const uint32_t grid = grid_for(n, BLOCK, grid_cap);
+      pbegin(false);
+#define CA(LL) hipLaunchKernelGGL((k_chain_shared_arc<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, arc_mask, in.nsurv)
+      CUTS_DISPATCH_PAIR(L, CA)
+#undef CA
+      HIPCHECK(hipGetLastError());
+      pend(1, (double)n * (4.0 * L + 4.0), false);
+    } else {
+      HIPCHECK(hipMemsetAsync(in.nsurv, 0, n * sizeof(uint32_t), c.stream));
+    }
     for (uint32_t i = 0; i < L && !aborted; ++i) {
-      if (i == kp.ch_u) continue;
+      if (i == kp.ch_u || ((arc_mask >> i) & 1)) continue;
       const uint32_t grid = grid_for(n, BLOCK, grid_cap);
       pbegin(false);
 #define CP(LL) hipLaunchKernelGGL((k_chain_prep<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, i, in.minlen, in.minsel)
@@ -2153,7 +2194,9 @@ struct Matcher {
 #undef CR
     HIPCHECK(hipGetLastError());
     pend(10, 0, true);
-    if (!(g.h2_off && kp.ch_ta <= 2)) common_pass(L, rows, n, in, CommonArgs{kp.ch_u, L - 1, kp.ch_ta, 1u}, nullptr);
+    const bool arc = g.arc_cn && ((kp.qadj[kp.ch_u] >> (L - 1)) & 1) && kp.ch_ta <= 2;
+    if (!(g.h2_off && kp.ch_ta <= 2) && !arc)
+      common_pass(L, rows, n, in, CommonArgs{kp.ch_u, L - 1, kp.ch_ta, 1u}, nullptr);
   }
 
   void run() {

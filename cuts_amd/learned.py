@@ -293,7 +293,12 @@ def _tail_mode(adj: List[int], s0: int, tail: Sequence[int], k: int, eng: Engine
             if last_b == 1 << t2:
                 u = b[1].bit_length() - 1
                 if (s0 >> u) & 1:
-                    return k - 3, not eng.two_hop
+                    # without the two-hop index, |N(u) & N(r_i)| of the other
+                    # prefix positions comes from the per-arc table only for
+                    # those adjacent to u in the query
+                    others = (s0 | (1 << t1)) & ~(1 << u)
+                    arc_all = eng.arc_common and (others & ~adj[u]) == 0
+                    return k - 3, not (eng.two_hop or arc_all)
     return (k - 2 if k >= 3 else 0), False
 
 
@@ -462,10 +467,13 @@ def judge(ctx: Context, dg, queries: Sequence[Tuple[int, List[Tuple[int, int]]]]
 
 def judge_by_time(ctx: Context, dg, queries: Sequence[Tuple[int, List[Tuple[int, int]]]], heuristic_plans,
                   heuristic_counts: Sequence[int], choices: Sequence[OrderChoice], roots=None, flags: int = 0,
-                  heuristic_work: Optional[Sequence[int]] = None, reps: int = 2) -> List["TimedJudged"]:
+                  heuristic_work: Optional[Sequence[int]] = None, reps: int = 2,
+                  margin: float = 0.03) -> List["TimedJudged"]:
     """The score order judges whether a proposal actually runs faster: both
     orders are run warm `reps` times on the same GPU and the proposal is kept
-    only if its best device time beats the score order's.  The proposal's
+    only if its best device time beats the score order's by `margin`
+    (relative; near-ties keep the score order, so the decision does not
+    flip with timing noise).  The proposal's
     runs are capped at 4x the score order's candidates (a runaway order is
     stopped, never waited for).  A kept order's count must equal the score
     order's (order never changes a count); anything else raises."""
@@ -482,7 +490,7 @@ def judge_by_time(ctx: Context, dg, queries: Sequence[Tuple[int, List[Tuple[int,
                 if any(r.count != hc for r in runs):
                     raise AssertionError(f"query {i}: learned order counted {runs[0].count}, score order {hc}")
                 tl = min(r.device_ms for r in runs)
-                if tl < th:
+                if tl < (1.0 - margin) * th:
                     kept = TimedJudged(lp, True, tl, th)
         out.append(kept or TimedJudged(hp, False, th, th))
     return out

@@ -23,6 +23,7 @@ from . import learned as LN
 from . import workloads as W
 
 TRAIN_MAX_WORK = 1 << 24
+TRAIN_TRIES = 10000  # candidates per size and graph before a quota is given up
 
 
 def train(ctx: Context, n_queries: int, seeds=(1, 2, 3, 4), epochs: int = 60, log=print):
@@ -33,7 +34,8 @@ def train(ctx: Context, n_queries: int, seeds=(1, 2, 3, 4), epochs: int = 60, lo
     for s in seeds:
         g = W.rmat18_graph(s)
         dg = ctx.upload(g)
-        qs, tried = W.admit(ctx, dg, g, per, seed=100 + s, kmin=5, kmax=10, max_work=TRAIN_MAX_WORK)
+        qs, tried = W.admit(ctx, dg, g, per, seed=100 + s, kmin=5, kmax=10, max_work=TRAIN_MAX_WORK,
+                            max_tries_per_k=TRAIN_TRIES)
         admitted += len(qs)
         tried_all[s] = tried
         parts.append(LN.log_tables(ctx, dg, LN.graph_features(g), qs, rng))
@@ -51,7 +53,8 @@ def train(ctx: Context, n_queries: int, seeds=(1, 2, 3, 4), epochs: int = 60, lo
     log(f"trained on {len(t)} samples: mse {hist[0]:.4f} -> {hist[-1]:.4f} ({t2 - t1:.0f}s)")
     info = {"graphs": [f"rmat scale=18 ef=16 a,b,c=.57,.19,.19 seed={s}" for s in seeds],
             "queries_requested": n_queries, "queries_admitted": admitted,
-            "admission": f"walk queries k 5-10, nested root samples, at most {TRAIN_MAX_WORK} candidates",
+            "admission": f"walk queries k 5-10, nested root samples, at most {TRAIN_MAX_WORK} candidates, "
+                         f"at most {TRAIN_TRIES} candidate queries per size and graph",
             "tries_per_size": {str(s): {str(k): v for k, v in tr.items()} for s, tr in tried_all.items()},
             "samples": int(len(t)), "epochs": epochs, "mse_first": hist[0], "mse_last": hist[-1],
             "log_s": t1 - t0, "train_s": t2 - t1}

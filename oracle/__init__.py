@@ -224,6 +224,25 @@ class _OOpts(C.Structure):
                 ("root_end", C.c_uint32), ("root_stride", C.c_uint32)]
 
 
+def enumerate_ex(g: OGraph, qa: np.ndarray, root_q: int, roots, cap: int = 1 << 20) -> np.ndarray:
+    """enumerate_embeddings restricted to query vertex root_q's images in
+    range(*roots).  Rows indexed by query vertex."""
+    L = lib()
+    L.oracle_enumerate_ex.restype = C.c_uint64
+    L.oracle_enumerate_ex.argtypes = [C.POINTER(_OGraph), C.c_uint32, C.POINTER(C.c_uint8), C.POINTER(_OOpts),
+                                      _u32p, C.c_uint64]
+    qa, p = _qptr(qa)
+    k = qa.shape[0]
+    o = _OOpts()
+    o.root_q = root_q
+    o.root_begin, o.root_end, o.root_stride = roots
+    rows = np.zeros((cap, k), dtype=np.uint32)
+    n = int(L.oracle_enumerate_ex(C.byref(g.c), k, p, C.byref(o), rows.ctypes.data_as(_u32p), cap))
+    if n > cap:
+        raise ValueError(f"{n} embeddings exceed cap {cap}")
+    return rows[:n]
+
+
 def count_ex(g: OGraph, qa: np.ndarray, thr_in=None, thr_out=None, root_q: int = -1, roots=(0, 0, 1),
              nthreads: int = 0) -> int:
     """oracle_count with explicit per-vertex signature thresholds and/or a

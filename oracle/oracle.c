@@ -244,11 +244,16 @@ static void dfs_enum(const oracle_graph *d, const oplan *p, uint32_t i, uint32_t
 
 uint64_t oracle_enumerate(const oracle_graph *d, uint32_t k, const uint8_t *qa, uint32_t *rows,
                           uint64_t cap) {
+  return oracle_enumerate_ex(d, k, qa, NULL, rows, cap);
+}
+
+uint64_t oracle_enumerate_ex(const oracle_graph *d, uint32_t k, const uint8_t *qa, const oracle_opts *o,
+                             uint32_t *rows, uint64_t cap) {
   if (k == 0 || k > MAXK) return 0;
   uint32_t ord[MAXK], f[MAXK];
   oplan p;
   oracle_order(k, qa, ord);
-  plan_from_order(k, qa, ord, &p, NULL);
+  plan_from_order(k, qa, ord, &p, o);
   sink s = {rows, cap, 0};
   dfs_enum(d, &p, 0, f, &s);
   return s.n;

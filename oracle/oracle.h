@@ -52,6 +52,9 @@ uint64_t oracle_count_ex(const oracle_graph *d, uint32_t k, const uint8_t *qa, c
  * returns the total number of embeddings (which may exceed cap). */
 uint64_t oracle_enumerate(const oracle_graph *d, uint32_t k, const uint8_t *qa, uint32_t *rows,
                           uint64_t cap);
+/* oracle_enumerate with oracle_count_ex's thresholds and root restriction. */
+uint64_t oracle_enumerate_ex(const oracle_graph *d, uint32_t k, const uint8_t *qa, const oracle_opts *o,
+                             uint32_t *rows, uint64_t cap);
 
 /* Score order restated independently of the product's host code. Returns 0
  * on success, -1 if some step has no vertex scoring above (0,0,0). */

@@ -272,6 +272,33 @@ def test_config2_full_counts_in_two_independent_modes(ctx, key):
     assert slow.level_sizes == fast.level_sizes
 
 
+PLAIN = INDEPENDENT | cuts_amd.FLAG_NO_SHORTCUT | cuts_amd.FLAG_NO_FUSE
+
+
+@pytest.mark.parametrize("key", ["config2a", "config2b"])
+def test_config2_top_hub_roots_default_vs_plain_expansion(ctx, key):
+    """The mid-size graph's two heaviest vertices (degrees 1,198 and 1,184)
+    as the only roots: 3.5e12 (4-cycle) and 1.9e11 (K4-e) embeddings each,
+    beyond the oracle.  The default mode (collapsed pair level or shared
+    chain candidates, dense two-hop rows of the hubs) must equal the plain
+    trie walk that expands and enumerates every level with no closed form at
+    all -- the reference's own algorithm -- and the mode in between."""
+    g = W.config2_graph()
+    deg = np.diff(g.view()["out_off"].astype(np.int64))
+    top = [int(v) for v in np.argsort(-deg, kind="stable")[:2]]
+    assert deg[top].tolist() == [1198, 1184]
+    shape = {"config2a": W.C4_2PEND, "config2b": W.K4E_TAIL2}[key]
+    p = cuts_amd.Plan(W.query_graph(shape))
+    dg = ctx.upload(g)
+    v = top[0] if key == "config2b" else top[1]
+    rt = (v, v + 1, 1)
+    fast = ctx.match(dg, p, roots=rt)
+    assert fast.count > 10 ** 11
+    assert ctx.match(dg, p, roots=rt, flags=INDEPENDENT).count == fast.count
+    plain = ctx.match(dg, p, roots=rt, flags=PLAIN)
+    assert plain.complete and plain.count == fast.count
+
+
 def test_cli_report_line_and_trunks(tmp_path):
     g = W.config1_graph()
     data = tmp_path / "cl4k.g"
