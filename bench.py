@@ -39,6 +39,7 @@ Other workloads (one JSON line each):
     --workload chunglu4k   triangle + 2-edge tail on Chung-Lu n=4,096 (full count)
     --workload batch256    256 random-walk queries, batched level queue in one hipGraph
                            launch, learned orders, autotuned launch shape
+    --workload write       4.8e8 embeddings written to pinned host memory, streamed vs synchronous
 """
 import argparse
 import json
@@ -123,7 +124,7 @@ def roofline(kern, workload):
     t = traffic_for(f"{workload}:{dname}")
     hbm = t["bytes_per_dispatch"] if t else None
     ach = hbm / avg_s / 1e9 if (hbm and avg_s) else None
-    return {"bound": "hbm", "kernel": dname, "kernels": t["kernels"] if t else None,
+    return {"bound": "hbm", "kernel": dname, "kernels": (t.get("kernels") or t.get("kernel")) if t else None,
             "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS if ach is not None else None, "traffic": hbm,
             "avg_launch_ms": avg_s * 1e3, "launches": dn,
@@ -343,8 +344,10 @@ def run_headline(args):
     want = [q["count"] for q in items]
     heur = [cuts_amd.Plan(cuts_amd.Graph.from_edges(k, e)) for k, e in qs]
     assert [p.order for p in heur] == [q["order"] for q in items], "score orders differ from the fixture's"
-    hres = [ctx.match(dg, p, roots=rt) for p, rt in zip(heur, roots)]
-    assert [r.count for r in hres] == want, "score-order counts differ from the fixture"
+    hres = None
+    if not args.profile_only:
+        hres = [ctx.match(dg, p, roots=rt) for p, rt in zip(heur, roots)]
+        assert [r.count for r in hres] == want, "score-order counts differ from the fixture"
     # learned orders: network forward over every connected prefix set (HIP),
     # engine-aware DP, first vertex kept (root samples), judged by time
     net, winfo = LN.load_weights(ctx, WEIGHTS)
@@ -493,6 +496,42 @@ def config2_fields(ctx, steps):
     return out
 
 
+def run_write(args):
+    """Enumeration output at scale: the 3-edge path on the 4,096-vertex
+    Chung-Lu graph, 484,209,988 embeddings written to pinned host memory
+    (7.7 GB), with the copies of each chunk overlapped with the next chunk's
+    expansion and, for comparison, copied before the next chunk runs."""
+    import cuts_amd
+    from cuts_amd import workloads as W
+    g = W.config1_graph()
+    want = 484209988
+    ctx = cuts_amd.Context(0)
+    dg = ctx.upload(g)
+    p = cuts_amd.Plan(cuts_amd.Graph.from_edges(4, [(0, 1), (1, 2), (2, 3)]))
+    buf = cuts_amd.pinned_array((want, 4))
+    res = {}
+    for name, flags in (("streamed", 0), ("sync", cuts_amd.FLAG_SYNC_WRITE)):
+        for _ in range(args.warmup):
+            ctx.match(dg, p, write=True, rows_out=buf, flags=flags)
+        ms = []
+        for i in range(args.steps):
+            r = ctx.match(dg, p, write=True, rows_out=buf, flags=flags)
+            if r.count != want or len(r.rows) != want:
+                raise AssertionError(f"{name}: count {r.count}, rows {len(r.rows)}, want {want}")
+            ms.append(r.device_ms)
+            log(f"{name} step {i + 1}/{args.steps}: {r.device_ms:.1f} ms")
+        res[name] = median(ms)
+    line = base_line("exact embeddings/sec + p50 time/query", want / (res["streamed"] / 1e3), "embeddings/s",
+                     args.steps, args.warmup, res["streamed"],
+                     "chung-lu n=4096 m=32768 gamma=2.3 seed=0; 3-edge path; every embedding written to pinned "
+                     "host memory (16 B each), chunks of 2^22 candidates",
+                     {"count": want, "ms_streamed": res["streamed"], "ms_sync": res["sync"],
+                      "host_GBs_streamed": want * 16 / (res["streamed"] / 1e3) / 1e9,
+                      "host_GBs_sync": want * 16 / (res["sync"] / 1e3) / 1e9,
+                      "speedup_streamed_vs_sync": res["sync"] / res["streamed"]})
+    emit(line)
+
+
 def run_chunglu36k(args):
     """The mid-size Chung-Lu graph and its two 6-vertex queries at full size,
     score order; the profiling pass runs after the timed steps."""
@@ -547,7 +586,8 @@ def main():
     ap.add_argument("--no-config2", action="store_true", help="headline: leave out the mid-size graph's fields")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run N dispatches of a 1 GiB plain copy (byte-counter yardstick)")
-    ap.add_argument("--workload", default="rmat20", choices=["rmat20", "chunglu36k", "chunglu4k", "batch256"])
+    ap.add_argument("--workload", default="rmat20",
+                    choices=["rmat20", "chunglu36k", "chunglu4k", "batch256", "write"])
     ap.add_argument("--eval-queries", type=int, default=500)
     ap.add_argument("--orders-out", default="", help="headline: write the judged learned orders to this file")
     ap.add_argument("--orders-in", default="", help="headline: take the learned orders from this file (no judging)")
@@ -561,7 +601,7 @@ def main():
               "the matcher shards by level-0 chunks, see DESIGN.md", file=sys.stderr)
         sys.exit(2)
     {"rmat20": run_headline, "chunglu36k": run_chunglu36k, "chunglu4k": run_chunglu4k,
-     "batch256": run_batch256}[args.workload](args)
+     "batch256": run_batch256, "write": run_write}[args.workload](args)
 
 
 if __name__ == "__main__":

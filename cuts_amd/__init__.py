@@ -34,6 +34,7 @@ FLAG_NO_CHAIN = 256
 FLAG_NO_TWO_HOP = 512
 FLAG_NO_SHARED = 1024
 FLAG_NO_ARC_COMMON = 2048
+FLAG_SYNC_WRITE = 4096
 MAX_QUERY = 16
 KERNEL_CLASSES = ["level0", "plan", "scan", "tile_index", "expand_write", "expand_count", "expand_fused",
                   "closed_form", "pair", "common", "chain"]
@@ -121,6 +122,8 @@ def lib():
             "cuts_match": (C.c_int, [vp, vp, vp, C.POINTER(_Opts), C.POINTER(_Result)]),
             "cuts_match_noalloc": (C.c_int, [vp, vp, vp, C.POINTER(_Opts), C.POINTER(_Result)]),
             "cuts_noalloc_scratch_bytes": (C.c_uint64, []),
+            "cuts_host_alloc": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+            "cuts_host_free": (None, [C.c_void_p]),
             "cuts_search": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.POINTER(C.c_uint64)]),
             "cuts_search_dfs_bfs_strategy": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_uint32,
                                                        C.POINTER(C.c_uint64)]),
@@ -150,6 +153,30 @@ class CutsError(RuntimeError):
 def _check(rc: int):
     if rc != 0:
         raise CutsError(rc, lib().cuts_last_error().decode())
+
+
+class _PinnedBlock:
+    """One cuts_host_alloc allocation, freed with its last reference."""
+
+    def __init__(self, nbytes: int):
+        p = C.c_void_p()
+        _check(lib().cuts_host_alloc(nbytes, C.byref(p)))
+        self.ptr = p
+
+    def __del__(self):
+        if getattr(self, "ptr", None) is not None and self.ptr.value and _lib is not None:
+            _lib.cuts_host_free(self.ptr)
+            self.ptr = None
+
+
+def pinned_array(shape, dtype=np.uint32) -> np.ndarray:
+    """A numpy array in pinned host memory (cuts_host_alloc); the memory is
+    released when the array and its views are gone."""
+    n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+    block = _PinnedBlock(max(n, 1))
+    buf = (C.c_char * max(n, 1)).from_address(block.ptr.value)
+    buf._block = block
+    return np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
 
 
 def _u32(a) -> np.ndarray:
@@ -357,9 +384,12 @@ class Context:
     def match(self, dg: DeviceGraph, plan: Plan, trunks: int = 0, flags: int = 0, budget: int = 0,
               roots: Optional[tuple] = None, write: bool = False, emb_cap: int = 0,
               blocks_per_cu: int = 0, time_limit_ms: float = 0.0, max_work: int = 0,
-              noalloc: bool = False) -> MatchResult:
+              noalloc: bool = False, rows_out: Optional[np.ndarray] = None) -> MatchResult:
         """One query (cuts_match).  noalloc=True calls cuts_match_noalloc: the
-        run's buffers come from the device's static scratch array."""
+        run's buffers come from the device's static scratch array.  With
+        write=True the rows go to rows_out (C-contiguous uint32, (cap, k);
+        pinned_array for overlapped copies) or to a new pinned array of
+        emb_cap rows."""
         o = _Opts()
         lib().cuts_match_options_default(C.byref(o))
         o.trunks, o.flags, o.budget_bytes, o.blocks_per_cu = trunks, flags, budget, blocks_per_cu
@@ -370,7 +400,15 @@ class Context:
         rows = None
         if write:
             o.flags |= FLAG_WRITE
-            rows = np.zeros((max(emb_cap, 1), plan.k), dtype=np.uint32)
+            if rows_out is not None:
+                if rows_out.dtype != np.uint32 or not rows_out.flags.c_contiguous or rows_out.ndim != 2 or \
+                        rows_out.shape[1] != plan.k:
+                    raise ValueError("rows_out must be a C-contiguous uint32 array of shape (cap, k)")
+                rows = rows_out
+                emb_cap = rows_out.shape[0]
+            else:
+                # pinned, so that the copies of one chunk overlap the next chunk
+                rows = pinned_array((max(emb_cap, 1), plan.k))
             o.emb_rows = _ptr(rows)
             o.emb_cap = emb_cap
         r = _Result()
@@ -381,7 +419,9 @@ class Context:
                           last_root=int(r.last_root), work_capped=bool(r.work_capped), work=int(r.work))
         res._raw = r
         if write:
-            res.rows = rows[: r.emb_written].copy()
+            # a view of the pinned buffer when it is mostly used, else a copy
+            res.rows = rows[: r.emb_written] if (rows_out is not None or 2 * r.emb_written >= len(rows)) \
+                else rows[: r.emb_written].copy()
         if o.flags & FLAG_PROFILE:
             res.kernels = {KERNEL_CLASSES[i]: (float(r.kernel_ms[i]), int(r.kernel_launches[i]),
                                                float(r.kernel_bytes[i])) for i in range(_NCLASS)}

@@ -549,6 +549,23 @@ int cuts_match_noalloc(cuts_context* ctx, const cuts_device_graph* dg, const cut
 
 uint64_t cuts_noalloc_scratch_bytes(void) { return cuts::noalloc_scratch_bytes(); }
 
+int cuts_host_alloc(uint64_t bytes, void** out) {
+  return guard([&]() -> int {
+    if (!out) return fail(CUTS_E_ARG, "null argument");
+    *out = nullptr;
+    if (hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      *out = nullptr;
+      return fail(CUTS_E_NOMEM, "pinned host allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    return CUTS_OK;
+  });
+}
+
+void cuts_host_free(void* p) {
+  if (p) (void)hipHostFree(p);
+}
+
 int cuts_log_tables(const char* path, const char* label, const cuts_plan* plan, const cuts_match_result* r) {
   return guard([&]() -> int {
     if (!path || !plan || !r) return fail(CUTS_E_ARG, "null argument");

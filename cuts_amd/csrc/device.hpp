@@ -125,6 +125,8 @@ struct DeviceGraph {
 struct LevelBuf {
   uint32_t* rows = nullptr;  // cap rows of (level+1) words
   uint64_t cap = 0;
+  uint32_t* rows2 = nullptr;    // second output buffer of the written last level
+  uint64_t rows2_cap = 0;
   uint32_t* minlen = nullptr;   // cap
   uint8_t* minsel = nullptr;    // cap
   uint64_t* work_off = nullptr; // cap+1
@@ -170,6 +172,11 @@ struct Context {
   // expansion
   unsigned long long* counters = nullptr;
   hipEvent_t kev[2] = {nullptr, nullptr};
+  // embedding output: rows of the last level are copied to the caller's
+  // buffer on their own stream while the next chunk expands; copy_ev[b] marks
+  // the end of the last copy out of output buffer b
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t copy_ev[2] = {nullptr, nullptr};
   uint64_t* host_pinned = nullptr;        // small pinned staging
   char* scratch = nullptr;                // this device's no-allocation scratch array
   ~Context();

@@ -489,10 +489,42 @@ __global__ __launch_bounds__(BLOCK) void k_scan_down(const uint32_t* __restrict_
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = bsum[nb];
 }
 
+// Levels of at most SCAN_TILE rows: the whole exclusive scan in one block
+// (out[0..n] with out[n] the total), one launch instead of three.
+__global__ __launch_bounds__(BLOCK) void k_scan_single(const uint32_t* __restrict__ in, uint64_t n,
+                                                      uint64_t* __restrict__ out) {
+  __shared__ uint64_t s_tmp[BLOCK / 64];
+  uint32_t v[SCAN_IPT];
+  uint64_t s = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_IPT; ++i) {
+    const uint64_t idx = (uint64_t)threadIdx.x * SCAN_IPT + i;
+    v[i] = idx < n ? in[idx] : 0;
+    s += v[i];
+  }
+  uint64_t tot;
+  uint64_t ex = block_excl_scan(s, s_tmp, &tot);
+#pragma unroll
+  for (int i = 0; i < SCAN_IPT; ++i) {
+    const uint64_t idx = (uint64_t)threadIdx.x * SCAN_IPT + i;
+    if (idx < n) out[idx] = ex;
+    ex += v[i];
+  }
+  if (threadIdx.x == 0) out[n] = tot;
+}
+
 // tile_p[t] = prefix owning work item w_begin + t*TILE (t < ntiles),
-// tile_p[ntiles] = prefix owning item w_end-1
+// tile_p[ntiles] = prefix owning item w_end-1.  Block 0 also clears the
+// counters the expansion that follows starts from (its eight tile counters
+// and, when given, its output row counter), which saves a fill launch each.
 __global__ __launch_bounds__(BLOCK) void k_tile_index(const uint64_t* __restrict__ off, uint64_t n, uint64_t w_begin,
-                                                     uint64_t w_end, uint64_t ntiles, uint32_t* __restrict__ tile_p) {
+                                                     uint64_t w_end, uint64_t ntiles, uint32_t* __restrict__ tile_p,
+                                                     unsigned int* __restrict__ tilectr,
+                                                     unsigned int* __restrict__ outctr) {
+  if (blockIdx.x == 0) {
+    if (threadIdx.x < 8) tilectr[threadIdx.x] = 0u;
+    if (outctr && threadIdx.x < 2) outctr[threadIdx.x] = 0u;
+  }
   for (uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; t <= ntiles; t += (uint64_t)gridDim.x * BLOCK) {
     const uint64_t w = t < ntiles ? w_begin + t * TILE : w_end - 1;
     uint64_t lo = 0, hi = n + 1;  // upper_bound over off[0..n]
@@ -649,10 +681,10 @@ __device__ __forceinline__ int64_t chain_closed(const GView& g, const KPlan& P, 
 // the chain count of the row + candidate.  Tiles as in k_expand.
 struct SibArgs {
   const uint32_t* rows;
-  const uint32_t* parent;      // parent prefix of every row
-  const uint64_t* coff;        // per parent: start of its block in cand
   const uint32_t* cand;
   const uint32_t* shared;      // per row: its own positions' intersections with u
+  const uint4* pb;             // per row: the chain closed form's row part (k_sib_base)
+  const uint64_t* rstart;      // per row: start of its parent's block in cand
   const uint64_t* work_off;
   const uint32_t* tile_p;
   uint64_t w_begin, w_end, ntiles;
@@ -1192,6 +1224,105 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
   }
 }
 
+// Per-row part of the chain closed form for the rows of level k-4 (L = k-3
+// words) whose position-(k-3) candidates are their parent's block: every
+// term of chain_closed that does not involve the candidate v, computed once
+// per row instead of once per (row, candidate).  With p = L the candidate's
+// position, u = r[ch_u], inu_old / Dold the older positions inside N(u)
+// meeting ta / meeting tb, the row's base is
+//   H(u) - sum_{inu_old} cnt_tb(r_l) - [deg u >= tb] na_old
+//        + sum_{i in Dold} sum_{j in inu_old, j != i} [r_i ~ r_j] - shared
+// and a candidate v adds, with inv = [v in N(u), deg v >= ta]:
+//   inv (-cnt_tb(v) + [deg u >= tb] + sum_{i in Dold} [r_i ~ v])
+//   + [deg v >= tb] sum_{j in inu_old} [v ~ r_j] - [deg v >= tb] |N(u) & N(v)|
+// (chain_sib_item); na = na_old - inv.  Stored per row: base (int64), na_old,
+// and the masks (inu_old in bits 0..14, Dold in 15..29, [deg u >= tb] in 30)
+// in pb, the start of the parent's block in rstart, its length in minlen.
+template <int L>
+__global__ __launch_bounds__(BLOCK) void k_sib_base(GView g, KPlan P, const uint32_t* __restrict__ rows, uint64_t n,
+                                                   const uint32_t* __restrict__ parent,
+                                                   const uint32_t* __restrict__ par_len,
+                                                   const uint64_t* __restrict__ par_off,
+                                                   const uint32_t* __restrict__ shared, ChainTables t,
+                                                   uint32_t* __restrict__ minlen, uint4* __restrict__ pb,
+                                                   uint64_t* __restrict__ rstart) {
+  uint64_t dummy = 0;
+  for (uint64_t q = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; q < n; q += (uint64_t)gridDim.x * BLOCK) {
+    const uint32_t par = parent[q];
+    minlen[q] = par_len[par];
+    rstart[q] = par_off[par];
+    uint32_t r[L], d[L], u = 0;
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      r[l] = rows[q * L + l];
+      if (l == (int)P.ch_u) u = r[l];
+    }
+#pragma unroll
+    for (int l = 0; l < L; ++l) d[l] = g.sig[2 * (uint64_t)r[l]];
+    const uint32_t du = g.off_out[u + 1] - g.off_out[u];
+    uint32_t inu = 0, dold = 0;
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      if (l == (int)P.ch_u) continue;
+      const bool in = ((P.qadj[P.ch_u] >> l) & 1) || adj_has(g, u, r[l], dummy);
+      if (in && d[l] >= P.ch_ta) inu |= 1u << l;
+      if (d[l] >= P.ch_tb) dold |= 1u << l;
+    }
+    const int64_t na_old = (int64_t)(t.ta ? t.ta[u] : du) - __popc(inu);
+    int64_t c = (int64_t)t.H[u];
+#pragma unroll
+    for (int l = 0; l < L; ++l)
+      if ((inu >> l) & 1) c -= t.tb ? t.tb[r[l]] : g.off_out[r[l] + 1] - g.off_out[r[l]];
+    const bool dtb = P.ch_tb <= 1 || du >= P.ch_tb;
+    if (dtb) c -= na_old;
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+      if (!((dold >> i) & 1)) continue;
+#pragma unroll
+      for (int j = 0; j < L; ++j) {
+        if (j == i || !((inu >> j) & 1)) continue;
+        if (((P.qadj[i] >> j) & 1) || adj_has(g, r[i], r[j], dummy)) ++c;
+      }
+    }
+    c -= (int64_t)shared[q];
+    pb[q] = make_uint4((uint32_t)(uint64_t)c, (uint32_t)((uint64_t)c >> 32), (uint32_t)na_old,
+                       inu | (dold << 15) | ((dtb ? 1u : 0u) << 30));
+  }
+}
+
+// The candidate's part of the chain closed form on top of the row's base
+// (k_sib_base); na receives na_old - inv.
+template <int L>
+__device__ __forceinline__ int64_t chain_sib_item(const GView& g, const KPlan& P, const uint32_t (&r)[L], uint32_t v,
+                                                  uint4 pbq, const ChainTables& t, int64_t& na, uint64_t& bytes) {
+  const uint32_t inu = pbq.w & 0x7FFFu, dold = (pbq.w >> 15) & 0x7FFFu;
+  const bool dtb = (pbq.w >> 30) & 1u;
+  int64_t c = (int64_t)(((uint64_t)pbq.y << 32) | pbq.x);
+  uint32_t u = 0;
+#pragma unroll
+  for (int l = 0; l < L; ++l)
+    if (l == (int)P.ch_u) u = r[l];
+  const uint32_t dv = g.sig[2 * (uint64_t)v];
+  bytes += 4;
+  const bool inv = dv >= P.ch_ta && (((P.qadj[P.ch_u] >> L) & 1) || adj_has(g, u, v, bytes));
+  na = (int64_t)pbq.z - (inv ? 1 : 0);
+  if (inv) {
+    c -= t.tb ? t.tb[v] : g.off_out[v + 1] - g.off_out[v];
+    bytes += t.tb ? 4 : 8;
+    if (dtb) ++c;
+#pragma unroll
+    for (int i = 0; i < L; ++i)
+      if (((dold >> i) & 1) && (((P.qadj[i] >> L) & 1) || adj_has(g, r[i], v, bytes))) ++c;
+  }
+  if (P.ch_tb <= 1 || dv >= P.ch_tb) {
+#pragma unroll
+    for (int j = 0; j < L; ++j)
+      if (((inu >> j) & 1) && (((P.qadj[L] >> j) & 1) || adj_has(g, v, r[j], bytes))) ++c;
+    c -= two_hop(g, u, v, bytes);
+  }
+  return c;
+}
+
 template <int L>
 __global__ __launch_bounds__(BLOCK) void k_chain_sib(GView g, KPlan P, SibArgs a, ChainTables t) {
   __shared__ uint64_t s_off[LDS_OFF + 1];
@@ -1229,32 +1360,79 @@ __global__ __launch_bounds__(BLOCK) void k_chain_sib(GView g, KPlan P, SibArgs a
     if (lds)
       for (uint32_t i = tid; i <= span; i += BLOCK) s_off[i] = a.work_off[plo + i];
     __syncthreads();
-    for (int it = 0; it < IPT; ++it) {
-      const uint64_t w = w0 + (uint64_t)it * BLOCK + tid;
-      if (w >= w1) continue;
-      uint32_t lo = 0, hi = span + 1;
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        const uint64_t o = lds ? s_off[mid] : a.work_off[plo + mid];
-        if (o <= w) lo = mid + 1;
-        else hi = mid;
-      }
-      const uint32_t pi = lo - 1;
-      const uint64_t q = (uint64_t)plo + pi;
-      const uint64_t qbase = lds ? s_off[pi] : a.work_off[q];
-      const uint32_t par = a.parent[q];
-      const uint32_t v = a.cand[a.coff[par] + (w - qbase)];
-      uint32_t r[L + 1];
+    if (a.stats) {
+      // profiling: the full closed form per item (it also counts the merge
+      // model's bytes)
+      for (int it = 0; it < IPT; ++it) {
+        const uint64_t w = w0 + (uint64_t)it * BLOCK + tid;
+        if (w >= w1) continue;
+        uint32_t lo = 0, hi = span + 1;
+        while (lo < hi) {
+          const uint32_t mid = (lo + hi) >> 1;
+          const uint64_t o = lds ? s_off[mid] : a.work_off[plo + mid];
+          if (o <= w) lo = mid + 1;
+          else hi = mid;
+        }
+        const uint32_t pi = lo - 1;
+        const uint64_t q = (uint64_t)plo + pi;
+        const uint64_t qbase = lds ? s_off[pi] : a.work_off[q];
+        const uint32_t v = a.cand[a.rstart[q] + (w - qbase)];
+        uint32_t r[L + 1];
 #pragma unroll
-      for (int l = 0; l < L; ++l) r[l] = a.rows[q * L + l];
-      r[L] = v;
-      if (a.stats) bytes += 4 * L + 16;
-      if (v == r[L - 1]) continue;
-      ++valid;
-      int64_t na = 0;
-      acc += (uint64_t)chain_closed<L + 1>(g, P, r, t, a.shared[q], true, na, a.stats != 0, mb, bytes);
-      pairs += (uint64_t)na;
-      if (a.stats) mb += 16;  // the survivor's node, written and read back
+        for (int l = 0; l < L; ++l) r[l] = a.rows[q * L + l];
+        r[L] = v;
+        bytes += 4 * L + 16;
+        if (v == r[L - 1]) continue;
+        ++valid;
+        int64_t na = 0;
+        acc += (uint64_t)chain_closed<L + 1>(g, P, r, t, a.shared[q], true, na, true, mb, bytes);
+        pairs += (uint64_t)na;
+        mb += 16;  // the survivor's node, written and read back
+      }
+    } else {
+      // the IPT items of a thread in lock-step stages (prefix search, row
+      // data and base, candidate word, candidate part of the closed form)
+      uint64_t w[IPT];
+      uint32_t pi[IPT];
+      bool in[IPT];
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        w[j] = w0 + (uint64_t)j * BLOCK + tid;
+        in[j] = w[j] < w1;
+        pi[j] = 0;
+      }
+      for (uint32_t nn = span + 1; nn > 1;) {
+        const uint32_t half = nn >> 1;
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+          const uint64_t o = lds ? s_off[pi[j] + half] : a.work_off[plo + pi[j] + half];
+          pi[j] = (o <= w[j]) ? pi[j] + half : pi[j];
+        }
+        nn -= half;
+      }
+      uint64_t q[IPT], cidx[IPT];
+      uint4 pbq[IPT];
+      uint32_t r[IPT][L], v[IPT];
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const uint32_t pj = min(pi[j], span - 1);
+        q[j] = (uint64_t)plo + pj;
+        const uint64_t qbase = lds ? s_off[pj] : a.work_off[q[j]];
+        cidx[j] = a.rstart[q[j]] + (w[j] - qbase);
+        pbq[j] = a.pb[q[j]];
+#pragma unroll
+        for (int l = 0; l < L; ++l) r[j][l] = a.rows[q[j] * L + l];
+      }
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) v[j] = in[j] ? a.cand[cidx[j]] : 0u;
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        if (!in[j] || v[j] == r[j][L - 1]) continue;
+        ++valid;
+        int64_t na = 0;
+        acc += (uint64_t)chain_sib_item<L>(g, P, r[j], v[j], pbq[j], t, na, bytes);
+        pairs += (uint64_t)na;
+      }
     }
     __syncthreads();
   }
@@ -1490,12 +1668,6 @@ __global__ __launch_bounds__(BLOCK) void k_chain_shared_h2(GView g, KPlan P, con
 // Chain level with shared candidates, rows of L = k-3 words (level k-4):
 // every row's work is the filled part of its parent's block of
 // position-(k-3) candidates (cnt = the parents' filled lengths).
-__global__ __launch_bounds__(BLOCK) void k_sib_len(const uint32_t* __restrict__ parent, const uint32_t* __restrict__ cnt,
-                                                  uint64_t n, uint32_t* __restrict__ minlen) {
-  for (uint64_t q = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; q < n; q += (uint64_t)gridDim.x * BLOCK)
-    minlen[q] = cnt[parent[q]];
-}
-
 // Chain tables for thresholds (ta, tb): H[u] = sum over x in N(u) with
 // deg(x) >= ta of |{y in N(x) : deg(y) >= tb}|, H[V + u] = sum of those
 // deg(x).  One wave per vertex.
@@ -1702,6 +1874,10 @@ struct Matcher {
   uint64_t level_cap[kMaxQuery] = {0};
   uint32_t grid_cap = 0;
 
+  // streamed embedding output: chunks of the last level copied so far (the
+  // output buffer of chunk i is i & 1), and whether copies are streamed
+  uint64_t emb_chunks = 0;
+  bool stream_emb = false;
   // where the run's buffers live: the context's level buffers and counters
   // (hipMalloc'd on demand), or a scratch arena and caller-owned level
   // buffers (the no-allocation entry point)
@@ -1767,6 +1943,12 @@ struct Matcher {
   // exclusive scan of b.minlen[0..n) into b.work_off[0..n], returns total
   uint64_t scan(LevelBuf& b, uint64_t n) {
     const uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nb <= 1) {
+      pbegin(false);
+      hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(BLOCK), 0, c.stream, b.minlen, n, b.work_off);
+      pend(2, 16.0 * n, false);
+      return read_u64(b.work_off + n);
+    }
     ensure_bsum(arena, b, nb);
     pbegin(false);
     hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)nb), dim3(BLOCK), 0, c.stream, b.minlen, n, b.bsum);
@@ -1840,8 +2022,17 @@ struct Matcher {
     if (mode == M_WRITE) {
       const uint64_t cap = level_cap[L];
       chunk = std::min<uint64_t>(chunk, std::max<uint64_t>(cap, 1));
+      // the written last level goes out in chunks of at most 2^22 candidates,
+      // so that the copy of one chunk overlaps the expansion of the next
+      if (L == k - 1 && o.emb_rows) chunk = std::min<uint64_t>(chunk, 1ull << 22);
+      // a buffer about to be replaced may still be the source of a copy
+      if (L == k - 1 && emb_chunks && (out.cap < std::min(chunk, W) || !out.rows2))
+        HIPCHECK(hipStreamSynchronize(c.copy_stream));
       ensure_rows_exact(arena, out, std::min<uint64_t>(chunk, W), L + 1);
       chunk = std::min<uint64_t>(chunk, out.cap);
+      if (L == k - 1 && stream_emb) {
+        ensure_aux(arena, out.rows2, out.rows2_cap, (uint64_t)out.cap * (L + 1));
+      }
     }
     const bool pair_common = mode == M_PAIR && kp.pr_a != kp.pr_b && !pair_cab_known();
     if (pair_common) {
@@ -1855,7 +2046,7 @@ struct Matcher {
       ensure_tiles(arena, in, nt);
       pbegin(false);
       hipLaunchKernelGGL(k_tile_index, dim3(grid_for(nt + 1, BLOCK, 65535)), dim3(BLOCK), 0, c.stream, in.work_off,
-                         n, w0, w1, nt, in.tile_p);
+                         n, w0, w1, nt, in.tile_p, (unsigned int*)(ctr + 4), mode == M_WRITE ? (unsigned int*)(ctr + 10) : nullptr);
       pend(3, (double)(nt + 1) * (8.0 * (64 - __builtin_clzll(n + 2)) + 4.0), false);
       ExpandArgs a{};
       a.rows = rows;
@@ -1879,15 +2070,17 @@ struct Matcher {
       a.ctab = ctab;
       a.ptab = ptbl;
       a.xcd = (o.flags & 8u) ? 0 : 1;
-      a.tilectr = (unsigned int*)(ctr + 4);  // 8 x uint32 in counters[4..7]
-      HIPCHECK(hipMemsetAsync(a.tilectr, 0, 8 * sizeof(unsigned int), c.stream));
+      a.tilectr = (unsigned int*)(ctr + 4);  // 8 x uint32 in counters[4..7], cleared by k_tile_index
       a.stats = prof ? 1 : 0;
       uint32_t grid = grid_for(nt, 1, grid_cap);
       r.chunks++;
       if (mode == M_WRITE) {
-        a.out_rows = out.rows;
+        const bool streamed = L == k - 1 && stream_emb;
+        const uint32_t ob = (uint32_t)(emb_chunks & 1);
+        a.out_rows = (streamed && ob) ? out.rows2 : out.rows;
         a.out_count = (uint32_t*)(ctr + 10);
-        HIPCHECK(hipMemsetAsync(ctr + 10, 0, 8, c.stream));
+        // the buffer's previous copy to the host must be done before it is refilled
+        if (streamed) HIPCHECK(hipStreamWaitEvent(c.stream, c.copy_ev[ob], 0));
         pbegin(true);
 #define EW(LL) hipLaunchKernelGGL((k_expand<LL, M_WRITE>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, a)
         CUTS_DISPATCH_L(L, EW)
@@ -1902,7 +2095,15 @@ struct Matcher {
           r.count += m;
           const uint64_t room = o.emb_cap > r.emb_written ? o.emb_cap - r.emb_written : 0;
           const uint64_t take = std::min(room, m);
-          if (take && o.emb_rows) {
+          if (take && o.emb_rows && streamed) {
+            // on the copy stream, while the next chunk expands (the count
+            // read above already waited for this chunk's rows)
+            HIPCHECK(hipMemcpyAsync(o.emb_rows + r.emb_written * k, a.out_rows, take * k * 4, hipMemcpyDeviceToHost,
+                                    c.copy_stream));
+            HIPCHECK(hipEventRecord(c.copy_ev[ob], c.copy_stream));
+            ++emb_chunks;
+            r.emb_written += take;
+          } else if (take && o.emb_rows) {
             HIPCHECK(hipMemcpyAsync(o.emb_rows + r.emb_written * k, out.rows, take * k * 4, hipMemcpyDeviceToHost,
                                     c.stream));
             HIPCHECK(hipStreamSynchronize(c.stream));
@@ -1979,7 +2180,7 @@ struct Matcher {
       ensure_tiles(arena, in, nt);
       pbegin(false);
       hipLaunchKernelGGL(k_tile_index, dim3(grid_for(nt + 1, BLOCK, 65535)), dim3(BLOCK), 0, c.stream, in.work_off,
-                         n, w0, w1, nt, in.tile_p);
+                         n, w0, w1, nt, in.tile_p, (unsigned int*)(ctr + 4), nullptr);
       pend(3, (double)(nt + 1) * (8.0 * (64 - __builtin_clzll(n + 2)) + 4.0), false);
       ExpandArgs a{};
       a.rows = rows;
@@ -1993,7 +2194,6 @@ struct Matcher {
       a.nsurv = weight;
       a.xcd = (o.flags & 8u) ? 0 : 1;
       a.tilectr = (unsigned int*)(ctr + 4);
-      HIPCHECK(hipMemsetAsync(a.tilectr, 0, 8 * sizeof(unsigned int), c.stream));
       a.stats = prof ? 1 : 0;
       r.chunks++;
       pbegin(true);
@@ -2062,7 +2262,7 @@ struct Matcher {
       const uint64_t nt = (w1 - w0 + TILE - 1) / TILE;
       ensure_tiles(arena, in, nt);
       hipLaunchKernelGGL(k_tile_index, dim3(grid_for(nt + 1, BLOCK, 65535)), dim3(BLOCK), 0, c.stream, in.work_off,
-                         n, w0, w1, nt, in.tile_p);
+                         n, w0, w1, nt, in.tile_p, (unsigned int*)(ctr + 4), nullptr);
       ExpandArgs a{};
       a.rows = rows;
       a.work_off = in.work_off;
@@ -2078,7 +2278,6 @@ struct Matcher {
       a.xcd = (o.flags & 8u) ? 0 : 1;
       a.tilectr = (unsigned int*)(ctr + 4);
       a.stats = prof ? 1 : 0;
-      HIPCHECK(hipMemsetAsync(a.tilectr, 0, 8 * sizeof(unsigned int), c.stream));
       r.chunks++;
       const uint32_t grid = grid_for(nt, 1, grid_cap);
       pbegin(true);
@@ -2144,10 +2343,16 @@ struct Matcher {
       HIPCHECK(hipGetLastError());
       r.merge_bytes[10] += (double)read_u64(merge);
     }
+    // per row: its parent's block (start and length) and the candidate-free
+    // part of the chain closed form
+    ensure_aux(arena, in.pbase, in.pbase_cap, n);
+    ensure_aux(arena, in.coff, in.coff_cap, n + 1);
     pbegin(false);
-    hipLaunchKernelGGL(k_sib_len, dim3(grid), dim3(BLOCK), 0, c.stream, in.parent, par.nsurv, n, in.minlen);
+#define SBB(LL) hipLaunchKernelGGL((k_sib_base<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.parent, par.nsurv, par.coff, in.nsurv, ctab, in.minlen, in.pbase, in.coff)
+    CUTS_DISPATCH_PAIR(L, SBB)
+#undef SBB
     HIPCHECK(hipGetLastError());
-    pend(1, (double)n * 20.0, false);
+    pend(1, (double)n * (4.0 * L + 60.0), false);
     const uint64_t W = scan(in, n);
     if (W == 0 || add_work(W)) return;
     for (uint64_t w0 = 0; w0 < W; w0 += MAX_CHUNK) {
@@ -2156,11 +2361,11 @@ struct Matcher {
       const uint64_t nt = (w1 - w0 + TILE - 1) / TILE;
       ensure_tiles(arena, in, nt);
       hipLaunchKernelGGL(k_tile_index, dim3(grid_for(nt + 1, BLOCK, 65535)), dim3(BLOCK), 0, c.stream, in.work_off,
-                         n, w0, w1, nt, in.tile_p);
+                         n, w0, w1, nt, in.tile_p, (unsigned int*)(ctr + 4), nullptr);
       SibArgs a{};
       a.rows = rows;
-      a.parent = in.parent;
-      a.coff = par.coff;
+      a.pb = in.pbase;
+      a.rstart = in.coff;
       a.cand = par.cand;
       a.shared = in.nsurv;
       a.work_off = in.work_off;
@@ -2170,7 +2375,6 @@ struct Matcher {
       a.ntiles = nt;
       a.counters = ctr;
       a.tilectr = (unsigned int*)(ctr + 4);
-      HIPCHECK(hipMemsetAsync(a.tilectr, 0, 8 * sizeof(unsigned int), c.stream));
       a.xcd = (o.flags & 8u) ? 0 : 1;
       a.stats = prof ? 1 : 0;
       r.chunks++;
@@ -2209,6 +2413,8 @@ struct Matcher {
     if (o.flags & 512u) g.h2_did = nullptr;
     if (o.flags & 2048u) g.arc_cn = nullptr;  // CUTS_FLAG_NO_ARC_COMMON
     write = (o.flags & 4u) != 0;
+    // CUTS_FLAG_SYNC_WRITE: each chunk's rows copied before the next chunk runs
+    stream_emb = write && o.emb_rows && !(o.flags & 4096u);
     prof = (o.flags & 16u) != 0;
     closed_last = !write && !(o.flags & 1u) && kp.sc_ok && k >= 2;
     fuse = closed_last && !(o.flags & 2u) && k >= 3;
@@ -2342,8 +2548,14 @@ struct Matcher {
       HIPCHECK(hipStreamSynchronize(c.stream));
       r.last_root = (uint32_t)(c.host_pinned[2] & 0xFFFFFFFFu);
     }
+    if (emb_chunks) {
+      // the timed window ends once every embedding row is in the caller's buffer
+      HIPCHECK(hipStreamWaitEvent(c.stream, c.copy_ev[0], 0));
+      HIPCHECK(hipStreamWaitEvent(c.stream, c.copy_ev[1], 0));
+    }
     HIPCHECK(hipEventRecord(c.ev1, c.stream));
     HIPCHECK(hipEventSynchronize(c.ev1));
+    if (emb_chunks) HIPCHECK(hipStreamSynchronize(c.copy_stream));
     float ms = 0;
     HIPCHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
     r.device_ms = ms;
@@ -2530,7 +2742,7 @@ DeviceGraph::~DeviceGraph() {
 Context::~Context() {
   (void)hipSetDevice(device);
   for (auto& b : lv) {
-    for (void* p : {(void*)b.rows, (void*)b.minlen, (void*)b.minsel, (void*)b.work_off, (void*)b.bsum,
+    for (void* p : {(void*)b.rows, (void*)b.rows2, (void*)b.minlen, (void*)b.minsel, (void*)b.work_off, (void*)b.bsum,
                     (void*)b.tile_p, (void*)b.cbase, (void*)b.pbase,
                     (void*)b.nsurv, (void*)b.parent, (void*)b.coff, (void*)b.cand})
       if (p) (void)hipFree(p);
@@ -2541,6 +2753,9 @@ Context::~Context() {
   if (ev1) (void)hipEventDestroy(ev1);
   for (hipEvent_t e : kev)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : copy_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (copy_stream) (void)hipStreamDestroy(copy_stream);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -2558,6 +2773,9 @@ void context_init(Context& c, int device) {
   HIPCHECK(hipEventCreate(&c.ev1));
   HIPCHECK(hipEventCreate(&c.kev[0]));
   HIPCHECK(hipEventCreate(&c.kev[1]));
+  HIPCHECK(hipStreamCreateWithFlags(&c.copy_stream, hipStreamNonBlocking));
+  HIPCHECK(hipEventCreateWithFlags(&c.copy_ev[0], hipEventDisableTiming));
+  HIPCHECK(hipEventCreateWithFlags(&c.copy_ev[1], hipEventDisableTiming));
   HIPCHECK(hipMalloc(&c.counters, 16 * sizeof(unsigned long long)));
   HIPCHECK(hipHostMalloc(&c.host_pinned, 64, hipHostMallocDefault));
   size_t fr = 0, tot = 0;

@@ -122,7 +122,9 @@ enum {
   CUTS_FLAG_NO_CHAIN = 1u << 8,       /* not as a chain either (k-1 hanging on k-2) */
   CUTS_FLAG_NO_TWO_HOP = 1u << 9,     /* intersections by list walks even where the two-hop index exists */
   CUTS_FLAG_NO_SHARED = 1u << 10,     /* chain level: position k-3's candidates found per row, not per parent */
-  CUTS_FLAG_NO_ARC_COMMON = 1u << 11  /* ignore the per-arc common-neighbour table */
+  CUTS_FLAG_NO_ARC_COMMON = 1u << 11, /* ignore the per-arc common-neighbour table */
+  CUTS_FLAG_SYNC_WRITE = 1u << 12     /* CUTS_FLAG_WRITE: copy each chunk's rows before the next chunk runs
+                                         (default: copies stream on a second stream, overlapped) */
 };
 
 /* Kernel classes reported by CUTS_FLAG_PROFILE. */
@@ -212,6 +214,11 @@ int cuts_match(cuts_context *ctx, const cuts_device_graph *dg, const cuts_plan *
 int cuts_match_noalloc(cuts_context *ctx, const cuts_device_graph *dg, const cuts_plan *plan,
                        const cuts_match_options *opt, cuts_match_result *res);
 uint64_t cuts_noalloc_scratch_bytes(void);
+/* Pinned (page-locked) host memory (no reference counterpart): an emb_rows
+ * buffer allocated here receives CUTS_FLAG_WRITE rows by asynchronous copies
+ * that overlap the expansion; pageable memory works but copies block. */
+int cuts_host_alloc(uint64_t bytes, void **out);
+void cuts_host_free(void *p);
 
 /* Table log (no reference counterpart; the reference keeps the level sizes
  * in `lengths`, src/host_funcs.cu:95,153, and drops them).  Appends one JSON

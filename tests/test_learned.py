@@ -245,3 +245,49 @@ def test_learned_orders_count_the_same_beat_random_orders_and_never_lose_after_j
         assert j.work <= r.work
     print(f"judged: {sum(j.learned for j in judged)} of {len(test)} proposals kept; candidates "
           f"{sum(j.work for j in judged)} vs score order {sum(r.work for _, _, _, r in test)}", flush=True)
+
+
+@pytest.mark.gpu
+def test_config5_train_on_rmat18_evaluate_on_rmat20(ctx):
+    """Config 5 at test scale.  A network is trained with the HIP kernels on
+    tables the solver logs for a small admitted set of walk queries on an
+    R-MAT scale-18 graph; its engine-aware orders for held-out queries of the
+    R-MAT scale-20 fixture (kept or fallen back after the score order's
+    timed judgement) count exactly the fixture's score-order counts.  With
+    the committed weights, the learned order of the config-3 query is the
+    pair order and at least 10x faster than the score order, and over the
+    judged evaluation queries the learned orders take less total time."""
+    import json
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "rmat20_queries.json")))
+    g18 = W.rmat18_graph(1)
+    dg18 = ctx.upload(g18)
+    qs18, _ = W.admit(ctx, dg18, g18, 40, seed=101, kmin=5, kmax=8, max_work=1 << 22, max_tries_per_k=400)
+    assert len(qs18) >= 20
+    x, A, meta, t = LN.log_tables(ctx, dg18, LN.graph_features(g18), qs18, np.random.default_rng(0))
+    net = LN.Mpnn(ctx, seed=3)
+    hist = net.fit(x, A, meta, t, epochs=30, batch=512, lr=2e-3)
+    assert hist[-1] < 0.5 * hist[0]
+    del dg18
+    g = W.config3_graph()
+    assert W.csr_fingerprint(g) == fx["fingerprint"]
+    dg = ctx.upload(g)
+    info = dg.info()
+    eng = LN.Engine(two_hop=info["two_hop_entries"] > 0, arc_common=bool(info["arc_common"]))
+    items = [fx["config3"]] + fx["queries"][:60]
+    qs = [(q["k"], [tuple(e) for e in q["edges"]]) for q in items]
+    roots = [tuple(q["roots"]) for q in items]
+    want = [q["count"] for q in items]
+    heur = [cuts_amd.Plan(cuts_amd.Graph.from_edges(k, e)) for k, e in qs]
+    work = [q["work"] for q in items]
+    gf = LN.graph_features(g)
+    committed, _ = LN.load_weights(ctx, os.path.join(os.path.dirname(LN.__file__), "data", "mpnn_rmat18.npy"))
+    for model in (net, committed):
+        choices = LN.choose_orders(model, qs, gf, [p.order for p in heur], keep_first=True, eng=eng)
+        judged = LN.judge_by_time(ctx, dg, qs, heur, want, choices, roots=roots, heuristic_work=work)
+        assert any(not j.learned for j in judged)  # the fallback path is exercised
+        for j, rt, w in zip(judged, roots, want):
+            assert ctx.match(dg, j.plan, roots=rt).count == w
+    # the committed network on the config-3 query and the judged set
+    c3 = judged[0]
+    assert c3.learned and c3.ms * 10 < c3.heuristic_ms, (c3.ms, c3.heuristic_ms)
+    assert sum(j.ms for j in judged[1:]) < sum(j.heuristic_ms for j in judged[1:])

@@ -365,3 +365,61 @@ def test_pair_level_from_the_arc_table(ctx, seed):
         for flags in (0, F.FLAG_NO_TWO_HOP | F.FLAG_NO_BITMAP, F.FLAG_NO_PAIR):
             assert ctx.match(dg, p, flags=flags).count == want, (name, flags)
         assert ctx.match(dg, p, flags=F.FLAG_NO_TWO_HOP, budget=2048).count == want
+
+
+def _chain_row_base(N, deg, R, cu, ta, tb):
+    """The candidate-free part of the chain closed form for the older
+    positions R (k_sib_base in cuts_amd/csrc/match.hip)."""
+    u = R[cu]
+    cnt = lambda x, t: sum(1 for y in N[x] if deg[y] >= t)
+    inu = {l for l in range(len(R)) if l != cu and R[l] in N[u] and deg[R[l]] >= ta}
+    dold = {i for i in range(len(R)) if i != cu and deg[R[i]] >= tb}
+    na_old = cnt(u, ta) - len(inu)
+    c = sum(cnt(x, tb) for x in N[u] if deg[x] >= ta) - sum(cnt(R[l], tb) for l in inu)
+    dtb = deg[u] >= tb
+    if dtb:
+        c -= na_old
+    c += sum(1 for i in dold for j in inu if j != i and R[j] in N[R[i]])
+    c -= sum(len(N[u] & N[R[i]]) for i in dold)  # the shared intersections
+    return c, na_old, inu, dold, dtb
+
+
+def _chain_item(N, deg, R, cu, v, base, ta, tb):
+    """The candidate's part on top of the row base (chain_sib_item)."""
+    c, na_old, inu, dold, dtb = base
+    u = R[cu]
+    cnt = lambda x, t: sum(1 for y in N[x] if deg[y] >= t)
+    inv = deg[v] >= ta and v in N[u]
+    if inv:
+        c += -cnt(v, tb) + (1 if dtb else 0) + sum(1 for i in dold if v in N[R[i]])
+    if deg[v] >= tb:
+        c += sum(1 for j in inu if R[j] in N[v])
+        c -= len(N[u] & N[v])
+    return c
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_chain_row_base_plus_item_vs_brute_force(seed):
+    """CPU pin of the split the shared-candidate chain kernel evaluates: the
+    per-row base plus each candidate's part equals the brute-force
+    completions of the row + candidate (loop-free symmetric graphs, ta <= 2)."""
+    rng = random.Random(4700 + seed)
+    n = rng.randint(12, 40)
+    N = [set() for _ in range(n)]
+    for _ in range(rng.randint(n, 4 * n)):
+        a, b = rng.randrange(n), rng.randrange(n)
+        if a != b:
+            N[a].add(b)
+            N[b].add(a)
+    deg = [len(x) for x in N]
+    for _ in range(30):
+        L = rng.randint(2, 5)
+        R = rng.sample(range(n), L)
+        cu = rng.randrange(L)
+        for ta in (1, 2):
+            for tb in (1, 2, 3):
+                base = _chain_row_base(N, deg, R, cu, ta, tb)
+                for v in range(n):
+                    if v in R:
+                        continue
+                    assert _chain_item(N, deg, R, cu, v, base, ta, tb) == _chain_brute(N, deg, R + [v], cu, ta, tb)
