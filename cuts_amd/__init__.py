@@ -121,6 +121,8 @@ def lib():
             "cuts_match_options_default": (None, [C.POINTER(_Opts)]),
             "cuts_match": (C.c_int, [vp, vp, vp, C.POINTER(_Opts), C.POINTER(_Result)]),
             "cuts_match_noalloc": (C.c_int, [vp, vp, vp, C.POINTER(_Opts), C.POINTER(_Result)]),
+            "cuts_match_many": (C.c_int, [vp, vp, C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(_Opts), C.c_uint32,
+                                          C.c_uint32, C.POINTER(_Result), C.POINTER(C.c_double)]),
             "cuts_noalloc_scratch_bytes": (C.c_uint64, []),
             "cuts_host_alloc": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
             "cuts_host_free": (None, [C.c_void_p]),
@@ -370,6 +372,9 @@ class Context:
             _lib.cuts_context_destroy(self._h)
             self._h = None
 
+    def match_many(self, dg: "DeviceGraph", plans, roots=None, flags: int = 0, workers: int = 4, max_work: int = 0):
+        return match_many(self, dg, plans, roots=roots, flags=flags, workers=workers, max_work=max_work)
+
     def copy_probe(self, nbytes: int, iters: int = 10) -> float:
         """Read+write GB/s of a plain copy of nbytes (HBM yardstick)."""
         gbps = C.c_double()
@@ -427,6 +432,34 @@ class Context:
                                                float(r.kernel_bytes[i])) for i in range(_NCLASS)}
             res.merge_bytes = {KERNEL_CLASSES[i]: float(r.merge_bytes[i]) for i in range(_NCLASS)}
         return res
+
+
+def _result_of(r, k: int) -> "MatchResult":
+    res = MatchResult(int(r.count), [int(x) for x in r.level_sizes[:k]], float(r.device_ms), int(r.chunks),
+                      complete=bool(r.complete), roots_done=int(r.roots_done), last_root=int(r.last_root),
+                      work_capped=bool(r.work_capped), work=int(r.work))
+    res._raw = r
+    return res
+
+
+def match_many(ctx: "Context", dg: "DeviceGraph", plans: Sequence["Plan"], roots=None, flags: int = 0,
+               workers: int = 4, max_work: int = 0):
+    """Many queries against one resident graph on `workers` concurrent streams
+    (cuts_match_many).  roots: None or one (first, end, stride) per query.
+    Returns (results in query order, host wall ms of the whole set)."""
+    n = len(plans)
+    opts = (_Opts * max(n, 1))()
+    for i in range(n):
+        lib().cuts_match_options_default(C.byref(opts[i]))
+        opts[i].flags = flags
+        opts[i].max_work = max_work
+        if roots is not None and roots[i] is not None:
+            opts[i].root_begin, opts[i].root_end, opts[i].root_stride = roots[i]
+    res = (_Result * max(n, 1))()
+    arr = (C.c_void_p * max(n, 1))(*[p._h.value for p in plans])
+    wall = C.c_double()
+    _check(lib().cuts_match_many(ctx._h, dg._h, arr, n, opts, n, workers, res, C.byref(wall)))
+    return [_result_of(res[i], plans[i].k) for i in range(n)], float(wall.value)
 
 
 @dataclass

@@ -9,6 +9,11 @@
 #include <iostream>
 #include <memory>
 #include <sstream>
+#include <atomic>
+#include <chrono>
+#include <mutex>
+#include <thread>
+#include <vector>
 
 #include "batch.hpp"
 #include "mpnn.hpp"
@@ -21,6 +26,10 @@ struct cuts_plan {
 };
 struct cuts_context {
   cuts::Context c;
+  // cuts_match_many: one context (stream, trie arena, counters) per worker
+  // thread, created on first use and kept
+  std::vector<std::unique_ptr<cuts::Context>> workers;
+  std::mutex workers_mu;
 };
 struct cuts_device_graph {
   cuts::DeviceGraph dg;
@@ -533,6 +542,64 @@ int cuts_mpnn_train_step(cuts_mpnn* m, uint32_t n, const float* x, const uint16_
 }
 
 void cuts_mpnn_free(cuts_mpnn* m) { delete m; }
+
+int cuts_match_many(cuts_context* ctx, const cuts_device_graph* dg, const cuts_plan* const* plans, uint32_t n,
+                    const cuts_match_options* opts, uint32_t n_opts, uint32_t workers, cuts_match_result* res,
+                    double* wall_ms) {
+  return guard([&]() -> int {
+    if (!ctx || !dg || (n && (!plans || !res))) return fail(CUTS_E_ARG, "null argument");
+    if (n_opts != 0 && n_opts != 1 && n_opts != n) return fail(CUTS_E_ARG, "n_opts must be 0, 1 or n");
+    if (n_opts && !opts) return fail(CUTS_E_ARG, "null options");
+    for (uint32_t i = 0; i < n; ++i)
+      if (!plans[i]) return fail(CUTS_E_ARG, "null plan");
+    for (uint32_t i = 0; i < n_opts; ++i)
+      if ((opts[i].flags & CUTS_FLAG_WRITE) && opts[i].emb_cap && !opts[i].emb_rows)
+        return fail(CUTS_E_ARG, "CUTS_FLAG_WRITE needs emb_rows");
+    if (workers == 0) workers = 4;
+    workers = std::min<uint32_t>(workers, 16);
+    {
+      std::lock_guard<std::mutex> lk(ctx->workers_mu);
+      while (ctx->workers.size() < workers) {
+        auto w = std::make_unique<cuts::Context>();
+        cuts::context_init(*w, ctx->c.device);
+        ctx->workers.push_back(std::move(w));
+      }
+      // the trie tables of the workers share the main context's budget
+      for (auto& w : ctx->workers) w->default_budget = ctx->c.default_budget / workers;
+    }
+    std::atomic<uint32_t> next{0};
+    std::mutex err_mu;
+    int err_code = CUTS_OK;
+    std::string err_msg;
+    auto body = [&](cuts::Context* wc) {
+      for (;;) {
+        const uint32_t i = next.fetch_add(1);
+        if (i >= n) break;
+        try {
+          cuts::MatchResult r;
+          const cuts_match_options* o = n_opts == 0 ? nullptr : (n_opts == 1 ? opts : opts + i);
+          cuts::run_match(*wc, const_cast<cuts::DeviceGraph&>(dg->dg), plans[i]->p, to_opts(o), r);
+          fill_result(r, res + i);
+        } catch (const cuts::Error& e) {
+          std::lock_guard<std::mutex> lk(err_mu);
+          if (err_code == CUTS_OK) { err_code = e.code; err_msg = e.what(); }
+          next = n;  // stop handing out queries
+        } catch (const std::exception& e) {
+          std::lock_guard<std::mutex> lk(err_mu);
+          if (err_code == CUTS_OK) { err_code = CUTS_E_ARG; err_msg = e.what(); }
+          next = n;
+        }
+      }
+    };
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::thread> th;
+    for (uint32_t t = 0; t < workers; ++t) th.emplace_back(body, ctx->workers[t].get());
+    for (auto& t : th) t.join();
+    if (wall_ms) *wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (err_code != CUTS_OK) return fail(err_code, err_msg);
+    return CUTS_OK;
+  });
+}
 
 int cuts_match_noalloc(cuts_context* ctx, const cuts_device_graph* dg, const cuts_plan* plan,
                        const cuts_match_options* opt, cuts_match_result* res) {

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -90,6 +91,7 @@ struct DeviceGraph {
   // aliases the out-side on symmetric graphs): a candidate's signature is
   // read at its list index, coalesced with the list word itself
   uint2 *nsig_out = nullptr, *nsig_in = nullptr;
+  std::recursive_mutex tables_mu;          // guards the two lazily built table maps below
   std::map<uint32_t, uint32_t*> cnt_tables; // degree threshold -> #neighbours meeting it
   // (ta, tb) -> 2V uint64: per vertex u, sum over x in N(u) with deg(x) >= ta
   // of |{y in N(x) : deg(y) >= tb}|, then of deg(x)

@@ -2584,6 +2584,8 @@ void* ScratchArena::carve(uint64_t bytes) {
 // Degree-threshold table: cached on the device graph, or, with an arena,
 // built in the scratch for this call only (and not cached).
 const uint32_t* cnt_table(Context& c, DeviceGraph& dg, ScratchArena* arena, uint32_t thr) {
+  // concurrent matchers (cuts_match_many) share the device graph
+  std::lock_guard<std::recursive_mutex> lk(dg.tables_mu);
   auto it = dg.cnt_tables.find(thr);
   if (it != dg.cnt_tables.end()) return it->second;
   uint32_t* t = nullptr;
@@ -2598,6 +2600,7 @@ const uint32_t* cnt_table(Context& c, DeviceGraph& dg, ScratchArena* arena, uint
 const uint32_t* cnt_table(Context& c, DeviceGraph& dg, uint32_t thr) { return cnt_table(c, dg, nullptr, thr); }
 
 const uint64_t* chain_table(Context& c, DeviceGraph& dg, ScratchArena* arena, uint32_t ta, uint32_t tb) {
+  std::lock_guard<std::recursive_mutex> lk(dg.tables_mu);
   const auto key = std::make_pair(ta, tb);
   auto it = dg.chain_tables.find(key);
   if (it != dg.chain_tables.end()) return it->second;

@@ -199,6 +199,19 @@ void cuts_match_options_default(cuts_match_options *o);
 int cuts_match(cuts_context *ctx, const cuts_device_graph *dg, const cuts_plan *plan,
                const cuts_match_options *opt, cuts_match_result *res);
 
+/* Many queries against one resident data graph, run as single-query
+ * matches on `workers` host threads (0 = 4, at most 16), each with its own
+ * stream and trie arena (created inside the context on first use and kept;
+ * their trie budgets split the context's), so that the small kernels of
+ * different queries share the GPU.  Queries are handed out dynamically in
+ * order; res[i] is query i's result.  opts: n_opts = 0 (defaults), 1 (the
+ * same options for every query) or n.  *wall_ms (may be NULL) receives the
+ * host wall time of the whole set.  No reference counterpart: the reference
+ * runs one query per process (src/host_funcs.cu:115-171). */
+int cuts_match_many(cuts_context *ctx, const cuts_device_graph *dg, const cuts_plan *const *plans, uint32_t n,
+                    const cuts_match_options *opts, uint32_t n_opts, uint32_t workers, cuts_match_result *res,
+                    double *wall_ms);
+
 /* No reference counterpart: cuts_match for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  Same arguments, semantics and result;
