@@ -250,6 +250,14 @@ def run_batch256(args):
                         arena_bytes=BATCH_ARENA, queue_cap=BATCH_QUEUE, roots=roots)
     rh = [bh.run() for _ in range(2)][-1]
     assert rh.counts == counts
+    # the same batch through the persistent kernel (one cooperative launch)
+    bp = cuts_amd.Batch(ctx, dg, plans, blocks_per_cu=best["blocks_per_cu"], item_work=best["item_work"],
+                        arena_bytes=BATCH_ARENA, queue_cap=BATCH_QUEUE, roots=roots, queue="persistent")
+    rp = [bp.run() for _ in range(max(args.warmup, 1) + args.steps)][max(args.warmup, 1):]
+    assert all(r.counts == counts for r in rp), "persistent-kernel counts differ from single-query counts"
+    ms_p = statistics.median([r.device_ms for r in rp])
+    log(f"batch learned: level queue {statistics.median([r.device_ms for r in runs]):.1f} ms, "
+        f"persistent kernel {ms_p:.1f} ms")
     ms = statistics.median([r.device_ms for r in runs])
     lat = runs[-1].latency_ms
     cum = list(np.cumsum(s_ms))
@@ -261,6 +269,8 @@ def run_batch256(args):
                    "replayed as one hipGraph; learned orders (same first vertex); autotuned launch shape",
                    {"queries": len(qs), "sum_count": sum(counts),
                     "batch_learned_ms": ms, "batch_heuristic_ms": rh.device_ms,
+                    "batch_learned_persistent_kernel_ms": ms_p,
+                    "p50_latency_ms_batch_learned_persistent_kernel": statistics.median(rp[-1].latency_ms),
                     "amortised_ms_per_query_batch_learned": amort,
                     "p50_ms_single_heuristic": statistics.median(s_ms),
                     "p99_ms_single_heuristic": sorted(s_ms)[int(0.99 * len(s_ms))],
@@ -443,7 +453,8 @@ def run_headline(args):
                       "learned_proposed": proposed, "learned_kept": kept,
                       "fallback_rate": 1.0 - kept / len(items),
                       "order_planning_s": plan_s, "judging_s": judge_s,
-                      "engine": {"two_hop_index": eng.two_hop, "arc_common": eng.arc_common},
+                      "engine": {"two_hop_index": eng.two_hop, "arc_common": eng.arc_common,
+                                 "hubs": info["hubs"], "hub_min_deg": info["hub_min_deg"]},
                       "weights": {"file": os.path.relpath(WEIGHTS, ROOT), **{k: winfo.get(k) for k in
                                   ("graphs", "queries_admitted", "samples", "mse_first", "mse_last")}},
                       "wall_s_timed_loop": wall, "upload_s": upload_s,

@@ -75,7 +75,8 @@ class _DGInfo(C.Structure):
 class _BatchOpts(C.Structure):
     _fields_ = [("flags", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("vchunk", C.c_uint32),
                 ("no_graph", C.c_uint32), ("queue_cap", C.c_uint64), ("arena_bytes", C.c_uint64),
-                ("ticket_queue", C.c_uint32), ("item_work", C.c_uint32), ("roots", C.POINTER(C.c_uint32))]
+                ("ticket_queue", C.c_uint32), ("item_work", C.c_uint32), ("roots", C.POINTER(C.c_uint32)),
+                ("persistent", C.c_uint32), ("barrier_timeout_ms", C.c_uint32)]
 
 
 class _BatchResult(C.Structure):
@@ -473,24 +474,30 @@ class BatchResult:
 
 class Batch:
     """Many queries against one resident data graph: a level-synchronous
-    device work queue, one persistent-grid launch per trie level, the whole
-    sequence captured once in a hipGraph.  ``queue="ticket"`` selects the
-    asynchronous ticket-queue kernel, which is disabled (creation fails)."""
+    device work queue of (query, level, chunk) items, the launch sequence
+    captured once in a hipGraph.  ``queue="level"``: one persistent-grid
+    launch per trie level.  ``queue="persistent"``: one cooperative launch
+    runs every level, its blocks meeting at a device-wide barrier between
+    levels (a wait longer than ``barrier_timeout_ms`` aborts the run).
+    ``queue="ticket"`` selects the asynchronous ticket-queue kernel, which is
+    disabled (creation fails)."""
 
     def __init__(self, ctx: Context, dg: DeviceGraph, plans: Sequence[Plan], flags: int = 0,
                  blocks_per_cu: int = 0, vchunk: int = 0, use_graph: bool = True, queue_cap: int = 0,
                  arena_bytes: int = 0, item_work: int = 0, queue: str = "level",
-                 roots: Optional[Sequence[Tuple[int, int, int]]] = None):
+                 roots: Optional[Sequence[Tuple[int, int, int]]] = None, barrier_timeout_ms: int = 0):
         """roots: per query (first, end, stride) of the data vertices that
         order position 0 may take (as Context.match's roots); None = all."""
-        if queue not in ("level", "ticket"):
-            raise ValueError("queue must be 'level' or 'ticket'")
+        if queue not in ("level", "persistent", "ticket"):
+            raise ValueError("queue must be 'level', 'persistent' or 'ticket'")
         self.ctx, self.dg, self.plans = ctx, dg, list(plans)
         o = _BatchOpts()
         lib().cuts_batch_options_default(C.byref(o))
         o.flags, o.blocks_per_cu, o.vchunk, o.no_graph = flags, blocks_per_cu, vchunk, 0 if use_graph else 1
         o.queue_cap, o.arena_bytes, o.item_work = queue_cap, arena_bytes, item_work
         o.ticket_queue = 1 if queue == "ticket" else 0
+        o.persistent = 1 if queue == "persistent" else 0
+        o.barrier_timeout_ms = barrier_timeout_ms
         if roots is not None:
             if len(roots) != len(self.plans):
                 raise ValueError("roots: one (first, end, stride) per query")

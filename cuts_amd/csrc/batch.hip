@@ -612,40 +612,13 @@ __device__ void process(const BatchArgs& a, Shared<WROW>& s, uint32_t L) {
   }
 }
 
-}  // namespace
-
-// Resets the queue on the device (so the sequence replays inside a hipGraph)
-// and publishes the level-0 items (up to vchunk roots of one query each).
-__global__ __launch_bounds__(BB) void k_batch_init(BatchArgs a) {
-  const uint64_t tid = (uint64_t)blockIdx.x * BB + threadIdx.x;
-  const uint64_t nthreads = (uint64_t)gridDim.x * BB;
-  for (uint64_t i = tid; i < (uint64_t)a.nq * kMaxQuery; i += nthreads) a.levels[i] = 0;
-  for (uint64_t q = tid; q < a.nq; q += nthreads) {
-    a.counts[q] = 0;
-    a.qpend[q] = a.qinit[q];
-    a.done[q] = 0;
-  }
-  for (uint64_t i = tid; i < a.n_init; i += nthreads) a.items[i] = a.init_items[i];
-  if (tid == 0) {
-    for (uint32_t l = 0; l <= kMaxQuery; ++l) {
-      a.st->head[l].v = 0;
-      a.st->tail[l].v = 0;
-      a.st->arena[l].v = 0;
-    }
-    a.st->tail[0].v = a.n_init;
-    a.st->abort.v = 0;
-    a.st->t_start.v = wall_clock64();
-  }
-}
-
-// Processes every item of level L (published by the earlier launches).
-// At least 6 waves per SIMD: 80 VGPRs; with row widths 7 and 11 the LDS also
-// fits 6 blocks of 4 waves per CU.
+// Processes every item of level L: published by the earlier launches, or by
+// the earlier rounds of the persistent kernel (read after its barrier).
 template <int WROW>
-__global__ __launch_bounds__(BB, 6) void k_batch_level(BatchArgs a, uint32_t L) {
-  __shared__ Shared<WROW> s;
+__device__ __forceinline__ void level_items(const BatchArgs& a, Shared<WROW>& s, uint32_t L) {
   const bool l0 = lane_id() == 0;
-  const unsigned long long n_items = uni64(std::min<unsigned long long>(a.st->tail[L].v, a.qcap));
+  const unsigned long long n_items = uni64(std::min<unsigned long long>(
+      __hip_atomic_load(&a.st->tail[L].v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), a.qcap));
   s.nstage = 0;  // same value from every thread
   s.wstage = 0;
   uint32_t prev_q = 0xFFFFFFFFu;  // query of the item finished in the last round
@@ -689,6 +662,95 @@ __global__ __launch_bounds__(BB, 6) void k_batch_level(BatchArgs a, uint32_t L) 
   }
 }
 
+// Device-wide barrier between two levels of the persistent kernel (round r
+// = the level just finished).  Wave 0 of every block releases the block's
+// writes (agent-scope fence: the XCDs' L2 caches are written back), counts
+// itself in, and waits until all gridDim.x blocks of the round are in, then
+// acquires (the L2 lines other XCDs wrote are invalidated).  The wait is
+// bounded: an abort raised anywhere, or a wait longer than the run's limit
+// (which raises abort 8), ends it, and the barrier returns false in every
+// block, so every block leaves.  The arrival counter only grows (round r
+// waits for (r+1) x gridDim.x); k_batch_init clears it.  Every branch
+// around the wait loop is wave-uniform.
+template <int WROW>
+__device__ bool level_barrier(const BatchArgs& a, Shared<WROW>& s, uint32_t round) {
+  __syncthreads();
+  if (wave_id() == 0) {
+    const bool l0 = lane_id() == 0;
+    __threadfence();
+    atomicAdd(&a.st->bar.v, l0 ? 1ull : 0ull);
+    const unsigned long long target = (unsigned long long)(round + 1) * gridDim.x;
+    const unsigned long long t0 = uni64(wall_clock64());
+    uint32_t stop = 0;
+    while (true) {
+      if (uni64(__hip_atomic_load(&a.st->abort.v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0) {
+        stop = 1;
+        break;
+      }
+      if (uni64(__hip_atomic_load(&a.st->bar.v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >= target) break;
+      if (uni64(wall_clock64()) - t0 > a.bar_ticks) {
+        atomicOr(&a.st->abort.v, l0 ? 8ull : 0ull);
+        stop = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(4);
+    }
+    __threadfence();
+    s.cmd = stop;
+  }
+  __syncthreads();
+  return uni(s.cmd) == 0;
+}
+
+}  // namespace
+
+// Resets the queue on the device (so the sequence replays inside a hipGraph)
+// and publishes the level-0 items (up to vchunk roots of one query each).
+__global__ __launch_bounds__(BB) void k_batch_init(BatchArgs a) {
+  const uint64_t tid = (uint64_t)blockIdx.x * BB + threadIdx.x;
+  const uint64_t nthreads = (uint64_t)gridDim.x * BB;
+  for (uint64_t i = tid; i < (uint64_t)a.nq * kMaxQuery; i += nthreads) a.levels[i] = 0;
+  for (uint64_t q = tid; q < a.nq; q += nthreads) {
+    a.counts[q] = 0;
+    a.qpend[q] = a.qinit[q];
+    a.done[q] = 0;
+  }
+  for (uint64_t i = tid; i < a.n_init; i += nthreads) a.items[i] = a.init_items[i];
+  if (tid == 0) {
+    for (uint32_t l = 0; l <= kMaxQuery; ++l) {
+      a.st->head[l].v = 0;
+      a.st->tail[l].v = 0;
+      a.st->arena[l].v = 0;
+    }
+    a.st->tail[0].v = a.n_init;
+    a.st->abort.v = 0;
+    a.st->bar.v = 0;
+    a.st->t_start.v = wall_clock64();
+  }
+}
+
+// Processes every item of level L (published by the earlier launches).
+// At least 6 waves per SIMD: 80 VGPRs; with row widths 7 and 11 the LDS also
+// fits 6 blocks of 4 waves per CU.
+template <int WROW>
+__global__ __launch_bounds__(BB, 6) void k_batch_level(BatchArgs a, uint32_t L) {
+  __shared__ Shared<WROW> s;
+  level_items<WROW>(a, s, L);
+}
+
+// The persistent kernel: one cooperative launch (every block resident at
+// once) runs levels 0 .. kmax-1, pulling the (query, level, chunk) items of
+// each level from the device queue, with a device-wide barrier between
+// levels in place of the launch boundary.
+template <int WROW>
+__global__ __launch_bounds__(BB, 6) void k_batch_all(BatchArgs a, uint32_t kmax) {
+  __shared__ Shared<WROW> s;
+  for (uint32_t L = 0; L < kmax; ++L) {
+    if (L > 0 && !level_barrier<WROW>(a, s, L - 1)) break;
+    level_items<WROW>(a, s, L);
+  }
+}
+
 }  // namespace cuts
 
 // ------------------------------------------------------------------ host
@@ -728,7 +790,8 @@ void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& p
   b.nq = (uint32_t)plans.size();
   b.vchunk = o.vchunk ? o.vchunk : 4096;
   b.use_graph = o.use_graph != 0;
-  const uint32_t bpc = o.blocks_per_cu ? std::min<uint32_t>(o.blocks_per_cu, 8) : 6;
+  b.persistent = o.persistent != 0;
+  uint32_t bpc = o.blocks_per_cu ? std::min<uint32_t>(o.blocks_per_cu, 8) : 6;
   b.grid = (uint32_t)c.cus * bpc;
   const uint64_t qcap = o.queue_cap ? o.queue_cap : (1ull << 22);
   // position-0 roots per query (first, stride, count) and the level-0 items
@@ -828,20 +891,63 @@ void batch_create(Context& c, DeviceGraph& dg, const std::vector<const Plan*>& p
   int khz = 0;
   if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c.device) == hipSuccess && khz > 0)
     b.tick_khz = khz;
+  // 0xFFFFFFFF: no wait at all (the first barrier aborts the run; tests of the abort path)
+  a.bar_ticks = o.barrier_timeout_ms == 0xFFFFFFFFu
+                    ? 0ull
+                    : (unsigned long long)((o.barrier_timeout_ms ? o.barrier_timeout_ms : 60000u) * b.tick_khz);
+  if (b.persistent) {
+    // every block of a cooperative launch must be resident at once
+    int coop = 0, nb = 0;
+    HIPCHECK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, c.device));
+    if (!coop) throw Error(-1, "batch: the device does not support cooperative launches (persistent kernel)");
+    const void* f = b.kmax <= 8    ? (const void*)k_batch_all<7>
+                    : b.kmax <= 12 ? (const void*)k_batch_all<11>
+                                   : (const void*)k_batch_all<15>;
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, BB, 0));
+    if (nb <= 0) throw Error(-1, "batch: the persistent kernel does not fit a CU");
+    bpc = std::min<uint32_t>(bpc, (uint32_t)nb);
+    b.grid = (uint32_t)c.cus * bpc;
+  }
   a.item_work = o.item_work ? o.item_work : 16384;
   a.flags = (o.flags & 3u) | ((o.flags & 64u) ? 4u : 0u) | ((o.flags & 128u) ? 8u : 0u) | ((o.flags & 256u) ? 16u : 0u);
   a.nq = b.nq;
 }
 
-static void batch_launch(Batch& b, hipStream_t s) {
+// Enqueues init + the level launches (or init + the persistent kernel);
+// returns the first launch error.
+static hipError_t batch_launch(Batch& b, hipStream_t s) {
   const uint64_t nitems = std::max<uint64_t>(b.args.n_init, b.nq);
   const uint32_t igrid = (uint32_t)std::min<uint64_t>((nitems + BB - 1) / BB, 4096);
   hipLaunchKernelGGL(k_batch_init, dim3(std::max<uint32_t>(igrid, 1)), dim3(BB), 0, s, b.args);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (b.persistent) {
+    hipLaunchAttribute attr[1];
+    std::memset(attr, 0, sizeof(attr));
+    attr[0].id = hipLaunchAttributeCooperative;
+    attr[0].val.cooperative = 1;
+    hipLaunchConfig_t cfg;
+    std::memset(&cfg, 0, sizeof(cfg));
+    cfg.gridDim = dim3(b.grid);
+    cfg.blockDim = dim3(BB);
+    cfg.stream = s;
+    cfg.attrs = attr;
+    cfg.numAttrs = 1;
+    uint32_t kmax = b.kmax;
+    void* args[] = {(void*)&b.args, (void*)&kmax};
+    const void* f = b.kmax <= 8    ? (const void*)k_batch_all<7>
+                    : b.kmax <= 12 ? (const void*)k_batch_all<11>
+                                   : (const void*)k_batch_all<15>;
+    return hipLaunchKernelExC(&cfg, f, args);
+  }
   for (uint32_t L = 0; L < b.kmax; ++L) {
     if (b.kmax <= 8) hipLaunchKernelGGL(k_batch_level<7>, dim3(b.grid), dim3(BB), 0, s, b.args, L);
     else if (b.kmax <= 12) hipLaunchKernelGGL(k_batch_level<11>, dim3(b.grid), dim3(BB), 0, s, b.args, L);
     else hipLaunchKernelGGL(k_batch_level<15>, dim3(b.grid), dim3(BB), 0, s, b.args, L);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
   }
+  return hipSuccess;
 }
 
 void batch_run(Batch& b, std::vector<BatchQueryResult>& out, double& device_ms, uint64_t& arena_words) {
@@ -849,13 +955,21 @@ void batch_run(Batch& b, std::vector<BatchQueryResult>& out, double& device_ms, 
   HIPCHECK(hipSetDevice(c.device));
   if (b.use_graph && !b.exec) {
     HIPCHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-    batch_launch(b, c.stream);
-    HIPCHECK(hipStreamEndCapture(c.stream, &b.graph));
+    const hipError_t le = batch_launch(b, c.stream);
+    hipGraph_t gr = nullptr;
+    const hipError_t ce = hipStreamEndCapture(c.stream, &gr);  // always ends the capture
+    if (le != hipSuccess || ce != hipSuccess) {
+      if (gr) (void)hipGraphDestroy(gr);
+      (void)hipGetLastError();
+      throw Error(-4, std::string("batch: capturing the launch sequence failed: ") +
+                          hipGetErrorString(le != hipSuccess ? le : ce));
+    }
+    b.graph = gr;
     HIPCHECK(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
   }
   HIPCHECK(hipEventRecord(c.ev0, c.stream));
   if (b.use_graph) HIPCHECK(hipGraphLaunch(b.exec, c.stream));
-  else batch_launch(b, c.stream);
+  else HIPCHECK(batch_launch(b, c.stream));
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipEventRecord(c.ev1, c.stream));
   HIPCHECK(hipEventSynchronize(c.ev1));
@@ -865,7 +979,11 @@ void batch_run(Batch& b, std::vector<BatchQueryResult>& out, double& device_ms, 
   BState st;
   HIPCHECK(hipMemcpy(&st, b.args.st, sizeof(BState), hipMemcpyDeviceToHost));
   if (st.abort.v) {
-    const char* why = (st.abort.v & 4) ? "corrupt work item" : (st.abort.v & 1) ? "work queue full" : "row arena full";
+    const char* why = (st.abort.v & 4)   ? "corrupt work item"
+                      : (st.abort.v & 8) ? "a block waited at the level barrier longer than the limit"
+                      : (st.abort.v & 1) ? "work queue full"
+                                         : "row arena full";
+    if (st.abort.v & 8) throw Error(-4, std::string("batch aborted: ") + why);
     throw Error(-5, std::string("batch aborted: ") + why + "; run fewer queries per batch or raise the limits");
   }
   std::vector<unsigned long long> cnt(b.nq), lev((size_t)b.nq * kMaxQuery), done(b.nq);

@@ -21,8 +21,9 @@ struct BState {
   PadCtr head[kMaxQuery + 1];   // items of level L taken
   PadCtr tail[kMaxQuery + 1];   // items of level L published
   PadCtr arena[kMaxQuery + 1];  // arena words of level L allocated (in half L & 1)
-  PadCtr abort;                 // 1 item queue full, 2 arena full, 4 corrupt item
+  PadCtr abort;                 // 1 item queue full, 2 arena full, 4 corrupt item, 8 barrier wait too long
   PadCtr t_start;
+  PadCtr bar;                   // persistent kernel: barrier arrivals (grows by the grid size per level)
 };
 
 struct BatchArgs {
@@ -54,6 +55,7 @@ struct BatchArgs {
   uint32_t flags;               // 1: no closed form, 2: no fusion, 4: no per-prefix base, 8: no pair level,
                                 // 16: no chain level
   uint32_t nq;                  // queries in the batch
+  unsigned long long bar_ticks; // persistent kernel: longest barrier wait (wall-clock ticks) before it aborts
 };
 
 struct BatchOpts {
@@ -66,6 +68,8 @@ struct BatchOpts {
   uint32_t ticket_queue = 0;
   uint32_t item_work = 0;
   const uint32_t* roots = nullptr;  // null or 3 words per query: first, end, stride
+  uint32_t persistent = 0;          // 1: one cooperative launch for every level
+  uint32_t barrier_timeout_ms = 0;  // persistent: longest barrier wait; 0 = 60000
 };
 
 struct BatchQueryResult {
@@ -84,6 +88,7 @@ struct Batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool use_graph = true;
+  bool persistent = false;
   double tick_khz = 100000.0;
   ~Batch();
 };

@@ -457,6 +457,8 @@ int cuts_batch_create(cuts_context* ctx, const cuts_device_graph* dg, const cuts
       o.ticket_queue = opt->ticket_queue;
       o.item_work = opt->item_work;
       o.roots = opt->roots;
+      o.persistent = opt->persistent;
+      o.barrier_timeout_ms = opt->barrier_timeout_ms;
     }
     auto b = std::make_unique<cuts_batch>();
     cuts::batch_create(ctx->c, const_cast<cuts::DeviceGraph&>(dg->dg), ps, o, b->b);

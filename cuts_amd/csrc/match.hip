@@ -84,6 +84,15 @@ constexpr int BLOCK = 256;  // 4 wave64s
 constexpr int IPT = 4;      // work items per thread per tile
 constexpr int TILE = BLOCK * IPT;
 constexpr int LDS_OFF = 2048;      // staged work offsets per tile
+// Items in lock-step per thread in the fused last-two-levels expansion, for
+// rows of up to 6 and of 7..10 words (more items keep more independent loads
+// in flight, at the cost of registers).
+#ifndef FUSED_G_SMALL
+#define FUSED_G_SMALL 2
+#endif
+#ifndef FUSED_G_LARGE
+#define FUSED_G_LARGE 1
+#endif
 constexpr int SCAN_IPT = 8;
 constexpr int SCAN_TILE = BLOCK * SCAN_IPT;
 constexpr uint64_t MAX_CHUNK = 1ull << 31;  // work items per launch sequence
@@ -795,8 +804,9 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
       // independent and in flight together instead of one dependent chain
       // after another.  Items past w1 are clamped to the tile's last prefix
       // and masked; their list word is never read.
-      constexpr int G = (MODE != M_COUNT && MODE != M_GWRITE) ? ((L <= 6) ? 2 : 1)
-                                                                                  : ((L <= 6) ? 4 : (L <= 10 ? 2 : 1));
+      constexpr int G = (MODE == M_FUSED) ? ((L <= 6) ? FUSED_G_SMALL : (L <= 10 ? FUSED_G_LARGE : 1))
+                        : (MODE != M_COUNT && MODE != M_GWRITE) ? ((L <= 6) ? 2 : 1)
+                                                                : ((L <= 6) ? 4 : (L <= 10 ? 2 : 1));
       for (int g0 = 0; g0 < IPT; g0 += G) {
         uint64_t w[G];
         uint32_t pi[G];
@@ -2831,7 +2841,10 @@ void upload_graph(Context& c, const Graph& g, DeviceGraph& dg) {
   if (g.symmetric && g.V >= 64) {
     size_t fr = 0, tot = 0;
     HIPCHECK(hipMemGetInfo(&fr, &tot));
-    const uint64_t cap = std::min<uint64_t>(fr / 32, 4ull << 30);
+    uint64_t cap = std::min<uint64_t>(fr / 32, 4ull << 30);
+    // CUTS_HUB_BITMAP_MB replaces the cap (at most half of free HBM)
+    if (const char* e = std::getenv("CUTS_HUB_BITMAP_MB"))
+      cap = std::min<uint64_t>((uint64_t)std::strtoull(e, nullptr, 10) << 20, fr / 2);
     const uint32_t words = (g.V + 63) / 64;
     std::vector<uint32_t> deg(g.V);
     for (uint32_t v = 0; v < g.V; ++v) deg[v] = g.out_off[v + 1] - g.out_off[v];

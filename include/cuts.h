@@ -243,12 +243,15 @@ int cuts_log_tables(const char *path, const char *label, const cuts_plan *plan, 
 
 /* ---- batched matching (no reference counterpart) --------------------- */
 /* Many queries against one resident data graph.  A level-synchronous device
- * work queue of (query, level, row chunk) items: one persistent-grid launch
- * per trie level processes the items of every query at that level and
- * publishes the next level's items; the whole sequence is captured once in
- * a hipGraph and replayed.  No block ever waits for another.  The reference
- * runs one query per process with a host sync per level
- * (src/host_funcs.cu:141-157). */
+ * work queue of (query, level, row chunk) items: by default one
+ * persistent-grid launch per trie level processes the items of every query
+ * at that level and publishes the next level's items, and no block ever
+ * waits for another; with `persistent`, one cooperative launch runs every
+ * level, its blocks meeting at a bounded device-wide barrier between levels.
+ * The launch sequence is captured once in a hipGraph and replayed.  The
+ * reference runs one query per process with a host sync per level
+ * (src/host_funcs.cu:141-157) and spreads level-0 trunks over MPI ranks
+ * (main_mpi.cu:257-376). */
 typedef struct cuts_batch cuts_batch;
 typedef struct {
   uint32_t flags;         /* CUTS_FLAG_NO_SHORTCUT / CUTS_FLAG_NO_FUSE / CUTS_FLAG_NO_BITMAP */
@@ -262,6 +265,15 @@ typedef struct {
   const uint32_t *roots;  /* NULL, or 3 words per query: first, end, stride of the data
                              vertices order position 0 may take (cuts_match_options'
                              root_begin/root_end/root_stride, per query) */
+  uint32_t persistent;    /* 1: one cooperative launch runs every level (the persistent
+                             kernel); blocks meet at a device-wide barrier between levels
+                             instead of a launch boundary.  blocks_per_cu is clamped to
+                             what one CU holds at once. */
+  uint32_t barrier_timeout_ms; /* persistent: a block that waits longer than this at a level
+                                  barrier aborts the run; every block then leaves, the run
+                                  fails with CUTS_E_HIP and the context stays usable;
+                                  0 = 60000; 0xFFFFFFFF = no wait (the first barrier
+                                  aborts: for tests of that path) */
 } cuts_batch_options;
 typedef struct {
   uint64_t count;

@@ -111,6 +111,73 @@ def test_expand_tile_loops_are_wave_uniform(tmp_path):
         check_block_loop(function_body(asm, name), f"k_chain_sib<{L}>")
 
 
+def barrier_loops(lines):
+    """{header label: (depth, first branch, first exec write before it)} of
+    every loop whose body (inner loops included) holds a block barrier, from
+    the loop-nest annotations LLVM writes next to block labels ("Parent
+    Loop", "Loop Header", "in Loop: Header")."""
+    bl, cur = [], None
+    for ln in lines:
+        m = re.match(r"^(\.L(BB\w+)|; %bb\.\d+):(.*)$", ln)
+        if m:
+            cur = {"label": m.group(2) or m.group(1), "notes": [m.group(3)], "ins": []}
+            bl.append(cur)
+        elif cur is not None:
+            st = ln.strip()
+            if st.startswith(";"):
+                if not cur["ins"]:
+                    cur["notes"].append(st)
+            elif st and not st.startswith("."):
+                cur["ins"].append(st)
+    for b in bl:
+        chain, own = [], None
+        for n in b["notes"]:
+            chain += re.findall(r"Parent Loop (BB\w+) Depth=\d+", n)
+            if re.search(r"This (?:Inner )?Loop Header: Depth=\d+", n):
+                own = b["label"]
+            h = re.search(r"in Loop: Header=(BB\w+) Depth=\d+", n)
+            if h:
+                own = h.group(1)
+        b["chain"] = chain + ([own] if own and own not in chain else [])
+    heads = {h for b in bl if any(i.startswith("s_barrier") for i in b["ins"]) for h in b["chain"]}
+    out = {}
+    for h in heads:
+        hb = next(b for b in bl if b["label"] == h)
+        br = ex = None
+        for ins in hb["ins"]:
+            if ins.startswith(("s_cbranch", "s_branch")):
+                br = ins
+                break
+            if "exec" in ins and ex is None:
+                ex = ins
+        out[h] = (len(hb["chain"]), br, ex)
+    return out
+
+
+@pytest.mark.parametrize("width", [7, 11, 15])
+def test_persistent_batch_loops_are_wave_uniform(tmp_path, width):
+    """The persistent kernel's level loop and item loop (the two outer loops
+    around its barriers, the device-wide barrier's wait included) branch on
+    scalar conditions before touching the exec mask, its first barrier sits
+    in the level loop, and the device-wide barrier's fences write back (release)
+    and invalidate (acquire) the XCDs' L2 caches."""
+    asm = assembly(os.path.join(CSRC, "batch.hip"), tmp_path)
+    name = f"_ZN4cuts11k_batch_allILi{width}EEEvNS_9BatchArgsEj"
+    lines = function_body(asm, name)
+    loops = barrier_loops(lines)
+    for depth in (1, 2):
+        outer = [(h, v) for h, v in loops.items() if v[0] == depth]
+        assert len(outer) == 1, f"k_batch_all<{width}>: barrier loops at depth {depth}: {outer}"
+        h, (_, br, ex) = outer[0]
+        assert br is not None and br.startswith(("s_cbranch_scc", "s_cbranch_vcc", "s_branch")), (h, br)
+        assert ex is None, f"k_batch_all<{width}>: loop header {h} writes exec before branching ({ex})"
+    first = next(b for b in blocks(lines) for ins in b[3] if ins.startswith("s_barrier"))
+    assert first[1] == 1, f"k_batch_all<{width}>: first barrier at loop depth {first[1]}"
+    body = [ln.strip() for ln in lines]
+    assert any(ln.startswith("buffer_wbl2 sc1") for ln in body)
+    assert any(ln.startswith("buffer_inv sc1") for ln in body)
+
+
 @pytest.mark.parametrize("src", ["batch.hip", "match.hip"])
 def test_no_sign_extended_wave_broadcasts(tmp_path, src):
     """`__builtin_amdgcn_readfirstlane` returns int; a 64-bit broadcast that
