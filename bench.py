@@ -235,10 +235,11 @@ def run_batch256(args):
     log(f"learned proposals {proposed}, kept by the judge {kept}; candidates {work_l} vs score order {work_h}")
     key = f"batch256:{W.csr_fingerprint(g)}:learned"
     tb = autotune.tune_batch(ctx, dg, plans, key, use_cache=False, log=log, arena_bytes=BATCH_ARENA,
+                             queues=("level", "persistent"),
                              queue_cap=BATCH_QUEUE, roots=roots)
     best = tb["best"]
     b = cuts_amd.Batch(ctx, dg, plans, blocks_per_cu=best["blocks_per_cu"], item_work=best["item_work"],
-                       arena_bytes=BATCH_ARENA, queue_cap=BATCH_QUEUE, roots=roots)
+                       arena_bytes=BATCH_ARENA, queue_cap=BATCH_QUEUE, roots=roots, queue=best["queue"])
     runs = []
     for _ in range(args.warmup):
         b.run()
@@ -329,6 +330,26 @@ def load_fixture(name):
 
 def median(xs):
     return statistics.median(xs) if xs else 0.0
+
+
+def concurrent_fields(ctx, dg, learned, heur, roots, want, emb_per_step, workers=4, reps=3):
+    """The same step as a throughput job: the queries handed out to
+    `workers` host threads, each running whole single-query matches on its
+    own stream (cuts_match_many); host wall time of the whole set, best of
+    `reps`, counts checked."""
+    out = {"workers": workers}
+    for name, plans in (("learned", learned), ("heuristic", heur)):
+        best = None
+        for _ in range(reps):
+            res, wall = ctx.match_many(dg, plans, roots=roots, workers=workers)
+            if [r.count for r in res] != want:
+                raise AssertionError(f"match_many ({name}): counts differ from the fixture")
+            best = wall if best is None else min(best, wall)
+        out[f"wall_ms_per_step_{name}"] = best
+        out[f"embeddings_per_s_{name}"] = emb_per_step / (best / 1e3)
+    log(f"concurrent ({workers} streams): learned {out['wall_ms_per_step_learned']:.1f} ms, "
+        f"score {out['wall_ms_per_step_heuristic']:.1f} ms per step")
+    return out
 
 
 def run_headline(args):
@@ -461,6 +482,7 @@ def run_headline(args):
                       "arc_common_build_s": info["arc_common_build_s"]})
     line["roofline"] = roofline(kern, "rmat20")
     line["kernels_ms_profile_pass"] = {k: v[0] for k, v in kern.items() if v[1]}
+    line["config"]["concurrent"] = concurrent_fields(ctx, dg, learned, heur, roots, want, emb / args.steps)
     if args.copy_probe:
         line["copy_probe_GBs"] = ctx.copy_probe(1 << 30, args.copy_probe)
     if not args.no_config2:

@@ -53,7 +53,7 @@ class _Opts(C.Structure):
     _fields_ = [("trunks", C.c_uint32), ("flags", C.c_uint32), ("budget_bytes", C.c_uint64),
                 ("root_begin", C.c_uint32), ("root_end", C.c_uint32), ("root_stride", C.c_uint32),
                 ("blocks_per_cu", C.c_uint32), ("emb_rows", _u32p), ("emb_cap", C.c_uint64),
-                ("time_limit_ms", C.c_double), ("max_work", C.c_uint64)]
+                ("time_limit_ms", C.c_double), ("max_work", C.c_uint64), ("expand_threads", C.c_uint32)]
 
 
 class _Result(C.Structure):
@@ -390,7 +390,8 @@ class Context:
     def match(self, dg: DeviceGraph, plan: Plan, trunks: int = 0, flags: int = 0, budget: int = 0,
               roots: Optional[tuple] = None, write: bool = False, emb_cap: int = 0,
               blocks_per_cu: int = 0, time_limit_ms: float = 0.0, max_work: int = 0,
-              noalloc: bool = False, rows_out: Optional[np.ndarray] = None) -> MatchResult:
+              noalloc: bool = False, rows_out: Optional[np.ndarray] = None,
+              expand_threads: int = 0) -> MatchResult:
         """One query (cuts_match).  noalloc=True calls cuts_match_noalloc: the
         run's buffers come from the device's static scratch array.  With
         write=True the rows go to rows_out (C-contiguous uint32, (cap, k);
@@ -401,6 +402,7 @@ class Context:
         o.trunks, o.flags, o.budget_bytes, o.blocks_per_cu = trunks, flags, budget, blocks_per_cu
         o.time_limit_ms = time_limit_ms
         o.max_work = max_work
+        o.expand_threads = expand_threads
         if roots is not None:
             o.root_begin, o.root_end, o.root_stride = roots
         rows = None

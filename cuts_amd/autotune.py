@@ -7,10 +7,19 @@ consistency check.  Results are cached per (device, graph fingerprint,
 workload) in ``$CUTS_TUNE_CACHE`` (default ``~/.cache/cuts_amd/tune.json``).
 
 Tunables:
-  single query -- blocks per CU of the expansion kernels (grid = CUs x n) and
-                  the XCD-contiguous tile mapping;
-  batch        -- blocks per CU of the level launches (2..8) and the candidate
-                  work at which a new work item is started.
+  single query -- blocks per CU of the expansion kernels (grid = CUs x n),
+                  the XCD-contiguous tile mapping, and the block size of the
+                  counting expansions (64, 128 or 256 threads; a tile stays
+                  1,024 work items, so the work items each lane carries --
+                  16, 8 or 4 -- change with it).  The reference's sub-wave
+                  groups (8-lane virtual warps sharing one intersection,
+                  src/device_funcs.cu:618-733) have no counterpart to tune:
+                  every lane here owns whole candidates of the flattened
+                  work list, and items per lane is the width that varies;
+  batch        -- blocks per CU of the level launches (2..8), the candidate
+                  work at which a new work item is started (the chunk size),
+                  the level-0 chunk (roots per initial item) and the queue
+                  (one launch per level, or the persistent kernel).
 """
 from __future__ import annotations
 
@@ -47,34 +56,36 @@ class CountMismatch(AssertionError):
 
 
 def tune_single(ctx: Context, dg: DeviceGraph, plans: Sequence[Plan], key: str, reps: int = 2,
-                grid: Sequence[int] = (2, 4, 8, 16), xcd: Sequence[bool] = (True, False), log=None,
-                use_cache: bool = True) -> Dict:
+                grid: Sequence[int] = (2, 4, 8, 16), xcd: Sequence[bool] = (True, False),
+                threads: Sequence[int] = (256, 128, 64), roots=None, log=None, use_cache: bool = True) -> Dict:
     cache = _load()
     if use_cache and key in cache:
         return cache[key]
+    roots = list(roots) if roots is not None else [None] * len(plans)
     points: List[Dict] = []
     ref_counts = None
     for bpc in grid:
         for x in xcd:
-            flags = 0 if x else FLAG_NO_XCD
-            ms = []
-            counts = []
-            for _ in range(reps):
-                t = 0.0
+            for nt in threads:
+                flags = 0 if x else FLAG_NO_XCD
+                ms = []
                 counts = []
-                for p in plans:
-                    r = ctx.match(dg, p, flags=flags, blocks_per_cu=bpc)
-                    t += r.device_ms
-                    counts.append(r.count)
-                ms.append(t)
-            if ref_counts is None:
-                ref_counts = counts
-            elif counts != ref_counts:
-                raise CountMismatch(f"counts differ at blocks_per_cu={bpc} xcd={x}")
-            pt = dict(blocks_per_cu=bpc, xcd=x, ms=min(ms))
-            points.append(pt)
-            if log:
-                log(f"tune single: {pt}")
+                for _ in range(reps):
+                    t = 0.0
+                    counts = []
+                    for p, rt in zip(plans, roots):
+                        r = ctx.match(dg, p, flags=flags, blocks_per_cu=bpc, expand_threads=nt, roots=rt)
+                        t += r.device_ms
+                        counts.append(r.count)
+                    ms.append(t)
+                if ref_counts is None:
+                    ref_counts = counts
+                elif counts != ref_counts:
+                    raise CountMismatch(f"counts differ at blocks_per_cu={bpc} xcd={x} threads={nt}")
+                pt = dict(blocks_per_cu=bpc, xcd=x, threads=nt, ms=min(ms))
+                points.append(pt)
+                if log:
+                    log(f"tune single: {pt}")
     best = min(points, key=lambda p: p["ms"])
     res = dict(best=best, points=points, counts=ref_counts)
     cache[key] = res
@@ -84,29 +95,34 @@ def tune_single(ctx: Context, dg: DeviceGraph, plans: Sequence[Plan], key: str, 
 
 def tune_batch(ctx: Context, dg: DeviceGraph, plans: Sequence[Plan], key: str, reps: int = 2,
                blocks: Sequence[int] = (2, 4, 6, 8), item_works: Sequence[int] = (4096, 16384, 65536),
+               vchunks: Sequence[int] = (0,), queues: Sequence[str] = ("level",),
                arena_bytes: int = 32 << 30, queue_cap: int = 0, roots=None, log=None,
                use_cache: bool = True) -> Dict:
+    """vchunk 0 is the default level-0 chunk (4,096 roots per item)."""
     cache = _load()
     if use_cache and key in cache:
         return cache[key]
     points: List[Dict] = []
     ref = None
-    for bpc in blocks:
-        for iw in item_works:
-            b = Batch(ctx, dg, plans, blocks_per_cu=bpc, item_work=iw, arena_bytes=arena_bytes, queue_cap=queue_cap,
-                      roots=roots)
-            runs = [b.run() for _ in range(reps)]
-            for r in runs:
-                if ref is None:
-                    ref = r.counts
-                elif r.counts != ref:
-                    raise CountMismatch(f"batch counts differ at blocks_per_cu={bpc} item_work={iw}")
-            pt = dict(blocks_per_cu=bpc, item_work=iw, ms=min(r.device_ms for r in runs),
-                      p50_latency_ms=statistics.median(runs[-1].latency_ms))
-            points.append(pt)
-            if log:
-                log(f"tune batch: {pt}")
-            del b
+    for queue in queues:
+        for bpc in blocks:
+            for iw in item_works:
+                for vc in vchunks:
+                    b = Batch(ctx, dg, plans, blocks_per_cu=bpc, item_work=iw, vchunk=vc, arena_bytes=arena_bytes,
+                              queue_cap=queue_cap, roots=roots, queue=queue)
+                    runs = [b.run() for _ in range(reps)]
+                    for r in runs:
+                        if ref is None:
+                            ref = r.counts
+                        elif r.counts != ref:
+                            raise CountMismatch(f"batch counts differ at queue={queue} blocks_per_cu={bpc} "
+                                                f"item_work={iw} vchunk={vc}")
+                    pt = dict(queue=queue, blocks_per_cu=bpc, item_work=iw, vchunk=vc,
+                              ms=min(r.device_ms for r in runs), p50_latency_ms=statistics.median(runs[-1].latency_ms))
+                    points.append(pt)
+                    if log:
+                        log(f"tune batch: {pt}")
+                    del b
     best = min(points, key=lambda p: p["ms"])
     res = dict(best=best, points=points)
     cache[key] = res

@@ -122,6 +122,7 @@ cuts::MatchOpts to_opts(const cuts_match_options* o) {
   m.emb_cap = o->emb_cap;
   m.time_limit_ms = o->time_limit_ms;
   m.max_work = o->max_work;
+  m.expand_threads = o->expand_threads;
   return m;
 }
 

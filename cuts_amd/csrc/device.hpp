@@ -194,6 +194,7 @@ struct MatchOpts {
   uint64_t emb_cap = 0;
   double time_limit_ms = 0;
   uint64_t max_work = 0;
+  uint32_t expand_threads = 0;
 };
 struct MatchResult {
   uint64_t count = 0;

@@ -88,10 +88,10 @@ constexpr int LDS_OFF = 2048;      // staged work offsets per tile
 // rows of up to 6 and of 7..10 words (more items keep more independent loads
 // in flight, at the cost of registers).
 #ifndef FUSED_G_SMALL
-#define FUSED_G_SMALL 2
+#define FUSED_G_SMALL 4
 #endif
 #ifndef FUSED_G_LARGE
-#define FUSED_G_LARGE 1
+#define FUSED_G_LARGE 2
 #endif
 constexpr int SCAN_IPT = 8;
 constexpr int SCAN_TILE = BLOCK * SCAN_IPT;
@@ -753,6 +753,10 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
   __shared__ uint64_t s_off[LDS_OFF + 1];
   __shared__ uint32_t s_w[BLOCK / 64], s_pre[BLOCK / 64], s_base;
   const uint32_t tid = threadIdx.x, wave = tid >> 6;
+  // threads per block: BLOCK for the writing expansion; 64, 128 or 256 for
+  // the counting ones (a tile is TILE items either way, ipt per lane)
+  const uint32_t NT = MODE == M_WRITE ? (uint32_t)BLOCK : (uint32_t)blockDim.x;
+  const uint32_t ipt = TILE / NT;
   const uint32_t nj = P.nj[L];
   const uint32_t tin = P.sig_in[L], tout = P.sig_out[L];
   uint64_t acc = 0, valid = 0, pairs = 0, trips = 0, bytes = 0, mb = 0;
@@ -795,7 +799,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
     const uint32_t span = phi - plo + 1;  // offsets plo .. phi+1
     const bool lds = span <= LDS_OFF;
     if (lds)
-      for (uint32_t i = tid; i <= span; i += BLOCK) s_off[i] = a.work_off[plo + i];
+      for (uint32_t i = tid; i <= span; i += NT) s_off[i] = a.work_off[plo + i];
     __syncthreads();
     if constexpr (MODE != M_WRITE) {
       // Counting modes: each thread's items are processed in lock-step
@@ -807,13 +811,13 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
       constexpr int G = (MODE == M_FUSED) ? ((L <= 6) ? FUSED_G_SMALL : (L <= 10 ? FUSED_G_LARGE : 1))
                         : (MODE != M_COUNT && MODE != M_GWRITE) ? ((L <= 6) ? 2 : 1)
                                                                 : ((L <= 6) ? 4 : (L <= 10 ? 2 : 1));
-      for (int g0 = 0; g0 < IPT; g0 += G) {
+      for (uint32_t g0 = 0; g0 < ipt; g0 += G) {
         uint64_t w[G];
         uint32_t pi[G];
         bool in[G];
 #pragma unroll
         for (int j = 0; j < G; ++j) {
-          w[j] = w0 + (uint64_t)(g0 + j) * BLOCK + tid;
+          w[j] = w0 + (uint64_t)(g0 + j) * NT + tid;
           in[j] = w[j] < w1;
           pi[j] = 0;
         }
@@ -2083,6 +2087,10 @@ struct Matcher {
       a.tilectr = (unsigned int*)(ctr + 4);  // 8 x uint32 in counters[4..7], cleared by k_tile_index
       a.stats = prof ? 1 : 0;
       uint32_t grid = grid_for(nt, 1, grid_cap);
+      // counting expansions: nthr threads per block, cgrow times the blocks
+      // (as many lanes), at most one block per tile
+      const uint32_t nthr = o.expand_threads ? o.expand_threads : (uint32_t)BLOCK;
+      const uint32_t cgrow = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(BLOCK / nthr, nt / std::max<uint64_t>(grid, 1)));
       r.chunks++;
       if (mode == M_WRITE) {
         const bool streamed = L == k - 1 && stream_emb;
@@ -2125,35 +2133,35 @@ struct Matcher {
         }
       } else if (mode == M_COUNT) {
         pbegin(true);
-#define EC(LL) hipLaunchKernelGGL((k_expand<LL, M_COUNT>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, a)
+#define EC(LL) hipLaunchKernelGGL((k_expand<LL, M_COUNT>), dim3(grid * cgrow), dim3(nthr), 0, c.stream, g, kp, a)
         CUTS_DISPATCH_L(L, EC)
 #undef EC
         HIPCHECK(hipGetLastError());
         pend(5, 0, true);
       } else if (mode == M_COLLAPSE) {
         pbegin(true);
-#define EK(LL) hipLaunchKernelGGL((k_expand<LL, M_COLLAPSE>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, a)
+#define EK(LL) hipLaunchKernelGGL((k_expand<LL, M_COLLAPSE>), dim3(grid * cgrow), dim3(nthr), 0, c.stream, g, kp, a)
         CUTS_DISPATCH_PAIR(L, EK)
 #undef EK
         HIPCHECK(hipGetLastError());
         pend(8, 0, true);
       } else if (mode == M_CHAIN) {
         pbegin(true);
-#define EC2(LL) hipLaunchKernelGGL((k_expand<LL, M_CHAIN>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, a)
+#define EC2(LL) hipLaunchKernelGGL((k_expand<LL, M_CHAIN>), dim3(grid * cgrow), dim3(nthr), 0, c.stream, g, kp, a)
         CUTS_DISPATCH_PAIR(L, EC2)
 #undef EC2
         HIPCHECK(hipGetLastError());
         pend(10, 0, true);
       } else if (mode == M_FUSED) {
         pbegin(true);
-#define EF(LL) hipLaunchKernelGGL((k_expand<LL, M_FUSED>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, a)
+#define EF(LL) hipLaunchKernelGGL((k_expand<LL, M_FUSED>), dim3(grid * cgrow), dim3(nthr), 0, c.stream, g, kp, a)
         CUTS_DISPATCH_L(L, EF)
 #undef EF
         HIPCHECK(hipGetLastError());
         pend(6, 0, true);
       } else {
         pbegin(true);
-#define EP(LL) hipLaunchKernelGGL((k_expand<LL, M_PAIR>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, a)
+#define EP(LL) hipLaunchKernelGGL((k_expand<LL, M_PAIR>), dim3(grid * cgrow), dim3(nthr), 0, c.stream, g, kp, a)
         CUTS_DISPATCH_PAIR(L, EP)
 #undef EP
         HIPCHECK(hipGetLastError());
@@ -2432,6 +2440,8 @@ struct Matcher {
     chain = fuse && !pair && kp.ch_ok && !(o.flags & 256u);
     // 6 blocks of 4 waves per CU: measured best on the power-law workloads
     // (the tiles are taken dynamically, so the grid only sets residency)
+    if (o.expand_threads != 0 && o.expand_threads != 64 && o.expand_threads != 128 && o.expand_threads != 256)
+      throw Error(-1, "expand_threads must be 64, 128 or 256 (or 0)");
     grid_cap = (uint32_t)c.cus * (o.blocks_per_cu ? o.blocks_per_cu : 6);
     budget = o.budget ? o.budget : c.default_budget;
     // split the budget evenly over the materialised levels 1..k-2 (or k-1

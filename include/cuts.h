@@ -165,6 +165,10 @@ typedef struct {
    * queries into fixed workloads.  The result then has work_capped = 1 and
    * no valid count. */
   uint64_t max_work;
+  /* threads per block of the counting expansions (the fused, pair, chain
+   * and last-level kernels): 64, 128 or 256 (0 = 256).  A tile stays 1,024
+   * work items, so each lane carries 16, 8 or 4 of them. */
+  uint32_t expand_threads;
 } cuts_match_options;
 
 typedef struct {

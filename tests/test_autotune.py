@@ -1,7 +1,8 @@
 """The launch-shape autotuner on small workloads: every point it sweeps
-(blocks per CU and XCD tile mapping for single queries; blocks per CU and
-item work for the batch) must count the same, the chosen point is one of
-the swept points, and the cache returns the stored result."""
+(blocks per CU, XCD tile mapping and block size of the counting expansions
+for single queries; queue kind, blocks per CU, item work and level-0 chunk
+for the batch) must count the same, the chosen point is one of the swept
+points, and the cache returns the stored result."""
 import pytest
 
 import cuts_amd
@@ -17,14 +18,23 @@ def test_tune_single_and_batch(ctx, tmp_path, monkeypatch):
     dg = ctx.upload(g)
     plans = [cuts_amd.Plan(W.query_graph(s)) for s in (W.TRI_TAIL2, W.C4_2PEND, W.K4E_TAIL2)]
     want = [ctx.match(dg, p).count for p in plans]
-    res = autotune.tune_single(ctx, dg, plans, "t-single", reps=1, grid=(2, 6))
+    print("tune single", flush=True)
+    res = autotune.tune_single(ctx, dg, plans, "t-single", reps=1, grid=(2, 6), threads=(256, 128, 64))
     assert res["counts"] == want
-    assert res["best"] in res["points"] and len(res["points"]) == 4
-    again = autotune.tune_single(ctx, dg, plans, "t-single", reps=1, grid=(2, 6))
+    assert res["best"] in res["points"] and len(res["points"]) == 12
+    again = autotune.tune_single(ctx, dg, plans, "t-single", reps=1, grid=(2, 6), threads=(256, 128, 64))
     assert again["best"] == res["best"]
+    best = res["best"]
+    assert [ctx.match(dg, p, blocks_per_cu=best["blocks_per_cu"], expand_threads=best["threads"],
+                      flags=0 if best["xcd"] else cuts_amd.FLAG_NO_XCD).count for p in plans] == want
+    with pytest.raises(cuts_amd.CutsError, match="expand_threads"):
+        ctx.match(dg, plans[0], expand_threads=96)
+    print("tune batch", flush=True)
     rb = autotune.tune_batch(ctx, dg, plans, "t-batch", reps=1, blocks=(2, 6), item_works=(4096, 65536),
-                             arena_bytes=1 << 30, queue_cap=1 << 20)
-    assert rb["best"] in rb["points"] and len(rb["points"]) == 4
-    b = cuts_amd.Batch(ctx, dg, plans, blocks_per_cu=rb["best"]["blocks_per_cu"],
-                       item_work=rb["best"]["item_work"], arena_bytes=1 << 30, queue_cap=1 << 20)
+                             vchunks=(0, 64), queues=("level", "persistent"), arena_bytes=1 << 30,
+                             queue_cap=1 << 20)
+    assert rb["best"] in rb["points"] and len(rb["points"]) == 16
+    bb = rb["best"]
+    b = cuts_amd.Batch(ctx, dg, plans, blocks_per_cu=bb["blocks_per_cu"], item_work=bb["item_work"],
+                       vchunk=bb["vchunk"], queue=bb["queue"], arena_bytes=1 << 30, queue_cap=1 << 20)
     assert b.run().counts == want
