@@ -15,11 +15,16 @@ scale-18 graphs by python -m cuts_amd.train, committed in cuts_amd/data)
 predicts |T| of every connected prefix set, a DP picks the order whose
 expanded levels are cheapest for this engine, and the score order judges
 it by running both (the proposal is kept only if it ran faster).
+Launch shape: before the timed steps the autotuner (cuts_amd.autotune)
+picks blocks per CU and the counting expansions' block size on every 5th
+evaluation query with the score orders; both orders then run with it.
 One step = every query matched once with the learned orders; the same
 queries with the score orders are timed in the same loop.  value = total
 embeddings / total device time of the learned passes (HIP events on the
 library's stream, first matcher kernel to count readback: the window of
 src/host_funcs.cu:136-162).  Every count is checked against the fixture.
+The same step run as a throughput job (queries handed to 4 host threads,
+each with its own stream, cuts_match_many) is reported beside it.
 
 The roofline of the dominant kernel class comes from a profiling pass after
 the timed steps (CUTS_FLAG_PROFILE: HIP events around every launch):
@@ -37,8 +42,9 @@ Other workloads (one JSON line each):
     --workload chunglu36k  the mid-size Chung-Lu graph (n=36,692) and its two 6-vertex queries,
                            full counts
     --workload chunglu4k   triangle + 2-edge tail on Chung-Lu n=4,096 (full count)
-    --workload batch256    256 random-walk queries, batched level queue in one hipGraph
-                           launch, learned orders, autotuned launch shape
+    --workload batch256    256 random-walk queries, batched (query, level, chunk) queue in one
+                           hipGraph launch, learned orders, autotuned launch shape and queue kind
+                           (one launch per level or the persistent kernel)
     --workload write       4.8e8 embeddings written to pinned host memory, streamed vs synchronous
 """
 import argparse
@@ -100,13 +106,13 @@ def traffic_for(key):
     return None if e is None else e
 
 
-def profile_pass(ctx, dg, plans, roots):
+def profile_pass(ctx, dg, plans, roots, shape=None):
     """One run of every plan with HIP events around every launch (outside the
     timed steps): per kernel class [ms, launches, demand bytes, merge bytes]."""
     import cuts_amd
     kern = {}
     for p, rt in zip(plans, roots):
-        r = ctx.match(dg, p, roots=rt, flags=cuts_amd.FLAG_PROFILE)
+        r = ctx.match(dg, p, roots=rt, flags=cuts_amd.FLAG_PROFILE, **(shape or {}))
         for name, (ms, n, b) in r.kernels.items():
             k = kern.setdefault(name, [0.0, 0, 0.0, 0.0])
             k[0] += ms
@@ -332,7 +338,7 @@ def median(xs):
     return statistics.median(xs) if xs else 0.0
 
 
-def concurrent_fields(ctx, dg, learned, heur, roots, want, emb_per_step, workers=4, reps=3):
+def concurrent_fields(ctx, dg, learned, heur, roots, want, emb_per_step, shape, workers=4, reps=3):
     """The same step as a throughput job: the queries handed out to
     `workers` host threads, each running whole single-query matches on its
     own stream (cuts_match_many); host wall time of the whole set, best of
@@ -341,7 +347,7 @@ def concurrent_fields(ctx, dg, learned, heur, roots, want, emb_per_step, workers
     for name, plans in (("learned", learned), ("heuristic", heur)):
         best = None
         for _ in range(reps):
-            res, wall = ctx.match_many(dg, plans, roots=roots, workers=workers)
+            res, wall = ctx.match_many(dg, plans, roots=roots, workers=workers, **shape)
             if [r.count for r in res] != want:
                 raise AssertionError(f"match_many ({name}): counts differ from the fixture")
             best = wall if best is None else min(best, wall)
@@ -376,8 +382,20 @@ def run_headline(args):
     heur = [cuts_amd.Plan(cuts_amd.Graph.from_edges(k, e)) for k, e in qs]
     assert [p.order for p in heur] == [q["order"] for q in items], "score orders differ from the fixture's"
     hres = None
+    shape, tuned = {}, None
+    if args.orders_in:
+        with open(args.orders_in) as f:
+            shape = json.load(f).get("shape", {})
+    elif not args.profile_only:
+        # launch shape picked by the autotuner on every 5th evaluation query
+        # (score orders; every point must count the same)
+        from cuts_amd import autotune
+        tuned = autotune.tune_single(ctx, dg, heur[1::5], "rmat20-headline", reps=2, grid=(2, 3, 4, 6), xcd=(True,),
+                                     threads=(256, 128), roots=roots[1::5], log=log, use_cache=False)
+        shape = dict(blocks_per_cu=tuned["best"]["blocks_per_cu"], expand_threads=tuned["best"]["threads"])
+    log(f"launch shape: {shape or 'defaults'}")
     if not args.profile_only:
-        hres = [ctx.match(dg, p, roots=rt) for p, rt in zip(heur, roots)]
+        hres = [ctx.match(dg, p, roots=rt, **shape) for p, rt in zip(heur, roots)]
         assert [r.count for r in hres] == want, "score-order counts differ from the fixture"
     # learned orders: network forward over every connected prefix set (HIP),
     # engine-aware DP, first vertex kept (root samples), judged by time
@@ -399,19 +417,19 @@ def run_headline(args):
         kept_flags = [bool(o) for o in kept_orders]
     else:
         judged = LN.judge_by_time(ctx, dg, qs, heur, want, choices, roots=roots,
-                                  heuristic_work=[r.work for r in hres])
+                                  heuristic_work=[r.work for r in hres], **shape)
         learned = [j.plan for j in judged]
         kept_flags = [j.learned for j in judged]
     judge_s = time.perf_counter() - t0
     if args.orders_out:
         with open(args.orders_out, "w") as f:
-            json.dump({"orders": [p.order if kf else None for p, kf in zip(learned, kept_flags)]}, f)
+            json.dump({"orders": [p.order if kf else None for p, kf in zip(learned, kept_flags)], "shape": shape}, f)
     proposed = sum(1 for c in choices if c.learned)
     kept = sum(kept_flags)
     log(f"learned orders: {proposed} proposed, {kept} kept by the judge ({plan_s:.1f}s planning, {judge_s:.1f}s judging)")
     if args.profile_only:
         # the profiling pass alone (rocprofv3 runs): one run of every learned plan
-        kern = profile_pass(ctx, dg, learned, roots)
+        kern = profile_pass(ctx, dg, learned, roots, shape)
         if args.copy_probe:
             ctx.copy_probe(1 << 30, args.copy_probe)
         emit({"profile_only": True, "roofline": roofline(kern, "rmat20"),
@@ -421,7 +439,7 @@ def run_headline(args):
 
     def one_pass(plans, sink=None):
         for i, (p, rt) in enumerate(zip(plans, roots)):
-            r = ctx.match(dg, p, roots=rt)
+            r = ctx.match(dg, p, roots=rt, **shape)
             if r.count != want[i]:
                 raise AssertionError(f"query {i}: count {r.count}, fixture {want[i]}")
             if sink is not None:
@@ -447,7 +465,7 @@ def run_headline(args):
     per_h = [median(m) for m in ms_h]
     ev_l, ev_h = per_l[1:], per_h[1:]
     ratio = [h / l for h, l in zip(per_h, per_l) if l > 0]
-    kern = profile_pass(ctx, dg, learned, roots)
+    kern = profile_pass(ctx, dg, learned, roots, shape)
     line = base_line("exact embeddings/sec + p50 time/query; learned vs heuristic order, same GPU",
                      emb / (tot_l / 1e3), "embeddings/s", args.steps, args.warmup, tot_l / args.steps,
                      "R-MAT scale 20 (config 3 graph: 2^20 vertices, ef 16, a,b,c=.57,.19,.19, seed 0, "
@@ -482,7 +500,9 @@ def run_headline(args):
                       "arc_common_build_s": info["arc_common_build_s"]})
     line["roofline"] = roofline(kern, "rmat20")
     line["kernels_ms_profile_pass"] = {k: v[0] for k, v in kern.items() if v[1]}
-    line["config"]["concurrent"] = concurrent_fields(ctx, dg, learned, heur, roots, want, emb / args.steps)
+    line["config"]["concurrent"] = concurrent_fields(ctx, dg, learned, heur, roots, want, emb / args.steps, shape)
+    line["config"]["launch_shape"] = {"used": shape or "defaults (6 blocks per CU, 256 threads)",
+                                      "tuned_points": tuned["points"] if tuned else None}
     if args.copy_probe:
         line["copy_probe_GBs"] = ctx.copy_probe(1 << 30, args.copy_probe)
     if not args.no_config2:

@@ -373,8 +373,10 @@ class Context:
             _lib.cuts_context_destroy(self._h)
             self._h = None
 
-    def match_many(self, dg: "DeviceGraph", plans, roots=None, flags: int = 0, workers: int = 4, max_work: int = 0):
-        return match_many(self, dg, plans, roots=roots, flags=flags, workers=workers, max_work=max_work)
+    def match_many(self, dg: "DeviceGraph", plans, roots=None, flags: int = 0, workers: int = 4, max_work: int = 0,
+                   blocks_per_cu: int = 0, expand_threads: int = 0):
+        return match_many(self, dg, plans, roots=roots, flags=flags, workers=workers, max_work=max_work,
+                          blocks_per_cu=blocks_per_cu, expand_threads=expand_threads)
 
     def copy_probe(self, nbytes: int, iters: int = 10) -> float:
         """Read+write GB/s of a plain copy of nbytes (HBM yardstick)."""
@@ -446,7 +448,7 @@ def _result_of(r, k: int) -> "MatchResult":
 
 
 def match_many(ctx: "Context", dg: "DeviceGraph", plans: Sequence["Plan"], roots=None, flags: int = 0,
-               workers: int = 4, max_work: int = 0):
+               workers: int = 4, max_work: int = 0, blocks_per_cu: int = 0, expand_threads: int = 0):
     """Many queries against one resident graph on `workers` concurrent streams
     (cuts_match_many).  roots: None or one (first, end, stride) per query.
     Returns (results in query order, host wall ms of the whole set)."""
@@ -456,6 +458,8 @@ def match_many(ctx: "Context", dg: "DeviceGraph", plans: Sequence["Plan"], roots
         lib().cuts_match_options_default(C.byref(opts[i]))
         opts[i].flags = flags
         opts[i].max_work = max_work
+        opts[i].blocks_per_cu = blocks_per_cu
+        opts[i].expand_threads = expand_threads
         if roots is not None and roots[i] is not None:
             opts[i].root_begin, opts[i].root_end, opts[i].root_stride = roots[i]
     res = (_Result * max(n, 1))()

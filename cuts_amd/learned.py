@@ -468,7 +468,7 @@ def judge(ctx: Context, dg, queries: Sequence[Tuple[int, List[Tuple[int, int]]]]
 def judge_by_time(ctx: Context, dg, queries: Sequence[Tuple[int, List[Tuple[int, int]]]], heuristic_plans,
                   heuristic_counts: Sequence[int], choices: Sequence[OrderChoice], roots=None, flags: int = 0,
                   heuristic_work: Optional[Sequence[int]] = None, reps: int = 2,
-                  margin: float = 0.03) -> List["TimedJudged"]:
+                  margin: float = 0.03, blocks_per_cu: int = 0, expand_threads: int = 0) -> List["TimedJudged"]:
     """The score order judges whether a proposal actually runs faster: both
     orders are run warm `reps` times on the same GPU and the proposal is kept
     only if its best device time beats the score order's by `margin`
@@ -476,16 +476,18 @@ def judge_by_time(ctx: Context, dg, queries: Sequence[Tuple[int, List[Tuple[int,
     flip with timing noise).  The proposal's
     runs are capped at 4x the score order's candidates (a runaway order is
     stopped, never waited for).  A kept order's count must equal the score
-    order's (order never changes a count); anything else raises."""
+    order's (order never changes a count); anything else raises.  Both
+    orders run with the given launch shape (0 = defaults)."""
+    shape = dict(blocks_per_cu=blocks_per_cu, expand_threads=expand_threads)
     out = []
     for i, ((k, edges), hp, hc, ch) in enumerate(zip(queries, heuristic_plans, heuristic_counts, choices)):
         rt = None if roots is None else roots[i]
-        th = min(ctx.match(dg, hp, roots=rt, flags=flags).device_ms for _ in range(reps))
+        th = min(ctx.match(dg, hp, roots=rt, flags=flags, **shape).device_ms for _ in range(reps))
         kept = None
         if ch.learned and list(ch.order) != list(hp.order):
             lp = Plan(Graph.from_edges(k, edges), ch.order)
             cap = 4 * heuristic_work[i] + 4096 if heuristic_work is not None else 0
-            runs = [ctx.match(dg, lp, roots=rt, flags=flags, max_work=cap) for _ in range(reps)]
+            runs = [ctx.match(dg, lp, roots=rt, flags=flags, max_work=cap, **shape) for _ in range(reps)]
             if not any(r.work_capped for r in runs):
                 if any(r.count != hc for r in runs):
                     raise AssertionError(f"query {i}: learned order counted {runs[0].count}, score order {hc}")

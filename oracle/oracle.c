@@ -332,7 +332,11 @@ static void trie_expand(trie_ctx *c, const uint32_t *R, uint64_t n, uint32_t len
       uint32_t lo, hi;
       source(d, p, len, f, &a, &lo, &hi);
       uint64_t L = hi - lo;
-      if (end > start && acc + L > c->max_rows && !last) break;
+      /* a group holds at most max_rows candidates (rows of the next level),
+       * or 64 x max_rows at the last level (embeddings examined, nothing
+       * stored), so the time cap is checked every fraction of a second even
+       * where one prefix level fans out into the last */
+      if (end > start && acc + L > (last ? 64 * c->max_rows : c->max_rows)) break;
       acc += L;
       ++end;
     }
