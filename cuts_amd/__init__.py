@@ -35,6 +35,7 @@ FLAG_NO_TWO_HOP = 512
 FLAG_NO_SHARED = 1024
 FLAG_NO_ARC_COMMON = 2048
 FLAG_SYNC_WRITE = 4096
+FLAG_SYNC_LEVELS = 8192   # read every level's work back before expanding it
 MAX_QUERY = 16
 KERNEL_CLASSES = ["level0", "plan", "scan", "tile_index", "expand_write", "expand_count", "expand_fused",
                   "closed_form", "pair", "common", "chain"]

@@ -546,6 +546,47 @@ __global__ __launch_bounds__(BLOCK) void k_tile_index(const uint64_t* __restrict
   }
 }
 
+// The same index without a host readback of the level's work: W = off[n] is
+// read on the device.  When it fits (W <= MAX_CHUNK, ceil(W / TILE) + 1 <=
+// tile_cap entries, W <= row_cap) the tiles of [0, W) are indexed, dyn =
+// {W, ntiles} and W is added to *work; otherwise dyn = {0, 0} -- the
+// expansion that reads dyn does nothing -- and bit 0 of *status is set, for
+// the host to redo the level with the work it then reads back.  Block 0 also
+// clears the expansion's counters, as k_tile_index does.
+__global__ __launch_bounds__(BLOCK) void k_tile_index_dyn(const uint64_t* __restrict__ off, uint64_t n,
+                                                         uint64_t tile_cap, uint64_t row_cap,
+                                                         uint32_t* __restrict__ tile_p,
+                                                         unsigned int* __restrict__ tilectr,
+                                                         unsigned int* __restrict__ outctr,
+                                                         unsigned long long* __restrict__ dyn,
+                                                         unsigned long long* __restrict__ status,
+                                                         unsigned long long* __restrict__ work) {
+  const uint64_t W = off[n];
+  const uint64_t nt = (W + TILE - 1) / TILE;
+  const bool fits = W <= MAX_CHUNK && nt + 1 <= tile_cap && W <= row_cap;
+  if (blockIdx.x == 0) {
+    if (threadIdx.x < 8) tilectr[threadIdx.x] = 0u;
+    if (outctr && threadIdx.x < 2) outctr[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) {
+      dyn[0] = fits ? W : 0ull;
+      dyn[1] = fits ? nt : 0ull;
+      if (fits) *work += W;
+      else *status |= 1ull;
+    }
+  }
+  if (!fits || W == 0) return;
+  for (uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; t <= nt; t += (uint64_t)gridDim.x * BLOCK) {
+    const uint64_t w = t < nt ? t * TILE : W - 1;
+    uint64_t lo = 0, hi = n + 1;  // upper_bound over off[0..n]
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (off[mid] <= w) lo = mid + 1;
+      else hi = mid;
+    }
+    tile_p[t] = (uint32_t)(lo - 1);
+  }
+}
+
 // ------------------------------------------------ closed-form last level
 // Number of candidates of the last position given all earlier positions
 // r[0..N): |{x in N(u) : deg(x) >= thr}| - |{i : r[i] in N(u), deg(r[i]) >= thr}|
@@ -746,6 +787,9 @@ struct ExpandArgs {
   uint32_t xcd;              // 1: XCD-contiguous tile mapping
   unsigned int* tilectr;     // 8 tile counters, zero at launch
   uint32_t stats;            // 1: accumulate demand bytes into counters[3]
+  // non-null: {w_end, ntiles} of this launch, written on the device by
+  // k_tile_index_dyn (w_begin is 0), in place of w_end and ntiles
+  const unsigned long long* dyn;
 };
 
 template <int L, int MODE>
@@ -757,6 +801,11 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
   // the counting ones (a tile is TILE items either way, ipt per lane)
   const uint32_t NT = MODE == M_WRITE ? (uint32_t)BLOCK : (uint32_t)blockDim.x;
   const uint32_t ipt = TILE / NT;
+  const uint64_t w_end_ = a.dyn ? a.dyn[0] : a.w_end;
+  const uint64_t ntiles_ = a.dyn ? a.dyn[1] : a.ntiles;
+  // a grid sized before the tile count was known: blocks beyond it leave
+  // at once (before any barrier or tile-counter atomic)
+  if (blockIdx.x >= ntiles_) return;
   const uint32_t nj = P.nj[L];
   const uint32_t tin = P.sig_in[L], tout = P.sig_out[L];
   uint64_t acc = 0, valid = 0, pairs = 0, trips = 0, bytes = 0, mb = 0;
@@ -777,7 +826,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
       uint64_t t = ~0ull;
       while (tries < nx) {
         const uint32_t xx = (myx + tries) % nx;
-        const uint64_t lo = a.ntiles * xx / nx, hi = a.ntiles * (xx + 1) / nx;
+        const uint64_t lo = ntiles_ * xx / nx, hi = ntiles_ * (xx + 1) / nx;
         uint32_t got = 0;
         if ((tid & 63) == 0) got = atomicAdd(&a.tilectr[xx], 1u);  // inside the wave-uniform branch
         const uint64_t i = uni(got);
@@ -794,7 +843,7 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
     if (tile == ~0ull) break;
     {
     const uint64_t w0 = a.w_begin + tile * TILE;
-    const uint64_t w1 = min(w0 + (uint64_t)TILE, a.w_end);
+    const uint64_t w1 = min(w0 + (uint64_t)TILE, w_end_);
     const uint32_t plo = a.tile_p[tile], phi = a.tile_p[tile + 1];
     const uint32_t span = phi - plo + 1;  // offsets plo .. phi+1
     const bool lds = span <= LDS_OFF;
@@ -1515,6 +1564,11 @@ __global__ __launch_bounds__(BLOCK) void k_common(GView g, KPlan P, ExpandArgs a
   __shared__ uint64_t s_tile;
   const uint32_t tid = threadIdx.x;
   uint64_t acc = 0, bytes = 0;
+  const uint64_t w_end_ = a.dyn ? a.dyn[0] : a.w_end;
+  const uint64_t ntiles_ = a.dyn ? a.dyn[1] : a.ntiles;
+  // a grid sized before the tile count was known: blocks beyond it leave
+  // at once (before any barrier or tile-counter atomic)
+  if (blockIdx.x >= ntiles_) return;
   const uint32_t nx = a.xcd ? 8u : 1u;
   const uint32_t myx = a.xcd ? (blockIdx.x & 7u) : 0u;
   uint32_t tries = 0;
@@ -1523,7 +1577,7 @@ __global__ __launch_bounds__(BLOCK) void k_common(GView g, KPlan P, ExpandArgs a
       uint64_t t = ~0ull;
       while (tries < nx) {
         const uint32_t xx = (myx + tries) % nx;
-        const uint64_t lo = a.ntiles * xx / nx, hi = a.ntiles * (xx + 1) / nx;
+        const uint64_t lo = ntiles_ * xx / nx, hi = ntiles_ * (xx + 1) / nx;
         uint32_t got = 0;
         if ((tid & 63) == 0) got = atomicAdd(&a.tilectr[xx], 1u);  // inside the wave-uniform branch
         const uint64_t i = uni(got);
@@ -1539,7 +1593,7 @@ __global__ __launch_bounds__(BLOCK) void k_common(GView g, KPlan P, ExpandArgs a
     const uint64_t tile = uni64(s_tile);
     if (tile == ~0ull) break;
     const uint64_t w0 = a.w_begin + tile * TILE;
-    const uint64_t w1 = min(w0 + (uint64_t)TILE, a.w_end);
+    const uint64_t w1 = min(w0 + (uint64_t)TILE, w_end_);
     const uint32_t plo = a.tile_p[tile], phi = a.tile_p[tile + 1];
     const uint32_t span = phi - plo + 1;
     const bool lds = span <= LDS_OFF;
@@ -1954,14 +2008,14 @@ struct Matcher {
     return (uint32_t)(c.host_pinned[0] & 0xFFFFFFFFu);
   }
 
-  // exclusive scan of b.minlen[0..n) into b.work_off[0..n], returns total
-  uint64_t scan(LevelBuf& b, uint64_t n) {
+  // exclusive scan of b.minlen[0..n) into b.work_off[0..n] (total at n)
+  void scan_launch(LevelBuf& b, uint64_t n) {
     const uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (nb <= 1) {
       pbegin(false);
       hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(BLOCK), 0, c.stream, b.minlen, n, b.work_off);
       pend(2, 16.0 * n, false);
-      return read_u64(b.work_off + n);
+      return;
     }
     ensure_bsum(arena, b, nb);
     pbegin(false);
@@ -1970,7 +2024,40 @@ struct Matcher {
     hipLaunchKernelGGL(k_scan_down, dim3((uint32_t)nb), dim3(BLOCK), 0, c.stream, b.minlen, n, b.bsum, nb,
                        b.work_off);
     pend(2, 16.0 * n, false);
+  }
+  // the same, returning the total (a readback)
+  uint64_t scan(LevelBuf& b, uint64_t n) {
+    scan_launch(b, n);
     return read_u64(b.work_off + n);
+  }
+
+  // Speculative levels: the level's work is not read back before it is
+  // expanded.  k_tile_index_dyn checks on the device that the work fits the
+  // tile index and the output rows already allocated, and the expansion
+  // takes its range from the device; a writing level reads back its row
+  // count together with the status word (one readback instead of two), a
+  // counting level none.  A writing level that did not fit is redone with
+  // the readback at once; a counting level that did not fit leaves the
+  // status bit for the end of the run, which then repeats the whole query
+  // with readbacks.  Off for profiling, work caps, deadlines, the
+  // no-allocation entry point and CUTS_FLAG_SYNC_LEVELS.
+  bool no_spec = false;
+  bool spec_overflow = false;
+  bool speculate() const {
+    return !no_spec && !prof && !o.max_work && !has_deadline && !arena && !(o.flags & 8192u);
+  }
+  // tile index over [0, W) with W read on the device; returns the launch
+  // shape bound (the most tiles the index may hold)
+  // status: ctr[11] for a writing level (read back with its row count),
+  // ctr[12] for the counting levels (read at the end of the run)
+  uint64_t tile_index_dyn(LevelBuf& in, uint64_t n, uint64_t row_cap, unsigned int* outctr) {
+    ensure_tiles(arena, in, std::max<uint64_t>(in.tile_cap, 4097) - 1);
+    const uint64_t tcap = in.tile_cap;
+    const uint32_t tg = (uint32_t)std::min<uint64_t>((tcap + BLOCK - 1) / BLOCK, 1024);
+    hipLaunchKernelGGL(k_tile_index_dyn, dim3(tg), dim3(BLOCK), 0, c.stream, in.work_off, n, tcap, row_cap,
+                       in.tile_p, (unsigned int*)(ctr + 4), outctr, ctr + 14, outctr ? ctr + 11 : ctr + 12, ctr + 13);
+    HIPCHECK(hipGetLastError());
+    return tcap - 1;
   }
 
   // rows: level L-1 table (n rows of L words) living in lv[L-1].rows (or an
@@ -2028,6 +2115,7 @@ struct Matcher {
       // the prefixes' list reads belong to the expansion that performs them
       if (merge) r.merge_bytes[mclass] += (double)read_u64(merge);
     }
+    if (speculate() && !(mode == M_WRITE && L == k - 1) && level_spec(L, rows, n, in, mode, use_cbase)) return;
     const uint64_t W = scan(in, n);
     if (W == 0) return;
     if (add_work(W)) return;
@@ -2171,11 +2259,103 @@ struct Matcher {
     if (pair_common && !aborted) pair_intersections(L, rows, n, in);
   }
 
+  // One level without the readback of its work (see speculate()).  Returns
+  // false when a writing level did not fit: the caller then runs it with the
+  // readback (its scan is repeated; nothing was written or counted).
+  bool level_spec(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in, int mode, bool use_cbase) {
+    scan_launch(in, n);
+    LevelBuf& out = lv[L];
+    uint64_t row_cap = ~0ull;
+    const bool want_parent = (chain && !chain_fused && mode == M_WRITE && L == k - 3) ||
+                             (sib && mode == M_WRITE && L == k - 4);
+    if (mode == M_WRITE) {
+      // the rows already allocated for this level (at least 64K, within
+      // the level's share of the budget)
+      ensure_rows_exact(arena, out, std::min<uint64_t>(std::max<uint64_t>(out.cap, 1ull << 16), level_cap[L]), L + 1);
+      row_cap = std::min<uint64_t>(out.cap, level_cap[L]);
+      if (want_parent) ensure_aux(arena, out.parent, out.parent_cap, out.cap);
+    }
+    const bool pair_common = mode == M_PAIR && kp.pr_a != kp.pr_b && !pair_cab_known();
+    if (pair_common) {
+      ensure_aux(arena, in.nsurv, in.nsurv_cap, n);
+      HIPCHECK(hipMemsetAsync(in.nsurv, 0, n * sizeof(uint32_t), c.stream));
+    }
+    const uint64_t tmax = tile_index_dyn(in, n, row_cap, mode == M_WRITE ? (unsigned int*)(ctr + 10) : nullptr);
+    ExpandArgs a{};
+    a.rows = rows;
+    a.work_off = in.work_off;
+    a.minsel = in.minsel;
+    a.tile_p = in.tile_p;
+    a.dyn = ctr + 14;
+    a.counters = ctr;
+    a.tbl = tbl;
+    a.cbase = use_cbase ? in.cbase : nullptr;
+    a.pbase = mode == M_PAIR ? in.pbase : nullptr;
+    a.nsurv = pair_common ? in.nsurv : nullptr;
+    a.out_parent = want_parent ? out.parent : nullptr;
+    a.shared = in.nsurv;
+    a.ctab = ctab;
+    a.ptab = ptbl;
+    a.xcd = (o.flags & 8u) ? 0 : 1;
+    a.tilectr = (unsigned int*)(ctr + 4);
+    a.stats = 0;
+    const uint32_t grid = grid_for(tmax, 1, grid_cap);
+    const uint32_t nthr = o.expand_threads ? o.expand_threads : (uint32_t)BLOCK;
+    const uint32_t cgrow = (uint32_t)(BLOCK / nthr);
+    r.chunks++;
+    if (mode == M_WRITE) {
+      a.out_rows = out.rows;
+      a.out_count = (uint32_t*)(ctr + 10);
+#define EW(LL) hipLaunchKernelGGL((k_expand<LL, M_WRITE>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, a)
+      CUTS_DISPATCH_L(L, EW)
+#undef EW
+      HIPCHECK(hipGetLastError());
+      // rows written (ctr[10]) and this level's status word (ctr[11]) in one readback
+      HIPCHECK(hipMemcpyAsync(c.host_pinned, ctr + 10, 16, hipMemcpyDeviceToHost, c.stream));
+      HIPCHECK(hipStreamSynchronize(c.stream));
+      const uint64_t m = c.host_pinned[0] & 0xFFFFFFFFu;
+      if (c.host_pinned[1] & 1ull) {
+        HIPCHECK(hipMemsetAsync(ctr + 11, 0, 8, c.stream));
+        r.chunks--;
+        return false;
+      }
+      r.level[L] += m;
+      if (m) level(L + 1, out.rows, m);
+      return true;
+    }
+#define EXS(LL, MD) hipLaunchKernelGGL((k_expand<LL, MD>), dim3(grid * cgrow), dim3(nthr), 0, c.stream, g, kp, a)
+#define ESC(LL) EXS(LL, M_COUNT)
+#define ESF(LL) EXS(LL, M_FUSED)
+#define ESK(LL) EXS(LL, M_COLLAPSE)
+#define ESH(LL) EXS(LL, M_CHAIN)
+#define ESP(LL) EXS(LL, M_PAIR)
+    if (mode == M_COUNT) {
+      CUTS_DISPATCH_L(L, ESC)
+    } else if (mode == M_FUSED) {
+      CUTS_DISPATCH_L(L, ESF)
+    } else if (mode == M_COLLAPSE) {
+      CUTS_DISPATCH_PAIR(L, ESK)
+    } else if (mode == M_CHAIN) {
+      CUTS_DISPATCH_PAIR(L, ESH)
+    } else {
+      CUTS_DISPATCH_PAIR(L, ESP)
+    }
+#undef ESP
+#undef ESH
+#undef ESK
+#undef ESF
+#undef ESC
+#undef EXS
+    HIPCHECK(hipGetLastError());
+    if (pair_common && !aborted) pair_intersections(L, rows, n, in, true);
+    return true;
+  }
+
   // Second pass of the pair level with u != w: subtracts, for every prefix
   // with survivors, survivors x |N(u) & N(w)| (degree-filtered).  The
   // planning arrays of the input level are reused once the expansion is
   // done.
-  void pair_intersections(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in) {
+  void pair_intersections(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in, bool spec = false) {
     const uint32_t grid = grid_for(n, BLOCK, grid_cap);
     pbegin(false);
 #define PP(LL) hipLaunchKernelGGL((k_pair_prep<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.nsurv, in.minlen, in.minsel)
@@ -2183,12 +2363,35 @@ struct Matcher {
 #undef PP
     HIPCHECK(hipGetLastError());
     pend(1, (double)n * (4.0 * 2 + 16.0 + 5.0), false);
-    common_pass(L, rows, n, in, CommonArgs{kp.pr_a, kp.pr_b, kp.pr_tab, 0u}, in.nsurv);
+    common_pass(L, rows, n, in, CommonArgs{kp.pr_a, kp.pr_b, kp.pr_tab, 0u}, in.nsurv, spec);
   }
 
   // Counts common neighbours over the work planned in in.minlen / in.minsel
   // (k_common; mode and weights as in ca), in chunks of the work space.
-  void common_pass(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in, CommonArgs ca, uint32_t* weight) {
+  void common_pass(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in, CommonArgs ca, uint32_t* weight,
+                   bool spec = false) {
+    if (spec) {
+      // the pair level's second pass without the readback of its work
+      scan_launch(in, n);
+      const uint64_t tmax = tile_index_dyn(in, n, ~0ull, nullptr);
+      ExpandArgs a{};
+      a.rows = rows;
+      a.work_off = in.work_off;
+      a.minsel = in.minsel;
+      a.tile_p = in.tile_p;
+      a.dyn = ctr + 14;
+      a.counters = ctr;
+      a.nsurv = weight;
+      a.xcd = (o.flags & 8u) ? 0 : 1;
+      a.tilectr = (unsigned int*)(ctr + 4);
+      a.stats = 0;
+      r.chunks++;
+#define PCS(LL) hipLaunchKernelGGL((k_common<LL>), dim3(grid_for(tmax, 1, grid_cap)), dim3(BLOCK), 0, c.stream, g, kp, a, ca)
+      CUTS_DISPATCH_L(L, PCS)
+#undef PCS
+      HIPCHECK(hipGetLastError());
+      return;
+    }
     const uint64_t W = scan(in, n);
     if (W == 0 || add_work(W)) return;
     for (uint64_t w0 = 0; w0 < W; w0 += MAX_CHUNK) {
@@ -2488,6 +2691,7 @@ struct Matcher {
     ensure_rows(arena, l0, std::max<uint32_t>(n0c, 1), 1);
     HIPCHECK(hipMemsetAsync(ctr, 0, 4 * 8, c.stream));
     HIPCHECK(hipMemsetAsync(ctr + 9, 0, 8, c.stream));
+    HIPCHECK(hipMemsetAsync(ctr + 11, 0, 3 * 8, c.stream));  // speculation status words, device-side work
     HIPCHECK(hipEventRecord(c.ev0, c.stream));
     const auto t_start = std::chrono::steady_clock::now();
     uint64_t n0 = 0;
@@ -2544,9 +2748,13 @@ struct Matcher {
         r.roots_done = std::min(n0, s + ts);
       }
       if (aborted || r.roots_done < n0) r.complete = 0;
+      // the work of speculative levels, summed on the device, and their status
+      HIPCHECK(hipMemcpyAsync(c.host_pinned + 8, ctr + 12, 16, hipMemcpyDeviceToHost, c.stream));
+      if (!write) HIPCHECK(hipMemcpyAsync(c.host_pinned, ctr, 24, hipMemcpyDeviceToHost, c.stream));
+      HIPCHECK(hipStreamSynchronize(c.stream));
+      spec_overflow = (c.host_pinned[8] & 1ull) != 0;
+      r.work += c.host_pinned[9];
       if (!write) {
-        HIPCHECK(hipMemcpyAsync(c.host_pinned, ctr, 24, hipMemcpyDeviceToHost, c.stream));
-        HIPCHECK(hipStreamSynchronize(c.stream));
         r.count = c.host_pinned[0];
         r.level[k - 1] = r.count;
         if (collapse_fused) {
@@ -2800,7 +3008,7 @@ void context_init(Context& c, int device) {
   HIPCHECK(hipEventCreateWithFlags(&c.copy_ev[0], hipEventDisableTiming));
   HIPCHECK(hipEventCreateWithFlags(&c.copy_ev[1], hipEventDisableTiming));
   HIPCHECK(hipMalloc(&c.counters, 16 * sizeof(unsigned long long)));
-  HIPCHECK(hipHostMalloc(&c.host_pinned, 64, hipHostMallocDefault));
+  HIPCHECK(hipHostMalloc(&c.host_pinned, 256, hipHostMallocDefault));
   size_t fr = 0, tot = 0;
   HIPCHECK(hipMemGetInfo(&fr, &tot));
   // trie tables may take 40% of the free HBM by default
@@ -2979,6 +3187,14 @@ void run_match(Context& c, DeviceGraph& dg, const Plan& p, const MatchOpts& o, M
   r = MatchResult();
   Matcher m(c, dg, p, o, r);
   m.run();
+  if (m.spec_overflow) {
+    // a counting level outgrew what the speculative pass had allocated:
+    // the query again, every level with its readback
+    r = MatchResult();
+    Matcher m2(c, dg, p, o, r);
+    m2.no_spec = true;
+    m2.run();
+  }
 }
 
 uint64_t noalloc_scratch_bytes() { return kScratchBytes; }

@@ -123,8 +123,11 @@ enum {
   CUTS_FLAG_NO_TWO_HOP = 1u << 9,     /* intersections by list walks even where the two-hop index exists */
   CUTS_FLAG_NO_SHARED = 1u << 10,     /* chain level: position k-3's candidates found per row, not per parent */
   CUTS_FLAG_NO_ARC_COMMON = 1u << 11, /* ignore the per-arc common-neighbour table */
-  CUTS_FLAG_SYNC_WRITE = 1u << 12     /* CUTS_FLAG_WRITE: copy each chunk's rows before the next chunk runs
+  CUTS_FLAG_SYNC_WRITE = 1u << 12,    /* CUTS_FLAG_WRITE: copy each chunk's rows before the next chunk runs
                                          (default: copies stream on a second stream, overlapped) */
+  CUTS_FLAG_SYNC_LEVELS = 1u << 13    /* read every level's work back before expanding it (default: a
+                                         level that fits the buffers already allocated is expanded
+                                         without that readback; results are identical) */
 };
 
 /* Kernel classes reported by CUTS_FLAG_PROFILE. */

@@ -44,6 +44,7 @@ MODES = [
     dict(budget=4096),  # a few hundred rows per level: many chunks
     dict(expand_threads=64),  # counting expansions with 64-thread blocks, 16 items per lane
     dict(expand_threads=128, flags=cuts_amd.FLAG_NO_PAIR),
+    dict(flags=cuts_amd.FLAG_SYNC_LEVELS),  # every level's work read back before it is expanded
 ]
 
 
