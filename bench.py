@@ -394,7 +394,7 @@ def run_headline(args):
                                      threads=(256, 128), roots=roots[1::5], log=log, use_cache=False)
         shape = dict(blocks_per_cu=tuned["best"]["blocks_per_cu"], expand_threads=tuned["best"]["threads"])
     log(f"launch shape: {shape or 'defaults'}")
-    if not args.profile_only:
+    if not args.profile_only and not args.plain_pass:
         hres = [ctx.match(dg, p, roots=rt, **shape) for p, rt in zip(heur, roots)]
         assert [r.count for r in hres] == want, "score-order counts differ from the fixture"
     # learned orders: network forward over every connected prefix set (HIP),
@@ -427,6 +427,17 @@ def run_headline(args):
     proposed = sum(1 for c in choices if c.learned)
     kept = sum(kept_flags)
     log(f"learned orders: {proposed} proposed, {kept} kept by the judge ({plan_s:.1f}s planning, {judge_s:.1f}s judging)")
+    if args.plain_pass:
+        # one step of the timed path alone, learned orders (for a rocprofv3
+        # kernel trace of the launch sequence as timed: no per-launch events)
+        t = 0.0
+        for i, (p, rt) in enumerate(zip(learned, roots)):
+            r = ctx.match(dg, p, roots=rt, **shape)
+            if r.count != want[i]:
+                raise AssertionError(f"query {i}: count {r.count}, fixture {want[i]}")
+            t += r.device_ms
+        emit({"plain_pass": True, "ms_learned": t, "shape": shape})
+        return
     if args.profile_only:
         # the profiling pass alone (rocprofv3 runs): one run of every learned plan
         kern = profile_pass(ctx, dg, learned, roots, shape)
@@ -645,6 +656,8 @@ def main():
     ap.add_argument("--orders-out", default="", help="headline: write the judged learned orders to this file")
     ap.add_argument("--orders-in", default="", help="headline: take the learned orders from this file (no judging)")
     ap.add_argument("--profile-only", action="store_true", help="headline: only the profiling pass (for rocprofv3)")
+    ap.add_argument("--plain-pass", action="store_true",
+                    help="headline: one step of the timed path with the learned orders (for rocprofv3)")
     ap.add_argument("--train-queries", type=int, default=1024)
     ap.add_argument("--score-only", action="store_true", help="batch256: score orders only, no training")
     ap.add_argument("--plain-launches", action="store_true", help="batch256 --score-only: no hipGraph")

@@ -95,7 +95,10 @@ constexpr int LDS_OFF = 2048;      // staged work offsets per tile
 #endif
 constexpr int SCAN_IPT = 8;
 constexpr int SCAN_TILE = BLOCK * SCAN_IPT;
-constexpr uint64_t MAX_CHUNK = 1ull << 31;  // work items per launch sequence
+constexpr uint64_t MAX_CHUNK = 1ull << 31;  // work items per launch sequence (written rows stay below 2^32)
+// work items of one speculative counting launch (counts, offsets and tile
+// numbers are 64-bit; the tile index of such a level is at most 1 GiB)
+constexpr uint64_t MAX_SPEC_ITEMS = 1ull << 38;
 
 struct GView {
   const uint32_t *off_out, *adj_out, *off_in, *adj_in, *sig;
@@ -547,14 +550,14 @@ __global__ __launch_bounds__(BLOCK) void k_tile_index(const uint64_t* __restrict
 }
 
 // The same index without a host readback of the level's work: W = off[n] is
-// read on the device.  When it fits (W <= MAX_CHUNK, ceil(W / TILE) + 1 <=
+// read on the device.  When it fits (W <= max_items, ceil(W / TILE) + 1 <=
 // tile_cap entries, W <= row_cap) the tiles of [0, W) are indexed, dyn =
 // {W, ntiles} and W is added to *work; otherwise dyn = {0, 0} -- the
 // expansion that reads dyn does nothing -- and bit 0 of *status is set, for
 // the host to redo the level with the work it then reads back.  Block 0 also
 // clears the expansion's counters, as k_tile_index does.
 __global__ __launch_bounds__(BLOCK) void k_tile_index_dyn(const uint64_t* __restrict__ off, uint64_t n,
-                                                         uint64_t tile_cap, uint64_t row_cap,
+                                                         uint64_t max_items, uint64_t tile_cap, uint64_t row_cap,
                                                          uint32_t* __restrict__ tile_p,
                                                          unsigned int* __restrict__ tilectr,
                                                          unsigned int* __restrict__ outctr,
@@ -563,7 +566,7 @@ __global__ __launch_bounds__(BLOCK) void k_tile_index_dyn(const uint64_t* __rest
                                                          unsigned long long* __restrict__ work) {
   const uint64_t W = off[n];
   const uint64_t nt = (W + TILE - 1) / TILE;
-  const bool fits = W <= MAX_CHUNK && nt + 1 <= tile_cap && W <= row_cap;
+  const bool fits = W <= max_items && nt + 1 <= tile_cap && W <= row_cap;
   if (blockIdx.x == 0) {
     if (threadIdx.x < 8) tilectr[threadIdx.x] = 0u;
     if (outctr && threadIdx.x < 2) outctr[threadIdx.x] = 0u;
@@ -2050,11 +2053,13 @@ struct Matcher {
   // shape bound (the most tiles the index may hold)
   // status: ctr[11] for a writing level (read back with its row count),
   // ctr[12] for the counting levels (read at the end of the run)
+  // (a writing level: at most MAX_CHUNK items; a counting one MAX_SPEC_ITEMS)
   uint64_t tile_index_dyn(LevelBuf& in, uint64_t n, uint64_t row_cap, unsigned int* outctr) {
     ensure_tiles(arena, in, std::max<uint64_t>(in.tile_cap, 4097) - 1);
     const uint64_t tcap = in.tile_cap;
     const uint32_t tg = (uint32_t)std::min<uint64_t>((tcap + BLOCK - 1) / BLOCK, 1024);
-    hipLaunchKernelGGL(k_tile_index_dyn, dim3(tg), dim3(BLOCK), 0, c.stream, in.work_off, n, tcap, row_cap,
+    hipLaunchKernelGGL(k_tile_index_dyn, dim3(tg), dim3(BLOCK), 0, c.stream, in.work_off, n,
+                       outctr ? MAX_CHUNK : MAX_SPEC_ITEMS, tcap, row_cap,
                        in.tile_p, (unsigned int*)(ctr + 4), outctr, ctr + 14, outctr ? ctr + 11 : ctr + 12, ctr + 13);
     HIPCHECK(hipGetLastError());
     return tcap - 1;
@@ -2119,6 +2124,9 @@ struct Matcher {
     const uint64_t W = scan(in, n);
     if (W == 0) return;
     if (add_work(W)) return;
+    // a counting level sizes the tile index for all of W, so that its next
+    // speculative run fits in one launch
+    if (mode != M_WRITE && W <= MAX_SPEC_ITEMS) ensure_tiles(arena, in, (W + TILE - 1) / TILE);
     uint64_t chunk = MAX_CHUNK;
     LevelBuf& out = lv[L];
     if (mode == M_WRITE) {
@@ -2394,6 +2402,7 @@ struct Matcher {
     }
     const uint64_t W = scan(in, n);
     if (W == 0 || add_work(W)) return;
+    if (W <= MAX_SPEC_ITEMS) ensure_tiles(arena, in, (W + TILE - 1) / TILE);
     for (uint64_t w0 = 0; w0 < W; w0 += MAX_CHUNK) {
       if (past_deadline()) return;
       const uint64_t w1 = std::min(W, w0 + MAX_CHUNK);
