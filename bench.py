@@ -392,7 +392,14 @@ def run_headline(args):
         from cuts_amd import autotune
         tuned = autotune.tune_single(ctx, dg, heur[1::5], "rmat20-headline", reps=2, grid=(2, 3, 4, 6), xcd=(True,),
                                      threads=(256, 128), roots=roots[1::5], log=log, use_cache=False)
-        shape = dict(blocks_per_cu=tuned["best"]["blocks_per_cu"], expand_threads=tuned["best"]["threads"])
+        threads = tuned["best"]["threads"]
+        # then blocks per CU per kernel class
+        tc = autotune.tune_classes(ctx, dg, heur[1::5], "rmat20-headline-classes", roots=roots[1::5],
+                                   grid=(1, 2, 3, 4, 6, 8), expand_threads=threads, log=log, use_cache=False)
+        tuned["classes"] = tc
+        shape = dict(blocks_per_cu=tc["blocks_per_cu"], expand_threads=threads)
+        if tc["class_blocks_per_cu"]:
+            shape["class_blocks_per_cu"] = tc["class_blocks_per_cu"]
     log(f"launch shape: {shape or 'defaults'}")
     if not args.profile_only and not args.plain_pass:
         hres = [ctx.match(dg, p, roots=rt, **shape) for p, rt in zip(heur, roots)]
@@ -513,7 +520,10 @@ def run_headline(args):
     line["kernels_ms_profile_pass"] = {k: v[0] for k, v in kern.items() if v[1]}
     line["config"]["concurrent"] = concurrent_fields(ctx, dg, learned, heur, roots, want, emb / args.steps, shape)
     line["config"]["launch_shape"] = {"used": shape or "defaults (6 blocks per CU, 256 threads)",
-                                      "tuned_points": tuned["points"] if tuned else None}
+                                      "tuned_points": tuned["points"] if tuned else None,
+                                      "class_sweep": ({k: tuned["classes"][k] for k in
+                                                       ("uniform_ms", "table_ms", "per_class_ms")}
+                                                      if tuned else None)}
     if args.copy_probe:
         line["copy_probe_GBs"] = ctx.copy_probe(1 << 30, args.copy_probe)
     if not args.no_config2:

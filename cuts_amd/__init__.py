@@ -54,7 +54,8 @@ class _Opts(C.Structure):
     _fields_ = [("trunks", C.c_uint32), ("flags", C.c_uint32), ("budget_bytes", C.c_uint64),
                 ("root_begin", C.c_uint32), ("root_end", C.c_uint32), ("root_stride", C.c_uint32),
                 ("blocks_per_cu", C.c_uint32), ("emb_rows", _u32p), ("emb_cap", C.c_uint64),
-                ("time_limit_ms", C.c_double), ("max_work", C.c_uint64), ("expand_threads", C.c_uint32)]
+                ("time_limit_ms", C.c_double), ("max_work", C.c_uint64), ("expand_threads", C.c_uint32),
+                ("class_blocks_per_cu", C.c_uint8 * 16)]
 
 
 class _Result(C.Structure):
@@ -375,9 +376,10 @@ class Context:
             self._h = None
 
     def match_many(self, dg: "DeviceGraph", plans, roots=None, flags: int = 0, workers: int = 4, max_work: int = 0,
-                   blocks_per_cu: int = 0, expand_threads: int = 0):
+                   blocks_per_cu: int = 0, expand_threads: int = 0, class_blocks_per_cu=None):
         return match_many(self, dg, plans, roots=roots, flags=flags, workers=workers, max_work=max_work,
-                          blocks_per_cu=blocks_per_cu, expand_threads=expand_threads)
+                          blocks_per_cu=blocks_per_cu, expand_threads=expand_threads,
+                          class_blocks_per_cu=class_blocks_per_cu)
 
     def copy_probe(self, nbytes: int, iters: int = 10) -> float:
         """Read+write GB/s of a plain copy of nbytes (HBM yardstick)."""
@@ -394,7 +396,7 @@ class Context:
               roots: Optional[tuple] = None, write: bool = False, emb_cap: int = 0,
               blocks_per_cu: int = 0, time_limit_ms: float = 0.0, max_work: int = 0,
               noalloc: bool = False, rows_out: Optional[np.ndarray] = None,
-              expand_threads: int = 0) -> MatchResult:
+              expand_threads: int = 0, class_blocks_per_cu: Optional[Sequence[int]] = None) -> MatchResult:
         """One query (cuts_match).  noalloc=True calls cuts_match_noalloc: the
         run's buffers come from the device's static scratch array.  With
         write=True the rows go to rows_out (C-contiguous uint32, (cap, k);
@@ -406,6 +408,7 @@ class Context:
         o.time_limit_ms = time_limit_ms
         o.max_work = max_work
         o.expand_threads = expand_threads
+        _set_class_bpc(o, class_blocks_per_cu)
         if roots is not None:
             o.root_begin, o.root_end, o.root_stride = roots
         rows = None
@@ -448,8 +451,22 @@ def _result_of(r, k: int) -> "MatchResult":
     return res
 
 
+def _set_class_bpc(o, table):
+    """class_blocks_per_cu: None, a sequence indexed by kernel class, or a
+    {class name: blocks} dict (names as in MatchResult.kernels)."""
+    if not table:
+        return
+    if isinstance(table, dict):
+        for name, b in table.items():
+            o.class_blocks_per_cu[KERNEL_CLASSES.index(name)] = int(b)
+    else:
+        for i, b in enumerate(table):
+            o.class_blocks_per_cu[i] = int(b)
+
+
 def match_many(ctx: "Context", dg: "DeviceGraph", plans: Sequence["Plan"], roots=None, flags: int = 0,
-               workers: int = 4, max_work: int = 0, blocks_per_cu: int = 0, expand_threads: int = 0):
+               workers: int = 4, max_work: int = 0, blocks_per_cu: int = 0, expand_threads: int = 0,
+               class_blocks_per_cu=None):
     """Many queries against one resident graph on `workers` concurrent streams
     (cuts_match_many).  roots: None or one (first, end, stride) per query.
     Returns (results in query order, host wall ms of the whole set)."""
@@ -461,6 +478,7 @@ def match_many(ctx: "Context", dg: "DeviceGraph", plans: Sequence["Plan"], roots
         opts[i].max_work = max_work
         opts[i].blocks_per_cu = blocks_per_cu
         opts[i].expand_threads = expand_threads
+        _set_class_bpc(opts[i], class_blocks_per_cu)
         if roots is not None and roots[i] is not None:
             opts[i].root_begin, opts[i].root_end, opts[i].root_stride = roots[i]
     res = (_Result * max(n, 1))()

@@ -16,6 +16,9 @@ Tunables:
                   src/device_funcs.cu:618-733) have no counterpart to tune:
                   every lane here owns whole candidates of the flattened
                   work list, and items per lane is the width that varies;
+  per class    -- blocks per CU of each tiled kernel class (tune_classes):
+                  one profiled sweep of uniform values, each class taking its
+                  fastest, kept if the table beats the best uniform value;
   batch        -- blocks per CU of the level launches (2..8), the candidate
                   work at which a new work item is started (the chunk size),
                   the level-0 chunk (roots per initial item) and the queue
@@ -29,7 +32,7 @@ import statistics
 import time
 from typing import Dict, List, Optional, Sequence
 
-from . import FLAG_NO_XCD, Batch, Context, DeviceGraph, Plan
+from . import FLAG_NO_XCD, FLAG_PROFILE, Batch, Context, DeviceGraph, Plan
 
 CACHE = os.environ.get("CUTS_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "cuts_amd", "tune.json"))
 
@@ -88,6 +91,83 @@ def tune_single(ctx: Context, dg: DeviceGraph, plans: Sequence[Plan], key: str, 
                     log(f"tune single: {pt}")
     best = min(points, key=lambda p: p["ms"])
     res = dict(best=best, points=points, counts=ref_counts)
+    cache[key] = res
+    _store(cache)
+    return res
+
+
+TILED_CLASSES = ("expand_write", "expand_count", "expand_fused", "pair", "common", "chain")
+
+
+def tune_classes(ctx: Context, dg: DeviceGraph, plans: Sequence[Plan], key: str, roots=None,
+                 grid: Sequence[int] = (1, 2, 3, 4, 6, 8), expand_threads: int = 0, reps: int = 2, log=None,
+                 use_cache: bool = True) -> Dict:
+    """Blocks per CU for each tiled kernel class from one profiled sweep.
+
+    Every plan runs at each uniform blocks-per-CU value with per-launch HIP
+    events (FLAG_PROFILE); each tiled class (expansions, intersection walks,
+    chain level) takes the value at which its summed time was least -- list
+    walks at random addresses often prefer fewer resident blocks (less L2
+    thrashing) than the counting expansions.  The table is then timed on the
+    plain path against the best uniform value and kept only if faster.
+    Every run must count the same."""
+    cache = _load()
+    if use_cache and key in cache:
+        return cache[key]
+    roots = list(roots) if roots is not None else [None] * len(plans)
+    ref_counts = None
+
+    def check(counts, what):
+        nonlocal ref_counts
+        if ref_counts is None:
+            ref_counts = counts
+        elif counts != ref_counts:
+            raise CountMismatch(f"counts differ at {what}")
+
+    def plain_ms(**kw):
+        best = None
+        for _ in range(reps):
+            t, counts = 0.0, []
+            for p, rt in zip(plans, roots):
+                r = ctx.match(dg, p, roots=rt, expand_threads=expand_threads, **kw)
+                t += r.device_ms
+                counts.append(r.count)
+            check(counts, str(kw))
+            best = t if best is None else min(best, t)
+        return best
+
+    per: Dict[int, Dict[str, float]] = {}
+    for b in grid:
+        cls: Dict[str, float] = {}
+        for _ in range(reps):
+            acc: Dict[str, float] = {}
+            counts = []
+            for p, rt in zip(plans, roots):
+                r = ctx.match(dg, p, roots=rt, flags=FLAG_PROFILE, blocks_per_cu=b, expand_threads=expand_threads)
+                counts.append(r.count)
+                for name, (ms, n, _) in r.kernels.items():
+                    if n:
+                        acc[name] = acc.get(name, 0.0) + ms
+            check(counts, f"blocks_per_cu={b} (profiled)")
+            for name, ms in acc.items():
+                cls[name] = min(cls.get(name, ms), ms)
+        per[b] = cls
+        if log:
+            log(f"tune classes: blocks_per_cu {b}: " + ", ".join(f"{k} {v:.1f}" for k, v in sorted(cls.items())))
+    table = {}
+    for name in TILED_CLASSES:
+        seen = [(per[b][name], b) for b in grid if name in per[b]]
+        if seen:
+            table[name] = min(seen)[1]
+    uniform = {b: plain_ms(blocks_per_cu=b) for b in grid}
+    ub = min(uniform, key=uniform.get)
+    t_table = plain_ms(blocks_per_cu=ub, class_blocks_per_cu=table)
+    keep = t_table < uniform[ub]
+    res = dict(blocks_per_cu=ub, class_blocks_per_cu=table if keep else {}, uniform_ms=uniform,
+               table_ms=t_table, per_class_ms={str(b): v for b, v in per.items()}, counts=ref_counts)
+    if log:
+        log(f"tune classes: uniform best {ub} ({uniform[ub]:.1f} ms), per-class table {table} "
+            f"({t_table:.1f} ms){' kept' if keep else ' not kept'}")
     cache[key] = res
     _store(cache)
     return res

@@ -123,6 +123,7 @@ cuts::MatchOpts to_opts(const cuts_match_options* o) {
   m.time_limit_ms = o->time_limit_ms;
   m.max_work = o->max_work;
   m.expand_threads = o->expand_threads;
+  for (int i = 0; i < 16; ++i) m.class_bpc[i] = o->class_blocks_per_cu[i];
   return m;
 }
 

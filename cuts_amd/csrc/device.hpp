@@ -195,6 +195,7 @@ struct MatchOpts {
   double time_limit_ms = 0;
   uint64_t max_work = 0;
   uint32_t expand_threads = 0;
+  uint8_t class_bpc[16] = {0};  // per kernel class blocks per CU (0: blocks_per_cu)
 };
 struct MatchResult {
   uint64_t count = 0;

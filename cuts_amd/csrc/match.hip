@@ -610,16 +610,86 @@ __device__ __forceinline__ uint64_t closed_form(const GView& g, const KPlan& P, 
   const bool need_deg = P.sc_thr > 1;
   const bool newest = P.sc_pos == (uint32_t)(N - 1);
   bytes += 8 + (tbl ? 4 : 0);
+  // Membership of every position's vertex in N(u) -- or of u in that
+  // vertex's list when u is the newest position -- as independent tests
+  // issued together: hub rows, then bit words and list bounds, then the
+  // lower-bound searches of the other lists in lock-step (one load per
+  // search per step), then their final compares.  Positions the query makes
+  // adjacent to u are members without a test.
+  const uint32_t all = (N >= 32) ? ~0u : ((1u << N) - 1u);
+  uint32_t inm = P.sc_adjmask & all;
+  const uint32_t need = ~P.sc_adjmask & all;
+  int32_t h[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    const uint32_t x = r[i];
-    bool in = (P.sc_adjmask >> i) & 1;
-    if (!in) {
-      if (newest && i != N - 1) in = adj_has(g, x, u, bytes);
-      else in = adj_has(g, u, x, bytes);
+    const uint32_t own = (newest && i != N - 1) ? r[i] : u;
+    h[i] = (g.hub_id && ((need >> i) & 1)) ? g.hub_id[own] : -1;
+  }
+  uint64_t bw[N];
+  uint32_t base[N], n[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const uint32_t own = (newest && i != N - 1) ? r[i] : u;
+    const uint32_t val = (newest && i != N - 1) ? u : r[i];
+    bw[i] = g.hub_id ? g.bits[h[i] >= 0 ? (uint64_t)h[i] * g.words + (val >> 6) : 0] : 0ull;
+    base[i] = g.off_out[own];
+    n[i] = g.off_out[own + 1] - base[i];
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const uint32_t val = (newest && i != N - 1) ? u : r[i];
+    if (!((need >> i) & 1)) {
+      n[i] = 0;
+    } else if (h[i] >= 0) {
+      if ((bw[i] >> (val & 63)) & 1ull) inm |= 1u << i;
+      n[i] = 0;
+      bytes += 12;
+    } else {
+      bytes += (g.hub_id ? 4 : 0) + 8 + bs_bytes(n[i]);
     }
-    if (in && need_deg) bytes += 4;
-    if (in && (!need_deg || g.sig[2 * (uint64_t)x] >= P.sc_thr)) --c;
+  }
+  // lock-step searches: while a list has more than one candidate slot, halve
+  // it (base stays <= the value's position < base + n)
+  while (true) {
+    bool more = false;
+#pragma unroll
+    for (int i = 0; i < N; ++i) more = more || n[i] > 1;
+    if (!more) break;
+    uint32_t w[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) w[i] = g.adj_out[n[i] > 1 ? base[i] + (n[i] >> 1) : 0];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const uint32_t val = (newest && i != N - 1) ? u : r[i];
+      if (n[i] > 1) {
+        const uint32_t half = n[i] >> 1;
+        if (w[i] <= val) base[i] += half;
+        n[i] -= half;
+      }
+    }
+  }
+  {
+    uint32_t w[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) w[i] = g.adj_out[n[i] ? base[i] : 0];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const uint32_t val = (newest && i != N - 1) ? u : r[i];
+      if (n[i] && w[i] == val) inm |= 1u << i;
+    }
+  }
+  if (need_deg) {
+    uint32_t d[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) d[i] = ((inm >> i) & 1) ? g.sig[2 * (uint64_t)r[i]] : 0u;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+      if ((inm >> i) & 1) {
+        bytes += 4;
+        if (d[i] >= P.sc_thr) --c;
+      }
+  } else {
+    c -= __popc(inm);
   }
   return (uint64_t)c;
 }
@@ -903,6 +973,23 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
             if (in[j] && idx[j] == 0) bytes += 4 * L + 9 + 8 * nj;
           }
         }
+        // hub rows of the first constraints' vertices: they depend on the
+        // row only, so their loads go out with the list base's instead of
+        // after the candidate's
+        constexpr int NJP = 3;
+        int32_t hpre[NJP][G];
+#pragma unroll
+        for (int cc = 0; cc < NJP; ++cc) {
+          const uint32_t jcc = cc < (int)nj ? P.jc[L][cc] : 0u;
+#pragma unroll
+          for (int j = 0; j < G; ++j) {
+            uint32_t uu = 0;
+#pragma unroll
+            for (int l = 0; l < L; ++l)
+              if (l == (int)(jcc & 31)) uu = row[j][l];
+            hpre[cc][j] = (g.hub_id && cc < (int)nj) ? g.hub_id[uu] : -1;
+          }
+        }
 #pragma unroll
         for (int j = 0; j < G; ++j) {
           jc[j] = P.jc[L][sel[j]];
@@ -937,7 +1024,13 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
             int32_t h[G];
             uint64_t bw[G];
 #pragma unroll
-            for (int j = 0; j < G; ++j) h[j] = g.hub_id[u2[j]];
+            for (int j = 0; j < G; ++j) {
+              int32_t hh = 0;
+#pragma unroll
+              for (int cc = 0; cc < NJP; ++cc)
+                if ((int)c == cc) hh = hpre[cc][j];
+              h[j] = c < (uint32_t)NJP ? hh : g.hub_id[u2[j]];
+            }
 #pragma unroll
             for (int j = 0; j < G; ++j)
               bw[j] = g.bits[h[j] >= 0 ? (uint64_t)h[j] * g.words + (v[j] >> 6) : 0];
@@ -1520,20 +1613,30 @@ struct CommonArgs {
   uint32_t pa, pb;  // row positions of u and w
   uint32_t thr;     // degree threshold of a common neighbour
   uint32_t mode;    // 0: count -= weight[p] per hit; 1: count -= 1 per hit; 2: weight[p] += hits
+  // null, or per prefix {first list word of the walked vertex, hub row of the
+  // tested one (~0u: none), its list bounds}: one 16-byte load per item in
+  // place of the row, selector, offset and hub-row loads
+  const uint4* desc;
 };
 
 // Pair level: the work of the rows (prefixes) that have survivors.
 template <int L>
 __global__ __launch_bounds__(BLOCK) void k_pair_prep(GView g, KPlan P, const uint32_t* __restrict__ rows, uint64_t n,
                                                     const uint32_t* __restrict__ nsurv, uint32_t* __restrict__ minlen,
-                                                    uint8_t* __restrict__ minsel) {
+                                                    uint8_t* __restrict__ minsel, uint4* __restrict__ desc) {
   for (uint64_t p = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (uint64_t)gridDim.x * BLOCK) {
     uint32_t len = 0, sel = 0;
     if (nsurv[p]) {
       const uint32_t u = rows[p * L + P.pr_a], w = rows[p * L + P.pr_b];
-      const uint32_t du = g.off_out[u + 1] - g.off_out[u], dw = g.off_out[w + 1] - g.off_out[w];
+      const uint32_t ul = g.off_out[u], uh = g.off_out[u + 1], wl = g.off_out[w], wh = g.off_out[w + 1];
+      const uint32_t du = uh - ul, dw = wh - wl;
       len = min(du, dw);
       sel = du <= dw ? 0u : 1u;
+      // walk the shorter list, test membership in the other
+      const uint32_t oth = sel ? u : w;
+      const int32_t h = g.hub_id ? g.hub_id[oth] : -1;
+      desc[p] = sel ? make_uint4(wl, h >= 0 ? (uint32_t)h : ~0u, ul, uh)
+                    : make_uint4(ul, h >= 0 ? (uint32_t)h : ~0u, wl, wh);
     }
     minlen[p] = len;
     minsel[p] = (uint8_t)sel;
@@ -1627,48 +1730,85 @@ __global__ __launch_bounds__(BLOCK) void k_common(GView g, KPlan P, ExpandArgs a
     }
     uint64_t p[IPT];
     uint32_t own[IPT], oth[IPT], li[IPT], x[IPT];
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-      const uint32_t pj = min(pi[j], span - 1);
-      p[j] = (uint64_t)plo + pj;
-      const uint64_t pbase = lds ? s_off[pj] : a.work_off[p[j]];
-      const uint32_t u = a.rows[p[j] * L + c.pa], wv = a.rows[p[j] * L + c.pb];
-      const bool sel = a.minsel[p[j]] != 0;
-      own[j] = sel ? wv : u;
-      oth[j] = sel ? u : wv;
-      li[j] = g.off_out[own[j]] + (uint32_t)(w[j] - pbase);
-    }
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) x[j] = in[j] ? g.adj_out[li[j]] : 0u;
     bool hit[IPT];
-    if (g.hub_id) {
-      int32_t h[IPT];
+    if (c.desc) {
+      // per-prefix descriptor: walked list's base, tested vertex's hub row
+      // or list bounds
+      uint4 d[IPT];
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const uint32_t pj = min(pi[j], span - 1);
+        p[j] = (uint64_t)plo + pj;
+        d[j] = c.desc[p[j]];
+      }
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) {
+        const uint32_t pj = min(pi[j], span - 1);
+        const uint64_t pbase = lds ? s_off[pj] : a.work_off[p[j]];
+        li[j] = d[j].x + (uint32_t)(w[j] - pbase);
+      }
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) x[j] = in[j] ? g.adj_out[li[j]] : 0u;
       uint64_t bw[IPT];
 #pragma unroll
-      for (int j = 0; j < IPT; ++j) h[j] = g.hub_id[oth[j]];
-#pragma unroll
-      for (int j = 0; j < IPT; ++j) bw[j] = (in[j] && h[j] >= 0) ? g.bits[(uint64_t)h[j] * g.words + (x[j] >> 6)] : 0;
+      for (int j = 0; j < IPT; ++j)
+        bw[j] = (in[j] && d[j].y != ~0u) ? g.bits[(uint64_t)d[j].y * g.words + (x[j] >> 6)] : 0ull;
 #pragma unroll
       for (int j = 0; j < IPT; ++j) {
         if (!in[j]) {
           hit[j] = false;
-        } else if (h[j] >= 0) {
+        } else if (d[j].y != ~0u) {
           hit[j] = (bw[j] >> (x[j] & 63)) & 1ull;
-          if (a.stats) bytes += 12;
+          if (a.stats) bytes += 8;
         } else {
-          const uint32_t lo2 = g.off_out[oth[j]], hi2 = g.off_out[oth[j] + 1];
-          hit[j] = member_safe(g.adj_out, lo2, hi2, x[j]);
-          if (a.stats) bytes += 12 + bs_bytes(hi2 - lo2);
+          hit[j] = member_safe(g.adj_out, d[j].z, d[j].w, x[j]);
+          if (a.stats) bytes += bs_bytes(d[j].w - d[j].z);
         }
+        if (a.stats && in[j]) bytes += 16;  // the descriptor
       }
     } else {
 #pragma unroll
       for (int j = 0; j < IPT; ++j) {
-        hit[j] = false;
-        if (!in[j]) continue;
-        const uint32_t lo2 = g.off_out[oth[j]], hi2 = g.off_out[oth[j] + 1];
-        hit[j] = member_safe(g.adj_out, lo2, hi2, x[j]);
-        if (a.stats) bytes += 8 + bs_bytes(hi2 - lo2);
+        const uint32_t pj = min(pi[j], span - 1);
+        p[j] = (uint64_t)plo + pj;
+        const uint64_t pbase = lds ? s_off[pj] : a.work_off[p[j]];
+        const uint32_t u = a.rows[p[j] * L + c.pa], wv = a.rows[p[j] * L + c.pb];
+        const bool sel = a.minsel[p[j]] != 0;
+        own[j] = sel ? wv : u;
+        oth[j] = sel ? u : wv;
+        li[j] = g.off_out[own[j]] + (uint32_t)(w[j] - pbase);
+      }
+#pragma unroll
+      for (int j = 0; j < IPT; ++j) x[j] = in[j] ? g.adj_out[li[j]] : 0u;
+      if (g.hub_id) {
+        int32_t h[IPT];
+        uint64_t bw[IPT];
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) h[j] = g.hub_id[oth[j]];
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) bw[j] = (in[j] && h[j] >= 0) ? g.bits[(uint64_t)h[j] * g.words + (x[j] >> 6)] : 0;
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+          if (!in[j]) {
+            hit[j] = false;
+          } else if (h[j] >= 0) {
+            hit[j] = (bw[j] >> (x[j] & 63)) & 1ull;
+            if (a.stats) bytes += 12;
+          } else {
+            const uint32_t lo2 = g.off_out[oth[j]], hi2 = g.off_out[oth[j] + 1];
+            hit[j] = member_safe(g.adj_out, lo2, hi2, x[j]);
+            if (a.stats) bytes += 12 + bs_bytes(hi2 - lo2);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+          hit[j] = false;
+          if (!in[j]) continue;
+          const uint32_t lo2 = g.off_out[oth[j]], hi2 = g.off_out[oth[j] + 1];
+          hit[j] = member_safe(g.adj_out, lo2, hi2, x[j]);
+          if (a.stats) bytes += 8 + bs_bytes(hi2 - lo2);
+        }
       }
     }
     // a common neighbour of two distinct vertices has degree >= 2, so only
@@ -1944,6 +2084,12 @@ struct Matcher {
   uint64_t budget = 0;
   uint64_t level_cap[kMaxQuery] = {0};
   uint32_t grid_cap = 0;
+  // blocks of a tiled kernel of class cls (CUTS_K_*): the class's own
+  // blocks per CU when one is set, else the run's
+  uint32_t gcap(int cls) const {
+    const uint32_t b = o.class_bpc[cls] ? o.class_bpc[cls] : (o.blocks_per_cu ? o.blocks_per_cu : 6u);
+    return (uint32_t)c.cus * b;
+  }
 
   // streamed embedding output: chunks of the last level copied so far (the
   // output buffer of chunk i is i & 1), and whether copies are streamed
@@ -2182,7 +2328,7 @@ struct Matcher {
       a.xcd = (o.flags & 8u) ? 0 : 1;
       a.tilectr = (unsigned int*)(ctr + 4);  // 8 x uint32 in counters[4..7], cleared by k_tile_index
       a.stats = prof ? 1 : 0;
-      uint32_t grid = grid_for(nt, 1, grid_cap);
+      uint32_t grid = grid_for(nt, 1, gcap(mclass));
       // counting expansions: nthr threads per block, cgrow times the blocks
       // (as many lanes), at most one block per tile
       const uint32_t nthr = o.expand_threads ? o.expand_threads : (uint32_t)BLOCK;
@@ -2307,7 +2453,8 @@ struct Matcher {
     a.xcd = (o.flags & 8u) ? 0 : 1;
     a.tilectr = (unsigned int*)(ctr + 4);
     a.stats = 0;
-    const uint32_t grid = grid_for(tmax, 1, grid_cap);
+    const int mclass = mode == M_WRITE ? 4 : mode == M_COUNT ? 5 : mode == M_FUSED ? 6 : (mode == M_PAIR || mode == M_COLLAPSE) ? 8 : 10;
+    const uint32_t grid = grid_for(tmax, 1, gcap(mclass));
     const uint32_t nthr = o.expand_threads ? o.expand_threads : (uint32_t)BLOCK;
     const uint32_t cgrow = (uint32_t)(BLOCK / nthr);
     r.chunks++;
@@ -2366,12 +2513,14 @@ struct Matcher {
   void pair_intersections(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in, bool spec = false) {
     const uint32_t grid = grid_for(n, BLOCK, grid_cap);
     pbegin(false);
-#define PP(LL) hipLaunchKernelGGL((k_pair_prep<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.nsurv, in.minlen, in.minsel)
+#define PP(LL) hipLaunchKernelGGL((k_pair_prep<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.nsurv, in.minlen, in.minsel, in.pbase)
     CUTS_DISPATCH_PAIR(L, PP)
 #undef PP
     HIPCHECK(hipGetLastError());
     pend(1, (double)n * (4.0 * 2 + 16.0 + 5.0), false);
-    common_pass(L, rows, n, in, CommonArgs{kp.pr_a, kp.pr_b, kp.pr_tab, 0u}, in.nsurv, spec);
+    // the pair level's bases (in.pbase) were read by the expansion that has
+    // run; the same per-prefix slots now take the walk's descriptors
+    common_pass(L, rows, n, in, CommonArgs{kp.pr_a, kp.pr_b, kp.pr_tab, 0u, in.pbase}, in.nsurv, spec);
   }
 
   // Counts common neighbours over the work planned in in.minlen / in.minsel
@@ -2394,7 +2543,7 @@ struct Matcher {
       a.tilectr = (unsigned int*)(ctr + 4);
       a.stats = 0;
       r.chunks++;
-#define PCS(LL) hipLaunchKernelGGL((k_common<LL>), dim3(grid_for(tmax, 1, grid_cap)), dim3(BLOCK), 0, c.stream, g, kp, a, ca)
+#define PCS(LL) hipLaunchKernelGGL((k_common<LL>), dim3(grid_for(tmax, 1, gcap(9))), dim3(BLOCK), 0, c.stream, g, kp, a, ca)
       CUTS_DISPATCH_L(L, PCS)
 #undef PCS
       HIPCHECK(hipGetLastError());
@@ -2427,7 +2576,7 @@ struct Matcher {
       a.stats = prof ? 1 : 0;
       r.chunks++;
       pbegin(true);
-#define PC(LL) hipLaunchKernelGGL((k_common<LL>), dim3(grid_for(nt, 1, grid_cap)), dim3(BLOCK), 0, c.stream, g, kp, a, ca)
+#define PC(LL) hipLaunchKernelGGL((k_common<LL>), dim3(grid_for(nt, 1, gcap(9))), dim3(BLOCK), 0, c.stream, g, kp, a, ca)
       CUTS_DISPATCH_L(L, PC)
 #undef PC
       HIPCHECK(hipGetLastError());
@@ -2476,7 +2625,7 @@ struct Matcher {
 #undef CP
       HIPCHECK(hipGetLastError());
       pend(1, (double)n * (4.0 * 2 + 16.0 + 5.0), false);
-      common_pass(L, rows, n, in, CommonArgs{kp.ch_u, i, kp.ch_ta, 2u}, in.nsurv);
+      common_pass(L, rows, n, in, CommonArgs{kp.ch_u, i, kp.ch_ta, 2u, nullptr}, in.nsurv);
     }
   }
 
@@ -2509,7 +2658,7 @@ struct Matcher {
       a.tilectr = (unsigned int*)(ctr + 4);
       a.stats = prof ? 1 : 0;
       r.chunks++;
-      const uint32_t grid = grid_for(nt, 1, grid_cap);
+      const uint32_t grid = grid_for(nt, 1, gcap(4));
       pbegin(true);
 #define GW(LL) hipLaunchKernelGGL((k_expand<LL, M_GWRITE>), dim3(grid), dim3(BLOCK), 0, c.stream, g, pk, a)
       CUTS_DISPATCH_PAIR(L, GW)
@@ -2609,7 +2758,7 @@ struct Matcher {
       a.stats = prof ? 1 : 0;
       r.chunks++;
       pbegin(true);
-#define SB(LL) hipLaunchKernelGGL((k_chain_sib<LL>), dim3(grid_for(nt, 1, grid_cap)), dim3(BLOCK), 0, c.stream, g, kp, a, ctab)
+#define SB(LL) hipLaunchKernelGGL((k_chain_sib<LL>), dim3(grid_for(nt, 1, gcap(10))), dim3(BLOCK), 0, c.stream, g, kp, a, ctab)
       CUTS_DISPATCH_PAIR(L, SB)
 #undef SB
       HIPCHECK(hipGetLastError());
@@ -2630,7 +2779,7 @@ struct Matcher {
     pend(10, 0, true);
     const bool arc = g.arc_cn && ((kp.qadj[kp.ch_u] >> (L - 1)) & 1) && kp.ch_ta <= 2;
     if (!(g.h2_off && kp.ch_ta <= 2) && !arc)
-      common_pass(L, rows, n, in, CommonArgs{kp.ch_u, L - 1, kp.ch_ta, 1u}, nullptr);
+      common_pass(L, rows, n, in, CommonArgs{kp.ch_u, L - 1, kp.ch_ta, 1u, nullptr}, nullptr);
   }
 
   void run() {
@@ -2654,6 +2803,9 @@ struct Matcher {
     // (the tiles are taken dynamically, so the grid only sets residency)
     if (o.expand_threads != 0 && o.expand_threads != 64 && o.expand_threads != 128 && o.expand_threads != 256)
       throw Error(-1, "expand_threads must be 64, 128 or 256 (or 0)");
+    if (o.blocks_per_cu > 64) throw Error(-1, "blocks_per_cu must be at most 64");
+    for (int i = 0; i < 16; ++i)
+      if (o.class_bpc[i] > 64) throw Error(-1, "class_blocks_per_cu entries must be at most 64");
     grid_cap = (uint32_t)c.cus * (o.blocks_per_cu ? o.blocks_per_cu : 6);
     budget = o.budget ? o.budget : c.default_budget;
     // split the budget evenly over the materialised levels 1..k-2 (or k-1

@@ -468,7 +468,8 @@ def judge(ctx: Context, dg, queries: Sequence[Tuple[int, List[Tuple[int, int]]]]
 def judge_by_time(ctx: Context, dg, queries: Sequence[Tuple[int, List[Tuple[int, int]]]], heuristic_plans,
                   heuristic_counts: Sequence[int], choices: Sequence[OrderChoice], roots=None, flags: int = 0,
                   heuristic_work: Optional[Sequence[int]] = None, reps: int = 2,
-                  margin: float = 0.03, blocks_per_cu: int = 0, expand_threads: int = 0) -> List["TimedJudged"]:
+                  margin: float = 0.03, blocks_per_cu: int = 0, expand_threads: int = 0,
+                  class_blocks_per_cu=None) -> List["TimedJudged"]:
     """The score order judges whether a proposal actually runs faster: both
     orders are run warm `reps` times on the same GPU and the proposal is kept
     only if its best device time beats the score order's by `margin`
@@ -478,7 +479,7 @@ def judge_by_time(ctx: Context, dg, queries: Sequence[Tuple[int, List[Tuple[int,
     stopped, never waited for).  A kept order's count must equal the score
     order's (order never changes a count); anything else raises.  Both
     orders run with the given launch shape (0 = defaults)."""
-    shape = dict(blocks_per_cu=blocks_per_cu, expand_threads=expand_threads)
+    shape = dict(blocks_per_cu=blocks_per_cu, expand_threads=expand_threads, class_blocks_per_cu=class_blocks_per_cu)
     out = []
     for i, ((k, edges), hp, hc, ch) in enumerate(zip(queries, heuristic_plans, heuristic_counts, choices)):
         rt = None if roots is None else roots[i]

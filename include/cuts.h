@@ -172,6 +172,11 @@ typedef struct {
    * and last-level kernels): 64, 128 or 256 (0 = 256).  A tile stays 1,024
    * work items, so each lane carries 16, 8 or 4 of them. */
   uint32_t expand_threads;
+  /* blocks per CU of the tiled kernels of each class (index CUTS_K_*: the
+   * expansions, the intersection walks, the chain level); 0 = blocks_per_cu.
+   * Kernels that walk lists at random addresses often run faster with fewer
+   * blocks (less L2 thrashing) than the counting ones. */
+  uint8_t class_blocks_per_cu[16];
 } cuts_match_options;
 
 typedef struct {
