@@ -343,73 +343,86 @@ __device__ __forceinline__ void pair_bases(const GView& g, const KPlan& P, const
 // (without it, k_common counts that afterwards, only for prefixes with
 // survivors).  When the pair level collapses (pr_collapse) it also counts the
 // prefix's completions into counters[0..2] and gives the prefix no work.
+// The planner of one prefix row p: the shortest constraint list (minlen,
+// minsel), the per-prefix closed-form base (cbase) and the pair level's
+// bases (pbase); a collapsing pair level's count is added to acc/surv/sna.
+// Returns the list length (0 for a collapsed prefix).
 template <int L>
-__global__ __launch_bounds__(BLOCK) void k_plan(GView g, KPlan P, const uint32_t* __restrict__ rows, uint64_t n,
-                                               uint32_t* __restrict__ minlen, uint8_t* __restrict__ minsel,
-                                               uint32_t* __restrict__ cbase, const uint32_t* __restrict__ tbl,
-                                               uint4* __restrict__ pbase, PairTables pt,
-                                               unsigned long long* __restrict__ merge,
-                                               unsigned long long* __restrict__ counters) {
+__device__ __forceinline__ uint32_t plan_row(const GView& g, const KPlan& P, const uint32_t* __restrict__ rows,
+                                             uint64_t p, uint8_t* __restrict__ minsel, uint32_t* __restrict__ cbase,
+                                             const uint32_t* __restrict__ tbl, uint4* __restrict__ pbase,
+                                             const PairTables& pt, bool collapse, bool merge, uint64_t& mb,
+                                             uint64_t& acc, uint64_t& surv, uint64_t& sna) {
   const uint32_t nj = P.nj[L];
-  const bool collapse = pbase && P.pr_collapse && (g.h2_off || (g.arc_cn && P.pr_adj));
-  uint64_t mb = 0, acc = 0, surv = 0, sna = 0;
-  for (uint64_t p = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (uint64_t)gridDim.x * BLOCK) {
-    uint32_t row[L];
+  uint32_t row[L];
 #pragma unroll
-    for (int l = 0; l < L; ++l) row[l] = rows[p * L + l];
-    uint32_t best = 0xFFFFFFFFu, sel = 0;
-    uint64_t sum = 0;
-    for (uint32_t c = 0; c < nj; ++c) {
-      const uint32_t jc = P.jc[L][c];
-      const uint32_t pos = jc & 31;
-      uint32_t u = 0;
+  for (int l = 0; l < L; ++l) row[l] = rows[p * L + l];
+  uint32_t best = 0xFFFFFFFFu, sel = 0;
+  uint64_t sum = 0;
+  for (uint32_t c = 0; c < nj; ++c) {
+    const uint32_t jc = P.jc[L][c];
+    const uint32_t pos = jc & 31;
+    uint32_t u = 0;
 #pragma unroll
-      for (int l = 0; l < L; ++l)
-        if (l == (int)pos) u = row[l];
-      const uint32_t* off = (jc >> 7) ? g.off_in : g.off_out;
-      const uint32_t len = off[u + 1] - off[u];
-      sum += len;
-      if (len < best) { best = len; sel = c; }
+    for (int l = 0; l < L; ++l)
+      if (l == (int)pos) u = row[l];
+    const uint32_t* off = (jc >> 7) ? g.off_in : g.off_out;
+    const uint32_t len = off[u + 1] - off[u];
+    sum += len;
+    if (len < best) { best = len; sel = c; }
+  }
+  mb += 8ull * nj + 4ull * sum;
+  minsel[p] = (uint8_t)sel;
+  if (cbase) {
+    uint32_t u = 0;
+#pragma unroll
+    for (int l = 0; l < L; ++l)
+      if (l == (int)P.sc_pos) u = row[l];
+    int64_t c = tbl ? (int64_t)tbl[u] : (int64_t)(g.off_out[u + 1] - g.off_out[u]);
+    uint64_t dummy = 0;
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+      const bool in = ((P.sc_adjmask >> l) & 1) || adj_has(g, u, row[l], dummy);
+      if (in && (P.sc_thr <= 1 || g.sig[2 * (uint64_t)row[l]] >= P.sc_thr)) --c;
     }
-    mb += 8ull * nj + 4ull * sum;
-    minlen[p] = best;
-    minsel[p] = (uint8_t)sel;
-    if (cbase) {
-      uint32_t u = 0;
-#pragma unroll
-      for (int l = 0; l < L; ++l)
-        if (l == (int)P.sc_pos) u = row[l];
-      int64_t c = tbl ? (int64_t)tbl[u] : (int64_t)(g.off_out[u + 1] - g.off_out[u]);
-      uint64_t dummy = 0;
-#pragma unroll
-      for (int l = 0; l < L; ++l) {
-        const bool in = ((P.sc_adjmask >> l) & 1) || adj_has(g, u, row[l], dummy);
-        if (in && (P.sc_thr <= 1 || g.sig[2 * (uint64_t)row[l]] >= P.sc_thr)) --c;
+    cbase[p] = (uint32_t)c;
+  }
+  if (pbase) {
+    uint32_t u = 0, w = 0;
+    int64_t ca = 0, cb = 0, cab = 0;
+    uint64_t dummy = 0;
+    pair_bases<L>(g, P, row, pt, u, w, ca, cb, cab, dummy);
+    pbase[p] = make_uint4((uint32_t)ca, (uint32_t)cb, (uint32_t)(int32_t)cab, 0u);
+    if (collapse) {
+      // position k-3's survivors are N(u) & N(w) minus the prefix (cab of
+      // them: with thresholds <= 2 every common neighbour qualifies), each
+      // inside both lists: the prefix's count without expanding it
+      const int64_t ns = cab;
+      acc += (uint64_t)(ns * ((ca - 1) * (cb - 1) - (cab - 1)));
+      surv += (uint64_t)ns;
+      sna += (uint64_t)(ns * (ca - 1));
+      if (merge) {
+        const uint64_t du = g.off_out[u + 1] - g.off_out[u], dw = g.off_out[w + 1] - g.off_out[w];
+        mb += (uint64_t)ns * (24 + 4 * du + (uint64_t)(ca - 1) * (24 + 4 * dw));
       }
-      cbase[p] = (uint32_t)c;
-    }
-    if (pbase) {
-      uint32_t u = 0, w = 0;
-      int64_t ca = 0, cb = 0, cab = 0;
-      uint64_t dummy = 0;
-      pair_bases<L>(g, P, row, pt, u, w, ca, cb, cab, dummy);
-      pbase[p] = make_uint4((uint32_t)ca, (uint32_t)cb, (uint32_t)(int32_t)cab, 0u);
-      if (collapse) {
-        // position k-3's survivors are N(u) & N(w) minus the prefix (cab of
-        // them: with thresholds <= 2 every common neighbour qualifies), each
-        // inside both lists: the prefix's count without expanding it
-        const int64_t ns = cab;
-        acc += (uint64_t)(ns * ((ca - 1) * (cb - 1) - (cab - 1)));
-        surv += (uint64_t)ns;
-        sna += (uint64_t)(ns * (ca - 1));
-        if (merge) {
-          const uint64_t du = g.off_out[u + 1] - g.off_out[u], dw = g.off_out[w + 1] - g.off_out[w];
-          mb += (uint64_t)ns * (24 + 4 * du + (uint64_t)(ca - 1) * (24 + 4 * dw));
-        }
-        minlen[p] = 0;
-      }
+      best = 0;
     }
   }
+  return best;
+}
+
+template <int L>
+__global__ __launch_bounds__(BLOCK) void k_plan(GView g, KPlan P, const uint32_t* __restrict__ rows, uint64_t n,
+                                                uint32_t* __restrict__ minlen, uint8_t* __restrict__ minsel,
+                                                uint32_t* __restrict__ cbase, const uint32_t* __restrict__ tbl,
+                                                uint4* __restrict__ pbase, PairTables pt,
+                                                unsigned long long* __restrict__ merge,
+                                                unsigned long long* __restrict__ counters) {
+  const bool collapse = pbase && P.pr_collapse && (g.h2_off || (g.arc_cn && P.pr_adj));
+  uint64_t mb = 0, acc = 0, surv = 0, sna = 0;
+  for (uint64_t p = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (uint64_t)gridDim.x * BLOCK)
+    minlen[p] = plan_row<L>(g, P, rows, p, minsel, cbase, tbl, pbase, pt, collapse, merge != nullptr, mb, acc, surv,
+                            sna);
   if (merge) add_wave(merge, mb);
   if (collapse) {
     add_wave(&counters[0], acc);
@@ -417,6 +430,7 @@ __global__ __launch_bounds__(BLOCK) void k_plan(GView g, KPlan P, const uint32_t
     add_wave(&counters[2], sna);
   }
 }
+
 
 // -------------------------------------------------- exclusive scan (u32->u64)
 __device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v) {
@@ -587,6 +601,91 @@ __global__ __launch_bounds__(BLOCK) void k_tile_index_dyn(const uint64_t* __rest
       else hi = mid;
     }
     tile_p[t] = (uint32_t)(lo - 1);
+  }
+}
+
+// A level of at most SCAN_TILE rows in one block and one launch, in place of
+// k_plan + k_scan_single + k_tile_index_dyn: every row planned (thread t
+// takes rows t*SCAN_IPT .. +SCAN_IPT-1, the scan's blocked layout), the
+// exclusive scan of the list lengths (kept in LDS and written to work_off),
+// then the speculative tile index over [0, W) from the LDS copy, with the
+// same fit test, dyn / status / work words and counter clears as
+// k_tile_index_dyn.
+template <int L>
+__global__ __launch_bounds__(BLOCK) void k_plan_small(GView g, KPlan P, const uint32_t* __restrict__ rows, uint64_t n,
+                                                      uint32_t* __restrict__ minlen, uint8_t* __restrict__ minsel,
+                                                      uint32_t* __restrict__ cbase, const uint32_t* __restrict__ tbl,
+                                                      uint4* __restrict__ pbase, PairTables pt,
+                                                      unsigned long long* __restrict__ counters,
+                                                      uint64_t* __restrict__ work_off, uint64_t max_items,
+                                                      uint64_t tile_cap, uint64_t row_cap, uint32_t* __restrict__ tile_p,
+                                                      unsigned int* __restrict__ tilectr,
+                                                      unsigned int* __restrict__ outctr,
+                                                      unsigned long long* __restrict__ dyn,
+                                                      unsigned long long* __restrict__ status,
+                                                      unsigned long long* __restrict__ work) {
+  __shared__ uint64_t s_off[SCAN_TILE + 1];
+  __shared__ uint32_t s_len[SCAN_TILE];
+  __shared__ uint64_t s_tmp[BLOCK / 64];
+  const bool collapse = pbase && P.pr_collapse && (g.h2_off || (g.arc_cn && P.pr_adj));
+  uint64_t mb = 0, acc = 0, surv = 0, sna = 0;
+  // plan with consecutive threads on consecutive rows (every thread busy
+  // for small n), then scan in the blocked layout from LDS
+  for (uint64_t p = threadIdx.x; p < n; p += BLOCK) {
+    const uint32_t len = plan_row<L>(g, P, rows, p, minsel, cbase, tbl, pbase, pt, collapse, false, mb, acc, surv, sna);
+    minlen[p] = len;
+    s_len[p] = len;
+  }
+  __syncthreads();
+  uint32_t v[SCAN_IPT];
+  uint64_t sum = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_IPT; ++i) {
+    const uint64_t p = (uint64_t)threadIdx.x * SCAN_IPT + i;
+    v[i] = p < n ? s_len[p] : 0u;
+    sum += v[i];
+  }
+  if (collapse) {
+    add_wave(&counters[0], acc);
+    add_wave(&counters[1], surv);
+    add_wave(&counters[2], sna);
+  }
+  uint64_t W;
+  uint64_t ex = block_excl_scan(sum, s_tmp, &W);
+#pragma unroll
+  for (int i = 0; i < SCAN_IPT; ++i) {
+    const uint64_t p = (uint64_t)threadIdx.x * SCAN_IPT + i;
+    if (p < n) {
+      work_off[p] = ex;
+      s_off[p] = ex;
+    }
+    ex += v[i];
+  }
+  if (threadIdx.x == 0) {
+    work_off[n] = W;
+    s_off[n] = W;
+  }
+  const uint64_t nt = (W + TILE - 1) / TILE;
+  const bool fits = W <= max_items && nt + 1 <= tile_cap && W <= row_cap;
+  if (threadIdx.x < 8) tilectr[threadIdx.x] = 0u;
+  if (outctr && threadIdx.x < 2) outctr[threadIdx.x] = 0u;
+  if (threadIdx.x == 0) {
+    dyn[0] = fits ? W : 0ull;
+    dyn[1] = fits ? nt : 0ull;
+    if (fits) *work += W;
+    else *status |= 1ull;
+  }
+  __syncthreads();
+  if (!fits || W == 0) return;
+  for (uint64_t t = threadIdx.x; t <= nt; t += BLOCK) {
+    const uint64_t w = t < nt ? t * TILE : W - 1;
+    uint32_t lo = 0, hi = (uint32_t)n + 1;  // upper_bound over s_off[0..n]
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (s_off[mid] <= w) lo = mid + 1;
+      else hi = mid;
+    }
+    tile_p[t] = lo - 1;
   }
 }
 
@@ -2253,7 +2352,11 @@ struct Matcher {
     else if (L == k - 2 && fuse) mode = M_FUSED;
     const int mclass = mode == M_WRITE ? 4 : mode == M_COUNT ? 5 : mode == M_FUSED ? 6 : (mode == M_PAIR || mode == M_COLLAPSE) ? 8 : 10;
     if (mode == M_PAIR) ensure_aux(arena, in.pbase, in.pbase_cap, n);
-    {
+    const bool spec = speculate() && !(mode == M_WRITE && L == k - 1);
+    // a speculative level of at most SCAN_TILE rows plans, scans and indexes
+    // its tiles in one launch (k_plan_small, inside level_spec)
+    const bool small = spec && n <= (uint64_t)SCAN_TILE;
+    if (!small) {
       const uint32_t grid = grid_for(n, BLOCK, grid_cap);
       unsigned long long* merge = prof ? ctr + 8 : nullptr;
       if (merge) HIPCHECK(hipMemsetAsync(merge, 0, 8, c.stream));
@@ -2266,7 +2369,8 @@ struct Matcher {
       // the prefixes' list reads belong to the expansion that performs them
       if (merge) r.merge_bytes[mclass] += (double)read_u64(merge);
     }
-    if (speculate() && !(mode == M_WRITE && L == k - 1) && level_spec(L, rows, n, in, mode, use_cbase)) return;
+    if (spec && level_spec(L, rows, n, in, mode, use_cbase, small)) return;
+    // (a small level that did not fit has been planned: the scan below reads it)
     const uint64_t W = scan(in, n);
     if (W == 0) return;
     if (add_work(W)) return;
@@ -2416,8 +2520,8 @@ struct Matcher {
   // One level without the readback of its work (see speculate()).  Returns
   // false when a writing level did not fit: the caller then runs it with the
   // readback (its scan is repeated; nothing was written or counted).
-  bool level_spec(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in, int mode, bool use_cbase) {
-    scan_launch(in, n);
+  bool level_spec(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in, int mode, bool use_cbase, bool small) {
+    if (!small) scan_launch(in, n);
     LevelBuf& out = lv[L];
     uint64_t row_cap = ~0ull;
     const bool want_parent = (chain && !chain_fused && mode == M_WRITE && L == k - 3) ||
@@ -2434,7 +2538,19 @@ struct Matcher {
       ensure_aux(arena, in.nsurv, in.nsurv_cap, n);
       HIPCHECK(hipMemsetAsync(in.nsurv, 0, n * sizeof(uint32_t), c.stream));
     }
-    const uint64_t tmax = tile_index_dyn(in, n, row_cap, mode == M_WRITE ? (unsigned int*)(ctr + 10) : nullptr);
+    uint64_t tmax;
+    if (small) {
+      // plan + scan + tile index in one launch
+      ensure_tiles(arena, in, std::max<uint64_t>(in.tile_cap, 4097) - 1);
+      tmax = in.tile_cap - 1;
+      unsigned int* oc = mode == M_WRITE ? (unsigned int*)(ctr + 10) : nullptr;
+#define PSM(LL) hipLaunchKernelGGL((k_plan_small<LL>), dim3(1), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.minlen, in.minsel, use_cbase ? in.cbase : nullptr, tbl, mode == M_PAIR ? in.pbase : nullptr, ptbl, ctr, in.work_off, oc ? MAX_CHUNK : MAX_SPEC_ITEMS, in.tile_cap, row_cap, in.tile_p, (unsigned int*)(ctr + 4), oc, ctr + 14, oc ? ctr + 11 : ctr + 12, ctr + 13)
+      CUTS_DISPATCH_L(L, PSM)
+#undef PSM
+      HIPCHECK(hipGetLastError());
+    } else {
+      tmax = tile_index_dyn(in, n, row_cap, mode == M_WRITE ? (unsigned int*)(ctr + 10) : nullptr);
+    }
     ExpandArgs a{};
     a.rows = rows;
     a.work_off = in.work_off;
