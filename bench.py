@@ -14,7 +14,9 @@ the learned order -- the table-size network (weights trained on R-MAT
 scale-18 graphs by python -m cuts_amd.train, committed in cuts_amd/data)
 predicts |T| of every connected prefix set, a DP picks the order whose
 expanded levels are cheapest for this engine, and the score order judges
-it by running both (the proposal is kept only if it ran faster).
+it by running both (the proposal is kept only if it ran faster); the DP
+proposes up to five orders per query (distinct last-three tails), each
+timed, the fastest kept.
 Launch shape: before the timed steps the autotuner (cuts_amd.autotune)
 picks blocks per CU and the counting expansions' block size on every 5th
 evaluation query with the score orders; both orders then run with it.
@@ -412,7 +414,7 @@ def run_headline(args):
     choices = []
     for i in range(0, len(qs), 64):
         choices += LN.choose_orders(net, qs[i:i + 64], gfeat, [p.order for p in heur[i:i + 64]], keep_first=True,
-                                    eng=eng)
+                                    eng=eng, top=args.learned_top, margin=args.learned_margin)
     plan_s = time.perf_counter() - t0
     t0 = time.perf_counter()
     if args.orders_in:
@@ -432,6 +434,7 @@ def run_headline(args):
         with open(args.orders_out, "w") as f:
             json.dump({"orders": [p.order if kf else None for p, kf in zip(learned, kept_flags)], "shape": shape}, f)
     proposed = sum(1 for c in choices if c.learned)
+    alternatives = sum(len(c.alternatives) for c in choices)
     kept = sum(kept_flags)
     log(f"learned orders: {proposed} proposed, {kept} kept by the judge ({plan_s:.1f}s planning, {judge_s:.1f}s judging)")
     if args.plain_pass:
@@ -507,7 +510,8 @@ def run_headline(args):
                                         "ms_learned": per_l[0], "ms_heuristic": per_h[0],
                                         "order_learned": learned[0].order, "order_heuristic": heur[0].order,
                                         "level_sizes": items[0]["level_sizes"]},
-                      "learned_proposed": proposed, "learned_kept": kept,
+                      "learned_proposed": proposed, "learned_alternatives": alternatives, "learned_kept": kept,
+                      "learned_top": args.learned_top, "learned_margin": args.learned_margin,
                       "fallback_rate": 1.0 - kept / len(items),
                       "order_planning_s": plan_s, "judging_s": judge_s,
                       "engine": {"two_hop_index": eng.two_hop, "arc_common": eng.arc_common,
@@ -666,6 +670,10 @@ def main():
     ap.add_argument("--orders-out", default="", help="headline: write the judged learned orders to this file")
     ap.add_argument("--orders-in", default="", help="headline: take the learned orders from this file (no judging)")
     ap.add_argument("--profile-only", action="store_true", help="headline: only the profiling pass (for rocprofv3)")
+    ap.add_argument("--learned-top", type=int, default=5,
+                    help="headline: order proposals per query (the judge times each, keeps the fastest)")
+    ap.add_argument("--learned-margin", type=float, default=0.0,
+                    help="headline: predicted saving a proposal needs over the score order")
     ap.add_argument("--plain-pass", action="store_true",
                     help="headline: one step of the timed path with the learned orders (for rocprofv3)")
     ap.add_argument("--train-queries", type=int, default=1024)

@@ -23,7 +23,7 @@ from __future__ import annotations
 import ctypes as C
 import itertools
 import math
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
@@ -328,9 +328,17 @@ def best_order(k: int, adj: List[int], f: Dict[int, float], first: Optional[int]
     connected), by dynamic programming over connected subsets for all but
     the last three positions and enumeration of those; with `first`, over
     the orders that start with that vertex."""
+    return best_orders(k, adj, f, first, eng, 1)[0]
+
+
+def best_orders(k: int, adj: List[int], f: Dict[int, float], first: Optional[int] = None,
+                eng: Engine = LAST_LEVEL_ONLY, top: int = 1) -> List[Tuple[List[int], float]]:
+    """The `top` cheapest orders of best_order's search, cheapest first:
+    one per distinct tail of the last three positions (each with the DP's
+    best prefix order for the rest)."""
     full = (1 << k) - 1
     if k == 1:
-        return [0], f[full]
+        return [([0], f[full])]
     best: Dict[int, Tuple[float, int]] = {}
     for sm in sorted(f, key=_pc):
         if first is not None and not (sm >> first) & 1:
@@ -352,7 +360,7 @@ def best_order(k: int, adj: List[int], f: Dict[int, float], first: Optional[int]
         if cand is not None:
             best[sm] = cand
     T = min(3, k - 1)
-    top = None
+    found = []
     for tail in itertools.permutations([v for v in range(k) if v != first], T):
         s0 = full
         for v in tail:
@@ -377,20 +385,22 @@ def best_order(k: int, adj: List[int], f: Dict[int, float], first: Optional[int]
                 c += f[sm]
             if extra and pos == k - 3:
                 c += f[sm]
-        if top is None or c < top[0]:
-            top = (c, s0, tail)
-    c, s0, tail = top
-    order = list(tail)[::-1]
-    sm = s0
-    while sm:
-        v = best[sm][1]
-        if v < 0:
-            order.append((sm & -sm).bit_length() - 1)
-            break
-        order.append(v)
-        sm &= ~(1 << v)
-    order.reverse()
-    return order, c
+        found.append((c, s0, tail))
+    found.sort(key=lambda t: t[0])
+    out = []
+    for c, s0, tail in found[:max(top, 1)]:
+        order = list(tail)[::-1]
+        sm = s0
+        while sm:
+            v = best[sm][1]
+            if v < 0:
+                order.append((sm & -sm).bit_length() - 1)
+                break
+            order.append(v)
+            sm &= ~(1 << v)
+        order.reverse()
+        out.append((order, c))
+    return out
 
 
 @dataclass
@@ -399,18 +409,21 @@ class OrderChoice:
     learned: bool          # False: fell back to the score order
     predicted_cost: float
     predicted_heuristic_cost: float
+    alternatives: List[List[int]] = field(default_factory=list)  # further proposals, next cheapest first
 
 
 def choose_orders(net: Mpnn, queries: Sequence[Tuple[int, List[Tuple[int, int]]]], gfeat: np.ndarray,
                   heuristic: Sequence[List[int]], margin: float = 0.25,
-                  keep_first: bool = False, eng: Engine = LAST_LEVEL_ONLY) -> List[OrderChoice]:
+                  keep_first: bool = False, eng: Engine = LAST_LEVEL_ONLY, top: int = 1) -> List[OrderChoice]:
     """One forward launch over every connected subset of every query, then
     the DP per query under the engine's cost (`eng`: what the matcher will
     count in closed form).  Falls back to the score order unless the network
     predicts at least `margin` (relative) fewer rows.  keep_first restricts
     the DP to orders that start with the score order's first vertex (needed
     when position 0 is matched on a sample of the data vertices, so that
-    both orders count the same embeddings)."""
+    both orders count the same embeddings).  top > 1 adds up to top-1
+    alternative proposals (the next cheapest orders, each also under the
+    margin) for a judge that times them all."""
     samples, spans = [], []
     adjs = []
     for k, edges in queries:
@@ -425,10 +438,12 @@ def choose_orders(net: Mpnn, queries: Sequence[Tuple[int, List[Tuple[int, int]]]
     out = []
     for (k, edges), adj, (s0, subs), h in zip(queries, adjs, spans, heuristic):
         f = {sm: float(sizes[s0 + i]) for i, sm in enumerate(subs)}
-        order, c = best_order(k, adj, f, first=h[0] if keep_first else None, eng=eng)
+        cands = best_orders(k, adj, f, first=h[0] if keep_first else None, eng=eng, top=top)
+        order, c = cands[0]
         ch = order_cost(h, adj, f, eng)
         if c < (1.0 - margin) * ch:
-            out.append(OrderChoice(order, True, c, ch))
+            alts = [o for o, co in cands[1:] if co < (1.0 - margin) * ch and list(o) != list(h)]
+            out.append(OrderChoice(order, True, c, ch, alts))
         else:
             out.append(OrderChoice(list(h), False, ch, ch))
     return out
@@ -485,15 +500,19 @@ def judge_by_time(ctx: Context, dg, queries: Sequence[Tuple[int, List[Tuple[int,
         rt = None if roots is None else roots[i]
         th = min(ctx.match(dg, hp, roots=rt, flags=flags, **shape).device_ms for _ in range(reps))
         kept = None
-        if ch.learned and list(ch.order) != list(hp.order):
-            lp = Plan(Graph.from_edges(k, edges), ch.order)
+        if ch.learned:
             cap = 4 * heuristic_work[i] + 4096 if heuristic_work is not None else 0
-            runs = [ctx.match(dg, lp, roots=rt, flags=flags, max_work=cap, **shape) for _ in range(reps)]
-            if not any(r.work_capped for r in runs):
+            for order in [ch.order] + list(ch.alternatives):
+                if list(order) == list(hp.order):
+                    continue
+                lp = Plan(Graph.from_edges(k, edges), order)
+                runs = [ctx.match(dg, lp, roots=rt, flags=flags, max_work=cap, **shape) for _ in range(reps)]
+                if any(r.work_capped for r in runs):
+                    continue
                 if any(r.count != hc for r in runs):
                     raise AssertionError(f"query {i}: learned order counted {runs[0].count}, score order {hc}")
                 tl = min(r.device_ms for r in runs)
-                if tl < (1.0 - margin) * th:
+                if tl < (1.0 - margin) * th and (kept is None or tl < kept.ms):
                     kept = TimedJudged(lp, True, tl, th)
         out.append(kept or TimedJudged(hp, False, th, th))
     return out

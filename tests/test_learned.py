@@ -99,6 +99,34 @@ def test_order_dp_is_exact(seed):
         assert math.isclose(cost, brute_best(k, adj, f, first, eng))
 
 
+@pytest.mark.parametrize("seed", range(12))
+def test_top_orders_are_valid_distinct_and_sorted(seed):
+    """best_orders: the first is best_order's; every proposal is a
+    connected order (with the fixed first vertex) whose stated cost is its
+    engine cost; costs never decrease; the last three positions differ."""
+    rng = random.Random(100 + seed)
+    k = rng.randint(4, 8)
+    edges = [(rng.randrange(v), v) for v in range(1, k)] + [tuple(rng.sample(range(k), 2)) for _ in range(rng.randint(0, 5))]
+    adj = LN.adjacency(k, edges)
+    f = {sm: rng.uniform(1, 1000) for sm in LN.connected_subsets(k, adj)}
+    for eng in ENGINES:
+        first = rng.randrange(k)
+        top = LN.best_orders(k, adj, f, first=first, eng=eng, top=4)
+        assert top[0] == LN.best_order(k, adj, f, first=first, eng=eng)
+        costs = [c for _, c in top]
+        assert costs == sorted(costs)
+        tails = set()
+        for order, c in top:
+            assert order[0] == first and sorted(order) == list(range(k))
+            seen = 1 << order[0]
+            for v in order[1:]:
+                assert adj[v] & seen  # every prefix connected
+                seen |= 1 << v
+            assert math.isclose(c, LN.order_cost(order, adj, f, eng))
+            tails.add(tuple(order[-3:]))
+        assert len(tails) == len(top)
+
+
 def test_engine_modes_follow_the_planner_rules():
     """The cost model's view of which positions the engine expands agrees with
     the planner rules restated in tests/test_pair.py (pair, collapse and
