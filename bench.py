@@ -340,6 +340,36 @@ def median(xs):
     return statistics.median(xs) if xs else 0.0
 
 
+def random_proposals_ablation(ctx, dg, qs, heur, want, roots, hres, shape, args, seed=7):
+    """What the network adds: the same judge given as many random connected
+    orders per query (same first vertex) as the network proposes, then the
+    kept orders timed like the learned ones."""
+    import numpy as np
+    from cuts_amd import learned as LN
+    rng = np.random.default_rng(seed)
+    choices = []
+    for (k, e), h in zip(qs, heur):
+        adj = LN.adjacency(k, e)
+        orders = [LN.random_connected_order(k, adj, rng, first=h.order[0]) for _ in range(args.learned_top)]
+        choices.append(LN.OrderChoice(orders[0], True, 0.0, 0.0, orders[1:]))
+    judged = LN.judge_by_time(ctx, dg, qs, heur, want, choices, roots=roots,
+                              heuristic_work=[r.work for r in hres], **shape)
+    plans = [j.plan for j in judged]
+    per = [[] for _ in plans]
+    for _ in range(args.steps):
+        for i, (p, rt) in enumerate(zip(plans, roots)):
+            r = ctx.match(dg, p, roots=rt, **shape)
+            if r.count != want[i]:
+                raise AssertionError(f"ablation query {i}: count {r.count}, fixture {want[i]}")
+            per[i].append(r.device_ms)
+    med = [median(m) for m in per]
+    out = {"proposals_per_query": args.learned_top, "kept": sum(j.learned for j in judged),
+           "ms_per_step": sum(sum(m) for m in per) / args.steps, "p50_ms_per_query": median(med)}
+    log(f"ablation (random proposals): {out['kept']} kept, {out['ms_per_step']:.1f} ms per step, "
+        f"p50 {out['p50_ms_per_query']:.3f} ms")
+    return out
+
+
 def concurrent_fields(ctx, dg, learned, heur, roots, want, emb_per_step, shape, workers=4, reps=3):
     """The same step as a throughput job: the queries handed out to
     `workers` host threads, each running whole single-query matches on its
@@ -486,6 +516,9 @@ def run_headline(args):
     per_h = [median(m) for m in ms_h]
     ev_l, ev_h = per_l[1:], per_h[1:]
     ratio = [h / l for h, l in zip(per_h, per_l) if l > 0]
+    ablation = None
+    if args.ablation and not args.orders_in:
+        ablation = random_proposals_ablation(ctx, dg, qs, heur, want, roots, hres, shape, args)
     kern = profile_pass(ctx, dg, learned, roots, shape)
     line = base_line("exact embeddings/sec + p50 time/query; learned vs heuristic order, same GPU",
                      emb / (tot_l / 1e3), "embeddings/s", args.steps, args.warmup, tot_l / args.steps,
@@ -522,6 +555,8 @@ def run_headline(args):
                       "arc_common_build_s": info["arc_common_build_s"]})
     line["roofline"] = roofline(kern, "rmat20")
     line["kernels_ms_profile_pass"] = {k: v[0] for k, v in kern.items() if v[1]}
+    if ablation:
+        line["config"]["ablation_random_proposals"] = ablation
     line["config"]["concurrent"] = concurrent_fields(ctx, dg, learned, heur, roots, want, emb / args.steps, shape)
     line["config"]["launch_shape"] = {"used": shape or "defaults (6 blocks per CU, 256 threads)",
                                       "tuned_points": tuned["points"] if tuned else None,
@@ -674,6 +709,8 @@ def main():
                     help="headline: order proposals per query (the judge times each, keeps the fastest)")
     ap.add_argument("--learned-margin", type=float, default=0.0,
                     help="headline: predicted saving a proposal needs over the score order")
+    ap.add_argument("--no-ablation", dest="ablation", action="store_false",
+                    help="headline: skip the random-proposal ablation of the learned order")
     ap.add_argument("--plain-pass", action="store_true",
                     help="headline: one step of the timed path with the learned orders (for rocprofv3)")
     ap.add_argument("--train-queries", type=int, default=1024)
