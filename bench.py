@@ -231,7 +231,8 @@ def run_batch256(args):
     gfeat = LN.graph_features(g)
     choices = []
     for i in range(0, len(qs), 32):
-        choices += LN.choose_orders(net, qs[i:i + 32], gfeat, [p.order for p in heur[i:i + 32]], keep_first=True)
+        choices += LN.choose_orders(net, qs[i:i + 32], gfeat, [p.order for p in heur[i:i + 32]], keep_first=True,
+                                    top=args.learned_top, margin=args.learned_margin)
     # the score order judges every proposal (one run capped at its work)
     judged = LN.judge(ctx, dg, qs, heur, single, choices, roots=roots)
     plans = [j.plan for j in judged]
@@ -265,6 +266,15 @@ def run_batch256(args):
     rp = [bp.run() for _ in range(max(args.warmup, 1) + args.steps)][max(args.warmup, 1):]
     assert all(r.counts == counts for r in rp), "persistent-kernel counts differ from single-query counts"
     ms_p = statistics.median([r.device_ms for r in rp])
+    # the same learned plans as concurrent single-query matches on four
+    # streams (the batches' arenas, 48 GiB each, are released first)
+    del b, bh, bp
+    many = []
+    for _ in range(max(args.steps, 1)):
+        res_m, wall_m = ctx.match_many(dg, plans, roots=roots, workers=4)
+        assert [r.count for r in res_m] == counts, "concurrent single-query counts differ"
+        many.append(wall_m)
+    ms_many = statistics.median(many)
     log(f"batch learned: level queue {statistics.median([r.device_ms for r in runs]):.1f} ms, "
         f"persistent kernel {ms_p:.1f} ms")
     ms = statistics.median([r.device_ms for r in runs])
@@ -279,6 +289,8 @@ def run_batch256(args):
                    {"queries": len(qs), "sum_count": sum(counts),
                     "batch_learned_ms": ms, "batch_heuristic_ms": rh.device_ms,
                     "batch_learned_persistent_kernel_ms": ms_p,
+                    "concurrent_single_query_learned_wall_ms": ms_many,
+                    "amortised_ms_per_query_concurrent_learned": ms_many / len(qs),
                     "p50_latency_ms_batch_learned_persistent_kernel": statistics.median(rp[-1].latency_ms),
                     "amortised_ms_per_query_batch_learned": amort,
                     "p50_ms_single_heuristic": statistics.median(s_ms),

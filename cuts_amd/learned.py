@@ -461,7 +461,8 @@ def judge(ctx: Context, dg, queries: Sequence[Tuple[int, List[Tuple[int, int]]]]
           heuristic_results, choices: Sequence[OrderChoice], roots=None, flags: int = 0) -> List[Judged]:
     """The score order judges every learned proposal: a proposal is kept
     only if its own run, capped at the score order's work, finishes with
-    fewer candidates examined.  A kept order's count must equal the score
+    fewer candidates examined; of several proposals (alternatives), the one
+    with the fewest candidates is kept.  A kept order's count must equal the score
     order's (order never changes a count); anything else raises.  `flags`
     are the match flags the score order's results were produced with (work
     depends on the execution mode)."""
@@ -469,13 +470,18 @@ def judge(ctx: Context, dg, queries: Sequence[Tuple[int, List[Tuple[int, int]]]]
     for i, ((k, edges), hp, hr, ch) in enumerate(zip(queries, heuristic_plans, heuristic_results, choices)):
         rt = None if roots is None else roots[i]
         kept = None
-        if ch.learned and list(ch.order) != list(hp.order):
-            lp = Plan(Graph.from_edges(k, edges), ch.order)
-            lr = ctx.match(dg, lp, roots=rt, max_work=hr.work, flags=flags)
-            if not lr.work_capped and lr.work < hr.work:
-                if lr.count != hr.count:
-                    raise AssertionError(f"query {i}: learned order counted {lr.count}, score order {hr.count}")
-                kept = Judged(lp, True, lr.work, lr.count)
+        if ch.learned:
+            # every proposal (alternatives too) capped at the best work so far
+            for order in [ch.order] + list(ch.alternatives):
+                if list(order) == list(hp.order):
+                    continue
+                best = kept.work if kept else hr.work
+                lp = Plan(Graph.from_edges(k, edges), order)
+                lr = ctx.match(dg, lp, roots=rt, max_work=best, flags=flags)
+                if not lr.work_capped and lr.work < best:
+                    if lr.count != hr.count:
+                        raise AssertionError(f"query {i}: learned order counted {lr.count}, score order {hr.count}")
+                    kept = Judged(lp, True, lr.work, lr.count)
         out.append(kept or Judged(hp, False, hr.work, hr.count))
     return out
 
@@ -486,7 +492,8 @@ def judge_by_time(ctx: Context, dg, queries: Sequence[Tuple[int, List[Tuple[int,
                   margin: float = 0.03, blocks_per_cu: int = 0, expand_threads: int = 0,
                   class_blocks_per_cu=None) -> List["TimedJudged"]:
     """The score order judges whether a proposal actually runs faster: both
-    orders are run warm `reps` times on the same GPU and the proposal is kept
+    orders are run once untimed (buffers grown, caches warm) and then `reps`
+    times on the same GPU, and the proposal is kept
     only if its best device time beats the score order's by `margin`
     (relative; near-ties keep the score order, so the decision does not
     flip with timing noise).  The proposal's
@@ -498,6 +505,7 @@ def judge_by_time(ctx: Context, dg, queries: Sequence[Tuple[int, List[Tuple[int,
     out = []
     for i, ((k, edges), hp, hc, ch) in enumerate(zip(queries, heuristic_plans, heuristic_counts, choices)):
         rt = None if roots is None else roots[i]
+        ctx.match(dg, hp, roots=rt, flags=flags, **shape)
         th = min(ctx.match(dg, hp, roots=rt, flags=flags, **shape).device_ms for _ in range(reps))
         kept = None
         if ch.learned:
@@ -506,6 +514,8 @@ def judge_by_time(ctx: Context, dg, queries: Sequence[Tuple[int, List[Tuple[int,
                 if list(order) == list(hp.order):
                     continue
                 lp = Plan(Graph.from_edges(k, edges), order)
+                if ctx.match(dg, lp, roots=rt, flags=flags, max_work=cap, **shape).work_capped:
+                    continue
                 runs = [ctx.match(dg, lp, roots=rt, flags=flags, max_work=cap, **shape) for _ in range(reps)]
                 if any(r.work_capped for r in runs):
                     continue
