@@ -439,7 +439,7 @@ def run_headline(args):
         threads = tuned["best"]["threads"]
         # then blocks per CU per kernel class
         tc = autotune.tune_classes(ctx, dg, heur[1::5], "rmat20-headline-classes", roots=roots[1::5],
-                                   grid=(1, 2, 3, 4, 6, 8), expand_threads=threads, log=log, use_cache=False)
+                                   grid=(2, 3, 4, 6, 8), expand_threads=threads, log=log, use_cache=False)
         tuned["classes"] = tc
         shape = dict(blocks_per_cu=tc["blocks_per_cu"], expand_threads=threads)
         if tc["class_blocks_per_cu"]:
