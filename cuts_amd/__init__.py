@@ -71,7 +71,7 @@ class _DGInfo(C.Structure):
     _fields_ = [("V", C.c_uint32), ("arcs", C.c_uint64), ("symmetric", C.c_int), ("self_loops", C.c_uint64),
                 ("hubs", C.c_uint32), ("hub_min_deg", C.c_uint32), ("two_hop_entries", C.c_uint64),
                 ("two_hop_dense_rows", C.c_uint32), ("arc_common", C.c_int), ("two_hop_build_s", C.c_double),
-                ("arc_common_build_s", C.c_double)]
+                ("arc_common_build_s", C.c_double), ("two_hop_on_device", C.c_int)]
 
 
 class _BatchOpts(C.Structure):
@@ -121,6 +121,8 @@ def lib():
             "cuts_device_graph_free": (None, [vp]),
             "cuts_device_graph_query": (C.c_int, [vp, C.POINTER(_DGInfo)]),
             "cuts_device_graph_arc_common": (C.c_int, [vp, _u32p, C.c_uint64]),
+            "cuts_device_graph_two_hop": (C.c_int, [vp, C.POINTER(C.c_uint64), _u32p, _u32p, C.c_uint64,
+                                                    C.POINTER(C.c_int32), C.POINTER(C.c_uint16), C.c_uint64]),
             "cuts_match_options_default": (None, [C.POINTER(_Opts)]),
             "cuts_match": (C.c_int, [vp, vp, vp, C.POINTER(_Opts), C.POINTER(_Result)]),
             "cuts_match_noalloc": (C.c_int, [vp, vp, vp, C.POINTER(_Opts), C.POINTER(_Result)]),
@@ -360,6 +362,27 @@ class DeviceGraph:
         out = np.zeros(max(n, 1), dtype=np.uint32)
         _check(lib().cuts_device_graph_arc_common(self._h, _ptr(out), n))
         return out[:n]
+
+    def two_hop(self, dense: bool = False):
+        """The two-hop index the upload built, read back: (off, v, c) as
+        Graph.two_hop() returns them, plus (dense_id, dense rows as a
+        rows x V uint16 array) when dense is true."""
+        i = self.info()
+        n, V, rows = i["two_hop_entries"], i["V"], i["two_hop_dense_rows"]
+        if n == 0:
+            raise CutsError(-1, "the device graph has no two-hop index")
+        off = np.zeros(V + 1, dtype=np.uint64)
+        v = np.zeros(max(n, 1), dtype=np.uint32)
+        c = np.zeros(max(n, 1), dtype=np.uint32)
+        did = np.zeros(max(V, 1), dtype=np.int32) if dense else None
+        rowbuf = np.zeros(max(rows * V, 1), dtype=np.uint16) if dense else None
+        _check(lib().cuts_device_graph_two_hop(
+            self._h, off.ctypes.data_as(C.POINTER(C.c_uint64)), _ptr(v), _ptr(c), n,
+            did.ctypes.data_as(C.POINTER(C.c_int32)) if dense else None,
+            rowbuf.ctypes.data_as(C.POINTER(C.c_uint16)) if dense else None, rows * V if dense else 0))
+        if not dense:
+            return off, v[:n], c[:n]
+        return off, v[:n], c[:n], did[:V], rowbuf[:rows * V].reshape(rows, V)
 
 
 class Context:

@@ -382,6 +382,7 @@ int cuts_device_graph_query(const cuts_device_graph* dg, cuts_device_graph_info*
     info->arc_common = d.arc_cn ? 1 : 0;
     info->two_hop_build_s = d.h2_build_s;
     info->arc_common_build_s = d.arc_cn_build_s;
+    info->two_hop_on_device = d.h2_on_gpu ? 1 : 0;
     return CUTS_OK;
   });
 }
@@ -394,6 +395,36 @@ int cuts_device_graph_arc_common(const cuts_device_graph* dg, uint32_t* out, uin
     if (n != d.A) return fail(CUTS_E_ARG, "n must equal the number of arcs");
     cuts::hip_check(hipSetDevice(d.device), "hipSetDevice");
     cuts::hip_check(hipMemcpy(out, d.arc_cn, n * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+    return CUTS_OK;
+  });
+}
+
+int cuts_device_graph_two_hop(const cuts_device_graph* dg, uint64_t* off, uint32_t* v, uint32_t* c, uint64_t cap,
+                              int32_t* dense_id, uint16_t* dense, uint64_t dense_cap) {
+  return guard([&]() -> int {
+    if (!dg || !off) return fail(CUTS_E_ARG, "null argument");
+    const cuts::DeviceGraph& d = dg->dg;
+    if (!d.h2_off) return fail(CUTS_E_ARG, "the device graph has no two-hop index");
+    const uint64_t n = d.h2_entries;
+    if (n > cap) return fail(CUTS_E_ARG, "two-hop index: " + std::to_string(n) + " entries exceed cap");
+    if (n && (!v || !c)) return fail(CUTS_E_ARG, "null output buffer");
+    const uint64_t dn = (uint64_t)d.h2_dense_rows * d.V;
+    if (dense_id && d.h2_dense_rows && (!dense || dense_cap < dn))
+      return fail(CUTS_E_ARG, "two-hop index: the dense rows need " + std::to_string(dn) + " entries");
+    cuts::hip_check(hipSetDevice(d.device), "hipSetDevice");
+    cuts::hip_check(hipMemcpy(off, d.h2_off, ((uint64_t)d.V + 1) * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+    if (n) {
+      cuts::hip_check(hipMemcpy(v, d.h2_v, n * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+      cuts::hip_check(hipMemcpy(c, d.h2_c, n * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+    }
+    if (dense_id) {
+      if (d.h2_dense_rows) {
+        cuts::hip_check(hipMemcpy(dense_id, d.h2_did, (uint64_t)d.V * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+        cuts::hip_check(hipMemcpy(dense, d.h2_dense, dn * 2, hipMemcpyDeviceToHost), "hipMemcpy");
+      } else {
+        std::fill(dense_id, dense_id + d.V, -1);
+      }
+    }
     return CUTS_OK;
   });
 }

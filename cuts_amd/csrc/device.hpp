@@ -114,6 +114,7 @@ struct DeviceGraph {
   uint16_t* h2_dense = nullptr;
   uint32_t h2_dense_rows = 0;
   double h2_build_s = 0;
+  bool h2_on_gpu = false;  // built by k_two_hop (V <= 2^17) rather than on the host
   // per-arc common-neighbour counts (symmetric loop-free graphs): arc_cn[i]
   // = |N(u) & N(w)| for the arc (u, w) at adj_out[i]; built on the device at
   // upload

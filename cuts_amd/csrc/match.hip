@@ -3067,6 +3067,79 @@ struct Matcher {
   }
 };
 
+// Two-hop index on the GPU (symmetric graphs): for every vertex u, the
+// vertices v != u that share a neighbour with u, ascending, with
+// |N(u) & N(v)|.  Each block owns a V-word counter row, zero between
+// vertices.  For its vertex u the block counts every wedge u-x-v with an
+// atomic (a wave per x, its lanes over N(x)), then reads the row in vertex
+// order -- each wave a contiguous quarter, so no barrier inside the sweeps:
+// pass 1 counts the distinct v (u's entries), pass 2 writes them at u's
+// offset (and into u's dense uint16 row when u has one) -- and clears it.
+// Counter reads are agent-scope atomic loads, so they see the atomics'
+// results at L2 rather than a stale L1 line.
+__global__ __launch_bounds__(BLOCK) void k_two_hop(const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
+                                                  uint32_t V, uint32_t* __restrict__ rows, int pass,
+                                                  unsigned long long* __restrict__ entries,
+                                                  const uint64_t* __restrict__ h2_off, uint32_t* __restrict__ hv,
+                                                  uint32_t* __restrict__ hc, const int32_t* __restrict__ did,
+                                                  uint16_t* __restrict__ dense) {
+  __shared__ uint32_t s_cnt[BLOCK / 64];
+  uint32_t* cnt = rows + (uint64_t)blockIdx.x * V;
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = uni(tid >> 6);
+  constexpr uint32_t NW = BLOCK / 64;
+  const uint32_t q0 = (uint32_t)((uint64_t)V * wave / NW), q1 = (uint32_t)((uint64_t)V * (wave + 1) / NW);
+  for (uint32_t u = blockIdx.x; u < V; u += gridDim.x) {
+    const uint32_t a0 = off[u], a1 = off[u + 1];
+    for (uint32_t i = a0 + wave; i < a1; i += NW) {
+      const uint32_t x = adj[i];
+      const uint32_t b1 = off[x + 1];
+      for (uint32_t j = off[x] + lane; j < b1; j += 64) {
+        const uint32_t v = adj[j];
+        if (v != u) atomicAdd(&cnt[v], 1u);
+      }
+    }
+    __syncthreads();
+    // sweep A: nonzero entries of this wave's quarter
+    uint32_t mine = 0;
+    for (uint32_t v0 = q0; v0 < q1; v0 += 64) {
+      const uint32_t v = v0 + lane;
+      const uint32_t c = v < q1 ? __hip_atomic_load(&cnt[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+      mine += (uint32_t)__popcll(__ballot(c != 0));
+      if (pass == 1 && c) __hip_atomic_store(&cnt[v], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (lane == 0) s_cnt[wave] = mine;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < NW; ++w) {
+      const uint32_t t = s_cnt[w];
+      if (w < wave) before += t;
+      total += t;
+    }
+    if (pass == 1) {
+      if (wave == 0) entries[u] = total;  // the same value from every lane of wave 0
+    } else {
+      // sweep B: write the entries in vertex order, clear the row
+      const int32_t dr = did ? did[u] : -1;
+      uint64_t o = h2_off[u] + before;
+      for (uint32_t v0 = q0; v0 < q1; v0 += 64) {
+        const uint32_t v = v0 + lane;
+        const uint32_t c = v < q1 ? __hip_atomic_load(&cnt[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+        const uint64_t m = __ballot(c != 0);
+        if (c) {
+          const uint64_t at = o + lane_rank(m);
+          hv[at] = v;
+          hc[at] = c;
+          if (dr >= 0) dense[(uint64_t)dr * V + v] = (uint16_t)c;
+          __hip_atomic_store(&cnt[v], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        o += (uint64_t)__popcll(m);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Plain streaming copy of n 16-byte words: the calibration kernel for HBM
 // byte counters (it moves exactly 16n bytes each way).
 __global__ __launch_bounds__(BLOCK) void k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst,
@@ -3411,27 +3484,73 @@ void upload_graph(Context& c, const Graph& g, DeviceGraph& dg) {
     // the host build keeps two V-word arrays per thread: bounded V
     if (wedges <= std::min<uint64_t>(fr / 16 / 8, 1ull << 28) && g.V <= (1u << 25)) {
       const uint32_t V = g.V;
-      std::vector<uint64_t> off;
-      std::vector<uint32_t> hv, hc;
-      build_two_hop(g, off, hv, hc);
-      dg.h2_entries = off[V];
       // dense rows for the longest lists (at least V/64 entries, or
       // V/CUTS_H2_DENSE_DIV; 1 turns them off), while their bytes stay within
       // 1/32 of free HBM; counts fit uint16 below degree 2^16.  A lookup in a
       // row is one load instead of a lower-bound search; below V/64 entries
       // the search is short and cached and rows gain nothing more
-      if (dg.max_deg < 65536) {
+      auto dense_rows = [&](const std::vector<uint64_t>& off) {
+        std::vector<uint32_t> cand;
+        if (dg.max_deg >= 65536) return cand;
         uint64_t div = 64;
         if (const char* e = std::getenv("CUTS_H2_DENSE_DIV")) div = std::max<uint64_t>(1, std::strtoull(e, nullptr, 10));
-        std::vector<uint32_t> cand;
         for (uint32_t u = 0; u < V; ++u)
           if ((off[u + 1] - off[u]) * div >= (uint64_t)V) cand.push_back(u);
         std::sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) {
           return off[a + 1] - off[a] > off[b + 1] - off[b] || (off[a + 1] - off[a] == off[b + 1] - off[b] && a < b);
         });
-        const uint64_t row_bytes = 2ull * V;
-        const uint64_t rows = std::min<uint64_t>(cand.size(), fr / 32 / std::max<uint64_t>(row_bytes, 1));
-        if (rows) {
+        const uint64_t rows = std::min<uint64_t>(cand.size(), fr / 32 / std::max<uint64_t>(2ull * V, 1));
+        cand.resize(rows);
+        return cand;
+      };
+      std::vector<uint64_t> off((size_t)V + 1, 0);
+      if (V <= (1u << 17)) {
+        // on the GPU (k_two_hop): pass 1 counts every vertex's entries, the
+        // host sums them into offsets and picks the dense rows, pass 2 writes
+        // the lists and the rows; one V-word counter row per block
+        dg.h2_on_gpu = true;
+        const uint32_t G = std::min<uint32_t>(V, 128);
+        uint32_t* cnt_rows = nullptr;
+        unsigned long long* ent = nullptr;
+        dev_alloc(cnt_rows, (uint64_t)G * V);
+        dev_alloc(ent, V);
+        HIPCHECK(hipMemsetAsync(cnt_rows, 0, (uint64_t)G * V * 4, c.stream));
+        hipLaunchKernelGGL(k_two_hop, dim3(G), dim3(BLOCK), 0, c.stream, dg.off_out, dg.adj_out, V, cnt_rows, 1, ent,
+                           (const uint64_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr, (const int32_t*)nullptr,
+                           (uint16_t*)nullptr);
+        HIPCHECK(hipGetLastError());
+        std::vector<unsigned long long> T(V);
+        HIPCHECK(hipMemcpyAsync(T.data(), ent, (uint64_t)V * 8, hipMemcpyDeviceToHost, c.stream));
+        HIPCHECK(hipStreamSynchronize(c.stream));
+        for (uint32_t u = 0; u < V; ++u) off[u + 1] = off[u] + T[u];
+        dg.h2_entries = off[V];
+        const std::vector<uint32_t> cand = dense_rows(off);
+        dev_alloc(dg.h2_off, V + 1);
+        HIPCHECK(hipMemcpy(dg.h2_off, off.data(), (V + 1) * 8, hipMemcpyHostToDevice));
+        dev_alloc(dg.h2_v, std::max<uint64_t>(off[V], 1));
+        dev_alloc(dg.h2_c, std::max<uint64_t>(off[V], 1));
+        if (!cand.empty()) {
+          std::vector<int32_t> did(V, -1);
+          for (uint64_t r = 0; r < cand.size(); ++r) did[cand[r]] = (int32_t)r;
+          dev_alloc(dg.h2_did, V);
+          HIPCHECK(hipMemcpy(dg.h2_did, did.data(), (uint64_t)V * 4, hipMemcpyHostToDevice));
+          dev_alloc(dg.h2_dense, cand.size() * V);
+          HIPCHECK(hipMemset(dg.h2_dense, 0, cand.size() * 2ull * V));
+          dg.h2_dense_rows = (uint32_t)cand.size();
+        }
+        hipLaunchKernelGGL(k_two_hop, dim3(G), dim3(BLOCK), 0, c.stream, dg.off_out, dg.adj_out, V, cnt_rows, 2, ent,
+                           (const uint64_t*)dg.h2_off, dg.h2_v, dg.h2_c, (const int32_t*)dg.h2_did, dg.h2_dense);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(c.stream));
+        (void)hipFree(cnt_rows);
+        (void)hipFree(ent);
+      } else {
+        std::vector<uint32_t> hv, hc;
+        build_two_hop(g, off, hv, hc);
+        dg.h2_entries = off[V];
+        const std::vector<uint32_t> cand = dense_rows(off);
+        if (!cand.empty()) {
+          const uint64_t rows = cand.size();
           std::vector<int32_t> did(V, -1);
           std::vector<uint16_t> dense(rows * V, 0);
 #pragma omp parallel for schedule(dynamic, 4)
@@ -3444,16 +3563,16 @@ void upload_graph(Context& c, const Graph& g, DeviceGraph& dg) {
           dev_alloc(dg.h2_did, V);
           HIPCHECK(hipMemcpy(dg.h2_did, did.data(), (uint64_t)V * 4, hipMemcpyHostToDevice));
           dev_alloc(dg.h2_dense, rows * V);
-          HIPCHECK(hipMemcpy(dg.h2_dense, dense.data(), rows * row_bytes, hipMemcpyHostToDevice));
+          HIPCHECK(hipMemcpy(dg.h2_dense, dense.data(), rows * 2ull * V, hipMemcpyHostToDevice));
           dg.h2_dense_rows = (uint32_t)rows;
         }
+        dev_alloc(dg.h2_off, V + 1);
+        HIPCHECK(hipMemcpy(dg.h2_off, off.data(), (V + 1) * 8, hipMemcpyHostToDevice));
+        dev_alloc(dg.h2_v, hv.size());
+        HIPCHECK(hipMemcpy(dg.h2_v, hv.data(), hv.size() * 4, hipMemcpyHostToDevice));
+        dev_alloc(dg.h2_c, hc.size());
+        HIPCHECK(hipMemcpy(dg.h2_c, hc.data(), hc.size() * 4, hipMemcpyHostToDevice));
       }
-      dev_alloc(dg.h2_off, V + 1);
-      HIPCHECK(hipMemcpy(dg.h2_off, off.data(), (V + 1) * 8, hipMemcpyHostToDevice));
-      dev_alloc(dg.h2_v, hv.size());
-      HIPCHECK(hipMemcpy(dg.h2_v, hv.data(), hv.size() * 4, hipMemcpyHostToDevice));
-      dev_alloc(dg.h2_c, hc.size());
-      HIPCHECK(hipMemcpy(dg.h2_c, hc.data(), hc.size() * 4, hipMemcpyHostToDevice));
       dg.h2_build_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
   }

@@ -102,6 +102,7 @@ typedef struct {
   uint32_t two_hop_dense_rows;
   int arc_common;               /* 1: per-arc common-neighbour counts present */
   double two_hop_build_s, arc_common_build_s;
+  int two_hop_on_device;        /* 1: the index was built on the device (V <= 2^17) */
 } cuts_device_graph_info;
 int cuts_device_graph_query(const cuts_device_graph *dg, cuts_device_graph_info *info);
 /* Per-arc common-neighbour counts (no reference counterpart): built on the
@@ -109,6 +110,14 @@ int cuts_device_graph_query(const cuts_device_graph *dg, cuts_device_graph_info 
  * for the arc (u, w) at position i of the out-lists.  n must equal the
  * number of arcs. */
 int cuts_device_graph_arc_common(const cuts_device_graph *dg, uint32_t *out, uint64_t n);
+/* The device graph's two-hop index read back (no reference counterpart):
+ * the layout of cuts_graph_two_hop (off: V+1 entries, v and c: cap entries,
+ * CUTS_E_ARG when the index holds more), plus its dense rows when dense_id
+ * is non-null: dense_id[u] = row or -1 (V entries) and row r at
+ * dense[r*V .. r*V+V) holding |N(u) & N(v)| as uint16 (dense_cap entries,
+ * at least rows*V).  CUTS_E_ARG when the upload built no index. */
+int cuts_device_graph_two_hop(const cuts_device_graph *dg, uint64_t *off, uint32_t *v, uint32_t *c, uint64_t cap,
+                              int32_t *dense_id, uint16_t *dense, uint64_t dense_cap);
 
 enum {
   CUTS_FLAG_NO_SHORTCUT = 1u << 0, /* last level: always scan candidate lists */
