@@ -71,7 +71,8 @@ class _DGInfo(C.Structure):
     _fields_ = [("V", C.c_uint32), ("arcs", C.c_uint64), ("symmetric", C.c_int), ("self_loops", C.c_uint64),
                 ("hubs", C.c_uint32), ("hub_min_deg", C.c_uint32), ("two_hop_entries", C.c_uint64),
                 ("two_hop_dense_rows", C.c_uint32), ("arc_common", C.c_int), ("two_hop_build_s", C.c_double),
-                ("arc_common_build_s", C.c_double), ("two_hop_on_device", C.c_int)]
+                ("arc_common_build_s", C.c_double), ("two_hop_on_device", C.c_int), ("hub_rows", C.c_uint32),
+                ("hub_rows_build_s", C.c_double)]
 
 
 class _BatchOpts(C.Structure):
@@ -123,6 +124,7 @@ def lib():
             "cuts_device_graph_arc_common": (C.c_int, [vp, _u32p, C.c_uint64]),
             "cuts_device_graph_two_hop": (C.c_int, [vp, C.POINTER(C.c_uint64), _u32p, _u32p, C.c_uint64,
                                                     C.POINTER(C.c_int32), C.POINTER(C.c_uint16), C.c_uint64]),
+            "cuts_device_graph_hub_rows": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_uint16), C.c_uint64]),
             "cuts_match_options_default": (None, [C.POINTER(_Opts)]),
             "cuts_match": (C.c_int, [vp, vp, vp, C.POINTER(_Opts), C.POINTER(_Result)]),
             "cuts_match_noalloc": (C.c_int, [vp, vp, vp, C.POINTER(_Opts), C.POINTER(_Result)]),
@@ -383,6 +385,19 @@ class DeviceGraph:
         if not dense:
             return off, v[:n], c[:n]
         return off, v[:n], c[:n], did[:V], rowbuf[:rows * V].reshape(rows, V)
+
+    def hub_rows(self):
+        """The hub rows the upload built on a graph too large for the
+        two-hop index: (row_id, rows x V uint16 array of |N(hub) & N(v)|)."""
+        i = self.info()
+        H, V = i["hub_rows"], i["V"]
+        if H == 0:
+            raise CutsError(-1, "the device graph has no hub rows")
+        rid = np.zeros(V, dtype=np.int32)
+        rows = np.zeros(H * V, dtype=np.uint16)
+        _check(lib().cuts_device_graph_hub_rows(self._h, rid.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                rows.ctypes.data_as(C.POINTER(C.c_uint16)), H * V))
+        return rid, rows.reshape(H, V)
 
 
 class Context:

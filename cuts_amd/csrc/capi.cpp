@@ -383,6 +383,8 @@ int cuts_device_graph_query(const cuts_device_graph* dg, cuts_device_graph_info*
     info->two_hop_build_s = d.h2_build_s;
     info->arc_common_build_s = d.arc_cn_build_s;
     info->two_hop_on_device = d.h2_on_gpu ? 1 : 0;
+    info->hub_rows = d.hr_rows;
+    info->hub_rows_build_s = d.hr_build_s;
     return CUTS_OK;
   });
 }
@@ -425,6 +427,20 @@ int cuts_device_graph_two_hop(const cuts_device_graph* dg, uint64_t* off, uint32
         std::fill(dense_id, dense_id + d.V, -1);
       }
     }
+    return CUTS_OK;
+  });
+}
+
+int cuts_device_graph_hub_rows(const cuts_device_graph* dg, int32_t* row_id, uint16_t* rows, uint64_t cap) {
+  return guard([&]() -> int {
+    if (!dg || !row_id || !rows) return fail(CUTS_E_ARG, "null argument");
+    const cuts::DeviceGraph& d = dg->dg;
+    if (!d.hr_rows) return fail(CUTS_E_ARG, "the device graph has no hub rows");
+    const uint64_t n = (uint64_t)d.hr_rows * d.V;
+    if (cap < n) return fail(CUTS_E_ARG, "hub rows: " + std::to_string(n) + " entries exceed cap");
+    cuts::hip_check(hipSetDevice(d.device), "hipSetDevice");
+    cuts::hip_check(hipMemcpy(row_id, d.hr_id, (uint64_t)d.V * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+    cuts::hip_check(hipMemcpy(rows, d.hr, n * 2, hipMemcpyDeviceToHost), "hipMemcpy");
     return CUTS_OK;
   });
 }

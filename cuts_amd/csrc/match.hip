@@ -112,11 +112,13 @@ struct GView {
   const int32_t* h2_did;   // dense rows of the index or null
   const uint16_t* h2_dense;
   const uint32_t* arc_cn;  // per arc (u, w) of adj_out: |N(u) & N(w)|, or null
+  const int32_t* hr_id;    // graphs without the two-hop index: hub row of a vertex or -1, or null
+  const uint16_t* hr;      // hub rows: row r = |N(hub_r) & N(v)| for every v (V uint16)
 };
 
 GView view_of(const DeviceGraph& dg) {
   return GView{dg.off_out, dg.adj_out, dg.off_in, dg.adj_in, dg.sig, dg.V, dg.hub_id, dg.nsig_out, dg.nsig_in, dg.bits,
-               dg.words, dg.h2_off, dg.h2_v, dg.h2_c, dg.h2_did, dg.h2_dense, dg.arc_cn};
+               dg.words, dg.h2_off, dg.h2_v, dg.h2_c, dg.h2_did, dg.h2_dense, dg.arc_cn, dg.hr_id, dg.hr};
 }
 
 __device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -185,6 +187,17 @@ __device__ __forceinline__ uint32_t two_hop(const GView& g, uint32_t u, uint32_t
     else hi = mid;
   }
   return (lo < end && g.h2_v[lo] == w) ? g.h2_c[lo] : 0u;
+}
+
+// |N(u) & N(w)| for u != w from a hub row when u or w has one, else ~0u.
+// Like the two-hop index it serves degree thresholds up to 2.
+__device__ __forceinline__ uint32_t hub_common(const GView& g, uint32_t u, uint32_t w) {
+  if (!g.hr_id) return ~0u;
+  const int32_t ru = g.hr_id[u];
+  if (ru >= 0) return g.hr[(uint64_t)ru * g.V + w];
+  const int32_t rw = g.hr_id[w];
+  if (rw >= 0) return g.hr[(uint64_t)rw * g.V + u];
+  return ~0u;
 }
 
 // |N(u) & N(w)| for adjacent u, w from the per-arc table: w's index in u's
@@ -1718,15 +1731,24 @@ struct CommonArgs {
   const uint4* desc;
 };
 
-// Pair level: the work of the rows (prefixes) that have survivors.
+// Pair level: the work of the rows (prefixes) that have survivors.  A
+// prefix whose u or w has a hub row takes survivors x |N(u) & N(w)| off the
+// count here and needs no walk.
 template <int L>
 __global__ __launch_bounds__(BLOCK) void k_pair_prep(GView g, KPlan P, const uint32_t* __restrict__ rows, uint64_t n,
                                                     const uint32_t* __restrict__ nsurv, uint32_t* __restrict__ minlen,
-                                                    uint8_t* __restrict__ minsel, uint4* __restrict__ desc) {
+                                                    uint8_t* __restrict__ minsel, uint4* __restrict__ desc,
+                                                    unsigned long long* __restrict__ counters) {
+  uint64_t acc = 0;
+  const bool hub = g.hr_id && P.pr_tab <= 2;
   for (uint64_t p = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (uint64_t)gridDim.x * BLOCK) {
     uint32_t len = 0, sel = 0;
-    if (nsurv[p]) {
-      const uint32_t u = rows[p * L + P.pr_a], w = rows[p * L + P.pr_b];
+    const uint32_t ns = nsurv[p];
+    const uint32_t u = ns ? rows[p * L + P.pr_a] : 0u, w = ns ? rows[p * L + P.pr_b] : 0u;
+    const uint32_t hc = (ns && hub) ? hub_common(g, u, w) : ~0u;
+    if (hc != ~0u) {
+      acc -= (uint64_t)ns * hc;
+    } else if (ns) {
       const uint32_t ul = g.off_out[u], uh = g.off_out[u + 1], wl = g.off_out[w], wh = g.off_out[w + 1];
       const uint32_t du = uh - ul, dw = wh - wl;
       len = min(du, dw);
@@ -1740,20 +1762,30 @@ __global__ __launch_bounds__(BLOCK) void k_pair_prep(GView g, KPlan P, const uin
     minlen[p] = len;
     minsel[p] = (uint8_t)sel;
   }
+  add_wave(&counters[0], acc);
 }
 
 // Chain level, shared part: the work of N(u) & N(r_i) for one older
 // position i of every parent prefix (rows of L words, u = r[ch_u]); zero
 // when deg(r_i) is below the last position's threshold (such r_i take no
-// completions away).
+// completions away), and when u or r_i has a hub row (the count is added to
+// the prefix's weight here).
 template <int L>
 __global__ __launch_bounds__(BLOCK) void k_chain_prep(GView g, KPlan P, const uint32_t* __restrict__ rows, uint64_t n,
                                                      uint32_t i, uint32_t* __restrict__ minlen,
-                                                     uint8_t* __restrict__ minsel) {
+                                                     uint8_t* __restrict__ minsel, uint32_t* __restrict__ weight) {
+  const bool hub = g.hr_id && P.ch_ta <= 2;
   for (uint64_t p = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; p < n; p += (uint64_t)gridDim.x * BLOCK) {
     const uint32_t u = rows[p * L + P.ch_u], w = rows[p * L + i];
     const uint32_t du = g.off_out[u + 1] - g.off_out[u], dw = g.off_out[w + 1] - g.off_out[w];
-    const bool need = P.ch_tb <= 1 || g.sig[2 * (uint64_t)w] >= P.ch_tb;
+    bool need = P.ch_tb <= 1 || g.sig[2 * (uint64_t)w] >= P.ch_tb;
+    if (need && hub) {
+      const uint32_t hc = hub_common(g, u, w);
+      if (hc != ~0u) {
+        weight[p] += hc;
+        need = false;
+      }
+    }
     minlen[p] = need ? min(du, dw) : 0u;
     minsel[p] = (uint8_t)(du <= dw ? 0u : 1u);
   }
@@ -2037,9 +2069,17 @@ __global__ __launch_bounds__(BLOCK) void k_chain_row(GView g, KPlan P, const uin
     const uint32_t v = r[L - 1];
     const uint32_t du = g.off_out[u + 1] - g.off_out[u], dv = g.off_out[v + 1] - g.off_out[v];
     const bool need = P.ch_tb <= 1 || g.sig[2 * (uint64_t)v] >= P.ch_tb;
+    bool known = h2 || arc;
     if (arc && need) c -= arc_common(g, u, v, bytes);
+    if (!known && need && g.hr_id && P.ch_ta <= 2) {
+      const uint32_t hc = hub_common(g, u, v);
+      if (hc != ~0u) {
+        c -= hc;
+        known = true;
+      }
+    }
     acc += (uint64_t)c;
-    minlen[p] = (!h2 && !arc && need) ? min(du, dv) : 0u;
+    minlen[p] = (!known && need) ? min(du, dv) : 0u;
     minsel[p] = (uint8_t)(du <= dv ? 0u : 1u);
   }
   add_wave(&counters[0], acc);
@@ -2629,7 +2669,7 @@ struct Matcher {
   void pair_intersections(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in, bool spec = false) {
     const uint32_t grid = grid_for(n, BLOCK, grid_cap);
     pbegin(false);
-#define PP(LL) hipLaunchKernelGGL((k_pair_prep<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.nsurv, in.minlen, in.minsel, in.pbase)
+#define PP(LL) hipLaunchKernelGGL((k_pair_prep<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.nsurv, in.minlen, in.minsel, in.pbase, ctr)
     CUTS_DISPATCH_PAIR(L, PP)
 #undef PP
     HIPCHECK(hipGetLastError());
@@ -2736,7 +2776,7 @@ struct Matcher {
       if (i == kp.ch_u || ((arc_mask >> i) & 1)) continue;
       const uint32_t grid = grid_for(n, BLOCK, grid_cap);
       pbegin(false);
-#define CP(LL) hipLaunchKernelGGL((k_chain_prep<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, i, in.minlen, in.minsel)
+#define CP(LL) hipLaunchKernelGGL((k_chain_prep<LL>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, rows, n, i, in.minlen, in.minsel, in.nsurv)
       CUTS_DISPATCH_PAIR(L, CP)
 #undef CP
       HIPCHECK(hipGetLastError());
@@ -2906,6 +2946,7 @@ struct Matcher {
     if (o.flags & 32u) g.hub_id = nullptr;  // CUTS_FLAG_NO_BITMAP
     if (o.flags & 512u) g.h2_off = nullptr;  // CUTS_FLAG_NO_TWO_HOP
     if (o.flags & 512u) g.h2_did = nullptr;
+    if (o.flags & 512u) g.hr_id = nullptr;
     if (o.flags & 2048u) g.arc_cn = nullptr;  // CUTS_FLAG_NO_ARC_COMMON
     write = (o.flags & 4u) != 0;
     // CUTS_FLAG_SYNC_WRITE: each chunk's rows copied before the next chunk runs
@@ -3140,6 +3181,35 @@ __global__ __launch_bounds__(BLOCK) void k_two_hop(const uint32_t* __restrict__ 
   }
 }
 
+// Hub rows (graphs without the two-hop index): row r holds |N(u) & N(v)|
+// for u = hubs[r] and every v != u as uint16 (a count stays below the
+// largest degree, < 2^16).  One block per job (row, first of up to HR_JOB
+// consecutive entries x of u's list): each wave takes an x, its lanes add
+// one to the count of every v in N(x).  Two counts share a 32-bit word, so
+// an add is a 32-bit atomic of 1 or 1 << 16 into its half, which never
+// carries (its count ends below 2^16).
+constexpr uint32_t HR_JOB = 64;
+__global__ __launch_bounds__(BLOCK) void k_hub_rows(const uint32_t* __restrict__ off, const uint32_t* __restrict__ adj,
+                                                   uint32_t V, const uint32_t* __restrict__ hubs,
+                                                   const uint2* __restrict__ jobs, uint32_t* __restrict__ words) {
+  const uint2 jb = jobs[blockIdx.x];
+  const uint32_t u = hubs[jb.x];
+  const uint32_t a1 = min(jb.y + HR_JOB, off[u + 1]);
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t base = (uint64_t)jb.x * V;
+  for (uint32_t i = jb.y + wave; i < a1; i += BLOCK / 64) {
+    const uint32_t x = adj[i];
+    const uint32_t b1 = off[x + 1];
+    for (uint32_t j = off[x] + lane; j < b1; j += 64) {
+      const uint32_t v = adj[j];
+      if (v != u) {
+        const uint64_t gi = base + v;
+        atomicAdd(&words[gi >> 1], 1u << ((uint32_t)(gi & 1) * 16));
+      }
+    }
+  }
+}
+
 // Plain streaming copy of n 16-byte words: the calibration kernel for HBM
 // byte counters (it moves exactly 16n bytes each way).
 __global__ __launch_bounds__(BLOCK) void k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst,
@@ -3315,7 +3385,7 @@ DeviceGraph::~DeviceGraph() {
   if (hub_id) (void)hipFree(hub_id);
   if (bits) (void)hipFree(bits);
   if (arc_cn) (void)hipFree(arc_cn);
-  for (void* p : {(void*)h2_off, (void*)h2_v, (void*)h2_c, (void*)h2_did, (void*)h2_dense})
+  for (void* p : {(void*)h2_off, (void*)h2_v, (void*)h2_c, (void*)h2_did, (void*)h2_dense, (void*)hr_id, (void*)hr})
     if (p) (void)hipFree(p);
   (void)hipSetDevice(prev);
 }
@@ -3481,8 +3551,16 @@ void upload_graph(Context& c, const Graph& g, DeviceGraph& dg) {
     }
     size_t fr = 0, tot = 0;
     HIPCHECK(hipMemGetInfo(&fr, &tot));
-    // the host build keeps two V-word arrays per thread: bounded V
-    if (wedges <= std::min<uint64_t>(fr / 16 / 8, 1ull << 28) && g.V <= (1u << 25)) {
+    // the host build keeps two V-word arrays per thread: bounded V.
+    // CUTS_H2_MAX_WEDGES lowers the wedge cap (0: no index, hub rows instead)
+    uint64_t wcap = std::min<uint64_t>(fr / 16 / 8, 1ull << 28);
+    bool no_index = false;
+    if (const char* e = std::getenv("CUTS_H2_MAX_WEDGES")) {
+      const uint64_t w = std::strtoull(e, nullptr, 10);
+      no_index = w == 0;
+      wcap = std::min<uint64_t>(wcap, w);
+    }
+    if (!no_index && wedges <= wcap && g.V <= (1u << 25)) {
       const uint32_t V = g.V;
       // dense rows for the longest lists (at least V/64 entries, or
       // V/CUTS_H2_DENSE_DIV; 1 turns them off), while their bytes stay within
@@ -3574,6 +3652,51 @@ void upload_graph(Context& c, const Graph& g, DeviceGraph& dg) {
         HIPCHECK(hipMemcpy(dg.h2_c, hc.data(), hc.size() * 4, hipMemcpyHostToDevice));
       }
       dg.h2_build_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    } else if (dg.max_deg < 65536 && dg.loops == 0) {
+      // hub rows in place of the index: the vertices of degree >= 256
+      // (CUTS_H2_HUB_MIN_DEG), longest lists first, at most CUTS_H2_HUB_ROWS
+      // rows (default 4096; 0 turns them off) within 1/32 of free HBM, built
+      // on the device
+      const uint32_t V = g.V;
+      uint64_t cap_rows = 4096, min_deg = 256;
+      if (const char* e = std::getenv("CUTS_H2_HUB_ROWS")) cap_rows = std::strtoull(e, nullptr, 10);
+      if (const char* e = std::getenv("CUTS_H2_HUB_MIN_DEG")) min_deg = std::max<uint64_t>(1, std::strtoull(e, nullptr, 10));
+      cap_rows = std::min<uint64_t>(cap_rows, fr / 32 / (2ull * V));
+      std::vector<uint32_t> hubs;
+      for (uint32_t u = 0; u < V; ++u)
+        if (g.out_off[u + 1] - g.out_off[u] >= min_deg) hubs.push_back(u);
+      auto deg = [&](uint32_t u) { return g.out_off[u + 1] - g.out_off[u]; };
+      std::sort(hubs.begin(), hubs.end(),
+                [&](uint32_t a, uint32_t b) { return deg(a) > deg(b) || (deg(a) == deg(b) && a < b); });
+      hubs.resize(std::min<uint64_t>(hubs.size(), cap_rows));
+      if (!hubs.empty()) {
+        const uint64_t H = hubs.size();
+        std::vector<uint2> jobs;
+        std::vector<int32_t> id(V, -1);
+        for (uint64_t r = 0; r < H; ++r) {
+          const uint32_t u = hubs[r];
+          id[u] = (int32_t)r;
+          for (uint32_t i = g.out_off[u]; i < g.out_off[u + 1]; i += HR_JOB) jobs.push_back(make_uint2((uint32_t)r, i));
+        }
+        uint32_t* d_hubs = nullptr;
+        uint2* d_jobs = nullptr;
+        dev_alloc(d_hubs, H);
+        dev_alloc(d_jobs, jobs.size());
+        dev_alloc(dg.hr_id, V);
+        dev_alloc(dg.hr, (H * V + 1) & ~1ull);
+        HIPCHECK(hipMemcpy(d_hubs, hubs.data(), H * 4, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(d_jobs, jobs.data(), jobs.size() * sizeof(uint2), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(dg.hr_id, id.data(), (uint64_t)V * 4, hipMemcpyHostToDevice));
+        HIPCHECK(hipMemsetAsync(dg.hr, 0, ((H * V + 1) & ~1ull) * 2, c.stream));
+        hipLaunchKernelGGL(k_hub_rows, dim3((uint32_t)jobs.size()), dim3(BLOCK), 0, c.stream, dg.off_out, dg.adj_out, V,
+                           d_hubs, d_jobs, (uint32_t*)dg.hr);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(c.stream));
+        (void)hipFree(d_hubs);
+        (void)hipFree(d_jobs);
+        dg.hr_rows = (uint32_t)H;
+        dg.hr_build_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      }
     }
   }
 }

@@ -103,6 +103,8 @@ typedef struct {
   int arc_common;               /* 1: per-arc common-neighbour counts present */
   double two_hop_build_s, arc_common_build_s;
   int two_hop_on_device;        /* 1: the index was built on the device (V <= 2^17) */
+  uint32_t hub_rows;            /* graphs without the index: rows of |N(hub) & N(v)| (0: none) */
+  double hub_rows_build_s;
 } cuts_device_graph_info;
 int cuts_device_graph_query(const cuts_device_graph *dg, cuts_device_graph_info *info);
 /* Per-arc common-neighbour counts (no reference counterpart): built on the
@@ -118,6 +120,13 @@ int cuts_device_graph_arc_common(const cuts_device_graph *dg, uint32_t *out, uin
  * at least rows*V).  CUTS_E_ARG when the upload built no index. */
 int cuts_device_graph_two_hop(const cuts_device_graph *dg, uint64_t *off, uint32_t *v, uint32_t *c, uint64_t cap,
                               int32_t *dense_id, uint16_t *dense, uint64_t dense_cap);
+/* Hub rows read back (no reference counterpart): on symmetric loop-free
+ * graphs too large for the two-hop index, the upload gives vertices of
+ * degree >= 256 (CUTS_H2_HUB_MIN_DEG), longest lists first, up to 4096
+ * (CUTS_H2_HUB_ROWS) a row of |N(u) & N(v)| for every v as uint16.
+ * row_id[u] = row or -1 (V entries); row r at rows[r*V .. r*V+V) (cap
+ * entries, at least hub_rows*V).  CUTS_E_ARG when the upload built none. */
+int cuts_device_graph_hub_rows(const cuts_device_graph *dg, int32_t *row_id, uint16_t *rows, uint64_t cap);
 
 enum {
   CUTS_FLAG_NO_SHORTCUT = 1u << 0, /* last level: always scan candidate lists */
