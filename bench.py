@@ -560,11 +560,12 @@ def run_headline(args):
                       "fallback_rate": 1.0 - kept / len(items),
                       "order_planning_s": plan_s, "judging_s": judge_s,
                       "engine": {"two_hop_index": eng.two_hop, "arc_common": eng.arc_common,
-                                 "hubs": info["hubs"], "hub_min_deg": info["hub_min_deg"]},
+                                 "hubs": info["hubs"], "hub_min_deg": info["hub_min_deg"],
+                                 "hub_rows": info["hub_rows"]},
                       "weights": {"file": os.path.relpath(WEIGHTS, ROOT), **{k: winfo.get(k) for k in
                                   ("graphs", "queries_admitted", "samples", "mse_first", "mse_last")}},
                       "wall_s_timed_loop": wall, "upload_s": upload_s,
-                      "arc_common_build_s": info["arc_common_build_s"]})
+                      "arc_common_build_s": info["arc_common_build_s"], "hub_rows_build_s": info["hub_rows_build_s"]})
     line["roofline"] = roofline(kern, "rmat20")
     line["kernels_ms_profile_pass"] = {k: v[0] for k, v in kern.items() if v[1]}
     if ablation:

@@ -121,7 +121,7 @@ struct DeviceGraph {
   uint32_t* arc_cn = nullptr;
   double arc_cn_build_s = 0;
   // hub rows (symmetric loop-free graphs too large for the two-hop index):
-  // for up to a few thousand vertices of degree >= 256, longest lists first,
+  // for up to 16,384 vertices of degree >= 256 (within 1/8 of free HBM), longest lists first,
   // |N(u) & N(v)| for every v as V uint16 (hr_id[u] = row or -1); built on
   // the device at upload.  They serve the intersections of the pair and
   // chain levels in which u or w has a row

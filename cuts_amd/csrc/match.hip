@@ -3655,13 +3655,13 @@ void upload_graph(Context& c, const Graph& g, DeviceGraph& dg) {
     } else if (dg.max_deg < 65536 && dg.loops == 0) {
       // hub rows in place of the index: the vertices of degree >= 256
       // (CUTS_H2_HUB_MIN_DEG), longest lists first, at most CUTS_H2_HUB_ROWS
-      // rows (default 4096; 0 turns them off) within 1/32 of free HBM, built
+      // rows (default 16384; 0 turns them off) within 1/8 of free HBM, built
       // on the device
       const uint32_t V = g.V;
-      uint64_t cap_rows = 4096, min_deg = 256;
+      uint64_t cap_rows = 16384, min_deg = 256;
       if (const char* e = std::getenv("CUTS_H2_HUB_ROWS")) cap_rows = std::strtoull(e, nullptr, 10);
       if (const char* e = std::getenv("CUTS_H2_HUB_MIN_DEG")) min_deg = std::max<uint64_t>(1, std::strtoull(e, nullptr, 10));
-      cap_rows = std::min<uint64_t>(cap_rows, fr / 32 / (2ull * V));
+      cap_rows = std::min<uint64_t>(cap_rows, fr / 8 / (2ull * V));
       std::vector<uint32_t> hubs;
       for (uint32_t u = 0; u < V; ++u)
         if (g.out_off[u + 1] - g.out_off[u] >= min_deg) hubs.push_back(u);

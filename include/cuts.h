@@ -122,7 +122,7 @@ int cuts_device_graph_two_hop(const cuts_device_graph *dg, uint64_t *off, uint32
                               int32_t *dense_id, uint16_t *dense, uint64_t dense_cap);
 /* Hub rows read back (no reference counterpart): on symmetric loop-free
  * graphs too large for the two-hop index, the upload gives vertices of
- * degree >= 256 (CUTS_H2_HUB_MIN_DEG), longest lists first, up to 4096
+ * degree >= 256 (CUTS_H2_HUB_MIN_DEG), longest lists first, up to 16384
  * (CUTS_H2_HUB_ROWS) a row of |N(u) & N(v)| for every v as uint16.
  * row_id[u] = row or -1 (V entries); row r at rows[r*V .. r*V+V) (cap
  * entries, at least hub_rows*V).  CUTS_E_ARG when the upload built none. */
