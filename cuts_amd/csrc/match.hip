@@ -977,6 +977,43 @@ struct ExpandArgs {
   const unsigned long long* dyn;
 };
 
+// The next tile of a block, computed by wave 0 (the same value in every
+// lane; ~0 once the block is done).  A launch with no more tiles than
+// blocks maps each block to one tile statically, no tile-counter atomic at
+// all, keeping each XCD's tiles contiguous: block b runs on XCD b mod 8, the
+// c_x = ceil((ntiles - x) / 8) blocks of XCD x take tiles P_x .. P_x + c_x
+// - 1 with P_x = c_0 + .. + c_{x-1}.  Otherwise tiles are fetched
+// dynamically: block b works first on XCD (b mod 8)'s contiguous share of
+// the tiles (neighbouring prefixes share adjacency lists in that XCD's L2)
+// and, once that share is drained, takes tiles from the other shares, one
+// atomic per 1,024-item tile and no static imbalance between blocks.  Lane
+// 0's atomic is nested in wave 0's wave-uniform branch and its result
+// broadcast.  `tries` starts at 0.
+__device__ __forceinline__ uint64_t next_tile(unsigned int* tilectr, uint64_t ntiles, uint32_t nx, uint32_t myx,
+                                              uint32_t& tries) {
+  if (ntiles <= gridDim.x) {
+    if (tries) return ~0ull;
+    tries = 1;
+    const uint32_t b = blockIdx.x;
+    if (b >= ntiles) return ~0ull;
+    if (nx == 1) return b;
+    const uint32_t x = b & 7u;
+    uint64_t p = 0;
+    for (uint32_t y = 0; y < x; ++y) p += (ntiles - y + 7) / 8;  // y < x <= b < ntiles
+    return p + (b >> 3);
+  }
+  while (tries < nx) {
+    const uint32_t xx = (myx + tries) % nx;
+    const uint64_t lo = ntiles * xx / nx, hi = ntiles * (xx + 1) / nx;
+    uint32_t got = 0;
+    if ((threadIdx.x & 63) == 0) got = atomicAdd(&tilectr[xx], 1u);
+    const uint64_t i = uni(got);
+    if (lo + i < hi) return lo + i;
+    ++tries;
+  }
+  return ~0ull;
+}
+
 template <int L, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a) {
   __shared__ uint64_t s_off[LDS_OFF + 1];
@@ -994,35 +1031,16 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
   const uint32_t nj = P.nj[L];
   const uint32_t tin = P.sig_in[L], tout = P.sig_out[L];
   uint64_t acc = 0, valid = 0, pairs = 0, trips = 0, bytes = 0, mb = 0;
-  // Dynamic, XCD-aware tile fetching: blocks are dispatched round-robin over
-  // the 8 XCDs, so block b works first on XCD (b mod 8)'s contiguous share of
-  // the tiles (neighbouring prefixes share adjacency lists in that XCD's L2)
-  // and, once that share is drained, takes tiles from the other shares.
-  // One atomic per 1,024-item tile; no static imbalance between blocks.
-  // The fetch runs in wave 0 under a wave-uniform branch; lane 0's atomic
-  // is nested inside it and its result broadcast, so the branches at the
-  // edges of this barrier loop are all wave-uniform.
+  // Tiles (next_tile): static for a launch with no more tiles than blocks,
+  // else dynamic and XCD-aware.  The fetch runs in wave 0 under a
+  // wave-uniform branch, so the branches at the edges of this barrier loop
+  // are all wave-uniform.
   __shared__ uint64_t s_tile;
   const uint32_t nx = a.xcd ? 8u : 1u;
   const uint32_t myx = a.xcd ? (blockIdx.x & 7u) : 0u;
   uint32_t tries = 0;
   while (true) {
-    if (uni(tid >> 6) == 0) {
-      uint64_t t = ~0ull;
-      while (tries < nx) {
-        const uint32_t xx = (myx + tries) % nx;
-        const uint64_t lo = ntiles_ * xx / nx, hi = ntiles_ * (xx + 1) / nx;
-        uint32_t got = 0;
-        if ((tid & 63) == 0) got = atomicAdd(&a.tilectr[xx], 1u);  // inside the wave-uniform branch
-        const uint64_t i = uni(got);
-        if (lo + i < hi) {
-          t = lo + i;
-          break;
-        }
-        ++tries;
-      }
-      s_tile = t;  // same value from every lane of wave 0
-    }
+    if (uni(tid >> 6) == 0) s_tile = next_tile(a.tilectr, ntiles_, nx, myx, tries);  // same value from every lane
     __syncthreads();
     const uint64_t tile = uni64(s_tile);
     if (tile == ~0ull) break;
@@ -1604,22 +1622,7 @@ __global__ __launch_bounds__(BLOCK) void k_chain_sib(GView g, KPlan P, SibArgs a
   const uint32_t myx = a.xcd ? (blockIdx.x & 7u) : 0u;
   uint32_t tries = 0;
   while (true) {
-    if (uni(tid >> 6) == 0) {
-      uint64_t tt = ~0ull;
-      while (tries < nx) {
-        const uint32_t xx = (myx + tries) % nx;
-        const uint64_t lo = a.ntiles * xx / nx, hi = a.ntiles * (xx + 1) / nx;
-        uint32_t got = 0;
-        if ((tid & 63) == 0) got = atomicAdd(&a.tilectr[xx], 1u);  // inside the wave-uniform branch
-        const uint64_t i = uni(got);
-        if (lo + i < hi) {
-          tt = lo + i;
-          break;
-        }
-        ++tries;
-      }
-      s_tile = tt;
-    }
+    if (uni(tid >> 6) == 0) s_tile = next_tile(a.tilectr, a.ntiles, nx, myx, tries);
     __syncthreads();
     const uint64_t tile = uni64(s_tile);
     if (tile == ~0ull) break;
@@ -1810,22 +1813,7 @@ __global__ __launch_bounds__(BLOCK) void k_common(GView g, KPlan P, ExpandArgs a
   const uint32_t myx = a.xcd ? (blockIdx.x & 7u) : 0u;
   uint32_t tries = 0;
   while (true) {
-    if (uni(tid >> 6) == 0) {
-      uint64_t t = ~0ull;
-      while (tries < nx) {
-        const uint32_t xx = (myx + tries) % nx;
-        const uint64_t lo = ntiles_ * xx / nx, hi = ntiles_ * (xx + 1) / nx;
-        uint32_t got = 0;
-        if ((tid & 63) == 0) got = atomicAdd(&a.tilectr[xx], 1u);  // inside the wave-uniform branch
-        const uint64_t i = uni(got);
-        if (lo + i < hi) {
-          t = lo + i;
-          break;
-        }
-        ++tries;
-      }
-      s_tile = t;
-    }
+    if (uni(tid >> 6) == 0) s_tile = next_tile(a.tilectr, ntiles_, nx, myx, tries);
     __syncthreads();
     const uint64_t tile = uni64(s_tile);
     if (tile == ~0ull) break;
@@ -3007,12 +2995,13 @@ struct Matcher {
     const uint32_t n0c = re > rb ? (re - rb + rs - 1) / rs : 0;
     LevelBuf& l0 = lv[0];
     ensure_rows(arena, l0, std::max<uint32_t>(n0c, 1), 1);
-    HIPCHECK(hipMemsetAsync(ctr, 0, 4 * 8, c.stream));
-    HIPCHECK(hipMemsetAsync(ctr + 9, 0, 8, c.stream));
-    HIPCHECK(hipMemsetAsync(ctr + 11, 0, 3 * 8, c.stream));  // speculation status words, device-side work
+    // counts, tile counters, merge bytes, level counts, rows written,
+    // speculation status words and device-side work: one clear
+    HIPCHECK(hipMemsetAsync(ctr, 0, 14 * 8, c.stream));
     HIPCHECK(hipEventRecord(c.ev0, c.stream));
     const auto t_start = std::chrono::steady_clock::now();
     uint64_t n0 = 0;
+    bool last_root_read = false;
     if (n0c) {
       pbegin(false);
       hipLaunchKernelGGL(k_level0_flags, dim3(grid_for(n0c, BLOCK, 65535)), dim3(BLOCK), 0, c.stream, dg.sig,
@@ -3066,29 +3055,36 @@ struct Matcher {
         r.roots_done = std::min(n0, s + ts);
       }
       if (aborted || r.roots_done < n0) r.complete = 0;
-      // the work of speculative levels, summed on the device, and their status
-      HIPCHECK(hipMemcpyAsync(c.host_pinned + 8, ctr + 12, 16, hipMemcpyDeviceToHost, c.stream));
-      if (!write) HIPCHECK(hipMemcpyAsync(c.host_pinned, ctr, 24, hipMemcpyDeviceToHost, c.stream));
+      // counts, the work of speculative levels (summed on the device) and
+      // their status, and the last processed root: one readback
+      const uint64_t* hp = c.host_pinned + 16;
+      HIPCHECK(hipMemcpyAsync(c.host_pinned + 16, ctr, 14 * 8, hipMemcpyDeviceToHost, c.stream));
+      if (r.roots_done)
+        HIPCHECK(hipMemcpyAsync(c.host_pinned + 2, l0.rows + r.roots_done - 1, 4, hipMemcpyDeviceToHost, c.stream));
       HIPCHECK(hipStreamSynchronize(c.stream));
-      spec_overflow = (c.host_pinned[8] & 1ull) != 0;
-      r.work += c.host_pinned[9];
+      if (r.roots_done) {
+        r.last_root = (uint32_t)(c.host_pinned[2] & 0xFFFFFFFFu);
+        last_root_read = true;
+      }
+      spec_overflow = (hp[12] & 1ull) != 0;
+      r.work += hp[13];
       if (!write) {
-        r.count = c.host_pinned[0];
+        r.count = hp[0];
         r.level[k - 1] = r.count;
         if (collapse_fused) {
-          r.level[k - 4] = c.host_pinned[1];
-          r.level[k - 3] = c.host_pinned[2];
-          r.level[k - 2] = read_u64(ctr + 9);
+          r.level[k - 4] = hp[1];
+          r.level[k - 3] = hp[2];
+          r.level[k - 2] = hp[9];
         } else if (pair || chain_fused) {
-          r.level[k - 3] = c.host_pinned[1];
-          r.level[k - 2] = c.host_pinned[2];
+          r.level[k - 3] = hp[1];
+          r.level[k - 2] = hp[2];
         } else if (fuse) {
-          r.level[k - 2] = c.host_pinned[1];
+          r.level[k - 2] = hp[1];
         }
       }
       if (r.work_capped) r.count = 0;
     }
-    if (r.roots_done) {
+    if (r.roots_done && !last_root_read) {
       // the last processed root, for callers that restate the sample
       HIPCHECK(hipMemcpyAsync(c.host_pinned + 2, l0.rows + r.roots_done - 1, 4, hipMemcpyDeviceToHost, c.stream));
       HIPCHECK(hipStreamSynchronize(c.stream));
