@@ -988,7 +988,7 @@ struct ExpandArgs {
 // and, once that share is drained, takes tiles from the other shares, one
 // atomic per 1,024-item tile and no static imbalance between blocks.  Lane
 // 0's atomic is nested in wave 0's wave-uniform branch and its result
-// broadcast.  `tries` starts at 0.
+// broadcast.  `tries` starts at 0 (then: the share offset being drained).
 __device__ __forceinline__ uint64_t next_tile(unsigned int* tilectr, uint64_t ntiles, uint32_t nx, uint32_t myx,
                                               uint32_t& tries) {
   if (ntiles <= gridDim.x) {
@@ -1002,14 +1002,26 @@ __device__ __forceinline__ uint64_t next_tile(unsigned int* tilectr, uint64_t nt
     for (uint32_t y = 0; y < x; ++y) p += (ntiles - y + 7) / 8;  // y < x <= b < ntiles
     return p + (b >> 3);
   }
+  const uint32_t lane = threadIdx.x & 63;
   while (tries < nx) {
     const uint32_t xx = (myx + tries) % nx;
     const uint64_t lo = ntiles * xx / nx, hi = ntiles * (xx + 1) / nx;
     uint32_t got = 0;
-    if ((threadIdx.x & 63) == 0) got = atomicAdd(&tilectr[xx], 1u);
+    if (lane == 0) got = atomicAdd(&tilectr[xx], 1u);
     const uint64_t i = uni(got);
     if (lo + i < hi) return lo + i;
-    ++tries;
+    // this share is drained: one counter load per lane finds the shares
+    // with tiles left (in the same order), so a block's last fetch costs
+    // two round trips rather than one per share.  A share once drained
+    // stays drained; a stale probe only costs another failed atomic.
+    bool left = false;
+    if (lane < nx) {
+      const uint32_t yy = (myx + lane) % nx;
+      const uint64_t c = __hip_atomic_load(&tilectr[yy], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      left = ntiles * yy / nx + c < ntiles * (yy + 1) / nx;
+    }
+    const uint64_t m = __ballot(left);
+    tries = m ? (uint32_t)__builtin_ctzll(m) : nx;
   }
   return ~0ull;
 }
