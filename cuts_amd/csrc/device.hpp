@@ -190,6 +190,12 @@ struct Context {
   hipStream_t copy_stream = nullptr;
   hipEvent_t copy_ev[2] = {nullptr, nullptr};
   uint64_t* host_pinned = nullptr;        // small pinned staging
+  // coherent host words that kernels write directly (system-scope stores):
+  // a writing level's row count and status, level 0's size, the run's
+  // counters, read after the stream synchronises with no copy; host_res_dev
+  // is its device address
+  uint64_t* host_res = nullptr;
+  uint64_t* host_res_dev = nullptr;
   char* scratch = nullptr;                // this device's no-allocation scratch array
   ~Context();
 };

@@ -294,6 +294,53 @@ __global__ __launch_bounds__(BLOCK) void k_level0_scatter(const uint32_t* __rest
     if (flags[i]) out[pos[i]] = rb + (uint32_t)i * rs;
 }
 
+// Level 0 of at most 64K candidate roots in one launch of one block: thread
+// t takes a contiguous run of candidates and counts those whose signature
+// meets position 0's, the block scans the counts in LDS, and each thread
+// writes its run's roots at its offset: the rows k_level0_flags, a scan and
+// k_level0_scatter write, in the same order.  The total goes to *total in
+// host memory (a system-scope store).
+__global__ __launch_bounds__(BLOCK) void k_level0_small(const uint32_t* __restrict__ sig, uint32_t tin, uint32_t tout,
+                                                       uint32_t rb, uint32_t re, uint32_t rs, uint32_t n,
+                                                       uint32_t* __restrict__ out, uint64_t* total) {
+  __shared__ uint32_t s[BLOCK];
+  const uint32_t t = threadIdx.x;
+  const uint32_t per = (n + BLOCK - 1) / BLOCK;
+  const uint32_t i0 = min(t * per, n), i1 = min(i0 + per, n);
+  uint32_t c = 0;
+  for (uint32_t i = i0; i < i1; ++i) {
+    const uint32_t u = rb + i * rs;
+    c += (u < re && sig[2 * (uint64_t)u] >= tin && sig[2 * (uint64_t)u + 1] >= tout) ? 1u : 0u;
+  }
+  s[t] = c;
+  __syncthreads();
+  for (uint32_t o = 1; o < BLOCK; o <<= 1) {
+    const uint32_t v = t >= o ? s[t - o] : 0u;
+    __syncthreads();
+    s[t] += v;
+    __syncthreads();
+  }
+  uint32_t pos = s[t] - c;
+  for (uint32_t i = i0; i < i1; ++i) {
+    const uint32_t u = rb + i * rs;
+    if (u < re && sig[2 * (uint64_t)u] >= tin && sig[2 * (uint64_t)u + 1] >= tout) out[pos++] = u;
+  }
+  if (t == BLOCK - 1) __hip_atomic_store(total, (uint64_t)s[t], __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// The run's counters (words 0..13) and, when last is non-null, the last
+// processed root, written to host memory (system-scope stores) for one
+// synchronisation with no copy.
+__global__ void k_publish_end(const unsigned long long* __restrict__ ctr, const uint32_t* __restrict__ last,
+                              uint64_t* pub) {
+  const uint32_t t = threadIdx.x;
+  if (t < 14)
+    __hip_atomic_store(&pub[t], (uint64_t)__hip_atomic_load(&ctr[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  else if (t == 14 && last)
+    __hip_atomic_store(&pub[14], (uint64_t)*last, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // ------------------------------------------------------------ planning
 // Degree-threshold tables of the pair level: |{x in N(u) : deg(x) >= t}| for
 // the thresholds of A, B and A & B (null: t <= 1, the plain degree).
@@ -975,6 +1022,12 @@ struct ExpandArgs {
   // non-null: {w_end, ntiles} of this launch, written on the device by
   // k_tile_index_dyn (w_begin is 0), in place of w_end and ntiles
   const unsigned long long* dyn;
+  // M_WRITE, non-null: the last block to finish writes the rows written
+  // (*out_count) and status[0] to publish[0..1] in host memory; done counts
+  // the finished blocks (zero at launch, left zero)
+  uint64_t* publish;
+  unsigned int* done;
+  const unsigned long long* status;
 };
 
 // The next tile of a block, computed by wave 0 (the same value in every
@@ -1511,6 +1564,22 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
       }
     }
     __syncthreads();
+    }
+  }
+  if constexpr (MODE == M_WRITE) {
+    if (a.publish && tid == 0) {
+      // every block that took part (the grid, or one block per tile when
+      // there are fewer tiles) counts itself once its reservations on
+      // out_count are done; the last one publishes the total
+      const unsigned int parts = (unsigned int)min<uint64_t>(ntiles_, gridDim.x);
+      const unsigned int prev = __hip_atomic_fetch_add(a.done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      if (prev + 1 == parts) {
+        const uint32_t m = __hip_atomic_load(a.out_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long st = __hip_atomic_load(a.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(a.done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&a.publish[1], (uint64_t)st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&a.publish[0], (uint64_t)m, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+      }
     }
   }
   if (MODE != M_WRITE) {
@@ -2617,15 +2686,29 @@ struct Matcher {
     if (mode == M_WRITE) {
       a.out_rows = out.rows;
       a.out_count = (uint32_t*)(ctr + 10);
+      // rows written (ctr[10]) and this level's status word (ctr[11]),
+      // written to host memory by the expansion's last block; with no
+      // block taking part (no work, or a level that did not fit) they are
+      // read back
+      a.publish = c.host_res_dev;
+      a.done = (unsigned int*)(ctr + 8);
+      a.status = ctr + 11;
+      c.host_res[0] = ~0ull;
 #define EW(LL) hipLaunchKernelGGL((k_expand<LL, M_WRITE>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, a)
       CUTS_DISPATCH_L(L, EW)
 #undef EW
       HIPCHECK(hipGetLastError());
-      // rows written (ctr[10]) and this level's status word (ctr[11]) in one readback
-      HIPCHECK(hipMemcpyAsync(c.host_pinned, ctr + 10, 16, hipMemcpyDeviceToHost, c.stream));
       HIPCHECK(hipStreamSynchronize(c.stream));
-      const uint64_t m = c.host_pinned[0] & 0xFFFFFFFFu;
-      if (c.host_pinned[1] & 1ull) {
+      const volatile uint64_t* hr = c.host_res;
+      uint64_t got[2] = {hr[0], hr[1]};
+      if (got[0] == ~0ull) {
+        HIPCHECK(hipMemcpyAsync(c.host_pinned, ctr + 10, 16, hipMemcpyDeviceToHost, c.stream));
+        HIPCHECK(hipStreamSynchronize(c.stream));
+        got[0] = c.host_pinned[0];
+        got[1] = c.host_pinned[1];
+      }
+      const uint64_t m = got[0] & 0xFFFFFFFFu;
+      if (got[1] & 1ull) {
         HIPCHECK(hipMemsetAsync(ctr + 11, 0, 8, c.stream));
         r.chunks--;
         return false;
@@ -3014,7 +3097,17 @@ struct Matcher {
     const auto t_start = std::chrono::steady_clock::now();
     uint64_t n0 = 0;
     bool last_root_read = false;
-    if (n0c) {
+    if (n0c && n0c <= (1u << 16)) {
+      pbegin(false);
+      c.host_res[2] = ~0ull;
+      hipLaunchKernelGGL(k_level0_small, dim3(1), dim3(BLOCK), 0, c.stream, dg.sig, kp.sig_in[0], kp.sig_out[0], rb,
+                         re, rs, n0c, l0.rows, c.host_res_dev + 2);
+      HIPCHECK(hipGetLastError());
+      pend(0, 16.0 * n0c, false);
+      HIPCHECK(hipStreamSynchronize(c.stream));
+      n0 = ((const volatile uint64_t*)c.host_res)[2];
+      if (n0 > n0c) throw Error(-4, "level 0: no size published");
+    } else if (n0c) {
       pbegin(false);
       hipLaunchKernelGGL(k_level0_flags, dim3(grid_for(n0c, BLOCK, 65535)), dim3(BLOCK), 0, c.stream, dg.sig,
                          kp.sig_in[0], kp.sig_out[0], rb, re, rs, n0c, l0.minlen);
@@ -3069,13 +3162,14 @@ struct Matcher {
       if (aborted || r.roots_done < n0) r.complete = 0;
       // counts, the work of speculative levels (summed on the device) and
       // their status, and the last processed root: one readback
-      const uint64_t* hp = c.host_pinned + 16;
-      HIPCHECK(hipMemcpyAsync(c.host_pinned + 16, ctr, 14 * 8, hipMemcpyDeviceToHost, c.stream));
-      if (r.roots_done)
-        HIPCHECK(hipMemcpyAsync(c.host_pinned + 2, l0.rows + r.roots_done - 1, 4, hipMemcpyDeviceToHost, c.stream));
+      const volatile uint64_t* hp = c.host_res + 16;
+      c.host_res[16] = ~0ull;
+      hipLaunchKernelGGL(k_publish_end, dim3(1), dim3(64), 0, c.stream, ctr,
+                         r.roots_done ? l0.rows + r.roots_done - 1 : nullptr, c.host_res_dev + 16);
+      HIPCHECK(hipGetLastError());
       HIPCHECK(hipStreamSynchronize(c.stream));
       if (r.roots_done) {
-        r.last_root = (uint32_t)(c.host_pinned[2] & 0xFFFFFFFFu);
+        r.last_root = (uint32_t)(hp[14] & 0xFFFFFFFFu);
         last_root_read = true;
       }
       spec_overflow = (hp[12] & 1ull) != 0;
@@ -3408,6 +3502,7 @@ Context::~Context() {
   }
   if (counters) (void)hipFree(counters);
   if (host_pinned) (void)hipHostFree(host_pinned);
+  if (host_res) (void)hipHostFree(host_res);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   for (hipEvent_t e : kev)
@@ -3437,6 +3532,8 @@ void context_init(Context& c, int device) {
   HIPCHECK(hipEventCreateWithFlags(&c.copy_ev[1], hipEventDisableTiming));
   HIPCHECK(hipMalloc(&c.counters, 16 * sizeof(unsigned long long)));
   HIPCHECK(hipHostMalloc(&c.host_pinned, 256, hipHostMallocDefault));
+  HIPCHECK(hipHostMalloc(&c.host_res, 256, hipHostMallocCoherent | hipHostMallocMapped));
+  HIPCHECK(hipHostGetDevicePointer((void**)&c.host_res_dev, c.host_res, 0));
   size_t fr = 0, tot = 0;
   HIPCHECK(hipMemGetInfo(&fr, &tot));
   // trie tables may take 40% of the free HBM by default
