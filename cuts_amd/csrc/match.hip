@@ -294,38 +294,40 @@ __global__ __launch_bounds__(BLOCK) void k_level0_scatter(const uint32_t* __rest
     if (flags[i]) out[pos[i]] = rb + (uint32_t)i * rs;
 }
 
-// Level 0 of at most 64K candidate roots in one launch of one block: thread
-// t takes a contiguous run of candidates and counts those whose signature
-// meets position 0's, the block scans the counts in LDS, and each thread
-// writes its run's roots at its offset: the rows k_level0_flags, a scan and
-// k_level0_scatter write, in the same order.  The total goes to *total in
-// host memory (a system-scope store).
-__global__ __launch_bounds__(BLOCK) void k_level0_small(const uint32_t* __restrict__ sig, uint32_t tin, uint32_t tout,
-                                                       uint32_t rb, uint32_t re, uint32_t rs, uint32_t n,
-                                                       uint32_t* __restrict__ out, uint64_t* total) {
-  __shared__ uint32_t s[BLOCK];
+// Level 0 of at most 64K candidate roots in one launch of one block of
+// L0_THREADS: thread t takes a contiguous run of at most 64 candidates and
+// keeps a bit per candidate whose signature meets position 0's, the block
+// scans the counts in LDS, and each thread writes its run's roots at its
+// offset: the rows k_level0_flags, a scan and k_level0_scatter write, in
+// the same order.  The total goes to *total in host memory (a system-scope
+// store).
+constexpr uint32_t L0_THREADS = 1024;
+__global__ __launch_bounds__(L0_THREADS) void k_level0_small(const uint32_t* __restrict__ sig, uint32_t tin,
+                                                            uint32_t tout, uint32_t rb, uint32_t re, uint32_t rs,
+                                                            uint32_t n, uint32_t* __restrict__ out, uint64_t* total) {
+  __shared__ uint32_t s[L0_THREADS];
   const uint32_t t = threadIdx.x;
-  const uint32_t per = (n + BLOCK - 1) / BLOCK;
+  const uint32_t per = (n + L0_THREADS - 1) / L0_THREADS;  // <= 64
   const uint32_t i0 = min(t * per, n), i1 = min(i0 + per, n);
-  uint32_t c = 0;
+  uint64_t keep = 0;
   for (uint32_t i = i0; i < i1; ++i) {
     const uint32_t u = rb + i * rs;
-    c += (u < re && sig[2 * (uint64_t)u] >= tin && sig[2 * (uint64_t)u + 1] >= tout) ? 1u : 0u;
+    const bool ok = u < re && sig[2 * (uint64_t)u] >= tin && sig[2 * (uint64_t)u + 1] >= tout;
+    keep |= (uint64_t)ok << (i - i0);
   }
+  const uint32_t c = (uint32_t)__popcll(keep);
   s[t] = c;
   __syncthreads();
-  for (uint32_t o = 1; o < BLOCK; o <<= 1) {
+  for (uint32_t o = 1; o < L0_THREADS; o <<= 1) {
     const uint32_t v = t >= o ? s[t - o] : 0u;
     __syncthreads();
     s[t] += v;
     __syncthreads();
   }
   uint32_t pos = s[t] - c;
-  for (uint32_t i = i0; i < i1; ++i) {
-    const uint32_t u = rb + i * rs;
-    if (u < re && sig[2 * (uint64_t)u] >= tin && sig[2 * (uint64_t)u + 1] >= tout) out[pos++] = u;
-  }
-  if (t == BLOCK - 1) __hip_atomic_store(total, (uint64_t)s[t], __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  for (uint32_t i = i0; i < i1; ++i)
+    if ((keep >> (i - i0)) & 1ull) out[pos++] = rb + i * rs;
+  if (t == L0_THREADS - 1) __hip_atomic_store(total, (uint64_t)s[t], __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // The run's counters (words 0..13) and, when last is non-null, the last
@@ -3100,7 +3102,7 @@ struct Matcher {
     if (n0c && n0c <= (1u << 16)) {
       pbegin(false);
       c.host_res[2] = ~0ull;
-      hipLaunchKernelGGL(k_level0_small, dim3(1), dim3(BLOCK), 0, c.stream, dg.sig, kp.sig_in[0], kp.sig_out[0], rb,
+      hipLaunchKernelGGL(k_level0_small, dim3(1), dim3(L0_THREADS), 0, c.stream, dg.sig, kp.sig_in[0], kp.sig_out[0], rb,
                          re, rs, n0c, l0.rows, c.host_res_dev + 2);
       HIPCHECK(hipGetLastError());
       pend(0, 16.0 * n0c, false);
