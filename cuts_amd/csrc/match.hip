@@ -86,12 +86,18 @@ constexpr int TILE = BLOCK * IPT;
 constexpr int LDS_OFF = 2048;      // staged work offsets per tile
 // Items in lock-step per thread in the fused last-two-levels expansion, for
 // rows of up to 6 and of 7..10 words (more items keep more independent loads
-// in flight, at the cost of registers).
+// in flight, at the cost of registers; 8 for short rows spilled and ran 14%
+// slower on the R-MAT 20 step, 4 for long rows ran 1% faster than 2), and in
+// the counting and writing expansions for rows of 7..10 words (4 ran 1%
+// faster than 2).  The pair, chain and collapsing expansions take 4 and 2.
 #ifndef FUSED_G_SMALL
 #define FUSED_G_SMALL 4
 #endif
 #ifndef FUSED_G_LARGE
-#define FUSED_G_LARGE 2
+#define FUSED_G_LARGE 4
+#endif
+#ifndef COUNT_G_LARGE
+#define COUNT_G_LARGE 4
 #endif
 constexpr int SCAN_IPT = 8;
 constexpr int SCAN_TILE = BLOCK * SCAN_IPT;
@@ -1171,8 +1177,8 @@ __global__ __launch_bounds__(BLOCK) void k_expand(GView g, KPlan P, ExpandArgs a
       // and masked; their list word is never read.
       constexpr int G = (MODE == M_FUSED) ? ((L <= 6) ? FUSED_G_SMALL : (L <= 10 ? FUSED_G_LARGE : 1))
                         : (MODE != M_COUNT && MODE != M_GWRITE && MODE != M_WRITE)
-                            ? ((L <= 6) ? 2 : 1)
-                            : ((L <= 6) ? 4 : (L <= 10 ? 2 : 1));
+                            ? ((L <= 6) ? 4 : 2)
+                            : ((L <= 6) ? 4 : (L <= 10 ? COUNT_G_LARGE : 1));
       for (uint32_t g0 = 0; g0 < ipt; g0 += G) {
         uint64_t w[G];
         uint32_t pi[G];
