@@ -315,11 +315,21 @@ __global__ __launch_bounds__(L0_THREADS) void k_level0_small(const uint32_t* __r
   const uint32_t t = threadIdx.x;
   const uint32_t per = (n + L0_THREADS - 1) / L0_THREADS;  // <= 64
   const uint32_t i0 = min(t * per, n), i1 = min(i0 + per, n);
+  // eight candidates' signatures in flight at a time
   uint64_t keep = 0;
-  for (uint32_t i = i0; i < i1; ++i) {
-    const uint32_t u = rb + i * rs;
-    const bool ok = u < re && sig[2 * (uint64_t)u] >= tin && sig[2 * (uint64_t)u + 1] >= tout;
-    keep |= (uint64_t)ok << (i - i0);
+  for (uint32_t b = i0; b < i1; b += 8) {
+    uint2 sg[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t i = b + k, u = rb + i * rs;
+      sg[k] = (i < i1 && u < re) ? *(const uint2*)(sig + 2 * (uint64_t)u) : make_uint2(0u, 0u);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t i = b + k, u = rb + i * rs;
+      const bool ok = i < i1 && u < re && sg[k].x >= tin && sg[k].y >= tout;
+      keep |= (uint64_t)ok << ((i - i0) & 63);
+    }
   }
   const uint32_t c = (uint32_t)__popcll(keep);
   s[t] = c;
@@ -581,6 +591,40 @@ __global__ __launch_bounds__(BLOCK) void k_scan_down(const uint32_t* __restrict_
     ex += v[i];
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = bsum[nb];
+}
+
+// k_scan_partials and k_scan_down in one launch for at most SCAN_FUSE_BLOCKS
+// tiles: each block sums the tile totals before it itself (a block-wide
+// reduction of at most that many words), and the last block writes the
+// total.
+constexpr uint64_t SCAN_FUSE_BLOCKS = 1024;
+__global__ __launch_bounds__(BLOCK) void k_scan_down_fused(const uint32_t* __restrict__ in, uint64_t n,
+                                                          const uint64_t* __restrict__ bsum, uint64_t nb,
+                                                          uint64_t* __restrict__ out) {
+  __shared__ uint64_t s_tmp[BLOCK / 64];
+  uint64_t part = 0;
+  for (uint64_t i = threadIdx.x; i < blockIdx.x; i += BLOCK) part += bsum[i];
+  uint64_t before;
+  block_excl_scan(part, s_tmp, &before);
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+  uint32_t v[SCAN_IPT];
+  uint64_t s = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_IPT; ++i) {
+    uint64_t idx = base + (uint64_t)threadIdx.x * SCAN_IPT + i;
+    v[i] = idx < n ? in[idx] : 0;
+    s += v[i];
+  }
+  uint64_t tot;
+  uint64_t ex = block_excl_scan(s, s_tmp, &tot) + before;
+#pragma unroll
+  for (int i = 0; i < SCAN_IPT; ++i) {
+    uint64_t idx = base + (uint64_t)threadIdx.x * SCAN_IPT + i;
+    if (idx < n) out[idx] = ex;
+    ex += v[i];
+  }
+  if (blockIdx.x == nb - 1 && threadIdx.x == 0) out[n] = before + tot;
 }
 
 // Levels of at most SCAN_TILE rows: the whole exclusive scan in one block
@@ -2354,9 +2398,14 @@ struct Matcher {
     ensure_bsum(arena, b, nb);
     pbegin(false);
     hipLaunchKernelGGL(k_scan_reduce, dim3((uint32_t)nb), dim3(BLOCK), 0, c.stream, b.minlen, n, b.bsum);
-    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(BLOCK), 0, c.stream, b.bsum, nb);
-    hipLaunchKernelGGL(k_scan_down, dim3((uint32_t)nb), dim3(BLOCK), 0, c.stream, b.minlen, n, b.bsum, nb,
-                       b.work_off);
+    if (nb <= SCAN_FUSE_BLOCKS) {
+      hipLaunchKernelGGL(k_scan_down_fused, dim3((uint32_t)nb), dim3(BLOCK), 0, c.stream, b.minlen, n, b.bsum, nb,
+                         b.work_off);
+    } else {
+      hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(BLOCK), 0, c.stream, b.bsum, nb);
+      hipLaunchKernelGGL(k_scan_down, dim3((uint32_t)nb), dim3(BLOCK), 0, c.stream, b.minlen, n, b.bsum, nb,
+                         b.work_off);
+    }
     pend(2, 16.0 * n, false);
   }
   // the same, returning the total (a readback)
