@@ -1,6 +1,9 @@
 // Internal interface between the C ABI layer (capi.cpp) and the HIP code
 // (match.hip, batch.hip).  Not part of the public ABI.
 #pragma once
+#include <array>
+#include <string>
+#include <unordered_map>
 #include <hip/hip_runtime_api.h>
 
 #include <cstdint>
@@ -196,6 +199,13 @@ struct Context {
   // is its device address
   uint64_t* host_res = nullptr;
   uint64_t* host_res_dev = nullptr;
+  // row counts of levels whose successor took its size from the device
+  // (16 words, written by k_plan_small), and the level sizes of the last
+  // complete run of each query (by device graph, plan, roots and flags):
+  // a writing level of at most one planning block's rows in that run hands
+  // its size to the next level on the device
+  unsigned long long* lvcnt = nullptr;
+  std::unordered_map<std::string, std::array<uint64_t, kMaxQuery>> level_hints;
   char* scratch = nullptr;                // this device's no-allocation scratch array
   ~Context();
 };

@@ -346,17 +346,20 @@ __global__ __launch_bounds__(L0_THREADS) void k_level0_small(const uint32_t* __r
   if (t == L0_THREADS - 1) __hip_atomic_store(total, (uint64_t)s[t], __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// The run's counters (words 0..13) and, when last is non-null, the last
-// processed root, written to host memory (system-scope stores) for one
-// synchronisation with no copy.
+// The run's counters (words 0..13), when last is non-null the last
+// processed root, and the device-side level counts (16 words, to pub[48..]),
+// written to host memory (system-scope stores) for one synchronisation with
+// no copy.
 __global__ void k_publish_end(const unsigned long long* __restrict__ ctr, const uint32_t* __restrict__ last,
-                              uint64_t* pub) {
+                              const unsigned long long* __restrict__ lvcnt, uint64_t* pub) {
   const uint32_t t = threadIdx.x;
   if (t < 14)
     __hip_atomic_store(&pub[t], (uint64_t)__hip_atomic_load(&ctr[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   else if (t == 14 && last)
     __hip_atomic_store(&pub[14], (uint64_t)*last, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  else if (t >= 32 && t < 32 + kMaxQuery)
+    __hip_atomic_store(&pub[16 + t], (uint64_t)lvcnt[t - 32], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // ------------------------------------------------------------ planning
@@ -734,7 +737,7 @@ __global__ __launch_bounds__(BLOCK) void k_tile_index_dyn(const uint64_t* __rest
 // same fit test, dyn / status / work words and counter clears as
 // k_tile_index_dyn.
 template <int L>
-__global__ __launch_bounds__(BLOCK) void k_plan_small(GView g, KPlan P, const uint32_t* __restrict__ rows, uint64_t n,
+__global__ __launch_bounds__(BLOCK) void k_plan_small(GView g, KPlan P, const uint32_t* __restrict__ rows, uint64_t n_host,
                                                       uint32_t* __restrict__ minlen, uint8_t* __restrict__ minsel,
                                                       uint32_t* __restrict__ cbase, const uint32_t* __restrict__ tbl,
                                                       uint4* __restrict__ pbase, PairTables pt,
@@ -745,11 +748,25 @@ __global__ __launch_bounds__(BLOCK) void k_plan_small(GView g, KPlan P, const ui
                                                       unsigned int* __restrict__ outctr,
                                                       unsigned long long* __restrict__ dyn,
                                                       unsigned long long* __restrict__ status,
-                                                      unsigned long long* __restrict__ work, uint64_t* publish) {
+                                                      unsigned long long* __restrict__ work, uint64_t* publish,
+                                                      const unsigned long long* __restrict__ dn,
+                                                      unsigned long long* __restrict__ lvc) {
   __shared__ uint64_t s_off[SCAN_TILE + 1];
   __shared__ uint32_t s_len[SCAN_TILE];
   __shared__ uint64_t s_tmp[BLOCK / 64];
   const bool collapse = pbase && P.pr_collapse && (g.h2_off || (g.arc_cn && P.pr_adj));
+  // dn non-null: the input level's row count is on the device (the low word
+  // of the previous expansion's row counter, read before this launch clears
+  // it), recorded in *lvc; more rows than one block plans leave the level
+  // undone with the status bit set
+  uint64_t n = n_host;
+  bool too_big = false;
+  if (dn) {
+    const uint64_t m = (uint32_t)*dn;
+    if (threadIdx.x == 0) *lvc = m;
+    too_big = m > (uint64_t)SCAN_TILE;
+    n = too_big ? 0 : m;
+  }
   uint64_t mb = 0, acc = 0, surv = 0, sna = 0;
   // plan with consecutive threads on consecutive rows (every thread busy
   // for small n), then scan in the blocked layout from LDS
@@ -788,7 +805,7 @@ __global__ __launch_bounds__(BLOCK) void k_plan_small(GView g, KPlan P, const ui
     s_off[n] = W;
   }
   const uint64_t nt = (W + TILE - 1) / TILE;
-  const bool fits = W <= max_items && nt + 1 <= tile_cap && W <= row_cap;
+  const bool fits = !too_big && W <= max_items && nt + 1 <= tile_cap && W <= row_cap;
   if (threadIdx.x < 8) tilectr[threadIdx.x] = 0u;
   if (outctr && threadIdx.x < 2) outctr[threadIdx.x] = 0u;
   if (threadIdx.x == 0) {
@@ -2426,6 +2443,24 @@ struct Matcher {
   // no-allocation entry point and CUTS_FLAG_SYNC_LEVELS.
   bool no_spec = false;
   bool spec_overflow = false;
+  // level sizes of the last complete run of this query, or null; bit l:
+  // level l's size was handed on on the device (read from lvcnt at the end)
+  const uint64_t* hint = nullptr;
+  uint32_t chained_mask = 0;
+  // level L can take its input size from the device: a speculative level
+  // planned by one block (k_plan_small) with no host step that needs the
+  // size (chain rows and shared sums, shared candidates, the last level's
+  // closed form, written embeddings, the pair level's intersection pass)
+  bool chainable(uint32_t L) const {
+    if (L >= k || !speculate() || o.trunks > 1) return false;
+    if (chain && !chain_fused && L == k - 2) return false;
+    if (chain && L == k - 3) return false;
+    if (sib && L == k - 4) return false;
+    if (L == k - 1 && closed_last) return false;
+    if (L == k - 1 && write) return false;
+    if (pair && L == k - 3 && !(collapse_fused && L == k - 4) && kp.pr_a != kp.pr_b && !pair_cab_known()) return false;
+    return true;
+  }
   bool speculate() const {
     return !no_spec && !prof && !o.max_work && !has_deadline && !arena && !(o.flags & 8192u);
   }
@@ -2442,21 +2477,24 @@ struct Matcher {
   // status: ctr[11] for a writing level (read back with its row count),
   // ctr[12] for the counting levels (read at the end of the run)
   // (a writing level: at most MAX_CHUNK items; a counting one MAX_SPEC_ITEMS)
-  uint64_t tile_index_dyn(LevelBuf& in, uint64_t n, uint64_t row_cap, unsigned int* outctr) {
+  uint64_t tile_index_dyn(LevelBuf& in, uint64_t n, uint64_t row_cap, unsigned int* outctr, uint64_t* publish = nullptr) {
     ensure_tiles(arena, in, std::max<uint64_t>(in.tile_cap, 4097) - 1);
     const uint64_t tcap = in.tile_cap;
     const uint32_t tg = (uint32_t)std::min<uint64_t>((tcap + BLOCK - 1) / BLOCK, 1024);
     hipLaunchKernelGGL(k_tile_index_dyn, dim3(tg), dim3(BLOCK), 0, c.stream, in.work_off, n,
                        outctr ? MAX_CHUNK : MAX_SPEC_ITEMS, tcap, row_cap,
                        in.tile_p, (unsigned int*)(ctr + 4), outctr, ctr + 14, outctr ? ctr + 11 : ctr + 12, ctr + 13,
-                       outctr ? c.host_res_dev : nullptr);
+                       publish);
     HIPCHECK(hipGetLastError());
     return tcap - 1;
   }
 
   // rows: level L-1 table (n rows of L words) living in lv[L-1].rows (or an
   // offset into it); produces position L.
-  void level(uint32_t L, const uint32_t* rows, uint64_t n) {
+  // dn non-null: the input level's size is on the device (its expansion's
+  // row counter) and n is only its bound; the caller has checked
+  // chainable(L)
+  void level(uint32_t L, const uint32_t* rows, uint64_t n, const unsigned long long* dn = nullptr) {
     if (n == 0) return;
     LevelBuf& in = lv[L - 1];
     if (chain && !chain_fused && L == k - 2) {
@@ -2513,7 +2551,7 @@ struct Matcher {
       // the prefixes' list reads belong to the expansion that performs them
       if (merge) r.merge_bytes[mclass] += (double)read_u64(merge);
     }
-    if (spec && level_spec(L, rows, n, in, mode, use_cbase, small)) return;
+    if (spec && level_spec(L, rows, n, in, mode, use_cbase, small, dn)) return;
     // (a small level that did not fit has been planned: the scan below reads it)
     const uint64_t W = scan(in, n);
     if (W == 0) return;
@@ -2664,7 +2702,8 @@ struct Matcher {
   // One level without the readback of its work (see speculate()).  Returns
   // false when a writing level did not fit: the caller then runs it with the
   // readback (its scan is repeated; nothing was written or counted).
-  bool level_spec(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in, int mode, bool use_cbase, bool small) {
+  bool level_spec(uint32_t L, const uint32_t* rows, uint64_t n, LevelBuf& in, int mode, bool use_cbase, bool small,
+                  const unsigned long long* dn = nullptr) {
     if (!small) scan_launch(in, n);
     LevelBuf& out = lv[L];
     uint64_t row_cap = ~0ull;
@@ -2682,21 +2721,28 @@ struct Matcher {
       ensure_aux(arena, in.nsurv, in.nsurv_cap, n);
       HIPCHECK(hipMemsetAsync(in.nsurv, 0, n * sizeof(uint32_t), c.stream));
     }
-    // a writing level's row count and status arrive in host memory, from
-    // the tile index (no tiles) or the expansion's last block
-    if (mode == M_WRITE) c.host_res[0] = ~0ull;
+    // a writing level's row count and status arrive in host memory (a slot
+    // per level), from the tile index (no tiles) or the expansion's last
+    // block
+    uint64_t* pub_dev = c.host_res_dev + 32 + 2 * L;
+    volatile uint64_t* pub = c.host_res + 32 + 2 * L;
+    if (mode == M_WRITE) pub[0] = ~0ull;
+    // a level whose input size is on the device cannot be redone here: a
+    // misfit goes to the end-of-run status word and repeats the run
+    unsigned long long* st_word = (mode == M_WRITE && !dn) ? ctr + 11 : ctr + 12;
     uint64_t tmax;
     if (small) {
       // plan + scan + tile index in one launch
       ensure_tiles(arena, in, std::max<uint64_t>(in.tile_cap, 4097) - 1);
       tmax = in.tile_cap - 1;
       unsigned int* oc = mode == M_WRITE ? (unsigned int*)(ctr + 10) : nullptr;
-#define PSM(LL) hipLaunchKernelGGL((k_plan_small<LL>), dim3(1), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.minlen, in.minsel, use_cbase ? in.cbase : nullptr, tbl, mode == M_PAIR ? in.pbase : nullptr, ptbl, ctr, in.work_off, oc ? MAX_CHUNK : MAX_SPEC_ITEMS, in.tile_cap, row_cap, in.tile_p, (unsigned int*)(ctr + 4), oc, ctr + 14, oc ? ctr + 11 : ctr + 12, ctr + 13, oc ? c.host_res_dev : nullptr)
+#define PSM(LL) hipLaunchKernelGGL((k_plan_small<LL>), dim3(1), dim3(BLOCK), 0, c.stream, g, kp, rows, n, in.minlen, in.minsel, use_cbase ? in.cbase : nullptr, tbl, mode == M_PAIR ? in.pbase : nullptr, ptbl, ctr, in.work_off, oc ? MAX_CHUNK : MAX_SPEC_ITEMS, in.tile_cap, row_cap, in.tile_p, (unsigned int*)(ctr + 4), oc, ctr + 14, st_word, ctr + 13, oc ? pub_dev : nullptr, dn, dn ? c.lvcnt + (L - 1) : nullptr)
       CUTS_DISPATCH_L(L, PSM)
 #undef PSM
       HIPCHECK(hipGetLastError());
     } else {
-      tmax = tile_index_dyn(in, n, row_cap, mode == M_WRITE ? (unsigned int*)(ctr + 10) : nullptr);
+      tmax = tile_index_dyn(in, n, row_cap, mode == M_WRITE ? (unsigned int*)(ctr + 10) : nullptr,
+                            mode == M_WRITE ? pub_dev : nullptr);
     }
     ExpandArgs a{};
     a.rows = rows;
@@ -2728,26 +2774,33 @@ struct Matcher {
       // written to host memory by the expansion's last block; with no
       // block taking part (no work, or a level that did not fit) they are
       // read back
-      a.publish = c.host_res_dev;
+      a.publish = pub_dev;
       a.done = (unsigned int*)(ctr + 8);
-      a.status = ctr + 11;
+      a.status = st_word;
 #define EW(LL) hipLaunchKernelGGL((k_expand<LL, M_WRITE>), dim3(grid), dim3(BLOCK), 0, c.stream, g, kp, a)
       CUTS_DISPATCH_L(L, EW)
 #undef EW
       HIPCHECK(hipGetLastError());
-      const volatile uint64_t* hr = c.host_res;
-      uint64_t got[2] = {~0ull, 0};
-      if (wait_published(0)) {
-        got[0] = hr[0];
-        got[1] = hr[1];
+      // the last complete run's size of this level fits one planning block:
+      // the next level takes it from the device, with no synchronisation
+      if (hint && hint[L] > 0 && hint[L] <= (uint64_t)SCAN_TILE && chainable(L + 1)) {
+        chained_mask |= 1u << L;
+        level(L + 1, out.rows, SCAN_TILE, ctr + 10);
+        return true;
       }
+      HIPCHECK(hipStreamSynchronize(c.stream));
+      uint64_t got[2] = {pub[0], pub[1]};
       if (got[0] == ~0ull) {
         HIPCHECK(hipMemcpyAsync(c.host_pinned, ctr + 10, 16, hipMemcpyDeviceToHost, c.stream));
         HIPCHECK(hipStreamSynchronize(c.stream));
         got[0] = c.host_pinned[0];
         got[1] = c.host_pinned[1];
+        if (dn) got[1] = 0;  // a misfit of this level is in the end-of-run word
       }
       const uint64_t m = got[0] & 0xFFFFFFFFu;
+      // a level whose input size was on the device and that did not fit:
+      // the end-of-run word repeats the run
+      if (dn && (got[1] & 1ull)) return true;
       if (got[1] & 1ull) {
         HIPCHECK(hipMemsetAsync(ctr + 11, 0, 8, c.stream));
         r.chunks--;
@@ -3123,6 +3176,17 @@ struct Matcher {
       ptbl.b = kp.pr_tb > 1 ? cnt_table(c, dg, arena, kp.pr_tb) : nullptr;
       ptbl.ab = kp.pr_tab > 1 ? cnt_table(c, dg, arena, kp.pr_tab) : nullptr;
     }
+    // level sizes of this query's last complete run (a hint only: a level
+    // that outgrows it repeats the run)
+    std::string hint_key = std::to_string((uintptr_t)&dg) + "/" + std::to_string((uintptr_t)&plan);
+    for (uint32_t v : plan.order) hint_key += "," + std::to_string(v);
+    for (uint32_t v : plan.par) hint_key += ";" + std::to_string(v);
+    hint_key += "/" + std::to_string(o.root_begin) + "/" + std::to_string(o.root_end) + "/" +
+                std::to_string(o.root_stride) + "/" + std::to_string(o.flags) + "/" + std::to_string(o.trunks);
+    if (!arena) {
+      const auto it = c.level_hints.find(hint_key);
+      hint = it == c.level_hints.end() ? nullptr : it->second.data();
+    }
     // level 0
     const uint32_t rb = o.root_stride ? o.root_begin : 0;
     const uint32_t re = o.root_stride ? std::min(o.root_end, dg.V) : dg.V;
@@ -3204,7 +3268,8 @@ struct Matcher {
       const volatile uint64_t* hp = c.host_res + 16;
       c.host_res[16] = ~0ull;
       hipLaunchKernelGGL(k_publish_end, dim3(1), dim3(64), 0, c.stream, ctr,
-                         r.roots_done ? l0.rows + r.roots_done - 1 : nullptr, c.host_res_dev + 16);
+                         r.roots_done ? l0.rows + r.roots_done - 1 : nullptr, (const unsigned long long*)c.lvcnt,
+                         c.host_res_dev + 16);
       HIPCHECK(hipGetLastError());
       HIPCHECK(hipStreamSynchronize(c.stream));
       if (r.roots_done) {
@@ -3213,6 +3278,9 @@ struct Matcher {
       }
       spec_overflow = (hp[12] & 1ull) != 0;
       r.work += hp[13];
+      // sizes of the levels whose successor read them on the device
+      for (uint32_t l = 0; l < k; ++l)
+        if ((chained_mask >> l) & 1) r.level[l] = hp[48 + l];
       if (!write) {
         r.count = hp[0];
         r.level[k - 1] = r.count;
@@ -3246,6 +3314,11 @@ struct Matcher {
     float ms = 0;
     HIPCHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
     r.device_ms = ms;
+    if (!arena && r.complete && !spec_overflow && !aborted && !r.work_capped) {
+      std::array<uint64_t, kMaxQuery> sizes{};
+      for (uint32_t l = 0; l < k; ++l) sizes[l] = r.level[l];
+      c.level_hints[hint_key] = sizes;
+    }
   }
 };
 
@@ -3542,6 +3615,7 @@ Context::~Context() {
   if (counters) (void)hipFree(counters);
   if (host_pinned) (void)hipHostFree(host_pinned);
   if (host_res) (void)hipHostFree(host_res);
+  if (lvcnt) (void)hipFree(lvcnt);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   for (hipEvent_t e : kev)
@@ -3571,7 +3645,8 @@ void context_init(Context& c, int device) {
   HIPCHECK(hipEventCreateWithFlags(&c.copy_ev[1], hipEventDisableTiming));
   HIPCHECK(hipMalloc(&c.counters, 16 * sizeof(unsigned long long)));
   HIPCHECK(hipHostMalloc(&c.host_pinned, 256, hipHostMallocDefault));
-  HIPCHECK(hipHostMalloc(&c.host_res, 256, hipHostMallocCoherent | hipHostMallocMapped));
+  HIPCHECK(hipHostMalloc(&c.host_res, 1024, hipHostMallocCoherent | hipHostMallocMapped));
+  HIPCHECK(hipMalloc(&c.lvcnt, kMaxQuery * sizeof(unsigned long long)));
   HIPCHECK(hipHostGetDevicePointer((void**)&c.host_res_dev, c.host_res, 0));
   size_t fr = 0, tot = 0;
   HIPCHECK(hipMemGetInfo(&fr, &tot));
