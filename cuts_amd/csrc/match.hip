@@ -310,7 +310,8 @@ __global__ __launch_bounds__(BLOCK) void k_level0_scatter(const uint32_t* __rest
 constexpr uint32_t L0_THREADS = 1024;
 __global__ __launch_bounds__(L0_THREADS) void k_level0_small(const uint32_t* __restrict__ sig, uint32_t tin,
                                                             uint32_t tout, uint32_t rb, uint32_t re, uint32_t rs,
-                                                            uint32_t n, uint32_t* __restrict__ out, uint64_t* total) {
+                                                            uint32_t n, uint32_t* __restrict__ out, uint64_t* total,
+                                                            unsigned long long* __restrict__ dev_total) {
   __shared__ uint32_t s[L0_THREADS];
   const uint32_t t = threadIdx.x;
   const uint32_t per = (n + L0_THREADS - 1) / L0_THREADS;  // <= 64
@@ -343,23 +344,30 @@ __global__ __launch_bounds__(L0_THREADS) void k_level0_small(const uint32_t* __r
   uint32_t pos = s[t] - c;
   for (uint32_t i = i0; i < i1; ++i)
     if ((keep >> (i - i0)) & 1ull) out[pos++] = rb + i * rs;
-  if (t == L0_THREADS - 1) __hip_atomic_store(total, (uint64_t)s[t], __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  if (t == L0_THREADS - 1) {
+    __hip_atomic_store(total, (uint64_t)s[t], __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (dev_total) *dev_total = s[t];
+  }
 }
 
-// The run's counters (words 0..13), when last is non-null the last
-// processed root, and the device-side level counts (16 words, to pub[48..]),
-// written to host memory (system-scope stores) for one synchronisation with
-// no copy.
+// The run's counters (words 0..13), the last processed root (*last, or
+// rows0[lvcnt[0] - 1] when level 0's size stayed on the device), and the
+// device-side level counts (16 words, to pub[48..]), written to host memory
+// (system-scope stores) for one synchronisation with no copy.
 __global__ void k_publish_end(const unsigned long long* __restrict__ ctr, const uint32_t* __restrict__ last,
-                              const unsigned long long* __restrict__ lvcnt, uint64_t* pub) {
+                              const uint32_t* __restrict__ rows0, const unsigned long long* __restrict__ lvcnt,
+                              uint64_t* pub) {
   const uint32_t t = threadIdx.x;
-  if (t < 14)
+  if (t < 14) {
     __hip_atomic_store(&pub[t], (uint64_t)__hip_atomic_load(&ctr[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-  else if (t == 14 && last)
-    __hip_atomic_store(&pub[14], (uint64_t)*last, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-  else if (t >= 32 && t < 32 + kMaxQuery)
+  } else if (t == 14 && (last || rows0)) {
+    const uint64_t n0 = rows0 ? lvcnt[0] : 1;
+    const uint32_t lr = last ? *last : (n0 ? rows0[n0 - 1] : 0u);
+    __hip_atomic_store(&pub[14], (uint64_t)lr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  } else if (t >= 32 && t < 32 + kMaxQuery) {
     __hip_atomic_store(&pub[16 + t], (uint64_t)lvcnt[t - 32], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
 }
 
 // ------------------------------------------------------------ planning
@@ -3201,15 +3209,24 @@ struct Matcher {
     const auto t_start = std::chrono::steady_clock::now();
     uint64_t n0 = 0;
     bool last_root_read = false;
+    // level 0 of at most one planning block's roots in the last complete
+    // run: level 1 takes its size from the device
+    const bool chain0 = hint && hint[0] > 0 && hint[0] <= (uint64_t)SCAN_TILE && k > 1 && !(o.time_limit_ms > 0) &&
+                        n0c && n0c <= (1u << 16) && chainable(1);
     if (n0c && n0c <= (1u << 16)) {
       pbegin(false);
       c.host_res[2] = ~0ull;
       hipLaunchKernelGGL(k_level0_small, dim3(1), dim3(L0_THREADS), 0, c.stream, dg.sig, kp.sig_in[0], kp.sig_out[0], rb,
-                         re, rs, n0c, l0.rows, c.host_res_dev + 2);
+                         re, rs, n0c, l0.rows, c.host_res_dev + 2, chain0 ? ctr + 10 : nullptr);
       HIPCHECK(hipGetLastError());
       pend(0, 16.0 * n0c, false);
-      if (!wait_published(2)) throw Error(-4, "level 0: no size published");
-      n0 = ((const volatile uint64_t*)c.host_res)[2];
+      if (chain0) {
+        chained_mask |= 1u;
+        n0 = SCAN_TILE;  // a bound; the size is read at the end
+      } else {
+        if (!wait_published(2)) throw Error(-4, "level 0: no size published");
+        n0 = ((const volatile uint64_t*)c.host_res)[2];
+      }
     } else if (n0c) {
       pbegin(false);
       hipLaunchKernelGGL(k_level0_flags, dim3(grid_for(n0c, BLOCK, 65535)), dim3(BLOCK), 0, c.stream, dg.sig,
@@ -3250,7 +3267,7 @@ struct Matcher {
           snap[1] = c.host_pinned[1];
           snap[2] = c.host_pinned[2];
         }
-        level(1, l0.rows + s, std::min(ts, n0 - s));
+        level(1, l0.rows + s, std::min(ts, n0 - s), (chained_mask & 1u) ? ctr + 10 : nullptr);
         if (aborted) {
           c.host_pinned[0] = snap[0];
           c.host_pinned[1] = snap[1];
@@ -3267,8 +3284,10 @@ struct Matcher {
       // their status, and the last processed root: one readback
       const volatile uint64_t* hp = c.host_res + 16;
       c.host_res[16] = ~0ull;
+      const bool dev0 = (chained_mask & 1u) != 0;
       hipLaunchKernelGGL(k_publish_end, dim3(1), dim3(64), 0, c.stream, ctr,
-                         r.roots_done ? l0.rows + r.roots_done - 1 : nullptr, (const unsigned long long*)c.lvcnt,
+                         (r.roots_done && !dev0) ? l0.rows + r.roots_done - 1 : nullptr,
+                         dev0 ? (const uint32_t*)l0.rows : nullptr, (const unsigned long long*)c.lvcnt,
                          c.host_res_dev + 16);
       HIPCHECK(hipGetLastError());
       HIPCHECK(hipStreamSynchronize(c.stream));
@@ -3278,9 +3297,15 @@ struct Matcher {
       }
       spec_overflow = (hp[12] & 1ull) != 0;
       r.work += hp[13];
-      // sizes of the levels whose successor read them on the device
+      // sizes of the levels whose successor read them on the device (level
+      // 0's: every root processed in one trunk)
       for (uint32_t l = 0; l < k; ++l)
         if ((chained_mask >> l) & 1) r.level[l] = hp[48 + l];
+      if (dev0) {
+        r.roots_done = r.level[0];
+        last_root_read = r.roots_done > 0;
+        if (last_root_read) r.last_root = (uint32_t)(hp[14] & 0xFFFFFFFFu);
+      }
       if (!write) {
         r.count = hp[0];
         r.level[k - 1] = r.count;
