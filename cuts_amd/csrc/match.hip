@@ -3340,6 +3340,9 @@ struct Matcher {
     HIPCHECK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
     r.device_ms = ms;
     if (!arena && r.complete && !spec_overflow && !aborted && !r.work_capped) {
+      // at most 2^16 queries remembered (about 10 MB); past that the
+      // table starts over
+      if (c.level_hints.size() >= (1u << 16) && !c.level_hints.count(hint_key)) c.level_hints.clear();
       std::array<uint64_t, kMaxQuery> sizes{};
       for (uint32_t l = 0; l < k; ++l) sizes[l] = r.level[l];
       c.level_hints[hint_key] = sizes;
