@@ -481,7 +481,13 @@ def run_headline(args):
     log(f"learned orders: {proposed} proposed, {kept} kept by the judge ({plan_s:.1f}s planning, {judge_s:.1f}s judging)")
     if args.plain_pass:
         # one step of the timed path alone, learned orders (for a rocprofv3
-        # kernel trace of the launch sequence as timed: no per-launch events)
+        # kernel trace of the launch sequence as timed: no per-launch
+        # events), after one step that leaves every query its level hints as
+        # the warm-up steps do; a 1 MiB copy_probe marks where the measured
+        # step starts in the trace
+        for p, rt in zip(learned, roots):
+            ctx.match(dg, p, roots=rt, **shape)
+        ctx.copy_probe(1 << 20, 1)
         t = 0.0
         for i, (p, rt) in enumerate(zip(learned, roots)):
             r = ctx.match(dg, p, roots=rt, **shape)
