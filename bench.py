@@ -737,8 +737,8 @@ def main():
     ap.add_argument("--plain-launches", action="store_true", help="batch256 --score-only: no hipGraph")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: only one GPU is used by this build (asked for --gpus {args.gpus}); "
-              "the matcher shards by level-0 chunks, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py: the matcher runs on one GPU (asked for --gpus {args.gpus}); "
+              "more GPUs would each take their share of the level-0 roots, which is not built", file=sys.stderr)
         sys.exit(2)
     {"rmat20": run_headline, "chunglu36k": run_chunglu36k, "chunglu4k": run_chunglu4k,
      "batch256": run_batch256, "write": run_write}[args.workload](args)
